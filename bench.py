@@ -1,0 +1,214 @@
+#!/usr/bin/env python
+"""Benchmark: images/s of the batched image + log-likelihood evaluation.
+
+Workload (one step): log-likelihoods of a batch of 64 parameter sets of the
+EPL + SHEAR deflector with SERSIC_ELLIPSE lens light and source, 180x180 pixels
+at 0.04"/pix, supersampling 3 (540x540 rays), 41x41 Moffat PSF, against one
+mock data image; parameters and data are resident in HBM before timing.  The
+headline value is the mixed-precision form (float32 profiles and convolution,
+float64 coordinates and chi^2 sums); the plain FP64 form is timed beside it.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_models(batch):
+    import torch
+
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model import LensModel
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+
+    cfg = W.EPL180
+    kernel = W.moffat_kernel(cfg["psf_size"], cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    draws = [W.epl180_draw(d) for d in range(batch)]
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+
+    probe = model("fp64")
+    truth = probe.image(*draws[0])
+    data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
+    im64 = model("fp64", data)
+    im32 = model("mixed", data)
+    kl = W.stack_kwargs([d[0] for d in draws])
+    ks = W.stack_kwargs([d[1] for d in draws])
+    kll = W.stack_kwargs([d[2] for d in draws])
+    params_np, n = im64.pack(kl, ks, kll)
+    params = torch.as_tensor(params_np, device="cuda")
+    return cfg, kernel, draws, data, params, params_np, im64, im32
+
+
+def time_steps(im, params, steps, warmup, backend):
+    import torch
+
+    logl = torch.empty(params.shape[0], dtype=torch.float64, device=params.device)
+    for _ in range(warmup):
+        im.likelihood_batch(params, logl=logl)
+    torch.cuda.synchronize()
+    backend.timing_read()  # drop warm-up records
+    backend.timing_enable(True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record()
+    for i in range(steps):
+        im.likelihood_batch(params, logl=logl)
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    backend.timing_enable(False)
+    render_ms, conv_ms, calls = backend.timing_read()
+    per_step = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(steps)])
+    total_ms = evs[0].elapsed_time(evs[-1])
+    return dict(total_ms=total_ms, wall_s=wall, per_step=per_step, render_ms=render_ms / max(calls, 1),
+                conv_ms=conv_ms / max(calls, 1), logl=logl.cpu().numpy())
+
+
+def render_work(params_np, n_ray):
+    """Algorithmic flops and special-function evaluations of one render launch:
+    per ray EPL 72 + 8 (N_terms - 1) flops and 6 specials (N_terms = the series
+    length chosen from the axis ratio), SHEAR 10, deflected position 2, two
+    SERSIC_ELLIPSE 17 + 4 specials each, pooling 1."""
+    from jaxtronomy_amd.Util.param_util import ellipticity2phi_q
+
+    F = S = 0.0
+    for p in params_np:
+        _, q = ellipticity2phi_q(p[0, 2], p[0, 3])
+        f = (1 - q) / (1 + q)
+        nterms = 1 if f == 0 else (10 if f < 0.12 else 20 if f < 0.4 else 35 if f < 0.6 else 50 if f < 0.73
+                                   else 100 if f < 0.86 else 200 if f < 0.92 else 500)
+        F += n_ray * (72 + 8 * (nterms - 1) + 10 + 2 + 17 + 17 + 1)
+        S += n_ray * (6 + 4 + 4)
+    return F, S
+
+
+def cpu_baseline(cfg, kernel, params_np, data, n_sample):
+    from oracle import c_oracle
+    from jaxtronomy_amd import workloads as W
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    g = W.grid_dict(cfg)
+    sample = params_np[:n_sample]
+    c_oracle.lib()
+    t0 = time.perf_counter()
+    ll = c_oracle.loglike(2, 1, 1, [1, 6, 33, 33], sample, g, data, kernel=kernel, psf_type=1,
+                          exposure_time=cfg["exposure_time"], background_rms=cfg["background_rms"],
+                          n_threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="images/s", cores=threads, kind="port",
+                sample="%d log-likelihood evaluations of the benchmark batch (C FP64 restatement of the reference "
+                       "algorithm, OpenMP, %.1f s wall)" % (n_sample, dt)), ll
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a design "
+              "note in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from jaxtronomy_amd import backend
+
+    backend.require_gpu()
+    torch.cuda.set_device(0)
+    log("bench: building models")
+    cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch)
+    n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
+
+    log("bench: timing mixed precision (%d steps)" % args.steps)
+    mixed = time_steps(im32, params, args.steps, args.warmup, backend)
+    log("bench: timing fp64 (%d steps)" % args.fp64_steps)
+    fp64 = time_steps(im64, params, args.fp64_steps, max(2, args.warmup // 2), backend)
+
+    # accuracy of the mixed form against the FP64 form of the same batch (the
+    # FP64 form itself is held to 1e-10 of the oracle by tests/test_gpu_parity.py)
+    ll_rel = float(np.max(np.abs(mixed["logl"] - fp64["logl"]) / np.abs(fp64["logl"])))
+    img64 = im64.image_batch(params[:4]).cpu().numpy()
+    img32 = im32.image_batch(params[:4]).double().cpu().numpy()
+    img_rel = float(max(np.max(np.abs(img32[i] - img64[i])) / np.max(np.abs(img64[i])) for i in range(4)))
+
+    value = args.batch * args.steps / (mixed["total_ms"] / 1e3)
+    value64 = args.batch * args.fp64_steps / (fp64["total_ms"] / 1e3)
+    F, S = render_work(params_np, n_ray)
+    achieved = (F + 8 * S) / (mixed["render_ms"] / 1e3) / 1e12
+    achieved64 = (F + 20 * S) / (fp64["render_ms"] / 1e3) / 1e12
+    conv_flops = 2.0 * kernel.size * cfg["numPix"] ** 2 * args.batch
+    line = {
+        "metric": "images/s + p99 latency, EPL+Sersic 180^2 ss3, served mixed-precision vs plain FP64",
+        "value": round(value, 1),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["total_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "EPL+SHEAR deflector, SERSIC_ELLIPSE lens light + source, 180x180 @0.04\", "
+                        "supersampling 3, 41x41 Moffat PSF, image + chi2 logL",
+            "batch": args.batch,
+            "precision": "mixed: f32 profiles/convolution, f64 coordinates and chi2 sums",
+        },
+        "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
+        "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4),
+        "fp64": {"value": round(value64, 1), "ms_per_step": round(fp64["total_ms"] / args.fp64_steps, 4),
+                 "p99_ms": round(float(np.percentile(fp64["per_step"], 99)), 4),
+                 "render_ms": round(fp64["render_ms"], 4), "conv_ms": round(fp64["conv_ms"], 4),
+                 "render_tflops_equiv": round(achieved64, 3)},
+        "mixed_over_fp64": round(value / value64, 2),
+        "accuracy": {"mixed_image_max_rel_vs_fp64": img_rel, "mixed_logl_max_rel_vs_fp64": ll_rel},
+        "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4)},
+        "roofline": {"bound": "valu", "kernel": "render_kernel<float>", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None,
+                     "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
+                     "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3)},
+    }
+    if not args.no_cpu:
+        log("bench: cpu baseline (%d images)" % args.cpu_sample)
+        cb, ll_cpu = cpu_baseline(cfg, kernel, params_np, data, args.cpu_sample)
+        line["cpu_baseline"] = cb
+        line["accuracy"]["fp64_logl_max_rel_vs_cpu_oracle"] = float(
+            np.max(np.abs(fp64["logl"][:args.cpu_sample] - ll_cpu) / np.abs(ll_cpu)))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

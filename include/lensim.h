@@ -1,0 +1,188 @@
+/*
+ * lensim.h -- C ABI of the MI355X (gfx950) strong-lensing image-simulation
+ * library (liblensim.so).
+ *
+ * One shared library, plain pointers and sizes, no torch types.  Every
+ * buffer argument named d_* is DEVICE memory on the current HIP device;
+ * every call is asynchronous on the given HIP stream (NULL = default stream)
+ * and allocates nothing: scratch memory comes in through `ws` (query its
+ * size with lensim_workspace_size).  All entry points return 0 on success or
+ * a negative lensim_status; lensim_last_error() returns a message for the
+ * last failure on the calling thread.
+ *
+ * Parameter blocks.  A model is an ordered list of profiles: n_lens
+ * deflectors, then n_source source-light profiles, then n_lens_light
+ * lens-light profiles.  Parameters of a batch of n_img models are one
+ * float64 array of shape [n_img][n_profiles][LENSIM_NPARAM]; each profile
+ * slot holds that profile's parameters in the order of the reference's
+ * `param_names` (unused trailing slots are ignored):
+ *   EPL            theta_E gamma e1 e2 center_x center_y    (LensModel/Profiles/epl.py:50)
+ *   SIE            theta_E e1 e2 center_x center_y           (LensModel/Profiles/sie.py:40)
+ *   NIE            theta_E e1 e2 s_scale center_x center_y   (LensModel/Profiles/nie.py)
+ *   NFW            Rs alpha_Rs center_x center_y             (LensModel/Profiles/nfw.py:48)
+ *   TNFW           Rs alpha_Rs r_trunc center_x center_y     (LensModel/Profiles/tnfw.py:27)
+ *   SHEAR          gamma1 gamma2 ra_0 dec_0                  (LensModel/Profiles/shear.py:16)
+ *   SERSIC_ELLIPSE amp R_sersic n_sersic e1 e2 center_x center_y
+ *                                                 (LightModel/Profiles/sersic_ellipse.py:25)
+ *   SERSIC         amp R_sersic n_sersic center_x center_y  (LightModel/Profiles/sersic.py)
+ *   GAUSSIAN       amp sigma center_x center_y               (LightModel/Profiles/gaussian.py)
+ */
+#ifndef LENSIM_H
+#define LENSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LENSIM_ABI_VERSION 1
+#define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
+#define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
+#define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
+
+typedef enum {
+  LENSIM_OK = 0,
+  LENSIM_EINVAL = -1,        /* bad argument (shape, pointer, size) */
+  LENSIM_EUNSUPPORTED = -2,  /* profile / option not supported */
+  LENSIM_EDEVICE = -3,       /* HIP error */
+  LENSIM_EWORKSPACE = -4     /* workspace too small */
+} lensim_status;
+
+typedef enum {
+  LENSIM_FP64 = 0,  /* plain FP64 form: every step in float64 */
+  LENSIM_MIXED = 1  /* float32 profiles and convolution, float64 coordinates and chi^2 sums */
+} lensim_precision;
+
+typedef enum {
+  LENSIM_NULL = 0,
+  /* deflectors */
+  LENSIM_EPL = 1,
+  LENSIM_SIE = 2,
+  LENSIM_NIE = 3,
+  LENSIM_NFW = 4,
+  LENSIM_TNFW = 5,
+  LENSIM_SHEAR = 6,
+  /* light profiles */
+  LENSIM_SERSIC = 32,
+  LENSIM_SERSIC_ELLIPSE = 33,
+  LENSIM_GAUSSIAN = 34
+} lensim_profile;
+
+typedef enum {
+  LENSIM_PSF_NONE = 0,
+  LENSIM_PSF_PIXEL = 1,    /* zero-padded 'same' convolution (ImSim/Numerics/convolution.py:74-86) */
+  LENSIM_PSF_GAUSSIAN = 2  /* edge-padded 'same' convolution (ImSim/Numerics/convolution.py:217-236) */
+} lensim_psf_type;
+
+/* Flags of lensim_image */
+#define LENSIM_IMAGE_SOURCE      1u  /* add the lensed source (source_add) */
+#define LENSIM_IMAGE_LENS_LIGHT  2u  /* add the lens light (lens_light_add) */
+#define LENSIM_IMAGE_UNCONVOLVED 4u  /* skip the PSF (unconvolved=True) */
+
+/* Model description: the profile lists of LensModel(lens_model_list),
+ * LightModel(source list) and LightModel(lens light list)
+ * (LensModel/lens_model.py:25-216, LightModel/light_model_base.py:63-228). */
+typedef struct {
+  int32_t n_lens;
+  int32_t n_source;
+  int32_t n_lens_light;
+  int32_t types[LENSIM_MAX_PROFILES]; /* lensim_profile, lens then source then lens light */
+  double alpha_scaling;               /* SinglePlane alpha_scaling (LensModel/single_plane.py:139) */
+  double flux_scaling;                /* ImageData flux_scaling (ImSim/image_model.py:60-63) */
+} lensim_model;
+
+/* Pixel grid + supersampling (ImSim/Numerics/grid.py:22-64): pixel (i, j)
+ * (column i fastest) has centre ra = ra_at_xy_0 + M00 i + M01 j,
+ * dec = dec_at_xy_0 + M10 i + M11 j; images are [ny][nx]. */
+typedef struct {
+  int32_t nx;
+  int32_t ny;
+  int32_t supersampling;
+  int32_t reserved;
+  double ra_at_xy_0;
+  double dec_at_xy_0;
+  double pix2angle[4]; /* M00 M01 M10 M11 */
+} lensim_grid;
+
+/* PSF convolution (ImSim/Numerics/numerics.py:102-146). */
+typedef struct {
+  int32_t type;           /* lensim_psf_type */
+  int32_t kx;             /* odd kernel width */
+  int32_t ky;             /* odd kernel height */
+  int32_t reserved;
+  const double *d_kernel; /* [ky][kx], float64 device memory */
+} lensim_psf;
+
+/* Data and noise of one image for the likelihood
+ * (Data/imaging_data.py:159-182, Data/image_noise.py:100-137). */
+typedef struct {
+  const double *d_data;         /* [ny][nx] observed image */
+  const double *d_noise_map;    /* [ny][nx] 1-sigma noise, or NULL */
+  const double *d_exposure_map; /* [ny][nx] exposure time, or NULL to use exposure_time */
+  const double *d_mask;         /* [ny][nx] likelihood mask (0/1), or NULL for all ones */
+  double exposure_time;         /* used when d_exposure_map is NULL */
+  double background_rms;
+} lensim_data;
+
+/* Library version and a message for the last failure on this thread. */
+int lensim_abi_version(void);
+const char *lensim_last_error(void);
+
+/* Bytes of scratch memory the calls below need for a batch of n_img models
+ * (grid and psf may be NULL for lensim_raytrace / lensim_surface_brightness;
+ * psf may be NULL for an unconvolved image).  Returns 0 on a bad argument. */
+size_t lensim_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                             int32_t n_img, int32_t precision);
+
+/* Ray shooting: beta = theta - alpha_scaling * sum_k alpha_k(theta) for every
+ * image of the batch at the same n_ray coordinates.
+ * Replaces LensModel.ray_shooting (LensModel/lens_model.py:236-252) ->
+ * SinglePlane.ray_shooting/alpha (LensModel/single_plane.py:38-56, :112-139).
+ * d_x, d_y: [n_ray]; d_beta_x, d_beta_y: [n_img][n_ray] float64. */
+int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n_img, const double *d_x,
+                    const double *d_y, int64_t n_ray, int32_t precision, double *d_beta_x, double *d_beta_y,
+                    void *ws, size_t ws_bytes, void *stream);
+
+/* Surface brightness of the model's source-light profiles (which = 0) or
+ * lens-light profiles (which = 1) at n_ray coordinates.
+ * Replaces LightModel.surface_brightness (LightModel/light_model_base.py:230-252).
+ * d_flux: [n_img][n_ray] float64. */
+int lensim_surface_brightness(const lensim_model *model, const double *d_params, int32_t n_img,
+                              int32_t which, const double *d_x, const double *d_y, int64_t n_ray,
+                              int32_t precision, double *d_flux, void *ws, size_t ws_bytes, void *stream);
+
+/* Model images of a batch: ray shooting + source light + lens light on the
+ * supersampled grid, s x s mean, PSF convolution, x pixel_width^2 x flux_scaling.
+ * Replaces ImageModel.image (ImSim/image_model.py:431-492) with its
+ * Numerics.re_size_convolve (ImSim/Numerics/numerics.py:158-177).
+ * d_out: [n_img][ny][nx], float64 for LENSIM_FP64 and float32 for LENSIM_MIXED. */
+int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                 const double *d_params, int32_t n_img, int32_t precision, uint32_t flags, void *d_out,
+                 void *ws, size_t ws_bytes, void *stream);
+
+/* log-likelihood of every model of a batch against one data image:
+ * -1/2 sum mask (model - data)^2 / (C_D(model) + |error|).
+ * Replaces ImageModel.likelihood_data_given_model (ImSim/image_model.py:162-221)
+ * -> ImageData._log_likelihood (Data/imaging_data.py:159-182).
+ * d_logl: [n_img] float64.  d_model_out: optional [n_img][ny][nx] model images
+ * (dtype as lensim_image) or NULL. */
+int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                   const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                   double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream);
+
+/* Kernel timing for benchmarks (no counterpart in the reference).  While
+ * enabled, every lensim_image / lensim_loglike call records HIP events on its
+ * own stream around its render kernel and its convolution kernel.
+ * lensim_timing_read waits for the recorded events, returns the summed
+ * milliseconds of each kernel and the number of calls since the last read, and
+ * resets the sums.  Off by default; not thread-safe. */
+int lensim_timing_enable(int on);
+int lensim_timing_read(double *render_ms, double *conv_ms, int64_t *calls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LENSIM_H */

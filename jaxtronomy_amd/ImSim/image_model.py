@@ -1,0 +1,305 @@
+"""ImageModel: lensed images and their likelihood on the GPU.
+
+Mirrors the reference's ``ImageModel`` (ImSim/image_model.py:23-694) for the
+analytic single-band path: ray shooting -> source light at the deflected
+supersampled grid -> lens light -> s x s pooling -> PSF convolution ->
+x pixel_width^2 x flux_scaling, and the diagonal chi^2 likelihood.  Each call
+is one ``lensim_image`` / ``lensim_loglike`` launch sequence on the current
+stream.
+
+Batches: parameter values may be 1-D arrays of length B (as under
+``jax.vmap``); ``image`` then returns [B, ny, nx] and the likelihood [B].
+``image_batch`` / ``likelihood_batch`` take an already packed device
+parameter block [B, n_profiles, 8] (see include/lensim.h).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import backend, profiles
+from ..Data.psf import cut_psf, gaussian_pixel_kernel
+from ..LensModel.lens_model import LensModel
+from ..LightModel.light_model import LightModel
+
+
+class ImageModel(object):
+    """This class uses functions of lens_model and source_model to make a lensed image."""
+
+    def __init__(self, data_class, psf_class, lens_model_class=None, source_model_class=None,
+                 lens_light_model_class=None, point_source_class=None, extinction_class=None, kwargs_numerics=None,
+                 likelihood_mask=None, psf_error_map_bool_list=None, kwargs_pixelbased=None, precision=None):
+        if kwargs_pixelbased is not None:
+            raise ValueError("pixelbased solver not supported")
+        if point_source_class is not None and len(getattr(point_source_class, "point_source_type_list", [])) > 0:
+            raise NotImplementedError("point sources are not part of this library")
+        if extinction_class is not None:
+            raise NotImplementedError("differential extinction is not supported")
+        self.type = "single-band"
+        self.num_bands = 1
+        self.PSF = psf_class
+        self.Data = data_class
+        self._flux_scaling = getattr(data_class, "flux_scaling", 1)
+        self.PSF.set_pixel_size(self.Data.pixel_width)
+        if lens_model_class is None:
+            lens_model_class = LensModel(lens_model_list=[])
+        if source_model_class is None:
+            source_model_class = LightModel(light_model_list=[])
+        if lens_light_model_class is None:
+            lens_light_model_class = LightModel(light_model_list=[])
+        self.LensModel = lens_model_class
+        self.SourceModel = source_model_class
+        self.LensLightModel = lens_light_model_class
+        if precision is None:
+            precision = getattr(lens_model_class, "precision", "fp64")
+        if precision not in backend.PRECISIONS:
+            raise ValueError("precision must be one of %s" % list(backend.PRECISIONS))
+        self.precision = precision
+        self._prec = backend.PRECISIONS[precision]
+        if kwargs_numerics is None:
+            kwargs_numerics = {}
+        self._kwargs_numerics = kwargs_numerics
+        self._init_numerics(**kwargs_numerics)
+
+        nx, ny = data_class.num_pixel_axes
+        if likelihood_mask is None:
+            likelihood_mask = np.ones((nx, ny))
+        self.likelihood_mask = np.array(likelihood_mask, dtype=bool)
+        self.num_data_evaluate = int(np.sum(self.likelihood_mask))
+        self._psf_error_map = False
+
+        # static device state: grid, PSF kernel, mask, profile lists
+        self._grid = backend.LensimGrid()
+        self._grid.nx, self._grid.ny = int(nx), int(ny)
+        self._grid.supersampling = self._s
+        self._grid.ra_at_xy_0, self._grid.dec_at_xy_0 = data_class.radec_at_xy_0
+        M = np.asarray(data_class.transform_pix2angle, dtype=np.float64)
+        for i, v in enumerate([M[0, 0], M[0, 1], M[1, 0], M[1, 1]]):
+            self._grid.pix2angle[i] = float(v)
+        self._psf = backend.LensimPsf()
+        self._kernel_dev = None
+        if self._kernel is None:
+            self._psf.type = backend.PSF_NONE
+        else:
+            self._kernel_dev = backend.to_device_f64(self._kernel)
+            self._psf.type = backend.PSF_GAUSSIAN if self._psf_type == "GAUSSIAN" else backend.PSF_PIXEL
+            self._psf.ky, self._psf.kx = self._kernel.shape
+            self._psf.d_kernel = self._kernel_dev.data_ptr()
+        self._mask_dev = None
+        self._lens_types = list(self.LensModel._types)
+        self._src_types = list(self.SourceModel._types)
+        self._light_types = list(self.LensLightModel._types)
+        self._names = (list(self.LensModel.param_name_list) + list(self.SourceModel.param_name_list) +
+                       list(self.LensLightModel.param_name_list))
+        self._model = backend.make_model(self._lens_types, self._src_types, self._light_types,
+                                         alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
+        self._submodels = {}
+        self._ws = backend.Workspace()
+
+    # ------------------------------------------------------------ numerics --
+    def _init_numerics(self, supersampling_factor=1, compute_mode="regular", supersampling_convolution=False,
+                       supersampling_kernel_size=5, flux_evaluate_indexes=None, supersampled_indexes=None,
+                       compute_indexes=None, point_source_supersampling_factor=1, convolution_kernel_size=None,
+                       convolution_type="fft", truncation_conv=None):
+        """Numerics.__init__, ImSim/Numerics/numerics.py:24-155."""
+        if compute_mode != "regular":
+            raise ValueError("compute_mode %s not supported; use 'regular'" % compute_mode)
+        if not isinstance(supersampling_factor, int):
+            raise TypeError("supersampling_factor needs to be an integer! Current type is %s"
+                            % type(supersampling_factor))
+        if supersampling_factor < 1:
+            raise ValueError("supersampling_factor must be >= 1")
+        if supersampling_factor == 1:
+            supersampling_convolution = False
+        if supersampling_convolution:
+            raise NotImplementedError("supersampling_convolution=True is not supported yet")
+        if flux_evaluate_indexes is not None:
+            raise NotImplementedError("flux_evaluate_indexes (sub-frame evaluation) is not supported yet")
+        if convolution_type not in ("fft", "grid", "fft_static"):
+            raise ValueError("convolution_type %s not supported!" % convolution_type)
+        self._s = supersampling_factor
+        self._psf_type = self.PSF.psf_type
+        pixel_width = self.Data.pixel_width
+        if self._psf_type == "PIXEL":
+            kernel = np.asarray(self.PSF.kernel_point_source, dtype=np.float64)
+            if convolution_kernel_size is not None:
+                size = convolution_kernel_size
+                if size % 2 == 0:
+                    size += 1
+                kernel = cut_psf(kernel, size)
+            self._kernel = np.ascontiguousarray(kernel)
+        elif self._psf_type == "GAUSSIAN":
+            trunc = self.PSF.truncation if truncation_conv is None else truncation_conv
+            self._kernel = gaussian_pixel_kernel(self.PSF.fwhm, pixel_width, trunc, supersampling_factor, False)
+        elif self._psf_type == "NONE":
+            self._kernel = None
+        else:
+            raise ValueError("psf_type %s not valid! Chose either NONE, GAUSSIAN or PIXEL." % self._psf_type)
+
+    @property
+    def kernel(self):
+        """The convolution kernel used for the PSF (None for psf_type NONE)."""
+        return self._kernel
+
+    # ------------------------------------------------------------- packing --
+    def pack(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None):
+        """Parameter block [B, n_profiles, 8] (float64, host) of lists of kwargs."""
+        kl = kwargs_lens or []
+        ks = kwargs_source or []
+        kll = kwargs_lens_light or []
+        if len(kl) != len(self._lens_types) and len(self._lens_types) > 0:
+            raise ValueError("expected %d lens kwargs, got %d" % (len(self._lens_types), len(kl)))
+        if len(ks) != len(self._src_types) and len(self._src_types) > 0:
+            raise ValueError("expected %d source kwargs, got %d" % (len(self._src_types), len(ks)))
+        if len(kll) != len(self._light_types) and len(self._light_types) > 0:
+            raise ValueError("expected %d lens light kwargs, got %d" % (len(self._light_types), len(kll)))
+        kl = kl if self._lens_types else []
+        ks = ks if self._src_types else []
+        kll = kll if self._light_types else []
+        n_img = profiles.batch_size([kl, ks, kll])
+        return profiles.pack(self._names, list(kl) + list(ks) + list(kll), n_img), n_img
+
+    def _submodel(self, lens_idx, src_idx, light_idx):
+        key = (tuple(lens_idx), tuple(src_idx), tuple(light_idx))
+        if key not in self._submodels:
+            nl, ns = len(self._lens_types), len(self._src_types)
+            rows = list(lens_idx) + [nl + i for i in src_idx] + [nl + ns + i for i in light_idx]
+            m = backend.make_model([self._lens_types[i] for i in lens_idx], [self._src_types[i] for i in src_idx],
+                                   [self._light_types[i] for i in light_idx],
+                                   alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
+            self._submodels[key] = (m, rows)
+        return self._submodels[key]
+
+    # ------------------------------------------------------------ batch API --
+    def workspace_bytes(self, n_img, model=None):
+        lib = backend.lib()
+        m = self._model if model is None else model
+        return lib.lensim_workspace_size(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf), n_img,
+                                         self._prec)
+
+    def image_batch(self, params, n_img=None, flags=backend.IMAGE_SOURCE | backend.IMAGE_LENS_LIGHT, out=None,
+                    model=None):
+        """Images of a packed device parameter block [B, n_profiles, 8]; returns
+        [B, ny, nx] in the compute dtype (float64 or float32)."""
+        lib = backend.lib()
+        m = self._model if model is None else model
+        if n_img is None:
+            n_img = params.shape[0]
+        dtype = torch.float64 if self._prec == backend.FP64 else torch.float32
+        if out is None:
+            out = torch.empty((n_img, self._grid.ny, self._grid.nx), dtype=dtype, device=params.device)
+        ws = self._ws.get(self.workspace_bytes(n_img, m))
+        rc = lib.lensim_image(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
+                              backend.ptr(params), n_img, self._prec, flags, backend.ptr(out), backend.ptr(ws),
+                              ws.numel(), backend.stream_handle())
+        backend.check(rc, "lensim_image")
+        return out
+
+    def _data_struct(self):
+        buf = self.Data.device_buffers()
+        if self._mask_dev is None:
+            self._mask_dev = backend.to_device_f64(self.likelihood_mask.astype(np.float64))
+        d = backend.LensimData()
+        d.d_data = buf["data"].data_ptr()
+        d.d_noise_map = None if buf["noise"] is None else buf["noise"].data_ptr()
+        d.d_exposure_map = None if buf["expo_map"] is None else buf["expo_map"].data_ptr()
+        d.d_mask = self._mask_dev.data_ptr()
+        d.exposure_time = buf["expo_time"]
+        d.background_rms = self.Data.background_rms
+        return d
+
+    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None):
+        """log-likelihoods [B] (float64) of a packed device parameter block."""
+        lib = backend.lib()
+        if n_img is None:
+            n_img = params.shape[0]
+        if logl is None:
+            logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
+        d = self._data_struct()
+        ws = self._ws.get(self.workspace_bytes(n_img))
+        rc = lib.lensim_loglike(ctypes.byref(self._model), ctypes.byref(self._grid), ctypes.byref(self._psf),
+                                ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
+                                backend.ptr(model_out), backend.ptr(ws), ws.numel(), backend.stream_handle())
+        backend.check(rc, "lensim_loglike")
+        return logl
+
+    # ------------------------------------------------------------ drop-in API --
+    @staticmethod
+    def _check_unsupported(kwargs_ps=None, kwargs_extinction=None):
+        if kwargs_ps:
+            raise NotImplementedError("point sources are not part of this library")
+        if kwargs_extinction:
+            raise ValueError("Extinction is not implemented")
+
+    def _render(self, lens_idx, src_idx, light_idx, kwargs_lens, kwargs_source, kwargs_lens_light, flags):
+        lens_idx, src_idx, light_idx = list(lens_idx), list(src_idx), list(light_idx)
+        kl = [kwargs_lens[i] for i in lens_idx] if lens_idx else []
+        ks = [kwargs_source[i] for i in src_idx] if src_idx else []
+        kll = [kwargs_lens_light[i] for i in light_idx] if light_idx else []
+        nl, ns = len(self._lens_types), len(self._src_types)
+        names = ([self._names[i] for i in lens_idx] + [self._names[nl + i] for i in src_idx] +
+                 [self._names[nl + ns + i] for i in light_idx])
+        n_img = profiles.batch_size([kwargs_lens or [], kwargs_source or [], kwargs_lens_light or []])
+        params = profiles.pack(names, kl + ks + kll, n_img)
+        model, _ = self._submodel(lens_idx, src_idx, light_idx)
+        pt = backend.to_device_f64(params)
+        img = self.image_batch(pt, n_img, flags=flags, model=model).to(torch.float64)
+        return img if n_img > 1 else img[0]
+
+    def image(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None, kwargs_ps=None,
+              kwargs_extinction=None, kwargs_special=None, unconvolved=False, source_add=True, lens_light_add=True,
+              point_source_add=True):
+        """ImageModel.image, ImSim/image_model.py:431-492."""
+        self._check_unsupported(kwargs_ps, kwargs_extinction)
+        flags = 0
+        if source_add:
+            flags |= backend.IMAGE_SOURCE
+        if lens_light_add:
+            flags |= backend.IMAGE_LENS_LIGHT
+        if unconvolved:
+            flags |= backend.IMAGE_UNCONVOLVED
+        nl, ns, nll = len(self._lens_types), len(self._src_types), len(self._light_types)
+        return self._render(range(nl), range(ns), range(nll), kwargs_lens, kwargs_source, kwargs_lens_light, flags)
+
+    def source_surface_brightness(self, kwargs_source, kwargs_lens=None, kwargs_extinction=None,
+                                  kwargs_special=None, unconvolved=False, de_lensed=False, k=None,
+                                  update_pixelbased_mapping=True):
+        """ImageModel.source_surface_brightness, ImSim/image_model.py:223-368."""
+        self._check_unsupported(None, kwargs_extinction)
+        ns = len(self._src_types)
+        if ns == 0:
+            nx, ny = self.Data.num_pixel_axes
+            return torch.zeros((nx, ny), dtype=torch.float64, device=backend.device())
+        src_idx = profiles.select(k, ns)
+        lens_idx = [] if de_lensed else range(len(self._lens_types))
+        flags = backend.IMAGE_SOURCE | (backend.IMAGE_UNCONVOLVED if unconvolved else 0)
+        return self._render(lens_idx, src_idx, [], kwargs_lens, kwargs_source, None, flags)
+
+    def lens_surface_brightness(self, kwargs_lens_light, unconvolved=False, k=None):
+        """ImageModel.lens_surface_brightness, ImSim/image_model.py:370-393."""
+        nll = len(self._light_types)
+        if nll == 0:
+            nx, ny = self.Data.num_pixel_axes
+            return torch.zeros((nx, ny), dtype=torch.float64, device=backend.device())
+        light_idx = profiles.select(k, nll)
+        flags = backend.IMAGE_LENS_LIGHT | (backend.IMAGE_UNCONVOLVED if unconvolved else 0)
+        return self._render([], [], light_idx, None, None, kwargs_lens_light, flags)
+
+    def likelihood_data_given_model(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None,
+                                    kwargs_ps=None, kwargs_extinction=None, kwargs_special=None, source_marg=False,
+                                    linear_prior=None, check_positive_flux=False):
+        """ImageModel.likelihood_data_given_model, ImSim/image_model.py:162-221."""
+        if check_positive_flux:
+            raise ValueError("check positive flux is not supported")
+        self._check_unsupported(kwargs_ps, kwargs_extinction)
+        params, n_img = self.pack(kwargs_lens, kwargs_source, kwargs_lens_light)
+        pt = backend.to_device_f64(params)
+        logl = self.likelihood_batch(pt, n_img)
+        return logl if n_img > 1 else logl[0]
+
+    def update_psf(self, psf_class):
+        raise ValueError("Updating psf class not supported. Create a new instance of ImageModel instead.")
+
+    def update_data(self, data_class):
+        raise ValueError("Updating data class not supported. Create a new instance of ImageModel instead.")

@@ -1,0 +1,53 @@
+"""Builds liblensim.so for gfx950 in-tree (jaxtronomy_amd/_lib/).
+
+hipcc cross-compiles without a GPU.  Packed-FP32 instructions are disabled on
+purpose (-target-feature -packed-fp32-ops); the host pass warns that it does
+not know that feature, which is harmless.
+"""
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "lensim.hip")
+OUT = os.path.join(HERE, "_lib", "liblensim.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build_library(verbose=False):
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    tmp = OUT + ".tmp%d" % os.getpid()
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, SRC]
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+        raise RuntimeError("hipcc failed (%d):\n%s" % (res.returncode, out))
+    if verbose and out.strip():
+        print(out)
+    os.replace(tmp, OUT)  # atomic: a process that already loaded the old .so keeps it
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build_library(verbose=True))
+    sys.exit(0)

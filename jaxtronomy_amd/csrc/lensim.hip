@@ -1,0 +1,754 @@
+// liblensim: HIP kernels for gfx950 and the C ABI declared in include/lensim.h.
+//
+// Pipeline of one batched image / likelihood call (all on one stream):
+//   prep_kernel     kwargs slots -> per-(image, profile) records (+ EPL series
+//                   coefficients), float64 and float32 copies
+//   psf_prep_kernel flip the PSF, pad its width to a multiple of the register
+//                   window, convert to the compute type
+//   render_kernel   one thread per pixel: s x s sub-rays generated from the grid
+//                   description, ray shooting through every deflector, source
+//                   light at the deflected position and lens light at the ray,
+//                   mean over the sub-pixel block, x pixel_width^2 x flux_scaling
+//   conv_kernel     LDS-tiled direct 'same' convolution (zero or edge padding)
+//                   with a register sliding window; optionally fused with the
+//                   per-pixel chi^2 and a block reduction
+//   logl_kernel     fixed-order sum of the per-tile chi^2 partials
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "profiles.hpp"
+
+using namespace lensim;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+  return fail(LENSIM_EDEVICE, "%s: %s", where, hipGetErrorString(e));
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+constexpr int kWin = 8;  // register window (outputs per thread) of the convolution
+
+// ---- optional kernel timing (lensim_timing_*) ----
+struct TimedCall {
+  hipEvent_t ev[3];  // before render, after render / before conv, after conv
+};
+bool g_timing = false;
+TimedCall *g_calls = nullptr;
+int g_ncalls = 0, g_capcalls = 0;
+
+hipEvent_t timing_event(int slot) {
+  if (!g_timing) return nullptr;
+  if (slot == 0) {
+    if (g_ncalls == g_capcalls) {
+      const int cap = g_capcalls ? 2 * g_capcalls : 256;
+      TimedCall *n = static_cast<TimedCall *>(realloc(g_calls, sizeof(TimedCall) * cap));
+      if (!n) return nullptr;
+      for (int i = g_capcalls; i < cap; ++i)
+        for (int k = 0; k < 3; ++k) n[i].ev[k] = nullptr;
+      g_calls = n;
+      g_capcalls = cap;
+    }
+    ++g_ncalls;
+  }
+  TimedCall &c = g_calls[g_ncalls - 1];
+  if (!c.ev[slot] && hipEventCreate(&c.ev[slot]) != hipSuccess) c.ev[slot] = nullptr;
+  return c.ev[slot];
+}
+
+void timing_mark(int slot, hipStream_t st) {
+  if (!g_timing) return;
+  hipEvent_t e = timing_event(slot);
+  if (e) (void)hipEventRecord(e, st);
+}
+
+struct ModelArgs {
+  int32_t n_lens, n_source, n_light, n_prof, n_epl;
+  int32_t types[LENSIM_MAX_PROFILES];
+  int32_t epl_index[LENSIM_MAX_PROFILES];
+  double alpha_scaling, flux_scaling;
+};
+
+struct Layout {
+  size_t recd, recf, coefd, coeff, psf, lowres, partial, total;
+  int n_tiles;
+};
+
+// Convolution tile shape for the compute type (outputs per block).
+template <typename T>
+struct ConvTile;
+template <>
+struct ConvTile<double> {
+  static constexpr int TY = 32, TX = 32;  // 128 threads
+};
+template <>
+struct ConvTile<float> {
+  static constexpr int TY = 32, TX = 64;  // 256 threads
+};
+
+int kxp_of(int kx) { return (kx + kWin - 1) / kWin * kWin; }
+
+template <typename T>
+int n_tiles_of(int nx, int ny) {
+  const int tx = (nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+  const int ty = (ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY;
+  return tx * ty;
+}
+
+Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision) {
+  Layout L{};
+  const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
+  size_t off = 0;
+  const size_t nrec = size_t(n_img) * m.n_prof * kRec;
+  L.recd = off;
+  off = align256(off + nrec * 8);
+  L.recf = off;
+  off = align256(off + nrec * 4);
+  const size_t ncoef = size_t(n_img) * m.n_epl * LENSIM_EPL_NMAX;
+  L.coefd = off;
+  off = align256(off + ncoef * 8);
+  L.coeff = off;
+  off = align256(off + ncoef * 4);
+  L.psf = off;
+  off = align256(off + size_t(ky) * kxp_of(kx) * tsz);
+  L.lowres = off;
+  off = align256(off + size_t(n_img) * nx * ny * tsz);
+  L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
+  L.partial = off;
+  off = align256(off + size_t(n_img) * L.n_tiles * 8);
+  L.total = off;
+  return L;
+}
+
+int model_args(const lensim_model *model, ModelArgs &m, bool need_lens_light_ok = true) {
+  (void)need_lens_light_ok;
+  if (!model) return fail(LENSIM_EINVAL, "model is NULL");
+  m = ModelArgs{};
+  m.n_lens = model->n_lens;
+  m.n_source = model->n_source;
+  m.n_light = model->n_lens_light;
+  if (m.n_lens < 0 || m.n_source < 0 || m.n_light < 0)
+    return fail(LENSIM_EINVAL, "negative profile count");
+  m.n_prof = m.n_lens + m.n_source + m.n_light;
+  if (m.n_prof > LENSIM_MAX_PROFILES)
+    return fail(LENSIM_EINVAL, "%d profiles exceed LENSIM_MAX_PROFILES=%d", m.n_prof, LENSIM_MAX_PROFILES);
+  for (int k = 0; k < m.n_prof; ++k) {
+    const int t = model->types[k];
+    m.types[k] = t;
+    m.epl_index[k] = -1;
+    const bool lens = k < m.n_lens;
+    const bool lens_ok = t == LENSIM_EPL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW ||
+                         t == LENSIM_TNFW || t == LENSIM_SHEAR;
+    const bool light_ok = t == LENSIM_SERSIC || t == LENSIM_SERSIC_ELLIPSE || t == LENSIM_GAUSSIAN;
+    if (lens && !lens_ok) return fail(LENSIM_EUNSUPPORTED, "profile %d: type %d is not a supported deflector", k, t);
+    if (!lens && !light_ok)
+      return fail(LENSIM_EUNSUPPORTED, "profile %d: type %d is not a supported light profile", k, t);
+    if (t == LENSIM_EPL) m.epl_index[k] = m.n_epl++;
+  }
+  m.alpha_scaling = model->alpha_scaling;
+  m.flux_scaling = model->flux_scaling;
+  return LENSIM_OK;
+}
+
+// ------------------------------------------------------------- kernels ---
+
+__global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int n_img, double *__restrict__ recd,
+                            float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_img * m.n_prof) return;
+  const int img = idx / m.n_prof, k = idx - img * m.n_prof;
+  const int type = m.types[k];
+  const double *p = params + size_t(idx) * LENSIM_NPARAM;
+  double rec[kRec];
+  double *coef = nullptr;
+  const int e = m.epl_index[k];
+  if (e >= 0) coef = coefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+  prepare_record(type, p, rec, coef);
+  for (int i = 0; i < kRec; ++i) {
+    recd[size_t(idx) * kRec + i] = rec[i];
+    recf[size_t(idx) * kRec + i] = float(rec[i]);
+  }
+  if (coef) {
+    const int nmax = (int)rec[EPL_NMAX];
+    float *cf = coeff + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+    for (int n = 0; n < nmax; ++n) cf[n] = float(coef[n]);
+  }
+}
+
+// Flip the kernel (so the convolution runs as a correlation), zero-pad its
+// width to kxp and convert it to T.
+template <typename T>
+__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kxp, T *__restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kxp * ky) return;
+  const int a = idx / kxp, c = idx - a * kxp;
+  if (!k) {  // identity kernel (no PSF / unconvolved)
+    out[idx] = (a == 0 && c == 0) ? T(1) : T(0);
+    return;
+  }
+  out[idx] = (c < kx) ? T(k[(ky - 1 - a) * kx + (kx - 1 - c)]) : T(0);
+}
+
+template <typename T>
+struct RecPtrs {
+  const double *recd;
+  const T *rect;
+  const double *coefd;
+  const T *coeft;
+};
+
+template <typename T>
+__device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double *recd, const float *recf,
+                                               const double *coefd, const float *coeff, int img) {
+  RecPtrs<T> r;
+  r.recd = recd + size_t(img) * m.n_prof * kRec;
+  if constexpr (sizeof(T) == sizeof(float)) {
+    r.rect = recf + size_t(img) * m.n_prof * kRec;
+    r.coeft = coeff;
+  } else {
+    r.rect = r.recd;
+    r.coeft = coefd;
+  }
+  r.coefd = coefd;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
+                                            double &ax, double &ay) {
+  for (int k = 0; k < m.n_lens; ++k) {
+    const int type = m.types[k];
+    const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
+    const int e = m.epl_index[k];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    add_deflection<T>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ double light_sum(const ModelArgs &m, const RecPtrs<T> &rp, int k0, int k1, double x,
+                                            double y) {
+  double f = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
+    f += light_value<T>(m.types[k], r, x, y);
+  }
+  return f;
+}
+
+struct GridArgs {
+  int nx, ny, s;
+  double ra0, dec0, m00, m01, m10, m11;
+  double scale;  // pixel_width^2 / s^2
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
+                                                     const float *__restrict__ recf,
+                                                     const double *__restrict__ coefd,
+                                                     const float *__restrict__ coeff, unsigned flags,
+                                                     T *__restrict__ out) {
+  const int img = blockIdx.y;
+  const int npix = g.nx * g.ny;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= npix) return;
+  const int i = pix % g.nx, j = pix / g.nx;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  // pixel centre (lenstronomy grid_from_coordinate_transform order)
+  const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
+  const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
+  const double inv_s = 1.0 / g.s;
+  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && m.n_source > 0;
+  const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
+  const int k_src0 = m.n_lens, k_src1 = m.n_lens + m.n_source, k_l1 = m.n_prof;
+  double acc = 0.0;
+  for (int sy = 0; sy < g.s; ++sy) {
+    const double oy = o0 + sy * inv_s;
+    for (int sx = 0; sx < g.s; ++sx) {
+      const double ox = o0 + sx * inv_s;
+      // make_subgrid offsets (ImSim/Numerics/grid.py:60-62)
+      const double x = ra_c + g.m00 * ox + g.m01 * oy;
+      const double y = dec_c + g.m10 * ox + g.m11 * oy;
+      if (do_src) {
+        double ax = 0.0, ay = 0.0;
+        deflect_all<T>(m, rp, img, x, y, ax, ay);
+        const double bx = x - ax * m.alpha_scaling;
+        const double by = y - ay * m.alpha_scaling;
+        acc += light_sum<T>(m, rp, k_src0, k_src1, bx, by);
+      }
+      if (do_light) acc += light_sum<T>(m, rp, k_src1, k_l1, x, y);
+    }
+  }
+  out[size_t(img) * npix + pix] = T(acc * (g.scale * m.flux_scaling));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double *__restrict__ recd,
+                                                       const float *__restrict__ recf,
+                                                       const double *__restrict__ coefd,
+                                                       const float *__restrict__ coeff, const double *__restrict__ xs,
+                                                       const double *__restrict__ ys, int64_t n_ray,
+                                                       double *__restrict__ bx, double *__restrict__ by) {
+  const int img = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_ray) return;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const double x = xs[i], y = ys[i];
+  double ax = 0.0, ay = 0.0;
+  deflect_all<T>(m, rp, img, x, y, ax, ay);
+  bx[size_t(img) * n_ray + i] = x - ax * m.alpha_scaling;
+  by[size_t(img) * n_ray + i] = y - ay * m.alpha_scaling;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void light_kernel(ModelArgs m, const double *__restrict__ recd,
+                                                    const float *__restrict__ recf, int k0, int k1,
+                                                    const double *__restrict__ xs, const double *__restrict__ ys,
+                                                    int64_t n_ray, double *__restrict__ flux) {
+  const int img = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_ray) return;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, nullptr, nullptr, img);
+  flux[size_t(img) * n_ray + i] = light_sum<T>(m, rp, k0, k1, xs[i], ys[i]);
+}
+
+struct ConvArgs {
+  int nx, ny, kx, ky, kxp, edge, n_tiles_x;
+  const double *data, *noise, *expo, *mask;
+  double exposure_time, bkg2;
+};
+
+template <typename T>
+__device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T model_t) {
+  const double m = double(model_t);
+  const double d = c.data[p];
+  double var;
+  if (c.noise) {
+    const double nm = c.noise[p];
+    var = nm * nm;
+  } else {
+    const double t = c.expo ? c.expo[p] : c.exposure_time;
+    const double mpos = (m >= 0.0) ? m : 0.0;
+    var = mpos / t + c.bkg2;
+  }
+  double chi = (m - d) * (m - d) / var;
+  if (c.mask) chi *= c.mask[p];
+  return chi;
+}
+
+// out[y][x] = sum_{a,c} in[y - cy + a][x - cx + c] * Kf[a][c]  (Kf = flipped kernel).
+// Block = TY rows x (TX / kWin) column groups; each thread computes kWin
+// consecutive outputs of one row from a register window over the LDS tile.
+template <typename T>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in, const T *__restrict__ kf,
+                                                   T *__restrict__ out, double *__restrict__ partial) {
+  constexpr int TY = ConvTile<T>::TY, TX = ConvTile<T>::TX, G = TX / kWin;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *tile = reinterpret_cast<T *>(smem_raw);
+  const int img = blockIdx.y;
+  const int tile_id = blockIdx.x;
+  const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
+  const int y0 = tyi * TY, x0 = txi * TX;
+  const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
+  const int TH = TY + c.ky - 1, TW = TX + c.kxp;
+  const T *src = in + size_t(img) * c.nx * c.ny;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < TH * TW; idx += TY * G) {
+    const int ly = idx / TW, lx = idx - ly * TW;
+    int gy = y0 - cy + ly, gx = x0 - cx + lx;
+    T v = T(0);
+    if (lx >= TX + c.kx - 1) {
+      // columns that only meet the zero padding of the kernel width
+    } else if (c.edge) {
+      gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
+      gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
+      v = src[gy * c.nx + gx];
+    } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
+      v = src[gy * c.nx + gx];
+    }
+    tile[idx] = v;
+  }
+  __syncthreads();
+  const int ty = tid / G, tg = tid - ty * G;
+  T acc[kWin];
+#pragma unroll
+  for (int r = 0; r < kWin; ++r) acc[r] = T(0);
+  for (int a = 0; a < c.ky; ++a) {
+    const T *row = tile + (ty + a) * TW + tg * kWin;
+    const T *krow = kf + a * c.kxp;
+    for (int c0 = 0; c0 < c.kxp; c0 += kWin) {
+      T buf[2 * kWin];
+#pragma unroll
+      for (int r = 0; r < 2 * kWin; ++r) buf[r] = row[c0 + r];
+#pragma unroll
+      for (int jj = 0; jj < kWin; ++jj) {
+        const T w = krow[c0 + jj];
+#pragma unroll
+        for (int r = 0; r < kWin; ++r) acc[r] += buf[r + jj] * w;
+      }
+    }
+  }
+  const int y = y0 + ty;
+  double chi = 0.0;
+  if (y < c.ny) {
+#pragma unroll
+    for (int r = 0; r < kWin; ++r) {
+      const int x = x0 + tg * kWin + r;
+      if (x < c.nx) {
+        const size_t p = size_t(y) * c.nx + x;
+        if (out) out[size_t(img) * c.nx * c.ny + p] = acc[r];
+        if (partial) chi += chi2_pixel<T>(c, p, acc[r]);
+      }
+    }
+  }
+  if (partial) {
+    // wave reduction then one value per wave through LDS (fixed order)
+    for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(smem_raw);
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) red[wave] = chi;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < (TY * G) / 64; ++w) s += red[w];
+      partial[size_t(img) * gridDim.x + tile_id] = s;
+    }
+  }
+}
+
+__global__ void logl_kernel(const double *__restrict__ partial, int n_tiles, int n_img, double *__restrict__ logl) {
+  const int img = blockIdx.x * blockDim.x + threadIdx.x;
+  if (img >= n_img) return;
+  double s = 0.0;
+  for (int t = 0; t < n_tiles; ++t) s += partial[size_t(img) * n_tiles + t];
+  logl[img] = -s / 2.0;
+}
+
+// ------------------------------------------------------------ launches ---
+
+int check_launch(const char *where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, where);
+  return LENSIM_OK;
+}
+
+int launch_prep(const ModelArgs &m, const double *params, int n_img, unsigned char *ws, const Layout &L,
+                hipStream_t st) {
+  const int n = n_img * m.n_prof;
+  if (n == 0) return LENSIM_OK;
+  prep_kernel<<<(n + 63) / 64, 64, 0, st>>>(m, params, n_img, reinterpret_cast<double *>(ws + L.recd),
+                                             reinterpret_cast<float *>(ws + L.recf),
+                                             reinterpret_cast<double *>(ws + L.coefd),
+                                             reinterpret_cast<float *>(ws + L.coeff));
+  return check_launch("prep_kernel");
+}
+
+int grid_args(const lensim_grid *grid, GridArgs &g) {
+  if (!grid) return fail(LENSIM_EINVAL, "grid is NULL");
+  if (grid->nx <= 0 || grid->ny <= 0 || grid->supersampling <= 0)
+    return fail(LENSIM_EINVAL, "grid sizes must be positive (nx=%d ny=%d s=%d)", grid->nx, grid->ny,
+                grid->supersampling);
+  if (int64_t(grid->nx) * grid->ny > (int64_t(1) << 30)) return fail(LENSIM_EINVAL, "grid too large");
+  g.nx = grid->nx;
+  g.ny = grid->ny;
+  g.s = grid->supersampling;
+  g.ra0 = grid->ra_at_xy_0;
+  g.dec0 = grid->dec_at_xy_0;
+  g.m00 = grid->pix2angle[0];
+  g.m01 = grid->pix2angle[1];
+  g.m10 = grid->pix2angle[2];
+  g.m11 = grid->pix2angle[3];
+  const double det = g.m00 * g.m11 - g.m01 * g.m10;
+  const double pixel_width = sqrt(fabs(det));  // lenstronomy PixelGrid.pixel_width
+  g.scale = pixel_width * pixel_width / (double(g.s) * double(g.s));
+  return LENSIM_OK;
+}
+
+int psf_dims(const lensim_psf *psf, uint32_t flags, int &kx, int &ky, int &edge) {
+  const bool none = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
+  if (none) {
+    kx = ky = 1;
+    edge = 0;
+    return LENSIM_OK;
+  }
+  if (psf->type != LENSIM_PSF_PIXEL && psf->type != LENSIM_PSF_GAUSSIAN)
+    return fail(LENSIM_EUNSUPPORTED, "psf type %d not supported", psf->type);
+  if (psf->kx <= 0 || psf->ky <= 0 || !(psf->kx & 1) || !(psf->ky & 1))
+    return fail(LENSIM_EINVAL, "psf kernel must have odd positive sizes (got %d x %d)", psf->ky, psf->kx);
+  if (!psf->d_kernel) return fail(LENSIM_EINVAL, "psf kernel pointer is NULL");
+  kx = psf->kx;
+  ky = psf->ky;
+  edge = psf->type == LENSIM_PSF_GAUSSIAN;
+  return LENSIM_OK;
+}
+
+template <typename T>
+size_t conv_smem(int ky, int kxp) {
+  return size_t(ConvTile<T>::TY + ky - 1) * (ConvTile<T>::TX + kxp) * sizeof(T);
+}
+
+// The image pipeline shared by lensim_image and lensim_loglike.
+template <typename T>
+int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+              const lensim_data *data, const double *params, int n_img, unsigned flags, T *out, double *logl,
+              unsigned char *ws, const Layout &L, hipStream_t st) {
+  int rc = launch_prep(m, params, n_img, ws, L, st);
+  if (rc) return rc;
+  const int kxp = kxp_of(kx);
+  T *kf = reinterpret_cast<T *>(ws + L.psf);
+  {
+    const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
+    const int n = kxp * ky;
+    psf_prep_kernel<T><<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kxp, kf);
+    rc = check_launch("psf_prep_kernel");
+    if (rc) return rc;
+  }
+  T *lowres = reinterpret_cast<T *>(ws + L.lowres);
+  const int npix = g.nx * g.ny;
+  timing_mark(0, st);
+  dim3 rgrid((npix + 255) / 256, n_img);
+  render_kernel<T><<<rgrid, 256, 0, st>>>(m, g, reinterpret_cast<const double *>(ws + L.recd),
+                                          reinterpret_cast<const float *>(ws + L.recf),
+                                          reinterpret_cast<const double *>(ws + L.coefd),
+                                          reinterpret_cast<const float *>(ws + L.coeff), flags, lowres);
+  rc = check_launch("render_kernel");
+  if (rc) return rc;
+  timing_mark(1, st);
+  ConvArgs c{};
+  c.nx = g.nx;
+  c.ny = g.ny;
+  c.kx = kx;
+  c.ky = ky;
+  c.kxp = kxp;
+  c.edge = edge;
+  c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+  const int n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
+  double *partial = nullptr;
+  if (data) {
+    c.data = data->d_data;
+    c.noise = data->d_noise_map;
+    c.expo = data->d_exposure_map;
+    c.mask = data->d_mask;
+    c.exposure_time = data->exposure_time;
+    c.bkg2 = data->background_rms * data->background_rms;
+    partial = reinterpret_cast<double *>(ws + L.partial);
+  }
+  const size_t smem = conv_smem<T>(ky, kxp);
+  if (smem > 65536) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
+  dim3 cgrid(n_tiles, n_img);
+  const int threads = ConvTile<T>::TY * (ConvTile<T>::TX / kWin);
+  conv_kernel<T><<<cgrid, threads, smem, st>>>(c, lowres, kf, out, partial);
+  rc = check_launch("conv_kernel");
+  if (rc) return rc;
+  timing_mark(2, st);
+  if (data) {
+    logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, n_tiles, n_img, logl);
+    rc = check_launch("logl_kernel");
+    if (rc) return rc;
+  }
+  return LENSIM_OK;
+}
+
+int check_ws(const Layout &L, void *ws, size_t ws_bytes) {
+  if (L.total > ws_bytes)
+    return fail(LENSIM_EWORKSPACE, "workspace of %zu B is smaller than the %zu B needed", ws_bytes, L.total);
+  if (L.total > 0 && !ws) return fail(LENSIM_EINVAL, "workspace pointer is NULL");
+  if (reinterpret_cast<uintptr_t>(ws) & 255) return fail(LENSIM_EINVAL, "workspace must be 256-byte aligned");
+  return LENSIM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- C ABI -----
+
+extern "C" {
+
+int lensim_abi_version(void) { return LENSIM_ABI_VERSION; }
+
+int lensim_timing_enable(int on) {
+  g_timing = on != 0;
+  return LENSIM_OK;
+}
+
+int lensim_timing_read(double *render_ms, double *conv_ms, int64_t *calls) {
+  double r = 0.0, c = 0.0;
+  int64_t n = 0;
+  for (int i = 0; i < g_ncalls; ++i) {
+    TimedCall &tc = g_calls[i];
+    if (!tc.ev[0] || !tc.ev[1] || !tc.ev[2]) continue;
+    hipError_t e = hipEventSynchronize(tc.ev[2]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, tc.ev[0], tc.ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&b, tc.ev[1], tc.ev[2]) != hipSuccess)
+      return fail(LENSIM_EDEVICE, "hipEventElapsedTime failed");
+    r += a;
+    c += b;
+    ++n;
+  }
+  g_ncalls = 0;  // events are kept and reused by later calls
+  if (render_ms) *render_ms = r;
+  if (conv_ms) *conv_ms = c;
+  if (calls) *calls = n;
+  return LENSIM_OK;
+}
+
+const char *lensim_last_error(void) { return g_err; }
+
+size_t lensim_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                             int32_t n_img, int32_t precision) {
+  ModelArgs m;
+  if (model_args(model, m)) return 0;
+  int nx = 1, ny = 1;
+  if (grid) {
+    nx = grid->nx;
+    ny = grid->ny;
+  }
+  int kx = 1, ky = 1, edge = 0;
+  if (psf_dims(psf, 0, kx, ky, edge)) return 0;
+  return layout(m, nx, ny, kx, ky, n_img, precision).total;
+}
+
+int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n_img, const double *d_x,
+                    const double *d_y, int64_t n_ray, int32_t precision, double *d_beta_x, double *d_beta_y,
+                    void *ws, size_t ws_bytes, void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  if (n_img < 0 || n_ray < 0) return fail(LENSIM_EINVAL, "negative sizes");
+  if (n_img == 0 || n_ray == 0) return LENSIM_OK;
+  if (n_ray > (int64_t(1) << 36)) return fail(LENSIM_EINVAL, "n_ray too large");
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_x || !d_y || !d_beta_x || !d_beta_y || (m.n_prof && !d_params))
+    return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const Layout L = layout(m, 1, 1, 1, 1, n_img, precision);
+  if ((rc = check_ws(L, ws, ws_bytes))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
+  dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
+  if (precision == LENSIM_FP64)
+    raytrace_kernel<double><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
+                                                   reinterpret_cast<float *>(w + L.recf),
+                                                   reinterpret_cast<double *>(w + L.coefd),
+                                                   reinterpret_cast<float *>(w + L.coeff), d_x, d_y, n_ray,
+                                                   d_beta_x, d_beta_y);
+  else
+    raytrace_kernel<float><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
+                                                  reinterpret_cast<float *>(w + L.recf),
+                                                  reinterpret_cast<double *>(w + L.coefd),
+                                                  reinterpret_cast<float *>(w + L.coeff), d_x, d_y, n_ray,
+                                                  d_beta_x, d_beta_y);
+  return check_launch("raytrace_kernel");
+}
+
+int lensim_surface_brightness(const lensim_model *model, const double *d_params, int32_t n_img, int32_t which,
+                              const double *d_x, const double *d_y, int64_t n_ray, int32_t precision,
+                              double *d_flux, void *ws, size_t ws_bytes, void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  if (n_img < 0 || n_ray < 0) return fail(LENSIM_EINVAL, "negative sizes");
+  if (which != 0 && which != 1) return fail(LENSIM_EINVAL, "which must be 0 (source) or 1 (lens light)");
+  if (n_img == 0 || n_ray == 0) return LENSIM_OK;
+  if (n_ray > (int64_t(1) << 36)) return fail(LENSIM_EINVAL, "n_ray too large");
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_x || !d_y || !d_flux || (m.n_prof && !d_params)) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const Layout L = layout(m, 1, 1, 1, 1, n_img, precision);
+  if ((rc = check_ws(L, ws, ws_bytes))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
+  const int k0 = which == 0 ? m.n_lens : m.n_lens + m.n_source;
+  const int k1 = which == 0 ? m.n_lens + m.n_source : m.n_prof;
+  dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
+  if (precision == LENSIM_FP64)
+    light_kernel<double><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
+                                                reinterpret_cast<float *>(w + L.recf), k0, k1, d_x, d_y, n_ray,
+                                                d_flux);
+  else
+    light_kernel<float><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
+                                               reinterpret_cast<float *>(w + L.recf), k0, k1, d_x, d_y, n_ray,
+                                               d_flux);
+  return check_launch("light_kernel");
+}
+
+int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf, const double *d_params,
+                 int32_t n_img, int32_t precision, uint32_t flags, void *d_out, void *ws, size_t ws_bytes,
+                 void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  GridArgs g;
+  if ((rc = grid_args(grid, g))) return rc;
+  int kx, ky, edge;
+  if ((rc = psf_dims(psf, flags, kx, ky, edge))) return rc;
+  if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
+  if (n_img == 0) return LENSIM_OK;
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_out || (m.n_prof && !d_params)) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  // kernel dims used by the layout: the real PSF unless unconvolved
+  int lkx = 1, lky = 1, le = 0;
+  psf_dims(psf, 0, lkx, lky, le);
+  const Layout L = layout(m, g.nx, g.ny, lkx, lky, n_img, precision);
+  if ((rc = check_ws(L, ws, ws_bytes))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  if (precision == LENSIM_FP64)
+    return run_image<double>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags,
+                             static_cast<double *>(d_out), nullptr, w, L, st);
+  return run_image<float>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags, static_cast<float *>(d_out),
+                          nullptr, w, L, st);
+}
+
+int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                   const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                   double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  GridArgs g;
+  if ((rc = grid_args(grid, g))) return rc;
+  int kx, ky, edge;
+  if ((rc = psf_dims(psf, 0, kx, ky, edge))) return rc;
+  if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
+  if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
+    return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
+  if (n_img == 0) return LENSIM_OK;
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_logl || (m.n_prof && !d_params)) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const Layout L = layout(m, g.nx, g.ny, kx, ky, n_img, precision);
+  if ((rc = check_ws(L, ws, ws_bytes))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  const unsigned flags = LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT;
+  if (precision == LENSIM_FP64)
+    return run_image<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags,
+                             static_cast<double *>(d_model_out), d_logl, w, L, st);
+  return run_image<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<float *>(d_model_out),
+                          d_logl, w, L, st);
+}
+
+}  // extern "C"

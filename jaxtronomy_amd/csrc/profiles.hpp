@@ -1,0 +1,505 @@
+// Device-side lens and light profiles for gfx950.
+//
+// Every profile has three pieces:
+//   * a host/device "prepare" step (prepare_record) that turns the reference's
+//     kwargs (one LENSIM_NPARAM slot of float64) into a record of derived
+//     constants (rotation, axis ratio, normalisations, EPL series
+//     coefficients), computed once per image and profile in float64;
+//   * a per-ray evaluation templated on the compute type T (double for the
+//     plain FP64 form, float for the mixed form).  Ray coordinates and the
+//     deflection sums stay in double in both forms; only the per-profile
+//     transcendental work runs in T.
+//
+// Formulas follow the reference profile files cited at each function; the
+// numerical floors, branches and NaN policies are kept exactly in the FP64
+// form.  The float form uses the same formulas except where float
+// cancellation would lose accuracy (noted at each place).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/lensim.h"
+
+namespace lensim {
+
+constexpr int kRec = 16;  // derived constants per (image, profile)
+
+// Record slot names shared by all profiles: centre first.
+enum : int { R_CX = 0, R_CY = 1 };
+
+// ---------------------------------------------------------------- maths ---
+
+template <typename T>
+struct Fn;
+
+template <>
+struct Fn<double> {
+  static __device__ __forceinline__ double sqrt_(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double exp_(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double log_(double x) { return ::log(x); }
+  static __device__ __forceinline__ double pow_(double a, double b) { return ::pow(a, b); }
+  static __device__ __forceinline__ double atan_(double x) { return ::atan(x); }
+  static __device__ __forceinline__ double atanh_(double x) { return ::atanh(x); }
+  static __device__ __forceinline__ double acosh_(double x) { return ::acosh(x); }
+  static __device__ __forceinline__ double acos_(double x) { return ::acos(x); }
+};
+
+template <>
+struct Fn<float> {
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
+  static __device__ __forceinline__ float log_(float x) { return __logf(x); }
+  static __device__ __forceinline__ float pow_(float a, float b) { return __expf(b * __logf(a)); }
+  static __device__ __forceinline__ float atan_(float x) { return ::atanf(x); }
+  static __device__ __forceinline__ float atanh_(float x) { return ::atanhf(x); }
+  static __device__ __forceinline__ float acosh_(float x) { return ::acoshf(x); }
+  static __device__ __forceinline__ float acos_(float x) { return ::acosf(x); }
+};
+
+// jnp.nan_to_num(v, posinf=1e10, neginf=-1e10) as used at LensModel/Profiles/epl.py:391-392
+template <typename T>
+__device__ __forceinline__ T nan_to_1e10(T v) {
+  if (v != v) return T(0);
+  if (v > T(1e10)) return (v == T(INFINITY)) ? T(1e10) : v;
+  if (v < T(-1e10)) return (v == T(-INFINITY)) ? T(-1e10) : v;
+  return v;
+}
+
+// ------------------------------------------------------ record layouts ----
+// EPL     (epl.py:109-124): cos phi, sin phi, q, b, t, f, 2/(1+q), nmax
+enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX };
+// NIE/SIE (nie.py:182-200, :280-295): cos, sin, q (clamped), q^2, s, b/sqrt(1-q^2),
+//          sqrt(1-q^2), q^2 s
+enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S };
+// NFW     (nfw.py:86-105, :257-279, :384-395): Rs (floored), 4 rho0 Rs, 1/Rs
+enum : int { NFW_RS = 2, NFW_A, NFW_IRS };
+// TNFW    (tnfw.py:67-87, :226-248, :329-349): Rs, 4 rho0 Rs, tau, tau^2,
+//          tau^2/(tau^2+1)^2, tau pi + (tau^2-1) ln tau, (tau^2-1)/tau, 0.001 Rs, tau^2+1, 1/Rs
+enum : int { TN_RS = 2, TN_A, TN_TAU, TN_TAU2, TN_PRE, TN_C0, TN_C1, TN_RMIN, TN_T2P1, TN_IRS };
+// SHEAR   (shear.py:37-53): ra_0, dec_0 in the centre slots, gamma1, gamma2
+enum : int { SH_G1 = 2, SH_G2 };
+// SERSIC  (sersic_ellipse.py:62-93, sersic_utils.py:83-110, :239-267,
+//          param_util.py:103-124): amp, the product-average matrix
+//          ((1-e1), -e2; -e2, (1+e1))/norm, max(R_sersic, 1e-4), 1/n, b_n, 1/R_sersic_
+enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS };
+// GAUSSIAN light (gaussian.py:27-42): amp/(2 pi sigma^2), 1/sigma^2
+enum : int { GA_C = 2, GA_IS2 };
+
+// ------------------------------------------------------------- prepare ---
+
+// EPL hypergeometric series length as chosen at LensModel/Profiles/epl.py:311-317.
+__host__ __device__ inline int epl_nmax(double f) {
+  if (f == 0.0) return 1;
+  if (f < 0.12) return 10;
+  if (f < 0.4) return 20;
+  if (f < 0.6) return 35;
+  if (f < 0.73) return 50;
+  if (f < 0.86) return 100;
+  if (f < 0.92) return 200;
+  return 500;
+}
+
+// Util/param_util.py:88-100
+__host__ __device__ inline void ellipticity2phi_q(double e1, double e2, double &phi, double &q) {
+  phi = atan2(e2, e1) / 2.0;
+  double c = sqrt(e1 * e1 + e2 * e2);
+  c = fmin(c, 0.9999);
+  q = (1.0 - c) / (1.0 + c);
+}
+
+// Turns one kwargs slot p[LENSIM_NPARAM] into the derived record rec[kRec].
+// For EPL also writes the series coefficients F_n = 2F1(-n, t/2; 2-t/2; 2),
+// n < nmax, from Gauss' contiguous recurrence (Util/hyp2f1_util.py:156-166).
+__device__ inline void prepare_record(int type, const double *p, double *rec, double *coef) {
+  for (int i = 0; i < kRec; ++i) rec[i] = 0.0;
+  switch (type) {
+    case LENSIM_EPL: {
+      const double theta_E = p[0], gamma = p[1], e1 = p[2], e2 = p[3];
+      double phi, q;
+      ellipticity2phi_q(e1, e2, phi, q);
+      const double t = gamma - 1.0;
+      const double b = theta_E * sqrt(q);
+      const double f = (1.0 - q) / (1.0 + q);
+      const int nmax = epl_nmax(f);
+      rec[R_CX] = p[4];
+      rec[R_CY] = p[5];
+      rec[EPL_C] = cos(phi);
+      rec[EPL_S] = sin(phi);
+      rec[EPL_Q] = q;
+      rec[EPL_B] = b;
+      rec[EPL_T] = t;
+      rec[EPL_F] = f;
+      rec[EPL_PREF] = 2.0 / (1.0 + q);
+      rec[EPL_NMAX] = (double)nmax;
+      if (coef) {
+        const double bb = t / 2.0, cc = 2.0 - bb;
+        double fm2 = 1.0, fm1 = 1.0 - 2.0 * bb / cc;
+        coef[0] = 1.0;
+        if (nmax > 1) coef[1] = fm1;
+        for (int n = 2; n < nmax; ++n) {
+          const double fn = ((n - 1.0) * fm2 - (2.0 * bb - cc) * fm1) / (cc + n - 1.0);
+          coef[n] = fn;
+          fm2 = fm1;
+          fm1 = fn;
+        }
+      }
+      break;
+    }
+    case LENSIM_SIE:
+    case LENSIM_NIE: {
+      const double theta_E = p[0], e1 = p[1], e2 = p[2];
+      const double s_scale = (type == LENSIM_SIE) ? 0.0000000001 : p[3];
+      const double cx = (type == LENSIM_SIE) ? p[3] : p[4];
+      const double cy = (type == LENSIM_SIE) ? p[4] : p[5];
+      double phi, q;
+      ellipticity2phi_q(e1, e2, phi, q);
+      const double theta_E_conv = theta_E / sqrt((1.0 + q * q) / (2.0 * q));
+      const double b = theta_E_conv * sqrt((1.0 + q * q) / 2.0);
+      const double s = s_scale / sqrt(q);
+      const double qc = (q >= 1.0) ? 0.99999999 : q;
+      const double sq = sqrt(1.0 - qc * qc);
+      rec[R_CX] = cx;
+      rec[R_CY] = cy;
+      rec[NIE_C] = cos(phi);
+      rec[NIE_S] = sin(phi);
+      rec[NIE_Q] = qc;
+      rec[NIE_Q2] = qc * qc;
+      rec[NIE_SS] = s;
+      rec[NIE_BSQ] = b / sq;
+      rec[NIE_SQ] = sq;
+      rec[NIE_Q2S] = qc * qc * s;
+      break;
+    }
+    case LENSIM_NFW: {
+      const double Rs0 = p[0], alpha_Rs = p[1];
+      const double rho0 = alpha_Rs / (4.0 * Rs0 * Rs0 * (1.0 + log(1.0 / 2.0)));
+      const double Rs = (Rs0 < 0.0000001) ? 0.0000001 : Rs0;
+      rec[R_CX] = p[2];
+      rec[R_CY] = p[3];
+      rec[NFW_RS] = Rs;
+      rec[NFW_A] = 4.0 * rho0 * Rs;
+      rec[NFW_IRS] = 1.0 / Rs;
+      break;
+    }
+    case LENSIM_TNFW: {
+      const double Rs = p[0], alpha_Rs = p[1], r_trunc = p[2];
+      const double rho0 = alpha_Rs / (4.0 * Rs * Rs * (1.0 + log(1.0 / 2.0)));
+      const double tau = r_trunc / Rs, tau2 = tau * tau;
+      rec[R_CX] = p[3];
+      rec[R_CY] = p[4];
+      rec[TN_RS] = Rs;
+      rec[TN_A] = 4.0 * rho0 * Rs;
+      rec[TN_TAU] = tau;
+      rec[TN_TAU2] = tau2;
+      rec[TN_PRE] = tau2 / ((tau2 + 1.0) * (tau2 + 1.0));
+      rec[TN_C0] = tau * M_PI + (tau2 - 1.0) * log(tau);
+      rec[TN_C1] = (tau2 - 1.0) / tau;
+      rec[TN_RMIN] = 0.001 * Rs;
+      rec[TN_T2P1] = tau2 + 1.0;
+      rec[TN_IRS] = 1.0 / Rs;
+      break;
+    }
+    case LENSIM_SHEAR: {
+      rec[R_CX] = p[2];
+      rec[R_CY] = p[3];
+      rec[SH_G1] = p[0];
+      rec[SH_G2] = p[1];
+      break;
+    }
+    case LENSIM_SERSIC:
+    case LENSIM_SERSIC_ELLIPSE: {
+      const bool ell = (type == LENSIM_SERSIC_ELLIPSE);
+      const double amp = p[0], n = p[2];
+      double Rs = p[1];
+      const double e1 = ell ? p[3] : 0.0, e2 = ell ? p[4] : 0.0;
+      if (ell) Rs = fmax(0.0, Rs);
+      const double norm = fmax(sqrt(fabs(1.0 - e1 * e1 - e2 * e2)), 0.000001);
+      const double Rs_ = fmax(0.0001, Rs);
+      rec[R_CX] = ell ? p[5] : p[3];
+      rec[R_CY] = ell ? p[6] : p[4];
+      rec[SE_AMP] = amp;
+      rec[SE_M00] = (1.0 - e1) / norm;
+      rec[SE_M01] = -e2 / norm;
+      rec[SE_M11] = (1.0 + e1) / norm;
+      rec[SE_RS] = Rs_;
+      rec[SE_INVN] = 1.0 / n;
+      rec[SE_BN] = fmax(1.9992 * n - 0.3271, 0.00001);
+      rec[SE_IRS] = 1.0 / Rs_;
+      break;
+    }
+    case LENSIM_GAUSSIAN: {
+      const double amp = p[0], sigma = p[1];
+      rec[R_CX] = p[2];
+      rec[R_CY] = p[3];
+      rec[GA_C] = amp / (2.0 * M_PI * sigma * sigma);
+      rec[GA_IS2] = 1.0 / (sigma * sigma);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// ------------------------------------------------------ per-ray lensing ---
+
+// Record access: centres always in double; the rest in the compute type.
+template <typename T>
+struct Rec {
+  const double *d;  // kRec doubles
+  const T *t;       // kRec values of T (== d for T = double)
+  __device__ __forceinline__ T operator[](int i) const { return t[i]; }
+};
+
+// EPL deflection, LensModel/Profiles/epl.py:178-200 and :364-394 (Tessore &
+// Metcalf 2015 eqs. 22-23).  The Lopez-Temme series (Util/hyp2f1_util.py:
+// 136-176) in w = z/(z-2) is summed by Horner's rule over the per-image
+// coefficients F_n; the reference's nan_to_num policy is kept.  ST is the type
+// of the series sum (double for long series in the mixed form).
+template <typename T, typename ST>
+__device__ __forceinline__ void epl_defl(const Rec<T> &r, const ST *__restrict__ coef, double x, double y,
+                                         double &ax, double &ay) {
+  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+  const double cd = r.d[EPL_C], sd = r.d[EPL_S];
+  const T xr = T(dx * cd + dy * sd);
+  const T yr = T(-dx * sd + dy * cd);
+  const T q = r[EPL_Q], t = r[EPL_T], f = r[EPL_F];
+  const T Zr = q * xr, Zi = yr;
+  const T R2 = Zr * Zr + Zi * Zi;
+  T R = Fn<T>::sqrt_(R2);
+  R = R > T(0.000000001) ? R : T(0.000000001);
+  T Or, Oi;
+  const int nmax = (int)r.d[EPL_NMAX];
+  if (nmax == 1) {  // f == 0: the reference returns 2F1 = 1 (epl.py:294, :317)
+    Or = Zr;
+    Oi = Zi;
+  } else {
+    // z = -f Z / conj(Z) = -f Z^2 / |Z|^2
+    const ST inv = ST(1) / ST(R2);
+    const ST zr = -ST(f) * (ST(Zr) * ST(Zr) - ST(Zi) * ST(Zi)) * inv;
+    const ST zi = -ST(f) * (ST(2) * ST(Zr) * ST(Zi)) * inv;
+    // w = z / (z - 2)
+    const ST dr = zr - ST(2), di = zi;
+    const ST dd = ST(1) / (dr * dr + di * di);
+    const ST wr = (zr * dr + zi * di) * dd;
+    const ST wi = (zi * dr - zr * di) * dd;
+    ST Sr = coef[nmax - 1], Si = ST(0);
+    for (int n = nmax - 2; n >= 0; --n) {
+      const ST nr = Sr * wr - Si * wi + coef[n];
+      Si = Sr * wi + Si * wr;
+      Sr = nr;
+    }
+    // hyp = S / (1 - z/2)
+    const ST er = ST(1) - zr * ST(0.5), ei = -zi * ST(0.5);
+    const ST ed = ST(1) / (er * er + ei * ei);
+    const T hr = T((Sr * er + Si * ei) * ed);
+    const T hi = T((Si * er - Sr * ei) * ed);
+    Or = Zr * hr - Zi * hi;
+    Oi = Zr * hi + Zi * hr;
+  }
+  const T scale = r[EPL_PREF] * Fn<T>::pow_(r[EPL_B] / R, t);
+  const T fr = nan_to_1e10(scale * Or);
+  const T fi = nan_to_1e10(scale * Oi);
+  const T c = r[EPL_C], s = r[EPL_S];
+  ax += double(fr * c - fi * s);
+  ay += double(fr * s + fi * c);
+}
+
+// NIE / SIE deflection, LensModel/Profiles/nie.py:86-107, :280-295, :333-342.
+template <typename T>
+__device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
+  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+  const double cd = r.d[NIE_C], sd = r.d[NIE_S];
+  const T xr = T(dx * cd + dy * sd);
+  const T yr = T(-dx * sd + dy * cd);
+  const T ss = r[NIE_SS];
+  const T psi = Fn<T>::sqrt_(r[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
+  const T sq = r[NIE_SQ], bsq = r[NIE_BSQ];
+  const T fx = bsq * Fn<T>::atan_(sq * xr / (psi + ss));
+  const T fy = bsq * Fn<T>::atanh_(sq * yr / (psi + r[NIE_Q2S]));
+  const T c = r[NIE_C], s = r[NIE_S];
+  ax += double(fx * c - fy * s);
+  ay += double(fx * s + fy * c);
+}
+
+// h(X) = acosh(1/X)/sqrt(1-X^2) (X<1), acos(1/X)/sqrt(X^2-1) (X>1), 1 at X=1.
+// It equals sum_k v^k/(2k+1) with v = 1 - X^2; near X = 1 the float form uses
+// that series because both closed branches cancel there.
+__device__ __forceinline__ float nfw_h_f32(float X) {
+  const float v = (1.0f - X) * (1.0f + X);
+  if (fabsf(v) < 0.1f) {
+    float s = 1.0f / 17.0f;
+    s = s * v + 1.0f / 15.0f;
+    s = s * v + 1.0f / 13.0f;
+    s = s * v + 1.0f / 11.0f;
+    s = s * v + 1.0f / 9.0f;
+    s = s * v + 1.0f / 7.0f;
+    s = s * v + 1.0f / 5.0f;
+    s = s * v + 1.0f / 3.0f;
+    s = s * v + 1.0f;
+    return s;
+  }
+  if (v > 0.0f) {
+    const float sv = __builtin_amdgcn_sqrtf(v);
+    return __logf((1.0f + sv) / X) / sv;  // acosh(1/X) = ln((1 + sqrt(1-X^2))/X)
+  }
+  const float sv = __builtin_amdgcn_sqrtf(-v);
+  return ::atanf(sv) / sv;  // acos(1/X) = atan(sqrt(X^2-1))
+}
+
+// NFW.g, LensModel/Profiles/nfw.py:345-365, exactly as the reference (FP64 form).
+__device__ __forceinline__ double nfw_g(double X) {
+  X = (X < 0.000001) ? 0.000001 : X;
+  if (X < 1.0) return ::log(X / 2.0) + 1.0 / ::sqrt(1.0 - X * X) * ::acosh(1.0 / X);
+  if (X == 1.0) return 1.0 + ::log(1.0 / 2.0);
+  return ::log(X / 2.0) + 1.0 / ::sqrt(X * X - 1.0) * ::acos(1.0 / X);
+}
+
+__device__ __forceinline__ float nfw_g(float X) {
+  X = (X < 0.000001f) ? 0.000001f : X;
+  return __logf(0.5f * X) + nfw_h_f32(X);
+}
+
+// NFW deflection, LensModel/Profiles/nfw.py:86-105 with nfw_alpha :257-279.
+template <typename T>
+__device__ __forceinline__ void nfw_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
+  const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
+  const T dx = T(dxd), dy = T(dyd);
+  T R = Fn<T>::sqrt_(dx * dx + dy * dy);
+  R = R > T(0.00000001) ? R : T(0.00000001);
+  // X = R / Rs by division in the FP64 form: g(X) has an exact X == 1 branch
+  // and cancels near it, so X must round as the reference's does.
+  const T X = (sizeof(T) == sizeof(double)) ? R / r[NFW_RS] : R * r[NFW_IRS];
+  const T g = nfw_g(X);
+  const T a = r[NFW_A] * g / (X * X);
+  ax += double(a * dx);
+  ay += double(a * dy);
+}
+
+// TNFW.F, LensModel/Profiles/tnfw.py:288-304 (FP64 form as the reference).
+__device__ __forceinline__ double tnfw_F(double x) {
+  if (x < 1.0) {
+    const double v = 1.0 - x * x;
+    return 1.0 / ::sqrt(v) * ::atanh(::sqrt(v));
+  }
+  if (x > 1.0) {
+    const double v = x * x - 1.0;
+    return 1.0 / ::sqrt(v) * ::atan(::sqrt(v));
+  }
+  return x;
+}
+__device__ __forceinline__ float tnfw_F(float x) { return nfw_h_f32(x); }
+
+// TNFW deflection, LensModel/Profiles/tnfw.py:67-87, :226-248, _g :329-349,
+// _L :276-286 (Baltz, Marshall & Oguri 2009).
+template <typename T>
+__device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
+  const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
+  const T dx = T(dxd), dy = T(dyd);
+  T R = Fn<T>::sqrt_(dx * dx + dy * dy);
+  const T rmin = r[TN_RMIN];
+  R = R > rmin ? R : rmin;
+  T X = (sizeof(T) == sizeof(double)) ? R / r[TN_RS] : R * r[TN_IRS];
+  X = X > T(0.001) ? X : T(0.001);
+  const T tau = r[TN_TAU], tau2 = r[TN_TAU2];
+  const T X2 = X * X;
+  const T sqtx = Fn<T>::sqrt_(tau2 + X2);
+  const T L = Fn<T>::log_(X / (tau + sqtx));
+  const T g = r[TN_PRE] * ((r[TN_T2P1] + T(2) * (X2 - T(1))) * tnfw_F(X) + r[TN_C0] +
+                           sqtx * (T(-M_PI) + L * r[TN_C1]));
+  const T a = r[TN_A] * g / X2;
+  ax += double(a * dx);
+  ay += double(a * dy);
+}
+
+// External shear, LensModel/Profiles/shear.py:37-53 (linear: kept in double).
+__device__ __forceinline__ void shear_defl(const double *rd, double x, double y, double &ax, double &ay) {
+  const double dx = x - rd[R_CX], dy = y - rd[R_CY];
+  const double g1 = rd[SH_G1], g2 = rd[SH_G2];
+  ax += g1 * dx + g2 * dy;
+  ay += g2 * dx - g1 * dy;
+}
+
+// Adds one deflector's alpha at (x, y).  coef_d / coef_t: this profile's EPL
+// series coefficients (NULL for other profiles).
+template <typename T>
+__device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const double *__restrict__ coef_d,
+                                               const T *__restrict__ coef_t, double x, double y, double &ax,
+                                               double &ay) {
+  switch (type) {
+    case LENSIM_EPL:
+      // The mixed form keeps long series (nmax >= 100, f >= 0.73) in double.
+      if (sizeof(T) == sizeof(float) && r.d[EPL_NMAX] >= 100.0)
+        epl_defl<T, double>(r, coef_d, x, y, ax, ay);
+      else
+        epl_defl<T, T>(r, coef_t, x, y, ax, ay);
+      break;
+    case LENSIM_SIE:
+    case LENSIM_NIE:
+      nie_defl<T>(r, x, y, ax, ay);
+      break;
+    case LENSIM_NFW:
+      nfw_defl<T>(r, x, y, ax, ay);
+      break;
+    case LENSIM_TNFW:
+      tnfw_defl<T>(r, x, y, ax, ay);
+      break;
+    case LENSIM_SHEAR:
+      shear_defl(r.d, x, y, ax, ay);
+      break;
+    default:
+      break;
+  }
+}
+
+// ---------------------------------------------------------- light --------
+
+// Sersic kernel, LensModel/Profiles/sersic_utils.py:239-267 (FP64 form).
+__device__ __forceinline__ double sersic_kernel(double R, const Rec<double> &r) {
+  const double R_ = fmax(0.0001, R);
+  const double frac = R_ / r.d[SE_RS];
+  const double e = -r.d[SE_BN] * (::pow(frac, r.d[SE_INVN]) - 1.0);
+  double v = (frac <= 1000.0) ? ::exp(e) : 0.0;
+  if (v != v) v = 0.0;
+  if (v == INFINITY) v = 1.7976931348623157e308;
+  return v;
+}
+
+// Float form: frac^(1/n) and exp() through the hardware base-2 exp/log.
+__device__ __forceinline__ float sersic_kernel(float R, const Rec<float> &r) {
+  const float R_ = fmaxf(0.0001f, R);
+  const float frac = R_ * r[SE_IRS];
+  const float pw = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(frac) * r[SE_INVN]);
+  const float e = -r[SE_BN] * (pw - 1.0f);
+  float v = (frac <= 1000.0f) ? __builtin_amdgcn_exp2f(e * 1.4426950408889634f) : 0.0f;
+  if (v != v) v = 0.0f;
+  return v;
+}
+
+// One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
+// sersic.py:53-73) and GAUSSIAN (gaussian.py:27-42).
+template <typename T>
+__device__ __forceinline__ double light_value(int type, const Rec<T> &r, double x, double y) {
+  const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
+  const T dx = T(dxd), dy = T(dyd);
+  switch (type) {
+    case LENSIM_SERSIC:
+    case LENSIM_SERSIC_ELLIPSE: {
+      const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
+      const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
+      const T R = Fn<T>::sqrt_(xe * xe + ye * ye);
+      return double(r[SE_AMP] * sersic_kernel(R, r));
+    }
+    case LENSIM_GAUSSIAN: {
+      const T r2 = dx * dx * r[GA_IS2] + dy * dy * r[GA_IS2];
+      return double(r[GA_C] * Fn<T>::exp_(-r2 / T(2)));
+    }
+    default:
+      return 0.0;
+  }
+}
+
+}  // namespace lensim

@@ -1,0 +1,118 @@
+"""Profile registry: reference profile names -> library type codes and the
+ordered parameter names of each profile (the order of a parameter slot).
+
+Names and parameter orders follow the reference's ``param_names``:
+EPL LensModel/Profiles/epl.py:50, SIE sie.py:40, NIE nie.py, NFW nfw.py:48,
+TNFW tnfw.py:27, SHEAR shear.py:16, SERSIC_ELLIPSE
+LightModel/Profiles/sersic_ellipse.py:25, SERSIC sersic.py, GAUSSIAN gaussian.py.
+"""
+
+import numpy as np
+import torch
+
+from . import backend
+
+LENS_PROFILES = {
+    "EPL": (1, ["theta_E", "gamma", "e1", "e2", "center_x", "center_y"]),
+    "SIE": (2, ["theta_E", "e1", "e2", "center_x", "center_y"]),
+    "NIE": (3, ["theta_E", "e1", "e2", "s_scale", "center_x", "center_y"]),
+    "NFW": (4, ["Rs", "alpha_Rs", "center_x", "center_y"]),
+    "TNFW": (5, ["Rs", "alpha_Rs", "r_trunc", "center_x", "center_y"]),
+    "SHEAR": (6, ["gamma1", "gamma2", "ra_0", "dec_0"]),
+}
+
+LIGHT_PROFILES = {
+    "SERSIC": (32, ["amp", "R_sersic", "n_sersic", "center_x", "center_y"]),
+    "SERSIC_ELLIPSE": (33, ["amp", "R_sersic", "n_sersic", "e1", "e2", "center_x", "center_y"]),
+    "GAUSSIAN": (34, ["amp", "sigma", "center_x", "center_y"]),
+}
+
+# keyword arguments with a default value in the reference's function signatures
+_DEFAULTS = {"center_x": 0.0, "center_y": 0.0, "ra_0": 0.0, "dec_0": 0.0}
+
+
+def lens_type(name):
+    if name not in LENS_PROFILES:
+        raise ValueError("%s is not a supported lens model. Supported are: %s." % (name, sorted(LENS_PROFILES)))
+    return LENS_PROFILES[name]
+
+
+def light_type(name):
+    if name not in LIGHT_PROFILES:
+        raise ValueError("Light model of type %s not supported. Supported are %s" % (name, sorted(LIGHT_PROFILES)))
+    return LIGHT_PROFILES[name]
+
+
+def _batch_of(value):
+    if isinstance(value, torch.Tensor):
+        return 1 if value.dim() == 0 else int(value.numel())
+    a = np.asarray(value)
+    return 1 if a.ndim == 0 else int(a.size)
+
+
+def batch_size(kwargs_lists):
+    """Batch size of a set of kwargs lists whose values are scalars or 1-D arrays."""
+    n = None
+    for kwargs_list in kwargs_lists:
+        for kw in kwargs_list or []:
+            for v in kw.values():
+                b = _batch_of(v)
+                if b != 1:
+                    if n is not None and n != b:
+                        raise ValueError("inconsistent batch sizes %d and %d in kwargs" % (n, b))
+                    n = b
+    return 1 if n is None else n
+
+
+def pack(names_lists, kwargs_lists, n_img):
+    """Pack kwargs into a float64 [n_img][n_profiles][NPARAM] parameter block.
+
+    names_lists: per profile, its ordered parameter names.  Values may be
+    python scalars, numpy arrays or torch tensors (scalar or length n_img).
+    Missing parameters without a reference default raise TypeError, unknown
+    ones raise ValueError, as the reference's profile calls do.
+    """
+    n_prof = len(names_lists)
+    cols = []
+    for names, kw in zip(names_lists, kwargs_lists):
+        for key in kw:
+            if key not in names:
+                raise ValueError("parameter %s is not part of profile parameters %s" % (key, names))
+        slot = []
+        for name in names:
+            if name in kw:
+                v = kw[name]
+            elif name in _DEFAULTS:
+                v = _DEFAULTS[name]
+            else:
+                raise TypeError("missing required parameter '%s'" % name)
+            if isinstance(v, torch.Tensor):
+                v = v.detach().to("cpu", torch.float64).numpy()
+            v = np.asarray(v, dtype=np.float64).reshape(-1)
+            if v.size == 1:
+                v = np.full(n_img, v[0])
+            elif v.size != n_img:
+                raise ValueError("parameter %s has %d values for a batch of %d" % (name, v.size, n_img))
+            slot.append(v)
+        while len(slot) < backend.NPARAM:
+            slot.append(np.zeros(n_img))
+        cols.append(np.stack(slot, axis=1))
+    if n_prof == 0:
+        return np.zeros((n_img, 0, backend.NPARAM))
+    return np.ascontiguousarray(np.stack(cols, axis=1))
+
+
+def select(k, n):
+    """convert_bool_list semantics: None -> all, int -> one, list -> those."""
+    if k is None:
+        return list(range(n))
+    if isinstance(k, (int, np.integer)):
+        if k < 0 or k >= n:
+            raise ValueError("k=%d out of range for %d profiles" % (k, n))
+        return [int(k)]
+    idx = list(k)
+    if all(isinstance(i, (bool, np.bool_)) for i in idx):
+        if len(idx) != n:
+            raise ValueError("bool list length %d does not match %d profiles" % (len(idx), n))
+        return [i for i, b in enumerate(idx) if b]
+    return [int(i) for i in idx]

@@ -1,0 +1,66 @@
+"""The C restatement of the oracle agrees with the numpy restatement (CPU)."""
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+from jaxtronomy_amd import workloads as W
+from oracle import c_oracle
+from oracle import lensim_oracle as O
+
+TYPES = {"EPL": 1, "SIE": 2, "NIE": 3, "NFW": 4, "TNFW": 5, "SHEAR": 6, "SERSIC": 32, "SERSIC_ELLIPSE": 33,
+         "GAUSSIAN": 34}
+
+
+def _pack(names, kwargs):
+    p = np.zeros((1, len(names), 8))
+    for k, (name, kw) in enumerate(zip(names, kwargs)):
+        pn = (O.LENS_FUNCS.get(name) or O.LIGHT_FUNCS.get(name))[1]
+        for i, n in enumerate(pn):
+            p[0, k, i] = kw.get(n, 0.0)
+    return p
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("lens", [["SIE", "SHEAR"], ["EPL", "SHEAR"], ["NFW", "SIE"], ["TNFW", "NIE"]])
+def test_c_oracle_image_matches_numpy(lens):
+    rng = np.random.default_rng(3)
+    kwl = []
+    for name in lens:
+        if name in ("SIE", "NIE"):
+            kw = dict(theta_E=1.1, e1=0.12, e2=-0.07, center_x=0.02, center_y=-0.01)
+            if name == "NIE":
+                kw["s_scale"] = 0.05
+        elif name == "EPL":
+            kw = dict(theta_E=1.2, gamma=2.1, e1=0.2, e2=0.05, center_x=0.0, center_y=0.0)
+        elif name == "SHEAR":
+            kw = dict(gamma1=0.03, gamma2=-0.01)
+        elif name == "NFW":
+            kw = dict(Rs=0.6, alpha_Rs=0.4, center_x=0.1, center_y=0.0)
+        else:
+            kw = dict(Rs=0.3, alpha_Rs=0.2, r_trunc=1.5, center_x=-0.2, center_y=0.1)
+        kwl.append(kw)
+    src = [dict(amp=20.0, R_sersic=0.3, n_sersic=2.5, e1=0.1, e2=0.0, center_x=0.05, center_y=0.0)]
+    light = [dict(amp=5.0, R_sersic=0.8, n_sersic=3.0, center_x=0.0, center_y=0.0)]
+    cfg = dict(W.SMALL, numPix=40, supersampling=2)
+    g = W.grid_dict(cfg)
+    ker = W.moffat_kernel(9, 2.0, 3.0)
+    want = O.image(g, lens, kwl, ["SERSIC_ELLIPSE"], src, ["SERSIC"], light, kernel=ker)
+    names = lens + ["SERSIC_ELLIPSE", "SERSIC"]
+    params = _pack(names, kwl + src + light)
+    got = c_oracle.image(len(lens), 1, 1, [TYPES[n] for n in names], params, g, kernel=ker, psf_type=1)
+    assert _rel(got[0], want) < 1e-12
+    # Gaussian PSF (edge padding) and the log-likelihood
+    gk = O.gaussian_kernel(0.1, 0.05, 6)
+    want = O.image(g, lens, kwl, ["SERSIC_ELLIPSE"], src, ["SERSIC"], light, psf_type="GAUSSIAN", kernel=gk)
+    got = c_oracle.image(len(lens), 1, 1, [TYPES[n] for n in names], params, g, kernel=gk, psf_type=2)
+    assert _rel(got[0], want) < 1e-12
+    data = want + rng.normal(0, 0.01, want.shape)
+    mask = (rng.uniform(size=want.shape) > 0.1).astype(float)
+    ll = c_oracle.loglike(len(lens), 1, 1, [TYPES[n] for n in names], params, g, data, kernel=gk, psf_type=2,
+                          exposure_time=1000.0, background_rms=0.01, mask=mask)
+    want_ll = O.log_likelihood(want, data, mask, background_rms=0.01, exposure_map=1000.0)
+    npt.assert_allclose(ll[0], want_ll, rtol=1e-11)

@@ -1,0 +1,295 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bars: the plain FP64 form within 1e-10 relative (image error measured as
+max|delta| / max|reference|); the mixed form within 1e-5 for images and 1e-6
+relative for the log-likelihood; deflections in the mixed form within 2e-6 of
+the deflection scale.
+"""
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+import torch
+
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.imaging_data import ImageData
+from jaxtronomy_amd.Data.psf import PSF
+from jaxtronomy_amd.ImSim.image_model import ImageModel
+from jaxtronomy_amd.LensModel.lens_model import LensModel
+from jaxtronomy_amd.LightModel.light_model import LightModel
+from oracle import lensim_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_IMG_TOL = 1e-5
+MIXED_LOGL_TOL = 1e-6
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def img_err(got, want):
+    return float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+
+
+def _points(seed=0, n=500, r=3.0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-r, r, n)
+    y = rng.uniform(-r, r, n)
+    # exact centres, axes and the origin are edge cases of every profile
+    x[:4] = [0.0, 0.0, 1e-12, 0.3]
+    y[:4] = [0.0, 0.5, 0.0, 0.0]
+    return x, y
+
+
+LENS_CASES = [
+    ("EPL", dict(theta_E=1.2, gamma=1.9, e1=0.1, e2=-0.05, center_x=0.1, center_y=-0.2)),
+    ("EPL", dict(theta_E=1.0, gamma=2.2, e1=0.3, e2=0.2, center_x=0.0, center_y=0.0)),
+    ("EPL", dict(theta_E=1.0, gamma=2.0, e1=0.0, e2=0.0, center_x=0.0, center_y=0.0)),  # f == 0
+    ("EPL", dict(theta_E=0.9, gamma=1.7, e1=-0.55, e2=0.45, center_x=0.0, center_y=0.0)),  # nmax 200-500
+    ("SIE", dict(theta_E=1.3, e1=0.12, e2=0.07, center_x=0.0, center_y=0.0)),
+    ("SIE", dict(theta_E=1.3, e1=0.0, e2=0.0, center_x=0.05, center_y=0.0)),  # q clamp
+    ("NIE", dict(theta_E=1.1, e1=-0.2, e2=0.1, s_scale=0.2, center_x=0.0, center_y=0.1)),
+    ("NFW", dict(Rs=0.8, alpha_Rs=0.35, center_x=0.0, center_y=0.0)),
+    ("NFW", dict(Rs=0.5, alpha_Rs=0.35, center_x=0.5, center_y=0.0)),  # X == 1 at (0, 0)
+    ("TNFW", dict(Rs=0.4, alpha_Rs=0.2, r_trunc=2.0, center_x=0.1, center_y=0.1)),
+    ("SHEAR", dict(gamma1=0.03, gamma2=-0.02, ra_0=0.1, dec_0=0.0)),
+]
+
+
+@pytest.mark.parametrize("name,kw", LENS_CASES)
+def test_ray_shooting_fp64(name, kw):
+    x, y = _points()
+    lm = LensModel([name])
+    bx, by = lm.ray_shooting(x, y, [kw])
+    func, names = O.LENS_FUNCS[name]
+    ax, ay = func(x, y, *[kw.get(n, 0.0) for n in names])
+    npt.assert_allclose(_np(bx), x - ax, rtol=FP64_TOL, atol=1e-13)
+    npt.assert_allclose(_np(by), y - ay, rtol=FP64_TOL, atol=1e-13)
+
+
+@pytest.mark.parametrize("name,kw", LENS_CASES)
+def test_ray_shooting_mixed(name, kw):
+    x, y = _points()
+    lm = LensModel([name], precision="mixed")
+    bx, by = lm.ray_shooting(x, y, [kw])
+    func, names = O.LENS_FUNCS[name]
+    ax, ay = func(x, y, *[kw.get(n, 0.0) for n in names])
+    scale = max(np.max(np.abs(ax)), np.max(np.abs(ay)), 1e-3)
+    assert np.max(np.abs(_np(bx) - (x - ax))) / scale < 2e-6
+    assert np.max(np.abs(_np(by) - (y - ay))) / scale < 2e-6
+
+
+def test_epl_q_sweep_batched():
+    """Scenario of test/test_LensModel/test_Profiles/test_epl.py:190-206: 200 axis
+    ratios in [0.03, 1] x 101 points, as one batch of 200 parameter sets."""
+    y = np.linspace(-10, 10, 101)
+    x = np.ones_like(y)
+    qs = np.linspace(0.03, 1, 200)[::-1]
+    c = (1 - qs) / (1 + qs)
+    kw = dict(theta_E=1.7, gamma=1.9, e1=c, e2=np.zeros_like(c), center_x=0.0, center_y=0.0)
+    bx, by = LensModel(["EPL"]).ray_shooting(x, y, [kw])
+    assert bx.shape == (200, 101)
+    for i in range(200):
+        ax, ay = O.epl_derivatives(x, y, 1.7, 1.9, c[i], 0.0)
+        npt.assert_allclose(_np(bx[i]), x - ax, rtol=FP64_TOL, atol=1e-12)
+        npt.assert_allclose(_np(by[i]), y - ay, rtol=FP64_TOL, atol=1e-12)
+
+
+def test_ray_shooting_profile_sum_and_k():
+    x, y = _points(1)
+    names = ["EPL", "SHEAR", "NFW"]
+    kws = [LENS_CASES[0][1], LENS_CASES[10][1], LENS_CASES[7][1]]
+    lm = LensModel(names)
+    bx, by = lm.ray_shooting(x, y, kws)
+    wx, wy = O.ray_shooting(x, y, names, kws)
+    npt.assert_allclose(_np(bx), wx, rtol=FP64_TOL, atol=1e-13)
+    npt.assert_allclose(_np(by), wy, rtol=FP64_TOL, atol=1e-13)
+    bx1, _ = lm.ray_shooting(x, y, kws, k=1)
+    wx1, _ = O.ray_shooting(x, y, ["SHEAR"], [kws[1]])
+    npt.assert_allclose(_np(bx1), wx1, rtol=FP64_TOL, atol=1e-13)
+
+
+LIGHT_CASES = [
+    ("SERSIC_ELLIPSE", dict(amp=23.0, R_sersic=0.6, n_sersic=7, e1=0.05, e2=0.03, center_x=0.0, center_y=0.0)),
+    ("SERSIC_ELLIPSE", dict(amp=3.0, R_sersic=0.1, n_sersic=1.2, e1=-0.3, e2=0.2, center_x=0.1, center_y=0.0)),
+    ("SERSIC", dict(amp=31.0, R_sersic=0.1, n_sersic=2, center_x=0.0, center_y=0.0)),
+    ("GAUSSIAN", dict(amp=2.0, sigma=0.3, center_x=0.2, center_y=-0.1)),
+]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("mixed", 2e-5)])
+@pytest.mark.parametrize("name,kw", LIGHT_CASES)
+def test_surface_brightness(name, kw, precision, tol):
+    x, y = _points(2)
+    f = LightModel([name], precision=precision).surface_brightness(x, y, [kw])
+    func, names = O.LIGHT_FUNCS[name]
+    want = func(x, y, *[kw.get(n, 0.0) for n in names])
+    assert img_err(_np(f), want) < tol
+
+
+# ------------------------------------------------------------------ images --
+
+def _small_model(psf_type, precision, draw=0):
+    cfg = W.SMALL
+    data = ImageData(**W.data_kwargs(cfg))
+    if psf_type == "GAUSSIAN":
+        psf = PSF("GAUSSIAN", fwhm=cfg["psf_fwhm"], truncation=cfg["psf_truncation"])
+    else:
+        psf = PSF("PIXEL", kernel_point_source=O.gaussian_kernel(cfg["psf_fwhm"], cfg["deltaPix"], 6))
+    im = ImageModel(data, psf, LensModel(["SIE", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+    return im, cfg
+
+
+@pytest.mark.parametrize("psf_type", ["GAUSSIAN", "PIXEL"])
+def test_small_config_images_fp64(psf_type):
+    """20 seeded draws of the 60x60 SIE+SHEAR+Sersic scene, against the oracle."""
+    im, cfg = _small_model(psf_type, "fp64")
+    assert im.kernel.shape == (11, 11)
+    g = W.grid_dict(cfg)
+    for d in range(20):
+        kl, ks = W.small_draw(d)
+        got = _np(im.image(kl, ks))
+        want = O.image(g, ["SIE", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, [], [], psf_type=psf_type,
+                       kernel=im.kernel)
+        assert img_err(got, want) < FP64_TOL, d
+
+
+def test_small_config_batch_equals_single_and_mixed():
+    im, cfg = _small_model("GAUSSIAN", "fp64")
+    imm, _ = _small_model("GAUSSIAN", "mixed")
+    draws = [W.small_draw(d) for d in range(16)]
+    kl = W.stack_kwargs([d[0] for d in draws])
+    ks = W.stack_kwargs([d[1] for d in draws])
+    batch = _np(im.image(kl, ks))
+    batch_m = _np(imm.image(kl, ks))
+    assert batch.shape == (16, 60, 60)
+    for i in (0, 7, 15):
+        single = _np(im.image(*draws[i]))
+        npt.assert_array_equal(batch[i], single)
+        assert img_err(batch_m[i], single) < MIXED_IMG_TOL
+
+
+def _epl180_model(precision):
+    cfg = W.EPL180
+    data = ImageData(**W.data_kwargs(cfg))
+    psf = PSF("PIXEL", kernel_point_source=W.moffat_kernel(cfg["psf_size"], cfg["psf_fwhm"] / cfg["deltaPix"],
+                                                           cfg["psf_beta"]))
+    im = ImageModel(data, psf, LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]),
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": cfg["supersampling"]},
+                    precision=precision)
+    return im, cfg
+
+
+def test_epl180_images_fp64_and_mixed():
+    im, cfg = _epl180_model("fp64")
+    imm, _ = _epl180_model("mixed")
+    g = W.grid_dict(cfg)
+    for d in range(3):
+        kl, ks, kll = W.epl180_draw(d)
+        want = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=im.kernel)
+        got = _np(im.image(kl, ks, kll))
+        assert img_err(got, want) < FP64_TOL, d
+        got_m = _np(imm.image(kl, ks, kll))
+        assert img_err(got_m, want) < MIXED_IMG_TOL, d
+
+
+def test_source_lens_light_split_and_unconvolved():
+    im, cfg = _epl180_model("fp64")
+    g = W.grid_dict(cfg)
+    kl, ks, kll = W.epl180_draw(5)
+    src = _np(im.source_surface_brightness(ks, kl))
+    want = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, [], [], kernel=im.kernel)
+    assert img_err(src, want) < FP64_TOL
+    ll = _np(im.lens_surface_brightness(kll, unconvolved=True))
+    want = O.image(g, [], [], [], [], ["SERSIC_ELLIPSE"], kll, unconvolved=True)
+    assert img_err(ll, want) < FP64_TOL
+    de = _np(im.source_surface_brightness(ks, kl, de_lensed=True))
+    want = O.image(g, [], [], ["SERSIC_ELLIPSE"], ks, [], [], kernel=im.kernel)
+    assert img_err(de, want) < FP64_TOL
+
+
+def test_reference_image_model_scene():
+    """Scenario of test/test_ImSim/test_image_model.py:28-110 without point sources:
+    100x100 at 0.05", SIE+SHEAR, SERSIC lens light, SERSIC_ELLIPSE source,
+    17x17 PIXEL kernel, supersampling 3."""
+    from jaxtronomy_amd.Util import param_util
+    from jaxtronomy_amd.Util.simulation_util import data_configure_simple
+
+    kwargs_data = data_configure_simple(100, 0.05, 100, 0.05, inverse=True)
+    kernel = np.zeros((17, 17))
+    kernel[5:-5, 5:-5] = 1
+    kernel[7:-7, 7:-7] = 3
+    kernel[9, 9] = 7
+    e1, e2 = param_util.phi_q2_ellipticity(0.2, 0.8)
+    kwargs_lens = [dict(theta_E=1.0, center_x=0, center_y=0, e1=e1, e2=e2), dict(gamma1=0.01, gamma2=0.01)]
+    e1s, e2s = param_util.phi_q2_ellipticity(0.2, 0.9)
+    kwargs_source = [dict(amp=23.0, R_sersic=0.6, n_sersic=7, center_x=0, center_y=0, e1=e1s, e2=e2s)]
+    kwargs_lens_light = [dict(amp=31.0, R_sersic=0.1, n_sersic=2, center_x=0, center_y=0)]
+    im = ImageModel(ImageData(**kwargs_data), PSF("PIXEL", kernel_point_source=kernel), LensModel(["SIE", "SHEAR"]),
+                    LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC"]),
+                    kwargs_numerics={"supersampling_factor": 3})
+    got = _np(im.image(kwargs_lens, kwargs_source, kwargs_lens_light))
+    g = dict(nx=100, ny=100, s=3, ra_at_xy_0=kwargs_data["ra_at_xy_0"], dec_at_xy_0=kwargs_data["dec_at_xy_0"],
+             transform_pix2angle=kwargs_data["transform_pix2angle"])
+    want = O.image(g, ["SIE", "SHEAR"], kwargs_lens, ["SERSIC_ELLIPSE"], kwargs_source, ["SERSIC"],
+                   kwargs_lens_light, kernel=kernel / kernel.sum())
+    assert img_err(got, want) < FP64_TOL
+    # likelihood of the model against noisy data drawn from it (image_model.py:162-221)
+    rng = np.random.default_rng(7)
+    data = W.noisy(want, 100.0, 0.05, rng)
+    kwargs_data["image_data"] = data
+    im2 = ImageModel(ImageData(**kwargs_data), PSF("PIXEL", kernel_point_source=kernel), LensModel(["SIE", "SHEAR"]),
+                     LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC"]),
+                     kwargs_numerics={"supersampling_factor": 3})
+    logl = float(im2.likelihood_data_given_model(kwargs_lens, kwargs_source, kwargs_lens_light))
+    want_ll = O.log_likelihood(want, data, background_rms=0.05, exposure_map=100.0)
+    npt.assert_allclose(logl, want_ll, rtol=FP64_TOL)
+
+
+# -------------------------------------------------------------- likelihood --
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("mixed", MIXED_LOGL_TOL)])
+def test_epl180_loglike_batch(precision, tol):
+    cfg = W.EPL180
+    g = W.grid_dict(cfg)
+    kernel = W.moffat_kernel(cfg["psf_size"], cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    truth = W.epl180_draw(0)
+    model_truth = O.image(g, ["EPL", "SHEAR"], truth[0], ["SERSIC_ELLIPSE"], truth[1], ["SERSIC_ELLIPSE"],
+                          truth[2], kernel=kernel)
+    rng = np.random.default_rng(11)
+    data = W.noisy(model_truth, cfg["exposure_time"], cfg["background_rms"], rng)
+    mask = np.ones_like(data)
+    mask[:10, :] = 0  # a masked strip
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 3}, likelihood_mask=mask, precision=precision)
+    draws = [truth] + [W.epl180_draw(d) for d in range(1, 4)]
+    kl = W.stack_kwargs([d[0] for d in draws])
+    ks = W.stack_kwargs([d[1] for d in draws])
+    kll = W.stack_kwargs([d[2] for d in draws])
+    logl = _np(im.likelihood_data_given_model(kl, ks, kll))
+    for i, d in enumerate(draws):
+        m = O.image(g, ["EPL", "SHEAR"], d[0], ["SERSIC_ELLIPSE"], d[1], ["SERSIC_ELLIPSE"], d[2], kernel=kernel)
+        want = O.log_likelihood(m, data, mask, background_rms=cfg["background_rms"],
+                                exposure_map=cfg["exposure_time"])
+        assert abs(logl[i] - want) / abs(want) < tol, (i, logl[i], want)
+
+
+def test_noise_map_likelihood_and_gaussian_psf():
+    im, cfg = _small_model("GAUSSIAN", "fp64")
+    g = W.grid_dict(cfg)
+    kl, ks = W.small_draw(3)
+    m = O.image(g, ["SIE", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, [], [], psf_type="GAUSSIAN", kernel=im.kernel)
+    rng = np.random.default_rng(5)
+    noise_map = np.full(m.shape, 0.02) + 0.01 * rng.uniform(size=m.shape)
+    data = m + rng.normal(0, 1, m.shape) * noise_map
+    kw = W.data_kwargs(cfg, data)
+    kw["noise_map"] = noise_map
+    im2 = ImageModel(ImageData(**kw), PSF("GAUSSIAN", fwhm=0.1, truncation=6), LensModel(["SIE", "SHEAR"]),
+                     LightModel(["SERSIC_ELLIPSE"]))
+    logl = float(im2.likelihood_data_given_model(kl, ks))
+    npt.assert_allclose(logl, O.log_likelihood(m, data, noise_map=noise_map), rtol=FP64_TOL)
