@@ -78,7 +78,7 @@ void timing_mark(int slot, hipStream_t st) {
 }
 
 struct ModelArgs {
-  int32_t n_lens, n_source, n_light, n_prof, n_epl;
+  int32_t n_lens, n_source, n_light, n_prof, n_epl, has_halo;
   int32_t types[LENSIM_MAX_PROFILES];
   int32_t epl_index[LENSIM_MAX_PROFILES];
   double alpha_scaling, flux_scaling;
@@ -125,7 +125,7 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   L.coeff = off;
   off = align256(off + ncoef * 4);
   L.psf = off;
-  off = align256(off + size_t(ky) * kxp_of(kx) * tsz);
+  off = align256(off + size_t(ky) * kxp_of(kx) * 8);
   L.lowres = off;
   off = align256(off + size_t(n_img) * nx * ny * tsz);
   L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
@@ -159,6 +159,7 @@ int model_args(const lensim_model *model, ModelArgs &m, bool need_lens_light_ok 
     if (!lens && !light_ok)
       return fail(LENSIM_EUNSUPPORTED, "profile %d: type %d is not a supported light profile", k, t);
     if (t == LENSIM_EPL) m.epl_index[k] = m.n_epl++;
+    if (t == LENSIM_NFW || t == LENSIM_TNFW) m.has_halo = 1;
   }
   m.alpha_scaling = model->alpha_scaling;
   m.flux_scaling = model->flux_scaling;
@@ -192,16 +193,15 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
 
 // Flip the kernel (so the convolution runs as a correlation), zero-pad its
 // width to kxp and convert it to T.
-template <typename T>
-__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kxp, T *__restrict__ out) {
+__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kxp, double *__restrict__ out) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= kxp * ky) return;
   const int a = idx / kxp, c = idx - a * kxp;
   if (!k) {  // identity kernel (no PSF / unconvolved)
-    out[idx] = (a == 0 && c == 0) ? T(1) : T(0);
+    out[idx] = (a == 0 && c == 0) ? 1.0 : 0.0;
     return;
   }
-  out[idx] = (c < kx) ? T(k[(ky - 1 - a) * kx + (kx - 1 - c)]) : T(0);
+  out[idx] = (c < kx) ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
 }
 
 template <typename T>
@@ -228,7 +228,7 @@ __device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double 
   return r;
 }
 
-template <typename T>
+template <typename T, bool HALO>
 __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
                                             double &ax, double &ay) {
   for (int k = 0; k < m.n_lens; ++k) {
@@ -236,7 +236,7 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
-    add_deflection<T>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+    add_deflection<T, HALO>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   }
 }
 
@@ -257,7 +257,7 @@ struct GridArgs {
   double scale;  // pixel_width^2 / s^2
 };
 
-template <typename T>
+template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
                                                      const float *__restrict__ recf,
                                                      const double *__restrict__ coefd,
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
         double ax = 0.0, ay = 0.0;
-        deflect_all<T>(m, rp, img, x, y, ax, ay);
+        deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
         const double bx = x - ax * m.alpha_scaling;
         const double by = y - ay * m.alpha_scaling;
         acc += light_sum<T>(m, rp, k_src0, k_src1, bx, by);
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   out[size_t(img) * npix + pix] = T(acc * (g.scale * m.flux_scaling));
 }
 
-template <typename T>
+template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double *__restrict__ recd,
                                                        const float *__restrict__ recf,
                                                        const double *__restrict__ coefd,
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   const double x = xs[i], y = ys[i];
   double ax = 0.0, ay = 0.0;
-  deflect_all<T>(m, rp, img, x, y, ax, ay);
+  deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
   bx[size_t(img) * n_ray + i] = x - ax * m.alpha_scaling;
   by[size_t(img) * n_ray + i] = y - ay * m.alpha_scaling;
 }
@@ -356,8 +356,9 @@ __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T mode
 // Block = TY rows x (TX / kWin) column groups; each thread computes kWin
 // consecutive outputs of one row from a register window over the LDS tile.
 template <typename T>
-__global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in, const T *__restrict__ kf,
-                                                   T *__restrict__ out, double *__restrict__ partial) {
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in,
+                                                   const double *__restrict__ kf, T *__restrict__ out,
+                                                   double *__restrict__ partial) {
   constexpr int TY = ConvTile<T>::TY, TX = ConvTile<T>::TX, G = TX / kWin;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);
@@ -386,21 +387,23 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   }
   __syncthreads();
   const int ty = tid / G, tg = tid - ty * G;
-  T acc[kWin];
+  // float64 accumulation in both forms (full-rate FMA on CDNA4); the tile is
+  // stored in T and widened once per loaded element
+  double acc[kWin];
 #pragma unroll
-  for (int r = 0; r < kWin; ++r) acc[r] = T(0);
+  for (int r = 0; r < kWin; ++r) acc[r] = 0.0;
   for (int a = 0; a < c.ky; ++a) {
     const T *row = tile + (ty + a) * TW + tg * kWin;
-    const T *krow = kf + a * c.kxp;
+    const double *krow = kf + a * c.kxp;
     for (int c0 = 0; c0 < c.kxp; c0 += kWin) {
-      T buf[2 * kWin];
+      double buf[2 * kWin];
 #pragma unroll
-      for (int r = 0; r < 2 * kWin; ++r) buf[r] = row[c0 + r];
+      for (int r = 0; r < 2 * kWin; ++r) buf[r] = double(row[c0 + r]);
 #pragma unroll
       for (int jj = 0; jj < kWin; ++jj) {
-        const T w = krow[c0 + jj];
+        const double w = krow[c0 + jj];
 #pragma unroll
-        for (int r = 0; r < kWin; ++r) acc[r] += buf[r + jj] * w;
+        for (int r = 0; r < kWin; ++r) acc[r] = fma(buf[r + jj], w, acc[r]);
       }
     }
   }
@@ -412,8 +415,9 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
       const int x = x0 + tg * kWin + r;
       if (x < c.nx) {
         const size_t p = size_t(y) * c.nx + x;
-        if (out) out[size_t(img) * c.nx * c.ny + p] = acc[r];
-        if (partial) chi += chi2_pixel<T>(c, p, acc[r]);
+        const T m = T(acc[r]);
+        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (partial) chi += chi2_pixel<T>(c, p, m);
       }
     }
   }
@@ -512,11 +516,11 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   int rc = launch_prep(m, params, n_img, ws, L, st);
   if (rc) return rc;
   const int kxp = kxp_of(kx);
-  T *kf = reinterpret_cast<T *>(ws + L.psf);
+  double *kf = reinterpret_cast<double *>(ws + L.psf);
   {
     const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
     const int n = kxp * ky;
-    psf_prep_kernel<T><<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kxp, kf);
+    psf_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kxp, kf);
     rc = check_launch("psf_prep_kernel");
     if (rc) return rc;
   }
@@ -524,10 +528,14 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   const int npix = g.nx * g.ny;
   timing_mark(0, st);
   dim3 rgrid((npix + 255) / 256, n_img);
-  render_kernel<T><<<rgrid, 256, 0, st>>>(m, g, reinterpret_cast<const double *>(ws + L.recd),
-                                          reinterpret_cast<const float *>(ws + L.recf),
-                                          reinterpret_cast<const double *>(ws + L.coefd),
-                                          reinterpret_cast<const float *>(ws + L.coeff), flags, lowres);
+  const double *recd = reinterpret_cast<const double *>(ws + L.recd);
+  const float *recf = reinterpret_cast<const float *>(ws + L.recf);
+  const double *coefd = reinterpret_cast<const double *>(ws + L.coefd);
+  const float *coeff = reinterpret_cast<const float *>(ws + L.coeff);
+  if (m.has_halo)
+    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres);
+  else
+    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres);
   rc = check_launch("render_kernel");
   if (rc) return rc;
   timing_mark(1, st);
@@ -645,18 +653,25 @@ int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n
   unsigned char *w = static_cast<unsigned char *>(ws);
   if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
   dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
-  if (precision == LENSIM_FP64)
-    raytrace_kernel<double><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
-                                                   reinterpret_cast<float *>(w + L.recf),
-                                                   reinterpret_cast<double *>(w + L.coefd),
-                                                   reinterpret_cast<float *>(w + L.coeff), d_x, d_y, n_ray,
-                                                   d_beta_x, d_beta_y);
-  else
-    raytrace_kernel<float><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
-                                                  reinterpret_cast<float *>(w + L.recf),
-                                                  reinterpret_cast<double *>(w + L.coefd),
-                                                  reinterpret_cast<float *>(w + L.coeff), d_x, d_y, n_ray,
-                                                  d_beta_x, d_beta_y);
+  const double *recd = reinterpret_cast<const double *>(w + L.recd);
+  const float *recf = reinterpret_cast<const float *>(w + L.recf);
+  const double *coefd = reinterpret_cast<const double *>(w + L.coefd);
+  const float *coeff = reinterpret_cast<const float *>(w + L.coeff);
+  if (precision == LENSIM_FP64) {
+    if (m.has_halo)
+      raytrace_kernel<double, true><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
+                                                          d_beta_y);
+    else
+      raytrace_kernel<double, false><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
+                                                           d_beta_y);
+  } else {
+    if (m.has_halo)
+      raytrace_kernel<float, true><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
+                                                         d_beta_y);
+    else
+      raytrace_kernel<float, false><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
+                                                          d_beta_y);
+  }
   return check_launch("raytrace_kernel");
 }
 

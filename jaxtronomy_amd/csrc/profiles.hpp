@@ -358,9 +358,19 @@ __device__ __forceinline__ double nfw_g(double X) {
   return ::log(X / 2.0) + 1.0 / ::sqrt(X * X - 1.0) * ::acos(1.0 / X);
 }
 
+// Float form of g.  For X < 0.8 the closed form ln(X/2) + h(X) cancels (both
+// terms grow like ln(2/X) while g ~ X^2 ln(2/X) / 2), so it is rewritten with
+// s = sqrt(1 - X^2), u = 1 - s = X^2 / (1 + s):
+//   g = (u / s) ln(2 / X) + log1p(-u / 2) / s,
+// which has no cancellation there.
 __device__ __forceinline__ float nfw_g(float X) {
   X = (X < 0.000001f) ? 0.000001f : X;
-  return __logf(0.5f * X) + nfw_h_f32(X);
+  if (X < 0.8f) {
+    const float s = __builtin_amdgcn_sqrtf((1.0f - X) * (1.0f + X));
+    const float u = X * X / (1.0f + s);
+    return (u * ::logf(2.0f / X) + ::log1pf(-0.5f * u)) / s;
+  }
+  return ::logf(0.5f * X) + nfw_h_f32(X);
 }
 
 // NFW deflection, LensModel/Profiles/nfw.py:86-105 with nfw_alpha :257-279.
@@ -404,6 +414,20 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
   R = R > rmin ? R : rmin;
   T X = (sizeof(T) == sizeof(double)) ? R / r[TN_RS] : R * r[TN_IRS];
   X = X > T(0.001) ? X : T(0.001);
+  if (sizeof(T) == sizeof(float) && X < T(1)) {
+    // Inside the scale radius the leading terms of g cancel (g ~ x^2 ln x while
+    // the terms grow like ln(1/x)); the float form evaluates them in double there.
+    const double Xd = double(X);
+    const double tau = r.d[TN_TAU], tau2 = r.d[TN_TAU2], X2 = Xd * Xd;
+    const double sqtx = ::sqrt(tau2 + X2);
+    const double L = ::log(Xd / (tau + sqtx));
+    const double g = r.d[TN_PRE] * ((r.d[TN_T2P1] + 2.0 * (X2 - 1.0)) * tnfw_F(Xd) + r.d[TN_C0] +
+                                    sqtx * (-M_PI + L * r.d[TN_C1]));
+    const double a = r.d[TN_A] * g / X2;
+    ax += a * double(dx);
+    ay += a * double(dy);
+    return;
+  }
   const T tau = r[TN_TAU], tau2 = r[TN_TAU2];
   const T X2 = X * X;
   const T sqtx = Fn<T>::sqrt_(tau2 + X2);
@@ -425,7 +449,9 @@ __device__ __forceinline__ void shear_defl(const double *rd, double x, double y,
 
 // Adds one deflector's alpha at (x, y).  coef_d / coef_t: this profile's EPL
 // series coefficients (NULL for other profiles).
-template <typename T>
+// HALO: whether the NFW / TNFW branches are compiled in (kernels for models
+// without them keep fewer registers).
+template <typename T, bool HALO>
 __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const double *__restrict__ coef_d,
                                                const T *__restrict__ coef_t, double x, double y, double &ax,
                                                double &ay) {
@@ -442,10 +468,10 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
       nie_defl<T>(r, x, y, ax, ay);
       break;
     case LENSIM_NFW:
-      nfw_defl<T>(r, x, y, ax, ay);
+      if constexpr (HALO) nfw_defl<T>(r, x, y, ax, ay);
       break;
     case LENSIM_TNFW:
-      tnfw_defl<T>(r, x, y, ax, ay);
+      if constexpr (HALO) tnfw_defl<T>(r, x, y, ax, ay);
       break;
     case LENSIM_SHEAR:
       shear_defl(r.d, x, y, ax, ay);
@@ -472,9 +498,9 @@ __device__ __forceinline__ double sersic_kernel(double R, const Rec<double> &r) 
 __device__ __forceinline__ float sersic_kernel(float R, const Rec<float> &r) {
   const float R_ = fmaxf(0.0001f, R);
   const float frac = R_ * r[SE_IRS];
-  const float pw = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(frac) * r[SE_INVN]);
+  const float pw = ::exp2f(::log2f(frac) * r[SE_INVN]);
   const float e = -r[SE_BN] * (pw - 1.0f);
-  float v = (frac <= 1000.0f) ? __builtin_amdgcn_exp2f(e * 1.4426950408889634f) : 0.0f;
+  float v = (frac <= 1000.0f) ? ::expf(e) : 0.0f;
   if (v != v) v = 0.0f;
   return v;
 }
