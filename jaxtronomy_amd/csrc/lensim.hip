@@ -89,19 +89,16 @@ struct Layout {
   int n_tiles;
 };
 
-// Convolution tile shape for the compute type (outputs per block).
+// Convolution tile shape (outputs per block): TX columns, TY rows; each
+// thread owns kWin consecutive rows of one column, so a block has
+// TX * TY / kWin threads and lanes of a wave read consecutive LDS words.
 template <typename T>
-struct ConvTile;
-template <>
-struct ConvTile<double> {
+struct ConvTile {
   static constexpr int TY = 32, TX = 32;  // 128 threads
 };
-template <>
-struct ConvTile<float> {
-  static constexpr int TY = 32, TX = 64;  // 256 threads
-};
 
-int kxp_of(int kx) { return (kx + kWin - 1) / kWin * kWin; }
+// kernel height padded to a multiple of the register window
+int kyp_of(int ky) { return (ky + kWin - 1) / kWin * kWin; }
 
 template <typename T>
 int n_tiles_of(int nx, int ny) {
@@ -125,7 +122,7 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   L.coeff = off;
   off = align256(off + ncoef * 4);
   L.psf = off;
-  off = align256(off + size_t(ky) * kxp_of(kx) * 8);
+  off = align256(off + size_t(kx) * kyp_of(ky) * 8);
   L.lowres = off;
   off = align256(off + size_t(n_img) * nx * ny * tsz);
   L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
@@ -191,17 +188,18 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
   }
 }
 
-// Flip the kernel (so the convolution runs as a correlation), zero-pad its
-// width to kxp and convert it to T.
-__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kxp, double *__restrict__ out) {
+// Flip the kernel (so the convolution runs as a correlation) and store it
+// transposed, column c of the flipped kernel contiguous: kt[c * kyp + a] =
+// K[ky-1-a][kx-1-c], zero for a >= ky.
+__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= kxp * ky) return;
-  const int a = idx / kxp, c = idx - a * kxp;
+  if (idx >= kyp * kx) return;
+  const int c = idx / kyp, a = idx - c * kyp;
   if (!k) {  // identity kernel (no PSF / unconvolved)
     out[idx] = (a == 0 && c == 0) ? 1.0 : 0.0;
     return;
   }
-  out[idx] = (c < kx) ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
+  out[idx] = (a < ky) ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
 }
 
 template <typename T>
@@ -329,7 +327,7 @@ __global__ __launch_bounds__(256) void light_kernel(ModelArgs m, const double *_
 }
 
 struct ConvArgs {
-  int nx, ny, kx, ky, kxp, edge, n_tiles_x;
+  int nx, ny, kx, ky, kyp, edge, n_tiles_x;  // kyp: kernel height padded to kWin
   const double *data, *noise, *expo, *mask;
   double exposure_time, bkg2;
 };
@@ -353,13 +351,14 @@ __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T mode
 }
 
 // out[y][x] = sum_{a,c} in[y - cy + a][x - cx + c] * Kf[a][c]  (Kf = flipped kernel).
-// Block = TY rows x (TX / kWin) column groups; each thread computes kWin
-// consecutive outputs of one row from a register window over the LDS tile.
+// Block = TX columns x (TY / kWin) row groups; each thread computes kWin
+// consecutive rows of one column from a register window that slides down the
+// LDS tile, so consecutive lanes read consecutive LDS words.
 template <typename T>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in,
-                                                   const double *__restrict__ kf, T *__restrict__ out,
+                                                   const double *__restrict__ kt, T *__restrict__ out,
                                                    double *__restrict__ partial) {
-  constexpr int TY = ConvTile<T>::TY, TX = ConvTile<T>::TX, G = TX / kWin;
+  constexpr int TY = ConvTile<T>::TY, TX = ConvTile<T>::TX, NT = TX * TY / kWin;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);
   const int img = blockIdx.y;
@@ -367,15 +366,17 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
   const int y0 = tyi * TY, x0 = txi * TX;
   const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
-  const int TH = TY + c.ky - 1, TW = TX + c.kxp;
+  const int kyp = c.kyp;
+  const int TH = TY + kyp - 1, TW = TX + c.kx - 1;
+  const int THp = TH | 1;  // column stride of the column-major tile: odd, so lanes hit distinct banks
   const T *src = in + size_t(img) * c.nx * c.ny;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < TH * TW; idx += TY * G) {
+  for (int idx = tid; idx < TH * TW; idx += NT) {
     const int ly = idx / TW, lx = idx - ly * TW;
     int gy = y0 - cy + ly, gx = x0 - cx + lx;
     T v = T(0);
-    if (lx >= TX + c.kx - 1) {
-      // columns that only meet the zero padding of the kernel width
+    if (ly >= TY + c.ky - 1) {
+      // rows that only meet the zero padding of the kernel height
     } else if (c.edge) {
       gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
       gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
@@ -383,37 +384,59 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
     } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
       v = src[gy * c.nx + gx];
     }
-    tile[idx] = v;
+    tile[lx * THp + ly] = v;
   }
   __syncthreads();
-  const int ty = tid / G, tg = tid - ty * G;
-  // float64 accumulation in both forms (full-rate FMA on CDNA4); the tile is
-  // stored in T and widened once per loaded element
+  const int lx = tid % TX, tg = tid / TX;
+  // float64 accumulation in both forms (full-rate FMA on CDNA4)
   double acc[kWin];
 #pragma unroll
   for (int r = 0; r < kWin; ++r) acc[r] = 0.0;
-  for (int a = 0; a < c.ky; ++a) {
-    const T *row = tile + (ty + a) * TW + tg * kWin;
-    const double *krow = kf + a * c.kxp;
-    for (int c0 = 0; c0 < c.kxp; c0 += kWin) {
-      double buf[2 * kWin];
+  // steps t = (kernel column cc, row chunk ch) flattened; the window of step
+  // t + 1 is loaded while step t is computed
+  const int nch = kyp / kWin, nsteps = c.kx * nch;
+  const T *colbase = tile + lx * THp + tg * kWin;
+  double buf[2 * kWin - 1];
 #pragma unroll
-      for (int r = 0; r < 2 * kWin; ++r) buf[r] = double(row[c0 + r]);
+  for (int q = 0; q < 2 * kWin - 1; ++q) buf[q] = double(colbase[q]);
+  double w[kWin];
 #pragma unroll
-      for (int jj = 0; jj < kWin; ++jj) {
-        const double w = krow[c0 + jj];
-#pragma unroll
-        for (int r = 0; r < kWin; ++r) acc[r] = fma(buf[r + jj], w, acc[r]);
-      }
+  for (int jj = 0; jj < kWin; ++jj) w[jj] = kt[jj];
+  int cc = 0, ch = 0;
+  for (int t = 0; t < nsteps; ++t) {
+    if (++ch == nch) {
+      ch = 0;
+      ++cc;
     }
+    // next step's weights and window are requested before this step's FMAs
+    // (the wait for them lands after the FMAs)
+    double wn[kWin];
+    T nb[2 * kWin - 1];
+    if (t + 1 < nsteps) {
+      const double *kw = kt + (t + 1) * kWin;  // = kt + cc * kyp + ch * kWin
+#pragma unroll
+      for (int jj = 0; jj < kWin; ++jj) wn[jj] = kw[jj];
+      const T *col = colbase + cc * THp + ch * kWin;
+#pragma unroll
+      for (int q = 0; q < 2 * kWin - 1; ++q) nb[q] = col[q];
+    }
+#pragma unroll
+    for (int jj = 0; jj < kWin; ++jj) {
+#pragma unroll
+      for (int r = 0; r < kWin; ++r) acc[r] = fma(buf[r + jj], w[jj], acc[r]);
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * kWin - 1; ++q) buf[q] = double(nb[q]);
+#pragma unroll
+    for (int jj = 0; jj < kWin; ++jj) w[jj] = wn[jj];
   }
-  const int y = y0 + ty;
+  const int x = x0 + lx;
   double chi = 0.0;
-  if (y < c.ny) {
+  if (x < c.nx) {
 #pragma unroll
     for (int r = 0; r < kWin; ++r) {
-      const int x = x0 + tg * kWin + r;
-      if (x < c.nx) {
+      const int y = y0 + tg * kWin + r;
+      if (y < c.ny) {
         const size_t p = size_t(y) * c.nx + x;
         const T m = T(acc[r]);
         if (out) out[size_t(img) * c.nx * c.ny + p] = m;
@@ -431,7 +454,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
     __syncthreads();
     if (tid == 0) {
       double s = 0.0;
-      for (int w = 0; w < (TY * G) / 64; ++w) s += red[w];
+      for (int w = 0; w < NT / 64; ++w) s += red[w];
       partial[size_t(img) * gridDim.x + tile_id] = s;
     }
   }
@@ -504,8 +527,8 @@ int psf_dims(const lensim_psf *psf, uint32_t flags, int &kx, int &ky, int &edge)
 }
 
 template <typename T>
-size_t conv_smem(int ky, int kxp) {
-  return size_t(ConvTile<T>::TY + ky - 1) * (ConvTile<T>::TX + kxp) * sizeof(T);
+size_t conv_smem(int kx, int kyp) {
+  return size_t((ConvTile<T>::TY + kyp - 1) | 1) * (ConvTile<T>::TX + kx - 1) * sizeof(T);
 }
 
 // The image pipeline shared by lensim_image and lensim_loglike.
@@ -515,12 +538,12 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
               unsigned char *ws, const Layout &L, hipStream_t st) {
   int rc = launch_prep(m, params, n_img, ws, L, st);
   if (rc) return rc;
-  const int kxp = kxp_of(kx);
+  const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
   {
     const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
-    const int n = kxp * ky;
-    psf_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kxp, kf);
+    const int n = kyp * kx;
+    psf_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kyp, kf);
     rc = check_launch("psf_prep_kernel");
     if (rc) return rc;
   }
@@ -544,7 +567,7 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   c.ny = g.ny;
   c.kx = kx;
   c.ky = ky;
-  c.kxp = kxp;
+  c.kyp = kyp;
   c.edge = edge;
   c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
   const int n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
@@ -558,10 +581,10 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     c.bkg2 = data->background_rms * data->background_rms;
     partial = reinterpret_cast<double *>(ws + L.partial);
   }
-  const size_t smem = conv_smem<T>(ky, kxp);
+  const size_t smem = conv_smem<T>(kx, kyp);
   if (smem > 65536) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
   dim3 cgrid(n_tiles, n_img);
-  const int threads = ConvTile<T>::TY * (ConvTile<T>::TX / kWin);
+  const int threads = ConvTile<T>::TX * ConvTile<T>::TY / kWin;
   conv_kernel<T><<<cgrid, threads, smem, st>>>(c, lowres, kf, out, partial);
   rc = check_launch("conv_kernel");
   if (rc) return rc;
