@@ -43,6 +43,7 @@ int hip_fail(hipError_t e, const char *where) {
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 constexpr int kWin = 8;  // register window (outputs per thread) of the convolution
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
 // ---- optional kernel timing (lensim_timing_*) ----
 struct TimedCall {
@@ -85,7 +86,7 @@ struct ModelArgs {
 };
 
 struct Layout {
-  size_t recd, recf, coefd, coeff, psf, lowres, partial, total;
+  size_t recd, recf, coefd, coeff, psf, lowres, partial, blkmax, bfrag, total;
   int n_tiles;
 };
 
@@ -99,6 +100,24 @@ struct ConvTile {
 
 // kernel height padded to a multiple of the register window
 int kyp_of(int ky) { return (ky + kWin - 1) / kWin * kWin; }
+
+// ---- MFMA convolution geometry (mixed form) ----
+// Each kernel row a contributes D(16x16) += A(16x32k) B(32k x16) per output
+// tile, with A = 16 image rows of the padded tile and B[k][n] = Kf[a][k - n]
+// (the band of kernel row a).  nks K-steps of 32 cover 16 + kx - 1 columns.
+constexpr int kMfWaves = 4;  // waves per block, each 16 output rows
+constexpr int kMfTiles = 2;  // 16-column tiles per wave
+constexpr int kMfTY = 16 * kMfWaves, kMfTX = 16 * kMfTiles;
+int mfma_nks(int kx) { return (16 + kx - 1 + 31) / 32; }
+int mfma_tw(int kx) { return 16 * (kMfTiles - 1) + 32 * mfma_nks(kx); }
+// LDS row stride in halves: >= tw and = 8 (mod 16), so the 8 rows read by
+// lanes 0..7 of a ds_read_b128 land on distinct banks
+int mfma_ls(int kx) {
+  const int tw = mfma_tw(kx);
+  return tw + ((8 - tw % 16) + 16) % 16;
+}
+size_t mfma_smem(int kx, int ky) { return size_t(kMfTY + ky - 1) * mfma_ls(kx) * 2 * sizeof(_Float16); }
+int mfma_tiles(int nx, int ny) { return ((nx + kMfTX - 1) / kMfTX) * ((ny + kMfTY - 1) / kMfTY); }
 
 template <typename T>
 int n_tiles_of(int nx, int ny) {
@@ -128,6 +147,10 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
   L.partial = off;
   off = align256(off + size_t(n_img) * L.n_tiles * 8);
+  L.blkmax = off;  // per render block max |value|, [n_img][ceil(nx ny / 256)]
+  off = align256(off + size_t(n_img) * ((size_t(nx) * ny + 255) / 256) * 4);
+  L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments
+  if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(ky) * mfma_nks(kx) * 2 * 64 * 16);
   L.total = off;
   return L;
 }
@@ -260,11 +283,12 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
                                                      const float *__restrict__ recf,
                                                      const double *__restrict__ coefd,
                                                      const float *__restrict__ coeff, unsigned flags,
-                                                     T *__restrict__ out) {
+                                                     T *__restrict__ out, float *__restrict__ blkmax) {
   const int img = blockIdx.y;
   const int npix = g.nx * g.ny;
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= npix) return;
+  const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = pix0 < npix;
+  const int pix = active ? pix0 : npix - 1;  // inactive lanes stay for the wave reduction
   const int i = pix % g.nx, j = pix / g.nx;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   // pixel centre (lenstronomy grid_from_coordinate_transform order)
@@ -276,7 +300,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
   const int k_src0 = m.n_lens, k_src1 = m.n_lens + m.n_source, k_l1 = m.n_prof;
   double acc = 0.0;
-  for (int sy = 0; sy < g.s; ++sy) {
+  for (int sy = 0; active && sy < g.s; ++sy) {
     const double oy = o0 + sy * inv_s;
     for (int sx = 0; sx < g.s; ++sx) {
       const double ox = o0 + sx * inv_s;
@@ -293,7 +317,22 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
       if (do_light) acc += light_sum<T>(m, rp, k_src1, k_l1, x, y);
     }
   }
-  out[size_t(img) * npix + pix] = T(acc * (g.scale * m.flux_scaling));
+  const T v = T(acc * (g.scale * m.flux_scaling));
+  if (active) out[size_t(img) * npix + pix] = v;
+  if (blkmax) {
+    // max |v| of this block (the MFMA convolution scales its fp16 operands
+    // by the image max); one slot per block, no atomics
+    __shared__ float wmax[256 / 64];
+    float mv = fabsf(float(v));
+    for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float b = wmax[0];
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
+      blkmax[size_t(img) * gridDim.x + blockIdx.x] = b;
+    }
+  }
 }
 
 template <typename T, bool HALO>
@@ -460,6 +499,158 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   }
 }
 
+
+// ------------------------------------------------ MFMA convolution (mixed) --
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// B fragments of the banded kernel rows, in the lane layout of
+// v_mfma_f32_16x16x32_f16 (lane l holds B[8 (l / 16) + j][l % 16], j < 8),
+// split into fp16 hi + lo of the kernel scaled by 2^sk (max |K| 2^sk in
+// [2^14, 2^15)).  One block; hdr[0] receives sk.
+__global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
+                                                            int *__restrict__ hdr, half8 *__restrict__ frag) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(k[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double mx = red[0];
+  const int sk = (mx > 0.0 && isfinite(mx)) ? 14 - ilogb(mx) : 0;
+  if (threadIdx.x == 0) hdr[0] = sk;
+  const int n_frag = ky * nks * 2 * 64;
+  for (int idx = threadIdx.x; idx < n_frag; idx += blockDim.x) {
+    const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
+    const int a = ak / nks, ks = ak - a * nks;
+    const int n = lane & 15, g = lane >> 4;
+    half8 v;
+    for (int j = 0; j < 8; ++j) {
+      const int c = 32 * ks + 8 * g + j - n;  // tap of the flipped kernel row a
+      const double w = (c >= 0 && c < kx) ? ldexp(k[(ky - 1 - a) * kx + (kx - 1 - c)], sk) : 0.0;
+      const _Float16 h = (_Float16)w;
+      v[j] = s == 0 ? h : (_Float16)(w - (double)h);
+    }
+    frag[idx] = v;
+  }
+}
+
+// Direct 'same' convolution on the matrix cores: block = 64 rows x 32 columns
+// (4 waves x 2 tiles of 16 x 16).  The padded tile sits in LDS as fp16 hi and
+// lo planes of the image scaled by 2^sa (sa from the per-image max); per
+// kernel row a and K-step, three MFMAs (hi.hi, hi.lo, lo.hi) accumulate in
+// float32, and each row's sum is added to a float64 accumulator.
+__global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int ls, const float *__restrict__ in,
+                                                        const int *__restrict__ hdr,
+                                                        const half8 *__restrict__ frag,
+                                                        const float *__restrict__ blkmax, int nblk,
+                                                        float *__restrict__ out, double *__restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int img = blockIdx.y, tile_id = blockIdx.x;
+  const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
+  const int Y0 = tyi * kMfTY, X0 = txi * kMfTX;
+  const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
+  const int TH = kMfTY + c.ky - 1, TW = 16 * (kMfTiles - 1) + 32 * nks;
+  _Float16 *lh = reinterpret_cast<_Float16 *>(smem_raw);
+  _Float16 *ll = lh + TH * ls;
+  const int tid = threadIdx.x;
+  // image max |value| from the render blocks' maxima
+  __shared__ float smax[4];
+  {
+    float mv = 0.f;
+    for (int b = tid; b < nblk; b += 256) mv = fmaxf(mv, blkmax[size_t(img) * nblk + b]);
+    for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
+    if ((tid & 63) == 0) smax[tid >> 6] = mv;
+    __syncthreads();
+  }
+  const float mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+  const int sa = (mx > 0.f && isfinite(mx)) ? 14 - ilogbf(mx) : 0;
+  const int sk = hdr[0];
+  const float *src = in + size_t(img) * c.nx * c.ny;
+  for (int idx = tid; idx < TH * TW; idx += 256) {
+    const int ly = idx / TW, lx = idx - ly * TW;
+    int gy = Y0 - cy + ly, gx = X0 - cx + lx;
+    float v = 0.f;
+    if (c.edge) {
+      gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
+      gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
+      v = src[gy * c.nx + gx];
+    } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
+      v = src[gy * c.nx + gx];
+    }
+    const float xs = ldexpf(v, sa);
+    const _Float16 h = (_Float16)xs;
+    lh[ly * ls + lx] = h;
+    ll[ly * ls + lx] = (_Float16)(xs - (float)h);
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  double dacc[kMfTiles][4];
+#pragma unroll
+  for (int t = 0; t < kMfTiles; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dacc[t][j] = 0.0;
+  const int rowbase = (16 * wave + li) * ls + 8 * lg;
+  for (int a = 0; a < c.ky; ++a) {
+    floatx4 acc[kMfTiles];
+#pragma unroll
+    for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < nks; ++ks) {
+      const half8 bh = frag[((a * nks + ks) * 2 + 0) * 64 + lane];
+      const half8 bl = frag[((a * nks + ks) * 2 + 1) * 64 + lane];
+      half8 ah[kMfTiles], al[kMfTiles];
+#pragma unroll
+      for (int t = 0; t < kMfTiles; ++t) {
+        const int off = rowbase + a * ls + 16 * t + 32 * ks;
+        ah[t] = *reinterpret_cast<const half8 *>(lh + off);
+        al[t] = *reinterpret_cast<const half8 *>(ll + off);
+      }
+#pragma unroll
+      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bh, acc[t], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bl, acc[t], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bh, acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < kMfTiles; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
+  }
+  // lane l holds D[4 (l / 16) + j][l % 16] of each tile
+  const double unscale = ldexp(1.0, -(sa + sk));
+  double chi = 0.0;
+#pragma unroll
+  for (int t = 0; t < kMfTiles; ++t) {
+    const int x = X0 + 16 * t + li;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int y = Y0 + 16 * wave + 4 * lg + j;
+      if (x < c.nx && y < c.ny) {
+        const size_t p = size_t(y) * c.nx + x;
+        const float m = float(dacc[t][j] * unscale);
+        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (partial) chi += chi2_pixel<float>(c, p, m);
+      }
+    }
+  }
+  if (partial) {
+    for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(smem_raw);
+    if (lane == 0) red[wave] = chi;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kMfWaves; ++w) s += red[w];
+      partial[size_t(img) * gridDim.x + tile_id] = s;
+    }
+  }
+}
+
 __global__ void logl_kernel(const double *__restrict__ partial, int n_tiles, int n_img, double *__restrict__ logl) {
   const int img = blockIdx.x * blockDim.x + threadIdx.x;
   if (img >= n_img) return;
@@ -531,6 +722,14 @@ size_t conv_smem(int kx, int kyp) {
   return size_t((ConvTile<T>::TY + kyp - 1) | 1) * (ConvTile<T>::TX + kx - 1) * sizeof(T);
 }
 
+// Whether the mixed form convolves on the matrix cores (any real PSF whose
+// MFMA tile fits in 64 KB of LDS) or on the vector ALUs.
+template <typename T>
+bool use_mfma(bool identity, int kx, int ky) {
+  if (sizeof(T) != sizeof(float) || identity) return false;
+  return mfma_smem(kx, ky) <= 65536;
+}
+
 // The image pipeline shared by lensim_image and lensim_loglike.
 template <typename T>
 int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
@@ -538,16 +737,23 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
               unsigned char *ws, const Layout &L, hipStream_t st) {
   int rc = launch_prep(m, params, n_img, ws, L, st);
   if (rc) return rc;
+  const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
+  const bool mfma = use_mfma<T>(identity, kx, ky);
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
-  {
-    const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
+  int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
+  half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
+  if (mfma) {
+    psf_mfma_prep_kernel<<<1, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
+    rc = check_launch("psf_mfma_prep_kernel");
+  } else {
     const int n = kyp * kx;
     psf_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kyp, kf);
     rc = check_launch("psf_prep_kernel");
-    if (rc) return rc;
   }
+  if (rc) return rc;
   T *lowres = reinterpret_cast<T *>(ws + L.lowres);
+  float *blkmax = mfma ? reinterpret_cast<float *>(ws + L.blkmax) : nullptr;
   const int npix = g.nx * g.ny;
   timing_mark(0, st);
   dim3 rgrid((npix + 255) / 256, n_img);
@@ -556,9 +762,9 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   const double *coefd = reinterpret_cast<const double *>(ws + L.coefd);
   const float *coeff = reinterpret_cast<const float *>(ws + L.coeff);
   if (m.has_halo)
-    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres);
+    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres, blkmax);
   else
-    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres);
+    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres, blkmax);
   rc = check_launch("render_kernel");
   if (rc) return rc;
   timing_mark(1, st);
@@ -569,8 +775,6 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   c.ky = ky;
   c.kyp = kyp;
   c.edge = edge;
-  c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
-  const int n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
   double *partial = nullptr;
   if (data) {
     c.data = data->d_data;
@@ -581,12 +785,31 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     c.bkg2 = data->background_rms * data->background_rms;
     partial = reinterpret_cast<double *>(ws + L.partial);
   }
-  const size_t smem = conv_smem<T>(kx, kyp);
-  if (smem > 65536) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
-  dim3 cgrid(n_tiles, n_img);
-  const int threads = ConvTile<T>::TX * ConvTile<T>::TY / kWin;
-  conv_kernel<T><<<cgrid, threads, smem, st>>>(c, lowres, kf, out, partial);
-  rc = check_launch("conv_kernel");
+  int n_tiles;
+  if (mfma) {
+    c.n_tiles_x = (g.nx + kMfTX - 1) / kMfTX;
+    n_tiles = mfma_tiles(g.nx, g.ny);
+    dim3 cgrid(n_tiles, n_img);
+    conv_mfma_kernel<<<cgrid, 256, mfma_smem(kx, ky), st>>>(c, mfma_nks(kx), mfma_ls(kx),
+                                                             reinterpret_cast<const float *>(lowres), hdr, frag,
+                                                             blkmax, (npix + 255) / 256,
+                                                             reinterpret_cast<float *>(out), partial);
+    rc = check_launch("conv_mfma_kernel");
+  } else {
+    c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+    n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
+    const size_t smem = conv_smem<T>(kx, kyp);
+    if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+      if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+    }
+    dim3 cgrid(n_tiles, n_img);
+    const int threads = ConvTile<T>::TX * ConvTile<T>::TY / kWin;
+    conv_kernel<T><<<cgrid, threads, smem, st>>>(c, lowres, kf, out, partial);
+    rc = check_launch("conv_kernel");
+  }
   if (rc) return rc;
   timing_mark(2, st);
   if (data) {
