@@ -11,11 +11,13 @@
  * last failure on the calling thread.
  *
  * Parameter blocks.  A model is an ordered list of profiles: n_lens
- * deflectors, then n_source source-light profiles, then n_lens_light
- * lens-light profiles.  Parameters of a batch of n_img models are one
- * float64 array of shape [n_img][n_profiles][LENSIM_NPARAM]; each profile
- * slot holds that profile's parameters in the order of the reference's
- * `param_names` (unused trailing slots are ignored):
+ * deflectors, then n_bulk further deflectors whose types come from a device
+ * array (many-deflector models, e.g. substructure), then n_source
+ * source-light profiles, then n_lens_light lens-light profiles.  Parameters
+ * of a batch of n_img models are one float64 array of shape
+ * [n_img][n_lens + n_bulk + n_source + n_lens_light][LENSIM_NPARAM]; each
+ * profile slot holds that profile's parameters in the order of the
+ * reference's `param_names` (unused trailing slots are ignored):
  *   EPL            theta_E gamma e1 e2 center_x center_y    (LensModel/Profiles/epl.py:50)
  *   SIE            theta_E e1 e2 center_x center_y           (LensModel/Profiles/sie.py:40)
  *   NIE            theta_E e1 e2 s_scale center_x center_y   (LensModel/Profiles/nie.py)
@@ -37,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LENSIM_ABI_VERSION 1
+#define LENSIM_ABI_VERSION 2
 #define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
 #define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
 #define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
@@ -83,7 +85,10 @@ typedef enum {
 
 /* Model description: the profile lists of LensModel(lens_model_list),
  * LightModel(source list) and LightModel(lens light list)
- * (LensModel/lens_model.py:25-216, LightModel/light_model_base.py:63-228). */
+ * (LensModel/lens_model.py:25-216, LightModel/light_model_base.py:63-228),
+ * plus an optional bulk deflector list as packed by
+ * SinglePlaneBulk.prepare_ray_shooting_kwargs (LensModel/single_plane_bulk.py:47-106):
+ * bulk types may be SIE, NIE, NFW, TNFW, SHEAR or LENSIM_NULL (padding). */
 typedef struct {
   int32_t n_lens;
   int32_t n_source;
@@ -91,6 +96,9 @@ typedef struct {
   int32_t types[LENSIM_MAX_PROFILES]; /* lensim_profile, lens then source then lens light */
   double alpha_scaling;               /* SinglePlane alpha_scaling (LensModel/single_plane.py:139) */
   double flux_scaling;                /* ImageData flux_scaling (ImSim/image_model.py:60-63) */
+  int32_t n_bulk;                     /* bulk deflectors after the n_lens inline ones */
+  int32_t bulk_type;                  /* their common lensim_profile, or LENSIM_NULL if mixed */
+  const int32_t *d_bulk_types;        /* [n_bulk] device array (read when bulk_type == LENSIM_NULL) */
 } lensim_model;
 
 /* Pixel grid + supersampling (ImSim/Numerics/grid.py:22-64): pixel (i, j)

@@ -21,6 +21,7 @@ import torch
 from .. import backend, profiles
 from ..Data.psf import cut_psf, gaussian_pixel_kernel
 from ..LensModel.lens_model import LensModel
+from ..LensModel.lens_model_bulk import LensModelBulk
 from ..LightModel.light_model import LightModel
 
 
@@ -87,11 +88,12 @@ class ImageModel(object):
             self._psf.ky, self._psf.kx = self._kernel.shape
             self._psf.d_kernel = self._kernel_dev.data_ptr()
         self._mask_dev = None
-        self._lens_types = list(self.LensModel._types)
+        self._bulk = isinstance(self.LensModel, LensModelBulk)
+        self._lens_types = [] if self._bulk else list(self.LensModel._types)
         self._src_types = list(self.SourceModel._types)
         self._light_types = list(self.LensLightModel._types)
-        self._names = (list(self.LensModel.param_name_list) + list(self.SourceModel.param_name_list) +
-                       list(self.LensLightModel.param_name_list))
+        self._names = ([] if self._bulk else list(self.LensModel.param_name_list)) + \
+            list(self.SourceModel.param_name_list) + list(self.LensLightModel.param_name_list)
         self._model = backend.make_model(self._lens_types, self._src_types, self._light_types,
                                          alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
         self._submodels = {}
@@ -143,8 +145,28 @@ class ImageModel(object):
         return self._kernel
 
     # ------------------------------------------------------------- packing --
+    def pack_bulk(self, kwargs_lens, kwargs_source=None, kwargs_lens_light=None):
+        """For a LensModelBulk deflector: (params [B, n, 8], B, model, keep-alive) where
+        kwargs_lens is the dict of LensModelBulk.prepare_ray_shooting_kwargs."""
+        n_img, it, ip, bt, bp = self.LensModel.pack(kwargs_lens)
+        ks = kwargs_source or []
+        kll = kwargs_lens_light or []
+        nb = profiles.batch_size([ks, kll])
+        if nb != 1 and n_img != 1 and nb != n_img:
+            raise ValueError("inconsistent batch sizes %d and %d" % (n_img, nb))
+        n = max(n_img, nb)
+        light = profiles.pack(self._names, list(ks if self._src_types else []) +
+                              list(kll if self._light_types else []), n)
+        ip = np.broadcast_to(ip, (n,) + ip.shape[1:])
+        bp = np.broadcast_to(bp, (n,) + bp.shape[1:])
+        params = np.ascontiguousarray(np.concatenate([ip, bp, light], axis=1))
+        m, keep = self.LensModel._model(it, bt, self._src_types, self._light_types, self._flux_scaling)
+        return params, n, m, keep
+
     def pack(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None):
         """Parameter block [B, n_profiles, 8] (float64, host) of lists of kwargs."""
+        if self._bulk:
+            raise ValueError("use pack_bulk with a LensModelBulk deflector")
         kl = kwargs_lens or []
         ks = kwargs_source or []
         kll = kwargs_lens_light or []
@@ -209,16 +231,17 @@ class ImageModel(object):
         d.background_rms = self.Data.background_rms
         return d
 
-    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None):
+    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None):
         """log-likelihoods [B] (float64) of a packed device parameter block."""
         lib = backend.lib()
+        m = self._model if model is None else model
         if n_img is None:
             n_img = params.shape[0]
         if logl is None:
             logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
         d = self._data_struct()
-        ws = self._ws.get(self.workspace_bytes(n_img))
-        rc = lib.lensim_loglike(ctypes.byref(self._model), ctypes.byref(self._grid), ctypes.byref(self._psf),
+        ws = self._ws.get(self.workspace_bytes(n_img, m))
+        rc = lib.lensim_loglike(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
                                 ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
                                 backend.ptr(model_out), backend.ptr(ws), ws.numel(), backend.stream_handle())
         backend.check(rc, "lensim_loglike")
@@ -259,6 +282,11 @@ class ImageModel(object):
             flags |= backend.IMAGE_LENS_LIGHT
         if unconvolved:
             flags |= backend.IMAGE_UNCONVOLVED
+        if self._bulk:
+            params, n_img, m, keep = self.pack_bulk(kwargs_lens, kwargs_source, kwargs_lens_light)
+            img = self.image_batch(backend.to_device_f64(params), n_img, flags=flags, model=m).to(torch.float64)
+            del keep
+            return img if n_img > 1 else img[0]
         nl, ns, nll = len(self._lens_types), len(self._src_types), len(self._light_types)
         return self._render(range(nl), range(ns), range(nll), kwargs_lens, kwargs_source, kwargs_lens_light, flags)
 
@@ -293,6 +321,11 @@ class ImageModel(object):
         if check_positive_flux:
             raise ValueError("check positive flux is not supported")
         self._check_unsupported(kwargs_ps, kwargs_extinction)
+        if self._bulk:
+            params, n_img, m, keep = self.pack_bulk(kwargs_lens, kwargs_source, kwargs_lens_light)
+            logl = self.likelihood_batch(backend.to_device_f64(params), n_img, model=m)
+            del keep
+            return logl if n_img > 1 else logl[0]
         params, n_img = self.pack(kwargs_lens, kwargs_source, kwargs_lens_light)
         pt = backend.to_device_f64(params)
         logl = self.likelihood_batch(pt, n_img)

@@ -15,6 +15,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
 
+ABI_VERSION = 2
 NPARAM = 8
 MAX_PROFILES = 64
 EPL_NMAX = 500
@@ -40,6 +41,9 @@ class LensimModel(ctypes.Structure):
         ("types", ctypes.c_int32 * MAX_PROFILES),
         ("alpha_scaling", ctypes.c_double),
         ("flux_scaling", ctypes.c_double),
+        ("n_bulk", ctypes.c_int32),
+        ("bulk_type", ctypes.c_int32),
+        ("d_bulk_types", ctypes.c_void_p),
     ]
 
 
@@ -114,7 +118,7 @@ def lib():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
-        if handle.lensim_abi_version() != 1:
+        if handle.lensim_abi_version() != ABI_VERSION:
             raise LensimError("liblensim ABI version mismatch")
         _lib = handle
     return _lib
@@ -158,7 +162,8 @@ class Workspace:
         return self._buf
 
 
-def make_model(lens_types, source_types, light_types, alpha_scaling=1.0, flux_scaling=1.0):
+def make_model(lens_types, source_types, light_types, alpha_scaling=1.0, flux_scaling=1.0, bulk=None):
+    """lensim_model for inline profile lists; bulk = (n_bulk, common_type, device int32 types or None)."""
     types = list(lens_types) + list(source_types) + list(light_types)
     if len(types) > MAX_PROFILES:
         raise ValueError("at most %d profiles per model" % MAX_PROFILES)
@@ -170,6 +175,11 @@ def make_model(lens_types, source_types, light_types, alpha_scaling=1.0, flux_sc
         m.types[i] = t
     m.alpha_scaling = float(alpha_scaling)
     m.flux_scaling = float(flux_scaling)
+    if bulk is not None:
+        n_bulk, common, d_types = bulk
+        m.n_bulk = int(n_bulk)
+        m.bulk_type = int(common)
+        m.d_bulk_types = None if d_types is None else d_types.data_ptr()
     return m
 
 

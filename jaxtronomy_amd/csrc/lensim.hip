@@ -78,12 +78,22 @@ void timing_mark(int slot, hipStream_t st) {
   if (e) (void)hipEventRecord(e, st);
 }
 
+// Record index space: [0, n_lens) inline deflectors, [n_lens, n_defl) bulk
+// deflectors, [n_defl, n_defl + n_source) source light, then lens light.
+// types[] holds the inline lens, source and lens-light types (no bulk).
 struct ModelArgs {
-  int32_t n_lens, n_source, n_light, n_prof, n_epl, has_halo;
+  int32_t n_lens, n_bulk, n_defl, n_source, n_light, n_prof, n_epl, has_halo, bulk_type;
+  const int32_t *bulk_types;
   int32_t types[LENSIM_MAX_PROFILES];
   int32_t epl_index[LENSIM_MAX_PROFILES];
   double alpha_scaling, flux_scaling;
 };
+
+__host__ __device__ inline int type_of(const ModelArgs &m, int k) {
+  if (k < m.n_lens) return m.types[k];
+  if (k < m.n_defl) return m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
+  return m.types[k - m.n_bulk];
+}
 
 struct Layout {
   size_t recd, recf, coefd, coeff, psf, lowres, partial, blkmax, bfrag, total;
@@ -155,19 +165,27 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   return L;
 }
 
-int model_args(const lensim_model *model, ModelArgs &m, bool need_lens_light_ok = true) {
-  (void)need_lens_light_ok;
+bool bulk_type_ok(int t) {
+  return t == LENSIM_NULL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW || t == LENSIM_TNFW ||
+         t == LENSIM_SHEAR;
+}
+
+int model_args(const lensim_model *model, ModelArgs &m) {
   if (!model) return fail(LENSIM_EINVAL, "model is NULL");
   m = ModelArgs{};
   m.n_lens = model->n_lens;
   m.n_source = model->n_source;
   m.n_light = model->n_lens_light;
-  if (m.n_lens < 0 || m.n_source < 0 || m.n_light < 0)
+  m.n_bulk = model->n_bulk;
+  if (m.n_lens < 0 || m.n_source < 0 || m.n_light < 0 || m.n_bulk < 0)
     return fail(LENSIM_EINVAL, "negative profile count");
-  m.n_prof = m.n_lens + m.n_source + m.n_light;
-  if (m.n_prof > LENSIM_MAX_PROFILES)
-    return fail(LENSIM_EINVAL, "%d profiles exceed LENSIM_MAX_PROFILES=%d", m.n_prof, LENSIM_MAX_PROFILES);
-  for (int k = 0; k < m.n_prof; ++k) {
+  const int n_inline = m.n_lens + m.n_source + m.n_light;
+  if (n_inline > LENSIM_MAX_PROFILES)
+    return fail(LENSIM_EINVAL, "%d profiles exceed LENSIM_MAX_PROFILES=%d", n_inline, LENSIM_MAX_PROFILES);
+  if (m.n_bulk > (1 << 20)) return fail(LENSIM_EINVAL, "n_bulk %d too large", m.n_bulk);
+  m.n_defl = m.n_lens + m.n_bulk;
+  m.n_prof = n_inline + m.n_bulk;
+  for (int k = 0; k < n_inline; ++k) {
     const int t = model->types[k];
     m.types[k] = t;
     m.epl_index[k] = -1;
@@ -181,6 +199,16 @@ int model_args(const lensim_model *model, ModelArgs &m, bool need_lens_light_ok 
     if (t == LENSIM_EPL) m.epl_index[k] = m.n_epl++;
     if (t == LENSIM_NFW || t == LENSIM_TNFW) m.has_halo = 1;
   }
+  if (m.n_bulk > 0) {
+    m.bulk_type = model->bulk_type;
+    m.bulk_types = model->d_bulk_types;
+    if (!bulk_type_ok(m.bulk_type))
+      return fail(LENSIM_EUNSUPPORTED, "bulk deflector type %d not supported (SIE, NIE, NFW, TNFW, SHEAR)",
+                  m.bulk_type);
+    if (m.bulk_type == LENSIM_NULL && !m.bulk_types)
+      return fail(LENSIM_EINVAL, "mixed bulk deflectors need d_bulk_types");
+    if (m.bulk_type == LENSIM_NFW || m.bulk_type == LENSIM_TNFW || m.bulk_type == LENSIM_NULL) m.has_halo = 1;
+  }
   m.alpha_scaling = model->alpha_scaling;
   m.flux_scaling = model->flux_scaling;
   return LENSIM_OK;
@@ -188,16 +216,22 @@ int model_args(const lensim_model *model, ModelArgs &m, bool need_lens_light_ok 
 
 // ------------------------------------------------------------- kernels ---
 
+__device__ inline bool bulk_type_ok_dev(int t) {
+  return t == LENSIM_NULL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW || t == LENSIM_TNFW ||
+         t == LENSIM_SHEAR;
+}
+
 __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int n_img, double *__restrict__ recd,
                             float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_img * m.n_prof) return;
   const int img = idx / m.n_prof, k = idx - img * m.n_prof;
-  const int type = m.types[k];
+  int type = type_of(m, k);
+  if (k >= m.n_lens && k < m.n_defl && !bulk_type_ok_dev(type)) type = LENSIM_NULL;
   const double *p = params + size_t(idx) * LENSIM_NPARAM;
   double rec[kRec];
   double *coef = nullptr;
-  const int e = m.epl_index[k];
+  const int e = k < m.n_lens ? m.epl_index[k] : -1;
   if (e >= 0) coef = coefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
   prepare_record(type, p, rec, coef);
   for (int i = 0; i < kRec; ++i) {
@@ -259,6 +293,20 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
     add_deflection<T, HALO>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   }
+  if (m.n_bulk == 0) return;
+  if (m.bulk_type == LENSIM_TNFW) {
+    // common case of substructure: one profile type, no per-deflector switch
+    if constexpr (HALO) {
+      for (int k = m.n_lens; k < m.n_defl; ++k)
+        tnfw_defl<T>(Rec<T>{rp.recd + k * kRec, rp.rect + k * kRec}, x, y, ax, ay);
+    }
+    return;
+  }
+  for (int k = m.n_lens; k < m.n_defl; ++k) {
+    const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
+    const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
+    add_deflection<T, HALO>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+  }
 }
 
 template <typename T>
@@ -267,7 +315,7 @@ __device__ __forceinline__ double light_sum(const ModelArgs &m, const RecPtrs<T>
   double f = 0.0;
   for (int k = k0; k < k1; ++k) {
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
-    f += light_value<T>(m.types[k], r, x, y);
+    f += light_value<T>(m.types[k - m.n_bulk], r, x, y);
   }
   return f;
 }
@@ -298,7 +346,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   const double o0 = -0.5 + 1.0 / (2.0 * g.s);
   const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && m.n_source > 0;
   const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
-  const int k_src0 = m.n_lens, k_src1 = m.n_lens + m.n_source, k_l1 = m.n_prof;
+  const int k_src0 = m.n_defl, k_src1 = m.n_defl + m.n_source, k_l1 = m.n_prof;
   double acc = 0.0;
   for (int sy = 0; active && sy < g.s; ++sy) {
     const double oy = o0 + sy * inv_s;
@@ -939,8 +987,8 @@ int lensim_surface_brightness(const lensim_model *model, const double *d_params,
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
   if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
-  const int k0 = which == 0 ? m.n_lens : m.n_lens + m.n_source;
-  const int k1 = which == 0 ? m.n_lens + m.n_source : m.n_prof;
+  const int k0 = which == 0 ? m.n_defl : m.n_defl + m.n_source;
+  const int k1 = which == 0 ? m.n_defl + m.n_source : m.n_prof;
   dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
   if (precision == LENSIM_FP64)
     light_kernel<double><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),

@@ -71,7 +71,11 @@ __device__ __forceinline__ T nan_to_1e10(T v) {
 
 // ------------------------------------------------------ record layouts ----
 // EPL     (epl.py:109-124): cos phi, sin phi, q, b, t, f, 2/(1+q), nmax
-enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX };
+enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX,
+              EPL_LOG2B,  // log2 b
+              EPL_NF,     // series length of the float form (truncation below 1e-8)
+              EPL_F2,     // f^2 = |z|^2
+              EPL_LNB };  // ln b
 // NIE/SIE (nie.py:182-200, :280-295): cos, sin, q (clamped), q^2, s, b/sqrt(1-q^2),
 //          sqrt(1-q^2), q^2 s
 enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S };
@@ -85,7 +89,11 @@ enum : int { SH_G1 = 2, SH_G2 };
 // SERSIC  (sersic_ellipse.py:62-93, sersic_utils.py:83-110, :239-267,
 //          param_util.py:103-124): amp, the product-average matrix
 //          ((1-e1), -e2; -e2, (1+e1))/norm, max(R_sersic, 1e-4), 1/n, b_n, 1/R_sersic_
-enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS };
+enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS,
+              SE_C1,      // invn / 2            (frac^(1/n) = exp2(C1 log2 R^2 + C2))
+              SE_C2,      // -invn log2 R_sersic_
+              SE_R2MAX,   // (1000 R_sersic_)^2  (the 1000 R_sersic window on R^2)
+              SE_LNRS };  // ln R_sersic_
 // GAUSSIAN light (gaussian.py:27-42): amp/(2 pi sigma^2), 1/sigma^2
 enum : int { GA_C = 2, GA_IS2 };
 
@@ -135,17 +143,32 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[EPL_F] = f;
       rec[EPL_PREF] = 2.0 / (1.0 + q);
       rec[EPL_NMAX] = (double)nmax;
+      rec[EPL_LOG2B] = log2(b);
+      rec[EPL_F2] = f * f;
+      rec[EPL_LNB] = log(b);
+      rec[EPL_NF] = (double)nmax;
       if (coef) {
         const double bb = t / 2.0, cc = 2.0 - bb;
         double fm2 = 1.0, fm1 = 1.0 - 2.0 * bb / cc;
         coef[0] = 1.0;
         if (nmax > 1) coef[1] = fm1;
+        double cmax = fmax(1.0, fabs(fm1));
         for (int n = 2; n < nmax; ++n) {
           const double fn = ((n - 1.0) * fm2 - (2.0 * bb - cc) * fm1) / (cc + n - 1.0);
           coef[n] = fn;
+          cmax = fmax(cmax, fabs(fn));
           fm2 = fm1;
           fm1 = fn;
         }
+        // |w| <= f / (2 - f) for |z| = f, so the terms n >= N add at most
+        // max|F_n| wmax^N / (1 - wmax) to a sum of size ~1
+        const double wmax = f / (2.0 - f);
+        int nf = nmax;
+        if (wmax > 0.0) {
+          double tail = cmax / (1.0 - wmax);
+          for (nf = 1; nf < nmax && tail >= 1e-8; ++nf) tail *= wmax;
+        }
+        rec[EPL_NF] = (double)nf;
       }
       break;
     }
@@ -229,6 +252,10 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[SE_INVN] = 1.0 / n;
       rec[SE_BN] = fmax(1.9992 * n - 0.3271, 0.00001);
       rec[SE_IRS] = 1.0 / Rs_;
+      rec[SE_C1] = 0.5 / n;
+      rec[SE_C2] = -log2(Rs_) / n;
+      rec[SE_R2MAX] = (1000.0 * Rs_) * (1000.0 * Rs_);
+      rec[SE_LNRS] = log(Rs_);
       break;
     }
     case LENSIM_GAUSSIAN: {
@@ -254,58 +281,129 @@ struct Rec {
   __device__ __forceinline__ T operator[](int i) const { return t[i]; }
 };
 
+// Sum_{n < N} c_n w^n for real c_n at complex w, by dividing by the real
+// quadratic (v - w)(v - conj w) = v^2 - s v + t (s = 2 Re w, t = |w|^2):
+// b_k = c_k + s b_{k+1} - t b_{k+2}; the sum is b_1 w + c_0 - t b_2.
+// Two FMAs per term instead of the four of a complex Horner step.
+template <typename ST>
+__device__ __forceinline__ void real_poly_at_complex(const ST *__restrict__ c, int N, ST wr, ST wi, ST &pr,
+                                                     ST &pi) {
+  if (N == 1) {
+    pr = c[0];
+    pi = ST(0);
+    return;
+  }
+  const ST sw = ST(2) * wr, tw = wr * wr + wi * wi;
+  ST b2 = ST(0), b1 = c[N - 1];
+  for (int k = N - 2; k >= 1; --k) {
+    const ST b0 = fma(sw, b1, fma(-tw, b2, c[k]));
+    b2 = b1;
+    b1 = b0;
+  }
+  pr = fma(b1, wr, fma(-tw, b2, c[0]));
+  pi = b1 * wi;
+}
+
 // EPL deflection, LensModel/Profiles/epl.py:178-200 and :364-394 (Tessore &
-// Metcalf 2015 eqs. 22-23).  The Lopez-Temme series (Util/hyp2f1_util.py:
-// 136-176) in w = z/(z-2) is summed by Horner's rule over the per-image
-// coefficients F_n; the reference's nan_to_num policy is kept.  ST is the type
-// of the series sum (double for long series in the mixed form).
-template <typename T, typename ST>
-__device__ __forceinline__ void epl_defl(const Rec<T> &r, const ST *__restrict__ coef, double x, double y,
-                                         double &ax, double &ay) {
+// Metcalf 2015 eqs. 22-23), FP64 form.  The Lopez-Temme series
+// (Util/hyp2f1_util.py:136-176) in w = z/(z-2) with the reference's nmax terms
+// over the per-image coefficients F_n; the reference's nan_to_num policy.
+__device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double *__restrict__ coef, double x,
+                                             double y, double &ax, double &ay) {
   const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
-  const double cd = r.d[EPL_C], sd = r.d[EPL_S];
-  const T xr = T(dx * cd + dy * sd);
-  const T yr = T(-dx * sd + dy * cd);
-  const T q = r[EPL_Q], t = r[EPL_T], f = r[EPL_F];
-  const T Zr = q * xr, Zi = yr;
-  const T R2 = Zr * Zr + Zi * Zi;
-  T R = Fn<T>::sqrt_(R2);
-  R = R > T(0.000000001) ? R : T(0.000000001);
-  T Or, Oi;
+  const double c = r.d[EPL_C], s = r.d[EPL_S];
+  const double xr = dx * c + dy * s;
+  const double yr = -dx * s + dy * c;
+  const double q = r.d[EPL_Q], t = r.d[EPL_T], f = r.d[EPL_F];
+  const double Zr = q * xr, Zi = yr;
+  const double R2 = Zr * Zr + Zi * Zi;
+  double R = ::sqrt(R2);
+  R = R > 0.000000001 ? R : 0.000000001;
+  double Or, Oi;
   const int nmax = (int)r.d[EPL_NMAX];
   if (nmax == 1) {  // f == 0: the reference returns 2F1 = 1 (epl.py:294, :317)
     Or = Zr;
     Oi = Zi;
   } else {
-    // z = -f Z / conj(Z) = -f Z^2 / |Z|^2
-    const ST inv = ST(1) / ST(R2);
-    const ST zr = -ST(f) * (ST(Zr) * ST(Zr) - ST(Zi) * ST(Zi)) * inv;
-    const ST zi = -ST(f) * (ST(2) * ST(Zr) * ST(Zi)) * inv;
+    // z = -f Z / conj(Z) = -f Z^2 / |Z|^2 (NaN at Z = 0, as the reference)
+    const double inv = 1.0 / R2;
+    const double zr = -f * (Zr * Zr - Zi * Zi) * inv;
+    const double zi = -f * (2.0 * Zr * Zi) * inv;
     // w = z / (z - 2)
-    const ST dr = zr - ST(2), di = zi;
-    const ST dd = ST(1) / (dr * dr + di * di);
-    const ST wr = (zr * dr + zi * di) * dd;
-    const ST wi = (zi * dr - zr * di) * dd;
-    ST Sr = coef[nmax - 1], Si = ST(0);
-    for (int n = nmax - 2; n >= 0; --n) {
-      const ST nr = Sr * wr - Si * wi + coef[n];
-      Si = Sr * wi + Si * wr;
-      Sr = nr;
-    }
-    // hyp = S / (1 - z/2)
-    const ST er = ST(1) - zr * ST(0.5), ei = -zi * ST(0.5);
-    const ST ed = ST(1) / (er * er + ei * ei);
-    const T hr = T((Sr * er + Si * ei) * ed);
-    const T hi = T((Si * er - Sr * ei) * ed);
+    const double dr = zr - 2.0;
+    const double dd = 1.0 / (dr * dr + zi * zi);
+    const double wr = (zr * dr + zi * zi) * dd;
+    const double wi = (zi * dr - zr * zi) * dd;
+    double Sr, Si;
+    real_poly_at_complex<double>(coef, nmax, wr, wi, Sr, Si);
+    // 2F1 = S / (1 - z/2)
+    const double er = 1.0 - zr * 0.5, ei = -zi * 0.5;
+    const double ed = 1.0 / (er * er + ei * ei);
+    const double hr = (Sr * er + Si * ei) * ed;
+    const double hi = (Si * er - Sr * ei) * ed;
     Or = Zr * hr - Zi * hi;
     Oi = Zr * hi + Zi * hr;
   }
-  const T scale = r[EPL_PREF] * Fn<T>::pow_(r[EPL_B] / R, t);
-  const T fr = nan_to_1e10(scale * Or);
-  const T fi = nan_to_1e10(scale * Oi);
-  const T c = r[EPL_C], s = r[EPL_S];
-  ax += double(fr * c - fi * s);
-  ay += double(fr * s + fi * c);
+  // (b/R)^t = exp(t (ln b - ln R))
+  const double scale = r.d[EPL_PREF] * ::exp(t * (r.d[EPL_LNB] - ::log(R)));
+  const double fr = nan_to_1e10(scale * Or);
+  const double fi = nan_to_1e10(scale * Oi);
+  ax += fr * c - fi * s;
+  ay += fr * s + fi * c;
+}
+
+// EPL, float form: the same formulas with |z| = f used exactly, the series
+// truncated at EPL_NF terms (tail below 1e-8 of the sum), hardware
+// reciprocal / log2 / exp2, and R^2 in place of R (R = max(|Z|, 1e-9) <=>
+// R^2 = max(|Z|^2, 1e-18)).  Long series (nmax >= 100) run in double.
+__device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *__restrict__ coef_f,
+                                             const double *__restrict__ coef_d, double x, double y, double &ax,
+                                             double &ay) {
+  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+  const double cd = r.d[EPL_C], sd = r.d[EPL_S];
+  const float xr = float(dx * cd + dy * sd);
+  const float yr = float(-dx * sd + dy * cd);
+  const float q = r[EPL_Q], f = r[EPL_F];
+  const float Zr = q * xr, Zi = yr;
+  const float R2 = fmaf(Zr, Zr, Zi * Zi);
+  float Or, Oi;
+  const int nf = (int)r[EPL_NF];
+  if (nf == 1 || R2 == 0.f) {
+    Or = Zr;  // f == 0, or the centre where the reference's NaN becomes 0
+    Oi = Zi;
+  } else {
+    const float inv = __builtin_amdgcn_rcpf(R2);
+    const float zr = -f * (Zr * Zr - Zi * Zi) * inv;
+    const float zi = -f * (2.f * Zr * Zi) * inv;
+    // w = z / (z - 2) with |z|^2 = f^2:  w = (f^2 - 2 z, -2 Im z) / (f^2 - 4 Re z + 4)
+    const float f2 = r[EPL_F2];
+    const float dd = __builtin_amdgcn_rcpf(fmaf(-4.f, zr, f2 + 4.f));
+    const float wr = fmaf(-2.f, zr, f2) * dd;
+    const float wi = -2.f * zi * dd;
+    float Sr, Si;
+    if (r.d[EPL_NMAX] >= 100.0) {
+      double Srd, Sid;
+      real_poly_at_complex<double>(coef_d, (int)r.d[EPL_NMAX], double(wr), double(wi), Srd, Sid);
+      Sr = float(Srd);
+      Si = float(Sid);
+    } else {
+      real_poly_at_complex<float>(coef_f, nf, wr, wi, Sr, Si);
+    }
+    // 2F1 = S / (1 - z/2)
+    const float er = fmaf(-0.5f, zr, 1.f), ei = -0.5f * zi;
+    const float ed = __builtin_amdgcn_rcpf(fmaf(er, er, ei * ei));
+    const float hr = fmaf(Sr, er, Si * ei) * ed;
+    const float hi = fmaf(Si, er, -Sr * ei) * ed;
+    Or = fmaf(Zr, hr, -Zi * hi);
+    Oi = fmaf(Zr, hi, Zi * hr);
+  }
+  const float R2c = fmaxf(R2, 1e-18f);
+  const float lg = fmaf(-0.5f, __builtin_amdgcn_logf(R2c), r[EPL_LOG2B]);  // log2(b / R)
+  const float scale = r[EPL_PREF] * __builtin_amdgcn_exp2f(r[EPL_T] * lg);
+  const float fr = scale * Or, fi = scale * Oi;
+  const float c = r[EPL_C], s = r[EPL_S];
+  ax += double(fmaf(fr, c, -fi * s));
+  ay += double(fmaf(fr, s, fi * c));
 }
 
 // NIE / SIE deflection, LensModel/Profiles/nie.py:86-107, :280-295, :333-342.
@@ -457,11 +555,10 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
                                                double &ay) {
   switch (type) {
     case LENSIM_EPL:
-      // The mixed form keeps long series (nmax >= 100, f >= 0.73) in double.
-      if (sizeof(T) == sizeof(float) && r.d[EPL_NMAX] >= 100.0)
-        epl_defl<T, double>(r, coef_d, x, y, ax, ay);
+      if constexpr (sizeof(T) == sizeof(float))
+        epl_defl_f32(r, coef_t, coef_d, x, y, ax, ay);
       else
-        epl_defl<T, T>(r, coef_t, x, y, ax, ay);
+        epl_defl_f64(r, coef_d, x, y, ax, ay);
       break;
     case LENSIM_SIE:
     case LENSIM_NIE:
@@ -483,26 +580,34 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
 
 // ---------------------------------------------------------- light --------
 
-// Sersic kernel, LensModel/Profiles/sersic_utils.py:239-267 (FP64 form).
-__device__ __forceinline__ double sersic_kernel(double R, const Rec<double> &r) {
-  const double R_ = fmax(0.0001, R);
-  const double frac = R_ / r.d[SE_RS];
-  const double e = -r.d[SE_BN] * (::pow(frac, r.d[SE_INVN]) - 1.0);
-  double v = (frac <= 1000.0) ? ::exp(e) : 0.0;
+// Sersic kernel, LensModel/Profiles/sersic_utils.py:239-267 (FP64 form), on
+// R^2: R_ = max(1e-4, R), frac = R_ / R_sersic_, frac^(1/n) = exp((ln R_ -
+// ln R_sersic_) / n), zero beyond frac = 1000, NaN -> 0.
+__device__ __forceinline__ double sersic_kernel(double R2, const Rec<double> &r) {
+  const double R2c = fmax(1e-8, R2);
+  const double pw = ::exp((0.5 * ::log(R2c) - r.d[SE_LNRS]) * r.d[SE_INVN]);
+  const double e = -r.d[SE_BN] * (pw - 1.0);
+  double v = (R2c <= r.d[SE_R2MAX]) ? ::exp(e) : 0.0;
   if (v != v) v = 0.0;
   if (v == INFINITY) v = 1.7976931348623157e308;
   return v;
 }
 
-// Float form: frac^(1/n) and exp() through the hardware base-2 exp/log.
-__device__ __forceinline__ float sersic_kernel(float R, const Rec<float> &r) {
-  const float R_ = fmaxf(0.0001f, R);
-  const float frac = R_ * r[SE_IRS];
-  const float pw = ::exp2f(::log2f(frac) * r[SE_INVN]);
-  const float e = -r[SE_BN] * (pw - 1.0f);
-  float v = (frac <= 1000.0f) ? ::expf(e) : 0.0f;
-  if (v != v) v = 0.0f;
-  return v;
+// exp(e) in float to ~2 ulp: exp2 of e log2(e) split into a float head and the
+// rounding error of the product, 2^hi (1 + lo ln 2).
+__device__ __forceinline__ float exp_accurate_f32(float e) {
+  const float L = 1.4426950408889634f;
+  const float hi = e * L;
+  const float lo = fmaf(e, L, -hi) + e * 1.925963033500011e-8f;  // log2(e) - L
+  return __builtin_amdgcn_exp2f(hi) * fmaf(lo, 0.6931471805599453f, 1.f);
+}
+
+// Float form on R^2: frac^(1/n) = exp2(C1 log2 R^2 + C2), window R^2 <= R2MAX.
+__device__ __forceinline__ float sersic_kernel(float R2, const Rec<float> &r) {
+  const float R2c = fmaxf(1e-8f, R2);
+  const float pw = __builtin_amdgcn_exp2f(fmaf(r[SE_C1], __builtin_amdgcn_logf(R2c), r[SE_C2]));
+  const float e = r[SE_BN] * (1.f - pw);
+  return (R2c <= r[SE_R2MAX]) ? exp_accurate_f32(e) : 0.f;
 }
 
 // One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
@@ -516,8 +621,7 @@ __device__ __forceinline__ double light_value(int type, const Rec<T> &r, double 
     case LENSIM_SERSIC_ELLIPSE: {
       const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
       const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
-      const T R = Fn<T>::sqrt_(xe * xe + ye * ye);
-      return double(r[SE_AMP] * sersic_kernel(R, r));
+      return double(r[SE_AMP] * sersic_kernel(xe * xe + ye * ye, r));
     }
     case LENSIM_GAUSSIAN: {
       const T r2 = dx * dx * r[GA_IS2] + dy * dy * r[GA_IS2];

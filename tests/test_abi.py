@@ -27,11 +27,11 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(backend._SIGNATURES), "ctypes signatures out of sync with the header"
-    assert lib.lensim_abi_version() == 1
+    assert lib.lensim_abi_version() == backend.ABI_VERSION
 
 
 def test_struct_layouts():
-    assert ctypes.sizeof(backend.LensimModel) == 12 + 4 * 64 + 4 + 16  # padded to 8 before the doubles
+    assert ctypes.sizeof(backend.LensimModel) == 12 + 4 * 64 + 4 + 16 + 8 + 8  # padded to 8 before the doubles
     assert ctypes.sizeof(backend.LensimGrid) == 16 + 16 + 32
     assert ctypes.sizeof(backend.LensimPsf) == 24
     assert ctypes.sizeof(backend.LensimData) == 48
