@@ -180,6 +180,21 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
                    const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
                    double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream);
 
+/* log-likelihood and its gradient with respect to every kwarg of every
+ * profile, for a batch: d_grad is [n_img][n_profiles][LENSIM_NPARAM] in the
+ * parameter-block layout (slots beyond a profile's kwargs are zero).
+ * Replaces jax.grad of ImageModel.likelihood_data_given_model
+ * (ImSim/image_model.py:162-221, as exercised by
+ * test/test_ImSim/test_image_model.py:274-335; EPL's custom JVP epl.py:321-340).
+ * Adjoint path: dlogL/dmodel (model-based variance included) -> correlation
+ * with the PSF -> per-ray forward-mode derivatives of each profile.
+ * Not supported: bulk deflector lists, the edge-padded GAUSSIAN PSF. */
+size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                  int32_t n_img, int32_t precision);
+int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                        const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                        double *d_logl, double *d_grad, void *ws, size_t ws_bytes, void *stream);
+
 /* Kernel timing for benchmarks (no counterpart in the reference).  While
  * enabled, every lensim_image / lensim_loglike call records HIP events on its
  * own stream around its render kernel and its convolution kernel.

@@ -247,6 +247,51 @@ class ImageModel(object):
         backend.check(rc, "lensim_loglike")
         return logl
 
+    def likelihood_and_grad_batch(self, params, n_img=None):
+        """(logL [B], dlogL/dparams [B, n_profiles, 8]) of a packed device parameter
+        block; the gradient slot layout is the parameter-block layout."""
+        lib = backend.lib()
+        if self._bulk:
+            raise NotImplementedError("gradients of bulk deflector lists are not supported")
+        if n_img is None:
+            n_img = params.shape[0]
+        params = params.contiguous()
+        logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
+        grad = torch.empty((n_img, params.shape[1], backend.NPARAM), dtype=torch.float64, device=params.device)
+        d = self._data_struct()
+        nbytes = lib.lensim_grad_workspace_size(ctypes.byref(self._model), ctypes.byref(self._grid),
+                                                ctypes.byref(self._psf), n_img, self._prec)
+        if nbytes == 0:
+            raise ValueError("gradient workspace query failed: %s" % lib.lensim_last_error().decode())
+        ws = self._ws.get(nbytes)
+        rc = lib.lensim_loglike_grad(ctypes.byref(self._model), ctypes.byref(self._grid), ctypes.byref(self._psf),
+                                     ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
+                                     backend.ptr(grad), backend.ptr(ws), ws.numel(), backend.stream_handle())
+        backend.check(rc, "lensim_loglike_grad")
+        return logl, grad
+
+    def log_likelihood_torch(self, params):
+        """Differentiable logL [B] of a packed parameter tensor [B, n_profiles, 8]:
+        torch.autograd calls the library's adjoint kernels in backward."""
+        return _LogLikelihood.apply(params, self)
+
+    def likelihood_gradient(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None):
+        """logL and its gradient in kwargs form: (logL, [grad dicts for lens],
+        [source], [lens light]), like jax.grad of likelihood_data_given_model
+        (test/test_ImSim/test_image_model.py:274-335)."""
+        params, n_img = self.pack(kwargs_lens, kwargs_source, kwargs_lens_light)
+        logl, grad = self.likelihood_and_grad_batch(backend.to_device_f64(params), n_img)
+        g = grad.cpu().numpy()
+        out, k = [], 0
+        for types in (self._lens_types, self._src_types, self._light_types):
+            lst = []
+            for _ in types:
+                names = self._names[k]
+                lst.append({nm: (g[:, k, i] if n_img > 1 else float(g[0, k, i])) for i, nm in enumerate(names)})
+                k += 1
+            out.append(lst)
+        return (logl if n_img > 1 else logl[0]), out[0], out[1], out[2]
+
     # ------------------------------------------------------------ drop-in API --
     @staticmethod
     def _check_unsupported(kwargs_ps=None, kwargs_extinction=None):
@@ -336,3 +381,18 @@ class ImageModel(object):
 
     def update_data(self, data_class):
         raise ValueError("Updating data class not supported. Create a new instance of ImageModel instead.")
+
+
+class _LogLikelihood(torch.autograd.Function):
+    """logL of packed parameters with the library's adjoint as the backward."""
+
+    @staticmethod
+    def forward(ctx, params, image_model):
+        logl, grad = image_model.likelihood_and_grad_batch(params.detach().to(torch.float64))
+        ctx.save_for_backward(grad)
+        return logl
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return grad_out[:, None, None] * grad, None

@@ -94,6 +94,9 @@ _SIGNATURES = {
                                     ctypes.c_size_t, _P]),
     "lensim_loglike": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                       ctypes.c_size_t, _P]),
+    "lensim_grad_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
+    "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
+                                           ctypes.c_size_t, _P]),
     "lensim_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "lensim_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64)]),

@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "grad.hpp"
 #include "profiles.hpp"
 
 using namespace lensim;
@@ -96,7 +97,9 @@ __host__ __device__ inline int type_of(const ModelArgs &m, int k) {
 }
 
 struct Layout {
-  size_t recd, recf, coefd, coeff, psf, lowres, partial, blkmax, bfrag, total;
+  size_t recd, recf, coefd, coeff, dcoefd, dcoeff, psf, lowres, partial, blkmax, bfrag;
+  size_t gmodel, glow, gpart;  // gradient: dlogL/dmodel, dlogL/dlowres, per-block partial gradients
+  size_t total;
   int n_tiles;
 };
 
@@ -150,6 +153,10 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   off = align256(off + ncoef * 8);
   L.coeff = off;
   off = align256(off + ncoef * 4);
+  L.dcoefd = off;
+  off = align256(off + ncoef * 8);
+  L.dcoeff = off;
+  off = align256(off + ncoef * 4);
   L.psf = off;
   off = align256(off + size_t(kx) * kyp_of(ky) * 8);
   L.lowres = off;
@@ -161,6 +168,22 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   off = align256(off + size_t(n_img) * ((size_t(nx) * ny + 255) / 256) * 4);
   L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments
   if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(ky) * mfma_nks(kx) * 2 * 64 * 16);
+  L.total = off;
+  return L;
+}
+
+// Layout of lensim_loglike_grad: the image layout plus the gradient buffers.
+Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision) {
+  Layout L = layout(m, nx, ny, kx, ky, n_img, precision);
+  const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
+  const size_t npix = size_t(nx) * ny;
+  size_t off = L.total;
+  L.gmodel = off;
+  off = align256(off + size_t(n_img) * npix * tsz);
+  L.glow = off;
+  off = align256(off + size_t(n_img) * npix * tsz);
+  L.gpart = off;
+  off = align256(off + size_t(n_img) * m.n_prof * ((npix + 255) / 256) * LENSIM_NPARAM * 8);
   L.total = off;
   return L;
 }
@@ -222,7 +245,8 @@ __device__ inline bool bulk_type_ok_dev(int t) {
 }
 
 __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int n_img, double *__restrict__ recd,
-                            float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff) {
+                            float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
+                            double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_img * m.n_prof) return;
   const int img = idx / m.n_prof, k = idx - img * m.n_prof;
@@ -232,8 +256,12 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
   double rec[kRec];
   double *coef = nullptr;
   const int e = k < m.n_lens ? m.epl_index[k] : -1;
-  if (e >= 0) coef = coefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
-  prepare_record(type, p, rec, coef);
+  double *dcoef = nullptr;
+  if (e >= 0) {
+    coef = coefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+    if (dcoefd) dcoef = dcoefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+  }
+  prepare_record(type, p, rec, coef, dcoef);
   for (int i = 0; i < kRec; ++i) {
     recd[size_t(idx) * kRec + i] = rec[i];
     recf[size_t(idx) * kRec + i] = float(rec[i]);
@@ -242,13 +270,18 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
     const int nmax = (int)rec[EPL_NMAX];
     float *cf = coeff + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
     for (int n = 0; n < nmax; ++n) cf[n] = float(coef[n]);
+    if (dcoef) {
+      float *dcf = dcoeff + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+      for (int n = 0; n < nmax; ++n) dcf[n] = float(dcoef[n]);
+    }
   }
 }
 
 // Flip the kernel (so the convolution runs as a correlation) and store it
 // transposed, column c of the flipped kernel contiguous: kt[c * kyp + a] =
 // K[ky-1-a][kx-1-c], zero for a >= ky.
-__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out) {
+__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out,
+                                int flip = 1) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= kyp * kx) return;
   const int c = idx / kyp, a = idx - c * kyp;
@@ -256,7 +289,9 @@ __global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, in
     out[idx] = (a == 0 && c == 0) ? 1.0 : 0.0;
     return;
   }
-  out[idx] = (a < ky) ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
+  // flip = 0 stores the kernel itself: the convolution then correlates with K,
+  // which is the adjoint of convolving with K (zero padding)
+  out[idx] = (a < ky) ? (flip ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : k[a * kx + c]) : 0.0;
 }
 
 template <typename T>
@@ -417,7 +452,28 @@ struct ConvArgs {
   int nx, ny, kx, ky, kyp, edge, n_tiles_x;  // kyp: kernel height padded to kWin
   const double *data, *noise, *expo, *mask;
   double exposure_time, bkg2;
+  void *gout;  // optional dlogL/dmodel per pixel (compute type)
 };
+
+// d logL / d model of one pixel: logL = -1/2 mask (m - d)^2 / V(m) with
+// V = max(m, 0)/t + bkg^2 (dV/dm = 1/t for m >= 0, the branch the reference's
+// jnp.where(data >= 0, ...) takes) or the noise map's square (dV/dm = 0).
+__device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m) {
+  const double d = c.data[p];
+  double var, dvar;
+  if (c.noise) {
+    var = c.noise[p] * c.noise[p];
+    dvar = 0.0;
+  } else {
+    const double t = c.expo ? c.expo[p] : c.exposure_time;
+    var = (m >= 0.0 ? m : 0.0) / t + c.bkg2;
+    dvar = m >= 0.0 ? 1.0 / t : 0.0;
+  }
+  const double r = (m - d) / var;
+  double gp = -(r - 0.5 * r * r * dvar);
+  if (c.mask) gp *= c.mask[p];
+  return gp;
+}
 
 template <typename T>
 __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T model_t) {
@@ -528,6 +584,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
         const T m = T(acc[r]);
         if (out) out[size_t(img) * c.nx * c.ny + p] = m;
         if (partial) chi += chi2_pixel<T>(c, p, m);
+        if (c.gout) static_cast<T *>(c.gout)[size_t(img) * c.nx * c.ny + p] = T(dlogl_dm(c, p, double(m)));
       }
     }
   }
@@ -682,6 +739,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int
         const float m = float(dacc[t][j] * unscale);
         if (out) out[size_t(img) * c.nx * c.ny + p] = m;
         if (partial) chi += chi2_pixel<float>(c, p, m);
+        if (c.gout) static_cast<float *>(c.gout)[size_t(img) * c.nx * c.ny + p] = float(dlogl_dm(c, p, double(m)));
       }
     }
   }
@@ -697,6 +755,161 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int
       partial[size_t(img) * gridDim.x + tile_id] = s;
     }
   }
+}
+
+
+// ------------------------------------------------------------- gradient ---
+// Number of kwargs of a profile type (its slots in a parameter block).
+__host__ __device__ inline int n_params_of(int type) {
+  switch (type) {
+    case LENSIM_EPL: return 6;
+    case LENSIM_SIE: return 5;
+    case LENSIM_NIE: return 6;
+    case LENSIM_NFW: return 4;
+    case LENSIM_TNFW: return 5;
+    case LENSIM_SHEAR: return 4;
+    case LENSIM_SERSIC: return 5;
+    case LENSIM_SERSIC_ELLIPSE: return 7;
+    case LENSIM_GAUSSIAN: return 4;
+    default: return 0;
+  }
+}
+
+struct GradArgs {
+  const double *params;   // [n_img][n_prof][8]
+  const double *coefd;    // EPL F_n (double)
+  const float *coeff;     // EPL F_n (float)
+  const double *dcoefd;   // EPL dF_n/dt
+  const float *dcoeff;
+  const void *glow;       // [n_img][ny][nx] dlogL / d(unconvolved image), compute type
+  double *gpart;          // [n_img][n_prof][nblk][8]
+};
+
+// Contribution of one ray to d flux / d(kwargs of profile k), with the
+// profile's own kwargs as N dual seeds; w = dlogL / d(ray flux).
+template <typename T, int N, bool HALO>
+__device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+                                         int k, int type, double x, double y, double w, double *acc) {
+  using S = gm::Dual<T, N>;
+  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  S ps[LENSIM_NPARAM];
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(T(pk[i]), i) : S(T(pk[i]));
+  S flux;
+  if (k < m.n_lens) {
+    // d/d(lens kwargs): the other deflectors in the forward form, this one in duals
+    double axp = 0.0, ayp = 0.0;
+    for (int j = 0; j < m.n_lens; ++j) {
+      if (j == k) continue;
+      const int e = m.epl_index[j];
+      const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+      add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + co,
+                              rp.coeft + co, x, y, axp, ayp);
+    }
+    S ax = S(T(axp)), ay = S(T(ayp));
+    const int e = m.epl_index[k];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    int nser = 1;
+    if (type == LENSIM_EPL)
+      nser = int(sizeof(T) == sizeof(float) ? rp.recd[k * kRec + EPL_NF] : rp.recd[k * kRec + EPL_NMAX]);
+    const T *c = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.coeff + co)
+                                            : reinterpret_cast<const T *>(ga.coefd + co);
+    const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
+                                             : reinterpret_cast<const T *>(ga.dcoefd + co);
+    const S xs = S(T(x)), ys = S(T(y));
+    gm::defl_g<T, S>(type, ps, c, dc, nser, xs, ys, ax, ay);
+    const S bx = xs - ax * T(m.alpha_scaling), by = ys - ay * T(m.alpha_scaling);
+    flux = S(T(0));
+    for (int j = m.n_defl; j < m.n_defl + m.n_source; ++j) {
+      const double *pj = ga.params + (size_t(img) * m.n_prof + j) * LENSIM_NPARAM;
+      S pl[LENSIM_NPARAM];
+#pragma unroll
+      for (int i = 0; i < LENSIM_NPARAM; ++i) pl[i] = S(T(pj[i]));
+      flux = flux + gm::light_g<T, S>(m.types[j - m.n_bulk], pl, bx, by);
+    }
+  } else if (k < m.n_defl + m.n_source) {
+    // d/d(source kwargs) at the deflected position
+    double ax = 0.0, ay = 0.0;
+    deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
+    const S bx = S(T(x - ax * m.alpha_scaling)), by = S(T(y - ay * m.alpha_scaling));
+    flux = gm::light_g<T, S>(type, ps, bx, by);
+  } else {
+    // d/d(lens light kwargs) at the ray
+    flux = gm::light_g<T, S>(type, ps, S(T(x)), S(T(y)));
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) acc[i] += w * double(flux.d[i]);
+}
+
+// One block = 256 pixels x one image x one profile (blockIdx.z).
+template <typename T, bool HALO>
+__global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
+                                                   const double *__restrict__ recd, const float *__restrict__ recf,
+                                                   const double *__restrict__ coefd, const float *__restrict__ coeff) {
+  const int img = blockIdx.y, k = blockIdx.z;
+  const int npix = g.nx * g.ny;
+  const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = pix0 < npix;
+  const int pix = active ? pix0 : npix - 1;
+  const int i = pix % g.nx, j = pix / g.nx;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const int type = type_of(m, k);
+  // the pixel's flux is scale * flux_scaling * (sum over its rays)
+  const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
+                                m.flux_scaling
+                          : 0.0;
+  double acc[LENSIM_NPARAM];
+#pragma unroll
+  for (int q = 0; q < LENSIM_NPARAM; ++q) acc[q] = 0.0;
+  const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
+  const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
+  const double inv_s = 1.0 / g.s;
+  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  // a lens profile only matters if a source is there to be lensed
+  const bool live = active && w != 0.0 && (k >= m.n_defl || m.n_source > 0);
+  for (int sy = 0; live && sy < g.s; ++sy) {
+    const double oy = o0 + sy * inv_s;
+    for (int sx = 0; sx < g.s; ++sx) {
+      const double ox = o0 + sx * inv_s;
+      const double x = ra_c + g.m00 * ox + g.m01 * oy;
+      const double y = dec_c + g.m10 * ox + g.m11 * oy;
+      switch (type) {
+        case LENSIM_EPL: ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_SIE: ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_NIE: ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_NFW: if constexpr (HALO) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_TNFW: if constexpr (HALO) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_SHEAR: ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_SERSIC: ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_SERSIC_ELLIPSE: ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_GAUSSIAN: ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        default: break;
+      }
+    }
+  }
+  // block sum of the 8 accumulators in a fixed order
+  __shared__ double red[256 / 64][LENSIM_NPARAM];
+#pragma unroll
+  for (int q = 0; q < LENSIM_NPARAM; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < LENSIM_NPARAM) {
+    double v = 0.0;
+    for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[wv][threadIdx.x];
+    ga.gpart[((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + threadIdx.x] = v;
+  }
+}
+
+__global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, int n, double *__restrict__ grad) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (img * n_prof + k) * 8 + q
+  if (idx >= n) return;
+  const int q = idx % LENSIM_NPARAM, ik = idx / LENSIM_NPARAM;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += gpart[(size_t(ik) * nblk + b) * LENSIM_NPARAM + q];
+  grad[idx] = s;
 }
 
 __global__ void logl_kernel(const double *__restrict__ partial, int n_tiles, int n_img, double *__restrict__ logl) {
@@ -716,13 +929,15 @@ int check_launch(const char *where) {
 }
 
 int launch_prep(const ModelArgs &m, const double *params, int n_img, unsigned char *ws, const Layout &L,
-                hipStream_t st) {
+                hipStream_t st, bool with_dcoef = false) {
   const int n = n_img * m.n_prof;
   if (n == 0) return LENSIM_OK;
   prep_kernel<<<(n + 63) / 64, 64, 0, st>>>(m, params, n_img, reinterpret_cast<double *>(ws + L.recd),
                                              reinterpret_cast<float *>(ws + L.recf),
                                              reinterpret_cast<double *>(ws + L.coefd),
-                                             reinterpret_cast<float *>(ws + L.coeff));
+                                             reinterpret_cast<float *>(ws + L.coeff),
+                                             with_dcoef ? reinterpret_cast<double *>(ws + L.dcoefd) : nullptr,
+                                             with_dcoef ? reinterpret_cast<float *>(ws + L.dcoeff) : nullptr);
   return check_launch("prep_kernel");
 }
 
@@ -782,8 +997,8 @@ bool use_mfma(bool identity, int kx, int ky) {
 template <typename T>
 int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
               const lensim_data *data, const double *params, int n_img, unsigned flags, T *out, double *logl,
-              unsigned char *ws, const Layout &L, hipStream_t st) {
-  int rc = launch_prep(m, params, n_img, ws, L, st);
+              unsigned char *ws, const Layout &L, hipStream_t st, T *gout = nullptr) {
+  int rc = launch_prep(m, params, n_img, ws, L, st, gout != nullptr);
   if (rc) return rc;
   const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
   const bool mfma = use_mfma<T>(identity, kx, ky);
@@ -831,6 +1046,7 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     c.mask = data->d_mask;
     c.exposure_time = data->exposure_time;
     c.bkg2 = data->background_rms * data->background_rms;
+    c.gout = gout;
     partial = reinterpret_cast<double *>(ws + L.partial);
   }
   int n_tiles;
@@ -866,6 +1082,65 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     if (rc) return rc;
   }
   return LENSIM_OK;
+}
+
+// log-likelihood and its gradient: the forward pipeline writes dlogL/dmodel,
+// the adjoint of the PSF (a correlation with the kernel) gives dlogL/d(pooled
+// image), and grad_kernel accumulates d(ray flux)/d(kwargs) per profile.
+template <typename T>
+int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+             const lensim_data *data, const double *params, int n_img, double *logl, double *grad,
+             unsigned char *ws, const Layout &L, hipStream_t st) {
+  T *gmodel = reinterpret_cast<T *>(ws + L.gmodel);
+  int rc = run_image<T>(m, g, psf, kx, ky, edge, data, params, n_img, LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT,
+                        nullptr, logl, ws, L, st, gmodel);
+  if (rc) return rc;
+  // adjoint of the convolution (zero padding): correlate with the kernel
+  const bool identity = !psf || psf->type == LENSIM_PSF_NONE;
+  const int kyp = kyp_of(ky);
+  double *kf = reinterpret_cast<double *>(ws + L.psf);
+  psf_prep_kernel<<<(kyp * kx + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kyp, kf, 0);
+  if ((rc = check_launch("psf_prep_kernel"))) return rc;
+  ConvArgs c{};
+  c.nx = g.nx;
+  c.ny = g.ny;
+  c.kx = kx;
+  c.ky = ky;
+  c.kyp = kyp;
+  c.edge = 0;
+  c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+  const int n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
+  const size_t smem = conv_smem<T>(kx, kyp);
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+  }
+  T *glow = reinterpret_cast<T *>(ws + L.glow);
+  conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, gmodel, kf, glow,
+                                                                                                 nullptr);
+  if ((rc = check_launch("conv_kernel (adjoint)"))) return rc;
+  // per-profile gradients
+  GradArgs ga;
+  ga.params = params;
+  ga.coefd = reinterpret_cast<const double *>(ws + L.coefd);
+  ga.coeff = reinterpret_cast<const float *>(ws + L.coeff);
+  ga.dcoefd = reinterpret_cast<const double *>(ws + L.dcoefd);
+  ga.dcoeff = reinterpret_cast<const float *>(ws + L.dcoeff);
+  ga.glow = glow;
+  ga.gpart = reinterpret_cast<double *>(ws + L.gpart);
+  const int npix = g.nx * g.ny, nblk = (npix + 255) / 256;
+  dim3 grid(nblk, n_img, m.n_prof);
+  const double *recd = reinterpret_cast<const double *>(ws + L.recd);
+  const float *recf = reinterpret_cast<const float *>(ws + L.recf);
+  if (m.has_halo)
+    grad_kernel<T, true><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+  else
+    grad_kernel<T, false><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+  if ((rc = check_launch("grad_kernel"))) return rc;
+  const int n = n_img * m.n_prof * LENSIM_NPARAM;
+  grad_reduce_kernel<<<(n + 255) / 256, 256, 0, st>>>(ga.gpart, nblk, n, grad);
+  return check_launch("grad_reduce_kernel");
 }
 
 int check_ws(const Layout &L, void *ws, size_t ws_bytes) {
@@ -1058,6 +1333,44 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
                              static_cast<double *>(d_model_out), d_logl, w, L, st);
   return run_image<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<float *>(d_model_out),
                           d_logl, w, L, st);
+}
+
+size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                  int32_t n_img, int32_t precision) {
+  ModelArgs m;
+  if (model_args(model, m) || !grid) return 0;
+  int kx = 1, ky = 1, edge = 0;
+  if (psf_dims(psf, 0, kx, ky, edge)) return 0;
+  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision).total;
+}
+
+int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                        const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                        double *d_logl, double *d_grad, void *ws, size_t ws_bytes, void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  if (m.n_bulk) return fail(LENSIM_EUNSUPPORTED, "gradients of bulk deflector lists are not supported");
+  GridArgs g;
+  if ((rc = grid_args(grid, g))) return rc;
+  int kx, ky, edge;
+  if ((rc = psf_dims(psf, 0, kx, ky, edge))) return rc;
+  if (edge) return fail(LENSIM_EUNSUPPORTED, "gradients with the edge-padded GAUSSIAN PSF are not supported");
+  if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
+  if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
+    return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
+  if (n_img == 0) return LENSIM_OK;
+  if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");
+  if (!d_logl || !d_grad || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision);
+  if ((rc = check_ws(L, ws, ws_bytes))) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  if (precision == LENSIM_FP64)
+    return run_grad<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, d_logl, d_grad, w, L, st);
+  return run_grad<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, d_logl, d_grad, w, L, st);
 }
 
 }  // extern "C"

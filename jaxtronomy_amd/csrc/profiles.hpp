@@ -122,7 +122,8 @@ __host__ __device__ inline void ellipticity2phi_q(double e1, double e2, double &
 // Turns one kwargs slot p[LENSIM_NPARAM] into the derived record rec[kRec].
 // For EPL also writes the series coefficients F_n = 2F1(-n, t/2; 2-t/2; 2),
 // n < nmax, from Gauss' contiguous recurrence (Util/hyp2f1_util.py:156-166).
-__device__ inline void prepare_record(int type, const double *p, double *rec, double *coef) {
+__device__ inline void prepare_record(int type, const double *p, double *rec, double *coef,
+                                      double *dcoef = nullptr) {
   for (int i = 0; i < kRec; ++i) rec[i] = 0.0;
   switch (type) {
     case LENSIM_EPL: {
@@ -162,6 +163,20 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
         }
         // |w| <= f / (2 - f) for |z| = f, so the terms n >= N add at most
         // max|F_n| wmax^N / (1 - wmax) to a sum of size ~1
+        if (dcoef) {
+          // d F_n / d t from the same recurrence: F_n = N_n / D_n with
+          // N_n = (n-1) F_{n-2} - (1.5 t - 2) F_{n-1}, D_n = 1 + n - t/2
+          double dm2 = 0.0, dm1 = -2.0 / (cc * cc);  // d/dt (1 - t / (2 - t/2))
+          dcoef[0] = 0.0;
+          if (nmax > 1) dcoef[1] = dm1;
+          for (int n = 2; n < nmax; ++n) {
+            const double Dn = cc + n - 1.0;
+            const double dn = ((n - 1.0) * dm2 - 1.5 * coef[n - 1] - (2.0 * bb - cc) * dm1) / Dn + 0.5 * coef[n] / Dn;
+            dcoef[n] = dn;
+            dm2 = dm1;
+            dm1 = dn;
+          }
+        }
         const double wmax = f / (2.0 - f);
         int nf = nmax;
         if (wmax > 0.0) {

@@ -1,0 +1,109 @@
+"""Gradients of the log-likelihood (adjoint kernels) against central finite
+differences: of the GPU's own FP64 forward (validated to 1e-10 against the
+oracle) at the 120x120 s=2 config, and of the CPU oracle on a small case."""
+
+import numpy as np
+import pytest
+import torch
+
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.imaging_data import ImageData
+from jaxtronomy_amd.Data.psf import PSF
+from jaxtronomy_amd.ImSim.image_model import ImageModel
+from jaxtronomy_amd.LensModel.lens_model import LensModel
+from jaxtronomy_amd.LightModel.light_model import LightModel
+from oracle import lensim_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FREE = {  # (profile index, slot) of the free parameters: EPL 6, SHEAR 2, source 5 + lens light 5
+    (0, 0): "theta_E", (0, 1): "gamma", (0, 2): "e1", (0, 3): "e2", (0, 4): "center_x", (0, 5): "center_y",
+    (1, 0): "gamma1", (1, 1): "gamma2",
+    (2, 0): "amp", (2, 1): "R_sersic", (2, 2): "n_sersic", (2, 3): "e1", (2, 4): "e2",
+    (3, 0): "amp", (3, 1): "R_sersic", (3, 2): "n_sersic", (3, 3): "e1", (3, 4): "e2",
+}
+
+
+def _model(precision, n=120, s=2, k=21):
+    cfg = dict(W.EPL180, numPix=n, supersampling=s)
+    kernel = W.moffat_kernel(k, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    g = W.grid_dict(cfg)
+    truth = W.epl180_draw(0)
+    m = O.image(g, ["EPL", "SHEAR"], truth[0], ["SERSIC_ELLIPSE"], truth[1], ["SERSIC_ELLIPSE"], truth[2],
+                kernel=kernel)
+    data = W.noisy(m, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(3))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": s}, precision=precision)
+    return im, truth
+
+
+def _perturbed(im, truth, n, seed):
+    rng = np.random.default_rng(seed)
+    base, _ = im.pack(*truth)
+    p = np.repeat(base, n, axis=0)
+    for (k, i) in FREE:
+        p[:, k, i] += rng.normal(0, 0.02, n) * max(abs(base[0, k, i]), 0.1)
+    return p
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-4), ("mixed", 1e-3)])
+def test_grad_vs_central_fd(precision, tol):
+    im, truth = _model(precision)
+    im64, _ = _model("fp64")
+    params = torch.as_tensor(_perturbed(im, truth, 4, 1), device="cuda")
+    logl, grad = im.likelihood_and_grad_batch(params)
+    logl64 = im64.likelihood_batch(params)
+    assert torch.allclose(logl, logl64, rtol=1e-6)
+    g = grad.cpu().numpy()
+    for (k, i) in FREE:
+        h = 1e-5 * max(abs(params[0, k, i].item()), 0.1)
+        pp, pm = params.clone(), params.clone()
+        pp[:, k, i] += h
+        pm[:, k, i] -= h
+        fd = ((im64.likelihood_batch(pp) - im64.likelihood_batch(pm)) / (2 * h)).cpu().numpy()
+        scale = np.maximum(np.abs(fd), 1e-3 * np.abs(logl64.cpu().numpy()) / max(abs(params[0, k, i].item()), 0.1))
+        err = np.max(np.abs(g[:, k, i] - fd) / scale)
+        assert err < tol, (k, i, FREE[(k, i)], g[:, k, i], fd)
+    # slots beyond a profile's kwargs stay zero
+    assert np.all(g[:, 1, 2:] == 0) and np.all(g[:, 0, 6:] == 0)
+
+
+def test_grad_vs_oracle_fd_small():
+    """Independent of the GPU forward: central differences of the CPU oracle."""
+    cfg = dict(W.EPL180, numPix=24, supersampling=1, deltaPix=0.15)
+    kernel = W.moffat_kernel(5, 1.5, 3.0)
+    g = W.grid_dict(cfg)
+    kl, ks, kll = W.epl180_draw(2)
+    m = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel)
+    data = W.noisy(m, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(5))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]))
+    kl = [dict(kl[0], theta_E=kl[0]["theta_E"] * 1.03, gamma=kl[0]["gamma"] - 0.02), kl[1]]
+    logl, glens, gsrc, glight = im.likelihood_gradient(kl, ks, kll)
+
+    def ll(klx, ksx, kllx):
+        mm = O.image(g, ["EPL", "SHEAR"], klx, ["SERSIC_ELLIPSE"], ksx, ["SERSIC_ELLIPSE"], kllx, kernel=kernel)
+        return O.log_likelihood(mm, data, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"])
+
+    assert abs(float(logl) - ll(kl, ks, kll)) < 1e-9 * abs(ll(kl, ks, kll))
+    for which, grads, kwl in ((0, glens, kl), (1, gsrc, ks), (2, glight, kll)):
+        for j, kw in enumerate(kwl):
+            for name in grads[j]:
+                h = 1e-6 * max(abs(kw[name]), 0.1)
+                lists = [[dict(d) for d in kl], [dict(d) for d in ks], [dict(d) for d in kll]]
+                lists[which][j][name] = kw[name] + h
+                up = ll(*lists)
+                lists[which][j][name] = kw[name] - h
+                dn = ll(*lists)
+                fd = (up - dn) / (2 * h)
+                assert abs(grads[j][name] - fd) <= 1e-5 * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
+
+
+def test_torch_autograd_backward():
+    im, truth = _model("mixed", n=60, s=2, k=11)
+    params = torch.as_tensor(_perturbed(im, truth, 3, 2), device="cuda").requires_grad_(True)
+    logl = im.log_likelihood_torch(params)
+    (2.0 * logl.sum()).backward()
+    _, grad = im.likelihood_and_grad_batch(params.detach())
+    assert torch.allclose(params.grad, 2.0 * grad)
