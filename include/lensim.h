@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LENSIM_ABI_VERSION 2
+#define LENSIM_ABI_VERSION 3
 #define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
 #define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
 #define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
@@ -132,6 +132,9 @@ typedef struct {
   const double *d_mask;         /* [ny][nx] likelihood mask (0/1), or NULL for all ones */
   double exposure_time;         /* used when d_exposure_map is NULL */
   double background_rms;
+  int64_t image_stride;         /* elements between the data (and noise map) of consecutive
+                                   batch images; 0: one observation for the whole batch.
+                                   The exposure map and mask are shared by the batch. */
 } lensim_data;
 
 /* Library version and a message for the last failure on this thread. */

@@ -218,7 +218,9 @@ class ImageModel(object):
         backend.check(rc, "lensim_image")
         return out
 
-    def _data_struct(self):
+    def _data_struct(self, data=None):
+        """lensim_data of the ImageData; ``data`` ([B, ny, nx] float64 device
+        tensor) gives each batch image its own observation instead."""
         buf = self.Data.device_buffers()
         if self._mask_dev is None:
             self._mask_dev = backend.to_device_f64(self.likelihood_mask.astype(np.float64))
@@ -229,17 +231,30 @@ class ImageModel(object):
         d.d_mask = self._mask_dev.data_ptr()
         d.exposure_time = buf["expo_time"]
         d.background_rms = self.Data.background_rms
+        if data is not None:
+            if data.dtype != torch.float64 or not data.is_contiguous() or \
+                    tuple(data.shape[-2:]) != (self._grid.ny, self._grid.nx):
+                raise ValueError("per-image data must be a contiguous float64 [B, %d, %d] tensor"
+                                 % (self._grid.ny, self._grid.nx))
+            if buf["noise"] is not None:
+                raise ValueError("per-image data with a noise map needs per-image noise maps")
+            d.d_data = data.data_ptr()
+            d.image_stride = self._grid.nx * self._grid.ny
         return d
 
-    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None):
-        """log-likelihoods [B] (float64) of a packed device parameter block."""
+    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None, data=None):
+        """log-likelihoods [B] (float64) of a packed device parameter block,
+        against the ImageData or, with ``data`` [B, ny, nx], one observation per
+        batch image."""
         lib = backend.lib()
         m = self._model if model is None else model
         if n_img is None:
             n_img = params.shape[0]
+        if data is not None and data.shape[0] < n_img:
+            raise ValueError("per-image data holds %d images for a batch of %d" % (data.shape[0], n_img))
         if logl is None:
             logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
-        d = self._data_struct()
+        d = self._data_struct(data)
         ws = self._ws.get(self.workspace_bytes(n_img, m))
         rc = lib.lensim_loglike(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
                                 ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),

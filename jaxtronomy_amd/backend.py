@@ -15,7 +15,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 NPARAM = 8
 MAX_PROFILES = 64
 EPL_NMAX = 500
@@ -77,6 +77,7 @@ class LensimData(ctypes.Structure):
         ("d_mask", ctypes.c_void_p),
         ("exposure_time", ctypes.c_double),
         ("background_rms", ctypes.c_double),
+        ("image_stride", ctypes.c_int64),
     ]
 
 

@@ -453,16 +453,18 @@ struct ConvArgs {
   const double *data, *noise, *expo, *mask;
   double exposure_time, bkg2;
   void *gout;  // optional dlogL/dmodel per pixel (compute type)
+  int64_t dstride;  // data / noise elements between batch images (0: shared)
 };
 
 // d logL / d model of one pixel: logL = -1/2 mask (m - d)^2 / V(m) with
 // V = max(m, 0)/t + bkg^2 (dV/dm = 1/t for m >= 0, the branch the reference's
 // jnp.where(data >= 0, ...) takes) or the noise map's square (dV/dm = 0).
-__device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m) {
-  const double d = c.data[p];
+__device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m, int img) {
+  const size_t q = p + size_t(img) * size_t(c.dstride);  // per-image observation
+  const double d = c.data[q];
   double var, dvar;
   if (c.noise) {
-    var = c.noise[p] * c.noise[p];
+    var = c.noise[q] * c.noise[q];
     dvar = 0.0;
   } else {
     const double t = c.expo ? c.expo[p] : c.exposure_time;
@@ -476,12 +478,13 @@ __device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m
 }
 
 template <typename T>
-__device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T model_t) {
+__device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T model_t, int img) {
+  const size_t q = p + size_t(img) * size_t(c.dstride);  // per-image observation
   const double m = double(model_t);
-  const double d = c.data[p];
+  const double d = c.data[q];
   double var;
   if (c.noise) {
-    const double nm = c.noise[p];
+    const double nm = c.noise[q];
     var = nm * nm;
   } else {
     const double t = c.expo ? c.expo[p] : c.exposure_time;
@@ -583,8 +586,8 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
         const size_t p = size_t(y) * c.nx + x;
         const T m = T(acc[r]);
         if (out) out[size_t(img) * c.nx * c.ny + p] = m;
-        if (partial) chi += chi2_pixel<T>(c, p, m);
-        if (c.gout) static_cast<T *>(c.gout)[size_t(img) * c.nx * c.ny + p] = T(dlogl_dm(c, p, double(m)));
+        if (partial) chi += chi2_pixel<T>(c, p, m, img);
+        if (c.gout) static_cast<T *>(c.gout)[size_t(img) * c.nx * c.ny + p] = T(dlogl_dm(c, p, double(m), img));
       }
     }
   }
@@ -738,8 +741,9 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int
         const size_t p = size_t(y) * c.nx + x;
         const float m = float(dacc[t][j] * unscale);
         if (out) out[size_t(img) * c.nx * c.ny + p] = m;
-        if (partial) chi += chi2_pixel<float>(c, p, m);
-        if (c.gout) static_cast<float *>(c.gout)[size_t(img) * c.nx * c.ny + p] = float(dlogl_dm(c, p, double(m)));
+        if (partial) chi += chi2_pixel<float>(c, p, m, img);
+        if (c.gout)
+          static_cast<float *>(c.gout)[size_t(img) * c.nx * c.ny + p] = float(dlogl_dm(c, p, double(m), img));
       }
     }
   }
@@ -1047,6 +1051,7 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     c.exposure_time = data->exposure_time;
     c.bkg2 = data->background_rms * data->background_rms;
     c.gout = gout;
+    c.dstride = data->image_stride;
     partial = reinterpret_cast<double *>(ws + L.partial);
   }
   int n_tiles;
@@ -1318,6 +1323,8 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
   if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
   if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
     return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (data->image_stride != 0 && data->image_stride < int64_t(g.nx) * g.ny)
+    return fail(LENSIM_EINVAL, "image_stride %lld is smaller than an image", (long long)data->image_stride);
   if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
   if (n_img == 0) return LENSIM_OK;
   if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
@@ -1359,6 +1366,8 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
   if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
     return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (data->image_stride != 0 && data->image_stride < int64_t(g.nx) * g.ny)
+    return fail(LENSIM_EINVAL, "image_stride %lld is smaller than an image", (long long)data->image_stride);
   if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
   if (n_img == 0) return LENSIM_OK;
   if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");

@@ -1,0 +1,375 @@
+"""Request server: continuous batching of simulate / log-likelihood requests.
+
+A *bucket* is one request class: an ImageModel configuration (grid,
+supersampling, PSF, profile lists, precision).  It owns
+
+* a pool of resident data slots: a log-likelihood request names the slot that
+  holds its observation, which stays on the device between requests;
+* ``n_lanes`` lanes, each a full set of static device buffers for up to
+  ``max_batch`` parameter sets (parameters, slot indices, gathered data,
+  workspace, results, pinned host staging) and one captured HIP graph per
+  (request kind, padded batch size): the gather of the batch's observations
+  plus the library's launch sequence, replayed for every batch.
+
+The scheduler keeps up to ``n_streams`` batches in flight.  Whenever a stream
+and a lane of the oldest waiting bucket are free, it admits every request of
+that bucket that has arrived so far (up to ``max_batch``), so requests that
+arrive while other batches run join the next batch, and batches of different
+buckets (or lanes of one bucket) overlap on the device.  Results are copied to
+pinned host memory on the lane's stream before its completion event, so
+retiring a batch reads host memory only.
+
+Each image of a batch is computed from its own parameters (and observation)
+alone, so a request's result does not depend on the requests it was batched
+with; batches are padded to a power of two with copies of their first request
+so that a handful of graphs covers every batch size.
+
+Admission: every bucket's device memory is charged against an HBM budget at
+registration; ``put_data`` returns None while all of a bucket's slots are in
+use.  All device work is on streams of the default priority.
+"""
+
+import collections
+import ctypes
+import threading
+import time
+
+import numpy as np
+import torch
+
+from . import backend
+
+
+class Future(object):
+    """Result handle of one request."""
+
+    __slots__ = ("value", "t_submit", "t_done", "_event")
+
+    def __init__(self):
+        self.value = None
+        self.t_submit = time.perf_counter()
+        self.t_done = None
+        self._event = threading.Event()
+
+    def done(self):
+        return self._event.is_set()
+
+    def result(self, timeout=None):
+        if not self._event.wait(timeout):
+            raise TimeoutError("request not completed")
+        return self.value
+
+    @property
+    def latency(self):
+        """Seconds from submit() to completion."""
+        return None if self.t_done is None else self.t_done - self.t_submit
+
+    def _set(self, value):
+        self.value = value
+        self.t_done = time.perf_counter()
+        self._event.set()
+
+
+def _pow2_at_least(n, cap):
+    m = 1
+    while m < n:
+        m *= 2
+    return min(m, cap)
+
+
+class _Lane(object):
+    """One set of static buffers of a bucket (and the graphs captured on them)."""
+
+    def __init__(self, bucket):
+        b = bucket
+        dev = backend.device()
+        ny, nx = b.shape
+        mb = b.max_batch
+        self.busy = False
+        self.params = torch.zeros((mb, b.n_prof, backend.NPARAM), dtype=torch.float64, device=dev)
+        self.params_host = torch.zeros((mb, b.n_prof, backend.NPARAM), dtype=torch.float64).pin_memory()
+        self.slot_idx = torch.zeros(mb, dtype=torch.long, device=dev)
+        self.slot_host = torch.zeros(mb, dtype=torch.long).pin_memory()
+        self.batch_data = torch.zeros((mb, ny, nx), dtype=torch.float64, device=dev)
+        self.logl = torch.zeros(mb, dtype=torch.float64, device=dev)
+        self.logl_host = torch.zeros(mb, dtype=torch.float64).pin_memory()
+        self.images = torch.zeros((mb, ny, nx), dtype=b.dtype, device=dev)
+        self.images_host = torch.zeros((mb, ny, nx), dtype=b.dtype).pin_memory()
+        self.ws = torch.empty(b.ws_bytes, dtype=torch.uint8, device=dev)
+        self.data = backend.LensimData()
+        ctypes.memmove(ctypes.byref(self.data), ctypes.byref(b.data_template), ctypes.sizeof(self.data))
+        self.data.d_data = self.batch_data.data_ptr()
+        self.data.image_stride = nx * ny
+        self.graphs = {}
+        self.nbytes = (self.params.numel() * 8 + self.slot_idx.numel() * 8 + self.batch_data.numel() * 8 +
+                       self.logl.numel() * 8 + self.images.numel() * self.images.element_size() + b.ws_bytes)
+
+
+class _Bucket(object):
+    def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs):
+        if image_model._bulk:
+            raise NotImplementedError("the server takes ImageModels with a plain lens model list")
+        self.key = key
+        self.im = image_model
+        self.max_batch = int(max_batch)
+        if not 1 <= self.max_batch <= 65535:
+            raise ValueError("max_batch must be in [1, 65535]")
+        self.use_graphs = use_graphs
+        self.shape = (image_model._grid.ny, image_model._grid.nx)
+        self.n_prof = len(image_model._names)
+        self.dtype = torch.float64 if image_model._prec == backend.FP64 else torch.float32
+        dev = backend.device()
+        buf = image_model.Data.device_buffers()
+        if buf["noise"] is not None:
+            raise NotImplementedError("the server computes the variance from the model; the ImageData "
+                                      "must not carry a noise map")
+        self.mask = backend.to_device_f64(image_model.likelihood_mask.astype(np.float64))
+        self.expo_map = buf["expo_map"]
+        d = backend.LensimData()
+        d.d_mask = self.mask.data_ptr()
+        d.d_exposure_map = None if self.expo_map is None else self.expo_map.data_ptr()
+        d.exposure_time = buf["expo_time"]
+        d.background_rms = image_model.Data.background_rms
+        self.data_template = d
+        self.ws_bytes = max(image_model.workspace_bytes(self.max_batch), 256)
+        self.slots = torch.zeros((int(n_slots),) + self.shape, dtype=torch.float64, device=dev)
+        self.free_slots = list(range(int(n_slots)))[::-1]
+        self.lanes = [_Lane(self) for _ in range(int(n_lanes))]
+        self.nbytes = self.slots.numel() * 8 + sum(l.nbytes for l in self.lanes)
+
+    def free_lane(self):
+        for lane in self.lanes:
+            if not lane.busy:
+                return lane
+        return None
+
+    def _launch(self, lane, kind, n):
+        """The launch sequence of one batch on the current stream."""
+        im = self.im
+        lib = backend.lib()
+        st = backend.stream_handle()
+        if kind == "loglike":
+            torch.index_select(self.slots, 0, lane.slot_idx[:n], out=lane.batch_data[:n])
+            rc = lib.lensim_loglike(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
+                                    ctypes.byref(lane.data), backend.ptr(lane.params), n, im._prec,
+                                    backend.ptr(lane.logl), None, backend.ptr(lane.ws), lane.ws.numel(), st)
+            backend.check(rc, "lensim_loglike")
+        else:
+            rc = lib.lensim_image(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
+                                  backend.ptr(lane.params), n, im._prec,
+                                  backend.IMAGE_SOURCE | backend.IMAGE_LENS_LIGHT, backend.ptr(lane.images),
+                                  backend.ptr(lane.ws), lane.ws.numel(), st)
+            backend.check(rc, "lensim_image")
+
+    def run(self, lane, kind, n, stream):
+        """Launch a batch whose first n rows of the lane's host staging are filled;
+        returns the completion event (results are then in the lane's host buffers)."""
+        npad = _pow2_at_least(n, self.max_batch)
+        if npad > n:  # padding rows: copies of the first request, results ignored
+            lane.params_host[n:npad] = lane.params_host[0]
+            lane.slot_host[n:npad] = lane.slot_host[0]
+        with torch.cuda.stream(stream):
+            lane.params[:npad].copy_(lane.params_host[:npad], non_blocking=True)
+            lane.slot_idx[:npad].copy_(lane.slot_host[:npad], non_blocking=True)
+            if not self.use_graphs:
+                self._launch(lane, kind, npad)
+            else:
+                g = lane.graphs.get((kind, npad))
+                if g is None:
+                    self._launch(lane, kind, npad)  # warm-up outside the capture
+                    stream.synchronize()
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=stream):
+                        self._launch(lane, kind, npad)
+                    lane.graphs[(kind, npad)] = g
+                g.replay()
+            if kind == "loglike":
+                lane.logl_host[:n].copy_(lane.logl[:n], non_blocking=True)
+            else:
+                lane.images_host[:n].copy_(lane.images[:n], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return ev
+
+
+class LensServer(object):
+    """Continuous-batching server over registered request classes (buckets).
+
+    ``register(key, image_model)`` sets up a bucket; ``put_data`` makes an
+    observation resident; ``submit`` queues a request and returns a Future;
+    ``step()`` / ``drain()`` drive the scheduler from the caller's thread and
+    ``start()`` / ``stop()`` from a background thread.
+    """
+
+    def __init__(self, n_streams=4, hbm_budget=64 << 30, use_graphs=True):
+        backend.require_gpu()
+        backend.timing_enable(False)  # event timing synchronises; it cannot sit in a graph
+        self.streams = [torch.cuda.Stream() for _ in range(int(n_streams))]
+        self.free_streams = list(range(int(n_streams)))
+        self.hbm_budget = int(hbm_budget)
+        self.hbm_used = 0
+        self.use_graphs = use_graphs
+        self.buckets = {}
+        self.pending = collections.OrderedDict()  # key -> deque of (kind, params, slot, future)
+        self.inflight = []
+        self.lock = threading.Lock()
+        self.batches = 0
+        self.requests_done = 0
+        self.batch_sizes = collections.Counter()
+        self._thread = None
+        self._stop = False
+
+    # ---------------------------------------------------------- admission --
+    def register(self, key, image_model, max_batch=64, n_slots=16, n_lanes=2):
+        if key in self.buckets:
+            raise ValueError("bucket %r is already registered" % (key,))
+        b = _Bucket(key, image_model, max_batch, n_slots, n_lanes, self.use_graphs)
+        if self.hbm_used + b.nbytes > self.hbm_budget:
+            raise MemoryError("bucket %r needs %d B; %d of %d B already used" % (key, b.nbytes, self.hbm_used,
+                                                                              self.hbm_budget))
+        self.hbm_used += b.nbytes
+        self.buckets[key] = b
+        return b
+
+    def put_data(self, key, image):
+        """Make an observation resident; returns its slot id, or None when all of
+        the bucket's slots are in use."""
+        b = self.buckets[key]
+        img = np.asarray(image, dtype=np.float64)
+        if img.shape != b.shape:
+            raise ValueError("observation of shape %s for a %s bucket" % (img.shape, b.shape))
+        with self.lock:
+            if not b.free_slots:
+                return None
+            slot = b.free_slots.pop()
+        b.slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(img)))
+        torch.cuda.current_stream().synchronize()
+        return slot
+
+    def release_data(self, key, slot):
+        """Return a slot to the pool; requests naming it must have completed."""
+        with self.lock:
+            self.buckets[key].free_slots.append(int(slot))
+
+    def submit(self, key, params, kind="loglike", slot=None):
+        """Queue one request: ``params`` is a [n_profiles, 8] parameter array in
+        the ImageModel.pack layout; ``kind`` is "loglike" (needs ``slot``) or
+        "image".  Returns a Future whose value is the float logL or the
+        [ny, nx] image (a CPU tensor)."""
+        if kind not in ("loglike", "image"):
+            raise ValueError("kind must be 'loglike' or 'image'")
+        b = self.buckets[key]
+        if kind == "loglike":
+            if slot is None or not 0 <= int(slot) < b.slots.shape[0]:
+                raise ValueError("a log-likelihood request needs a valid data slot")
+        p = np.asarray(params, dtype=np.float64).reshape(b.n_prof, backend.NPARAM)
+        fut = Future()
+        with self.lock:
+            self.pending.setdefault(key, collections.deque()).append((kind, p, int(slot or 0), fut))
+        return fut
+
+    # ---------------------------------------------------------- scheduling --
+    def _retire(self):
+        still = []
+        for item in self.inflight:
+            ev, b, lane, kind, reqs, si = item
+            if not ev.query():
+                still.append(item)
+                continue
+            if kind == "loglike":
+                vals = lane.logl_host[:len(reqs)].numpy().copy()
+                for i, r in enumerate(reqs):
+                    r[3]._set(float(vals[i]))
+            else:
+                imgs = lane.images_host[:len(reqs)].clone()
+                for i, r in enumerate(reqs):
+                    r[3]._set(imgs[i])
+            lane.busy = False
+            self.free_streams.append(si)
+            self.requests_done += len(reqs)
+        self.inflight = still
+
+    def _dispatch(self):
+        launched = False
+        while self.free_streams:
+            with self.lock:
+                pick = None
+                for key, q in self.pending.items():
+                    if q:
+                        lane = self.buckets[key].free_lane()
+                        if lane is not None:
+                            pick = key
+                            break
+                if pick is None:
+                    return launched
+                q = self.pending[pick]
+                b = self.buckets[pick]
+                kind = q[0][0]
+                reqs = []
+                while q and len(reqs) < b.max_batch and q[0][0] == kind:
+                    reqs.append(q.popleft())
+                self.pending.move_to_end(pick)  # round-robin between buckets
+            for i, r in enumerate(reqs):
+                lane.params_host[i].copy_(torch.from_numpy(r[1]))
+                lane.slot_host[i] = r[2]
+            si = self.free_streams.pop(0)
+            lane.busy = True
+            ev = b.run(lane, kind, len(reqs), self.streams[si])
+            self.inflight.append((ev, b, lane, kind, reqs, si))
+            self.batches += 1
+            self.batch_sizes[len(reqs)] += 1
+            launched = True
+        return launched
+
+    def step(self):
+        """One scheduler iteration: retire finished batches, launch new ones.
+        Returns True when it launched a batch."""
+        self._retire()
+        return self._dispatch()
+
+    def idle(self):
+        with self.lock:
+            return not self.inflight and not any(self.pending.values())
+
+    def drain(self, timeout=120.0):
+        """Run the scheduler until every queued request has completed."""
+        t0 = time.perf_counter()
+        while not self.idle():
+            if not self.step():
+                time.sleep(20e-6)
+            if time.perf_counter() - t0 > timeout:
+                raise TimeoutError("server did not drain in %.0f s" % timeout)
+
+    def start(self):
+        """Run the scheduler in a background thread."""
+        if self._thread is not None:
+            return
+        self._stop = False
+        device = self.streams[0].device
+
+        def loop():
+            torch.cuda.set_device(device)
+            while not self._stop:
+                if not self.step():
+                    time.sleep(20e-6)
+
+        self._thread = threading.Thread(target=loop, daemon=True)
+        self._thread.start()
+
+    def stop(self):
+        """Stop the background scheduler and finish the batches in flight."""
+        self._stop = True
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+        for s in self.streams:
+            s.synchronize()
+        self._retire()
+
+
+def latency_percentiles(futures, qs=(50, 99)):
+    """Latency percentiles in milliseconds of completed futures."""
+    lat = np.array([f.latency for f in futures if f.latency is not None]) * 1e3
+    if lat.size == 0:
+        return {q: None for q in qs}
+    return {q: float(np.percentile(lat, q)) for q in qs}

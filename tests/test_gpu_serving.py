@@ -1,0 +1,156 @@
+"""Request server: each request's result is the same whether computed alone or
+batched with others (continuous batching, padding, graph replay), and graph
+replays with eager work between them match eager launches."""
+
+import numpy as np
+import pytest
+import torch
+
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.imaging_data import ImageData
+from jaxtronomy_amd.Data.psf import PSF
+from jaxtronomy_amd.ImSim.image_model import ImageModel
+from jaxtronomy_amd.LensModel.lens_model import LensModel
+from jaxtronomy_amd.LightModel.light_model import LightModel
+from jaxtronomy_amd.serving import LensServer, latency_percentiles
+
+pytestmark = pytest.mark.gpu
+
+
+def _image_model(precision, n=64, s=2, k=13):
+    cfg = dict(W.EPL180, numPix=n, supersampling=s)
+    kernel = W.moffat_kernel(k, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": s}, precision=precision)
+    return im, cfg
+
+
+def _requests(im, n_req, seed):
+    params = []
+    for d in range(n_req):
+        p, _ = im.pack(*W.epl180_draw(seed + d))
+        params.append(p[0])
+    return np.stack(params)
+
+
+def _observations(im, cfg, n_obs, seed):
+    obs = []
+    for d in range(n_obs):
+        p, _ = im.pack(*W.epl180_draw(1000 + seed + d))
+        clean = im.image_batch(torch.as_tensor(p, device="cuda")).double().cpu().numpy()[0]
+        obs.append(W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(seed + d)))
+    return obs
+
+
+def _alone_logl(im, p, obs):
+    data = torch.as_tensor(obs[None], device="cuda").contiguous()
+    return im.likelihood_batch(torch.as_tensor(p[None], device="cuda"), data=data).item()
+
+
+def _alone_image(im, p):
+    return im.image_batch(torch.as_tensor(p[None], device="cuda")).double().cpu()[0]
+
+
+@pytest.mark.parametrize("precision,use_graphs", [("fp64", True), ("mixed", True), ("mixed", False)])
+def test_mix_equals_alone(precision, use_graphs):
+    im, cfg = _image_model(precision)
+    srv = LensServer(n_streams=3, use_graphs=use_graphs)
+    srv.register("a", im, max_batch=16, n_slots=4, n_lanes=2)
+    obs = _observations(im, cfg, 3, 7)
+    slots = [srv.put_data("a", o) for o in obs]
+    params = _requests(im, 37, 11)
+    futs = []
+    # arrivals in uneven waves: the scheduler forms batches of varied sizes
+    # (padded to powers of two) while earlier batches are in flight
+    waves = [1, 5, 16, 3, 9, 2, 1]
+    i = 0
+    for w in waves:
+        for _ in range(w):
+            kind = "image" if i % 4 == 3 else "loglike"
+            futs.append((i, kind, srv.submit("a", params[i], kind=kind, slot=slots[i % 3])))
+            i += 1
+        srv.step()
+    assert i == len(params)
+    srv.drain()
+    assert srv.requests_done == len(params)
+    assert len(srv.batch_sizes) > 1  # really batched in different mixes
+    for i, kind, f in futs:
+        assert f.done()
+        if kind == "loglike":
+            ref = _alone_logl(im, params[i], obs[i % 3])
+            assert f.result() == ref, (i, f.result(), ref)
+        else:
+            ref = _alone_image(im, params[i])
+            assert torch.equal(f.result().double(), ref), i
+    p = latency_percentiles([f for _, _, f in futs])
+    assert p[50] is not None and p[99] >= p[50]
+
+
+def test_two_buckets_and_background_thread():
+    im64, cfg = _image_model("fp64", n=48, s=1, k=9)
+    imx, _ = _image_model("mixed", n=64, s=2, k=13)
+    srv = LensServer(n_streams=4)
+    srv.register("fp64", im64, max_batch=8, n_slots=2)
+    srv.register("mixed", imx, max_batch=8, n_slots=2)
+    o64 = _observations(im64, cfg, 1, 3)[0]
+    ox = _observations(imx, cfg, 1, 4)[0]
+    s64 = srv.put_data("fp64", o64)
+    sx = srv.put_data("mixed", ox)
+    p64 = _requests(im64, 20, 21)
+    px = _requests(imx, 20, 41)
+    srv.start()
+    try:
+        f64 = [srv.submit("fp64", p, slot=s64) for p in p64]
+        fx = [srv.submit("mixed", p, slot=sx) for p in px]
+        r64 = [f.result(timeout=60) for f in f64]
+        rx = [f.result(timeout=60) for f in fx]
+    finally:
+        srv.stop()
+    for p, r in zip(p64, r64):
+        assert r == _alone_logl(im64, p, o64)
+    for p, r in zip(px, rx):
+        assert r == _alone_logl(imx, p, ox)
+
+
+def test_slot_pool_and_budget():
+    im, cfg = _image_model("mixed", n=32, s=1, k=5)
+    srv = LensServer(n_streams=1)
+    b = srv.register("a", im, max_batch=4, n_slots=2, n_lanes=1)
+    z = np.zeros((32, 32))
+    s0, s1 = srv.put_data("a", z), srv.put_data("a", z)
+    assert {s0, s1} == {0, 1} and srv.put_data("a", z) is None
+    srv.release_data("a", s0)
+    assert srv.put_data("a", z) == s0
+    with pytest.raises(ValueError):
+        srv.submit("a", _requests(im, 1, 0)[0], kind="loglike", slot=None)
+    with pytest.raises(ValueError):
+        srv.put_data("a", np.zeros((31, 32)))
+    with pytest.raises(MemoryError):
+        LensServer(n_streams=1, hbm_budget=b.nbytes - 1).register("a", im, max_batch=4, n_slots=2, n_lanes=1)
+
+
+def test_graph_replays_with_eager_work_between():
+    """Every replay of a captured batch matches the eager launch, with unrelated
+    eager device work and synchronisations between the replays."""
+    im, cfg = _image_model("mixed")
+    obs = _observations(im, cfg, 2, 5)
+    params = _requests(im, 8, 77)
+    eager = LensServer(n_streams=1, use_graphs=False)
+    eager.register("a", im, max_batch=8, n_slots=2, n_lanes=1)
+    graph = LensServer(n_streams=1, use_graphs=True)
+    graph.register("a", im, max_batch=8, n_slots=2, n_lanes=1)
+    for srv in (eager, graph):
+        for o in obs:
+            srv.put_data("a", o)
+    ref = [eager.submit("a", p, slot=j % 2) for j, p in enumerate(params)]
+    eager.drain()
+    ref = [f.result() for f in ref]
+    scratch = torch.randn(1 << 20, device="cuda")
+    for rep in range(4):
+        futs = [graph.submit("a", p, slot=j % 2) for j, p in enumerate(params)]
+        graph.drain()
+        assert [f.result() for f in futs] == ref, rep
+        scratch = (scratch * 1.0001 + 1.0).sum() + scratch  # eager work between replays
+        torch.cuda.synchronize()
+    assert len(graph.buckets["a"].lanes[0].graphs) == 1
