@@ -62,12 +62,12 @@ def build_models(batch):
     return cfg, kernel, draws, data, params, params_np, im64, im32
 
 
-def time_steps(im, params, steps, warmup, backend):
+def time_steps(im, params, steps, warmup, backend, model=None):
     import torch
 
     logl = torch.empty(params.shape[0], dtype=torch.float64, device=params.device)
     for _ in range(warmup):
-        im.likelihood_batch(params, logl=logl)
+        im.likelihood_batch(params, logl=logl, model=model)
     torch.cuda.synchronize()
     backend.timing_read()  # drop warm-up records
     backend.timing_enable(True)
@@ -76,7 +76,7 @@ def time_steps(im, params, steps, warmup, backend):
     t0 = time.perf_counter()
     evs[0].record()
     for i in range(steps):
-        im.likelihood_batch(params, logl=logl)
+        im.likelihood_batch(params, logl=logl, model=model)
         evs[i + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -88,11 +88,11 @@ def time_steps(im, params, steps, warmup, backend):
                 conv_ms=conv_ms / max(calls, 1), logl=logl.cpu().numpy())
 
 
-def render_work(params_np, n_ray):
+def render_work(params_np, n_ray, lens_light=True):
     """Algorithmic flops and special-function evaluations of one render launch:
     per ray EPL 72 + 8 (N_terms - 1) flops and 6 specials (N_terms = the series
     length chosen from the axis ratio), SHEAR 10, deflected position 2, two
-    SERSIC_ELLIPSE 17 + 4 specials each, pooling 1."""
+    SERSIC_ELLIPSE (source and lens light) 17 + 4 specials each, pooling 1."""
     from jaxtronomy_amd.Util.param_util import ellipticity2phi_q
 
     F = S = 0.0
@@ -101,8 +101,9 @@ def render_work(params_np, n_ray):
         f = (1 - q) / (1 + q)
         nterms = 1 if f == 0 else (10 if f < 0.12 else 20 if f < 0.4 else 35 if f < 0.6 else 50 if f < 0.73
                                    else 100 if f < 0.86 else 200 if f < 0.92 else 500)
-        F += n_ray * (72 + 8 * (nterms - 1) + 10 + 2 + 17 + 17 + 1)
-        S += n_ray * (6 + 4 + 4)
+        nl = 2 if lens_light else 1
+        F += n_ray * (72 + 8 * (nterms - 1) + 10 + 2 + 17 * nl + 1)
+        S += n_ray * (6 + 4 * nl)
     return F, S
 
 
@@ -124,6 +125,116 @@ def cpu_baseline(cfg, kernel, params_np, data, n_sample):
                        "algorithm, OpenMP, %.1f s wall)" % (n_sample, dt)), ll
 
 
+def build_substructure(batch, n_sub):
+    import torch
+
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+
+    cfg = dict(W.SUBSTRUCTURE, n_sub=n_sub)
+    kernel = W.moffat_kernel(cfg["psf_size"], cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    draws = [W.substructure_draw(d, cfg=cfg) for d in range(batch)]
+    names = ["EPL", "SHEAR"] + ["TNFW"] * n_sub
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModelBulk(["EPL", "SHEAR", "TNFW"], precision=precision),
+                          LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+
+    probe = model("fp64")
+    kl0, subs0, ks0 = draws[0]
+    truth = probe.image(probe.LensModel.prepare_ray_shooting_kwargs(names, kl0 + subs0), ks0)
+    data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
+    im64 = model("fp64", data)
+    im32 = model("mixed", data)
+    kl = W.stack_kwargs([d[0] + d[1] for d in draws])
+    ks = W.stack_kwargs([d[2] for d in draws])
+    rsk = im64.LensModel.prepare_ray_shooting_kwargs(names, kl)
+    params_np, n, m, keep = im64.pack_bulk(rsk, ks)
+    params = torch.as_tensor(params_np, device="cuda")
+    return cfg, kernel, names, data, params, params_np, (m, keep), im64, im32
+
+
+def substructure_work(params_np, n_ray, n_sub):
+    """Per ray: EPL 72 + 8 (N_terms - 1) flops and 6 specials, SHEAR 10, n_sub
+    TNFW at 36 flops and 7 specials each, deflected position 2,
+    SERSIC_ELLIPSE 17 + 4 specials, pooling 1."""
+    F, S = render_work(params_np, n_ray, lens_light=False)
+    n_img = params_np.shape[0]
+    return F + n_img * n_ray * 36.0 * n_sub, S + n_img * n_ray * 7.0 * n_sub
+
+
+def main_substructure(args):
+    import torch
+
+    from jaxtronomy_amd import backend
+
+    log("bench: building substructure models (%d subhalos, batch %d)" % (args.n_sub, args.batch))
+    cfg, kernel, names, data, params, params_np, mk, im64, im32 = build_substructure(args.batch, args.n_sub)
+    m = mk[0]
+    n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
+    log("bench: timing mixed precision (%d steps)" % args.steps)
+    mixed = time_steps(im32, params, args.steps, args.warmup, backend, model=m)
+    log("bench: timing fp64 (%d steps)" % args.fp64_steps)
+    fp64 = time_steps(im64, params, args.fp64_steps, max(1, args.warmup // 2), backend, model=m)
+    ll_rel = float(np.max(np.abs(mixed["logl"] - fp64["logl"]) / np.abs(fp64["logl"])))
+    value = args.batch * args.steps / (mixed["total_ms"] / 1e3)
+    value64 = args.batch * args.fp64_steps / (fp64["total_ms"] / 1e3)
+    F, S = substructure_work(params_np, n_ray, args.n_sub)
+    achieved = (F + 8 * S) / (mixed["render_ms"] / 1e3) / 1e12
+    achieved64 = (F + 20 * S) / (fp64["render_ms"] / 1e3) / 1e12
+    line = {
+        "metric": "realisations/s, EPL+SHEAR + %d TNFW subhalos, 300^2 ss3, mixed precision" % args.n_sub,
+        "value": round(value, 2),
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["total_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "EPL+SHEAR + %d TNFW (log10 M ~ U[6,9], positions U[-3,3]\", r_trunc = 5 Rs), "
+                               "SERSIC_ELLIPSE source, 300x300 @0.03\", supersampling 3, 41x41 Moffat PSF, "
+                               "image + chi2 logL" % args.n_sub, "batch": args.batch},
+        "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
+        "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4),
+        "fp64": {"value": round(value64, 2), "ms_per_step": round(fp64["total_ms"] / args.fp64_steps, 4),
+                 "render_ms": round(fp64["render_ms"], 4), "render_tflops_equiv": round(achieved64, 3)},
+        "accuracy": {"mixed_logl_max_rel_vs_fp64": ll_rel},
+        "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4)},
+        "roofline": {"bound": "valu", "kernel": "render_kernel<float>", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None,
+                     "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S)},
+    }
+    if not args.no_cpu:
+        from oracle import c_oracle
+        from jaxtronomy_amd import workloads as W
+
+        threads = min(16, len(os.sched_getaffinity(0)))
+        g = W.grid_dict(cfg)
+        types = [1, 6] + [5] * args.n_sub + [33]
+        cp = params_np[:1]  # [EPL, SHEAR] inline, TNFW bulk, source: the oracle's lens-list order
+        c_oracle.lib()
+        t0 = time.perf_counter()
+        ll = c_oracle.loglike(2 + args.n_sub, 1, 0, types, cp, g, data, kernel=kernel, psf_type=1,
+                              exposure_time=cfg["exposure_time"], background_rms=cfg["background_rms"],
+                              n_threads=threads)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = dict(value=1.0 / dt, unit="images/s", cores=threads, kind="port",
+                                    sample="1 log-likelihood evaluation (C FP64 restatement, OpenMP, %.1f s)" % dt)
+        line["accuracy"]["fp64_logl_rel_vs_cpu_oracle"] = float(abs(fp64["logl"][0] - ll[0]) / abs(ll[0]))
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +244,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("epl180", "substructure"), default="epl180")
+    ap.add_argument("--n-sub", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a design "
@@ -145,6 +258,8 @@ def main():
 
     backend.require_gpu()
     torch.cuda.set_device(0)
+    if args.workload == "substructure":
+        return main_substructure(args)
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2

@@ -148,7 +148,8 @@ class ImageModel(object):
     def pack_bulk(self, kwargs_lens, kwargs_source=None, kwargs_lens_light=None):
         """For a LensModelBulk deflector: (params [B, n, 8], B, model, keep-alive) where
         kwargs_lens is the dict of LensModelBulk.prepare_ray_shooting_kwargs."""
-        n_img, it, ip, bt, bp = self.LensModel.pack(kwargs_lens)
+        n_img, it, ip, bt, bp = self.LensModel.pack(
+            kwargs_lens, max_inline=backend.MAX_PROFILES - len(self._src_types) - len(self._light_types))
         ks = kwargs_source or []
         kll = kwargs_lens_light or []
         nb = profiles.batch_size([ks, kll])

@@ -9,7 +9,10 @@ deflectors inside one thread per ray and keeps the deflection sum in
 registers (float64), instead of stacking one deflection field per deflector.
 
 EPL entries run as the kernel's inline deflectors (they carry per-image series
-coefficients; at most 64); all other profiles go to the bulk list.  A kwargs
+coefficients).  Of the other profiles, the most frequent type goes to the bulk
+list and the rest run inline too while they fit (at most 64 inline profiles,
+light profiles included), so that a substructure list (a main deflector plus
+many subhalos of one type) keeps the kernel's single-type bulk loop.  A kwargs
 array may also be 2-D, [B, n_deflectors], for B models at once.
 """
 
@@ -66,8 +69,10 @@ class LensModelBulk(object):
         return {"all_kwargs": all_kwargs, "index_list": np.array(index_list)}
 
     # ------------------------------------------------------------ packing --
-    def pack(self, ray_shooting_kwargs):
+    def pack(self, ray_shooting_kwargs, max_inline=None):
         """(n_img, inline types, inline params [B][n_inline][8], bulk types, bulk params [B][n_bulk][8])."""
+        if max_inline is None:
+            max_inline = backend.MAX_PROFILES
         all_kwargs = ray_shooting_kwargs["all_kwargs"]
         index_list = np.asarray(ray_shooting_kwargs["index_list"], dtype=int)
         n_def = len(index_list)
@@ -82,22 +87,28 @@ class LensModelBulk(object):
             a = np.asarray(v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else v, dtype=np.float64)
             cols[k] = np.broadcast_to(a, (n_img, n_def)) if a.ndim == 2 else np.broadcast_to(a, (n_img, n_def))
         n_unique = len(self.unique_lens_model_list)
-        inline_idx, bulk_idx = [], []
-        for d, u in enumerate(index_list):
-            if u < n_unique and self.unique_lens_model_list[u] == "EPL":
-                inline_idx.append(d)
-            else:
-                bulk_idx.append(d)
-        if len(inline_idx) > backend.MAX_PROFILES:
-            raise ValueError("at most %d EPL deflectors per bulk model" % backend.MAX_PROFILES)
+        # NULL padding deflects by zero: it is dropped
+        names = [self.unique_lens_model_list[u] if u < n_unique else None for u in index_list]
+        inline_idx = [d for d, nm in enumerate(names) if nm == "EPL"]
+        rest = [d for d, nm in enumerate(names) if nm not in ("EPL", None)]
+        if len(inline_idx) > max_inline:
+            raise ValueError("at most %d EPL deflectors per bulk model" % max_inline)
+        counts = {}
+        for d in rest:
+            counts[names[d]] = counts.get(names[d], 0) + 1
+        bulk_idx = rest
+        if counts:
+            major = max(sorted(counts), key=lambda nm: counts[nm])
+            minor = [d for d in rest if names[d] != major]
+            if len(inline_idx) + len(minor) <= max_inline:
+                inline_idx = sorted(inline_idx + minor)
+                bulk_idx = [d for d in rest if names[d] == major]
 
         def block(idx):
             p = np.zeros((n_img, len(idx), backend.NPARAM))
             types = np.zeros(len(idx), dtype=np.int32)
             for j, d in enumerate(idx):
                 u = index_list[d]
-                if u >= n_unique:
-                    continue  # NULL padding
                 name = self.unique_lens_model_list[u]
                 types[j] = profiles.LENS_PROFILES[name][0]
                 for i, pn in enumerate(self.param_name_list[u]):

@@ -100,3 +100,75 @@ def noisy(image, exposure_time, background_rms, rng):
     img = np.asarray(image, dtype=np.float64)
     counts = rng.poisson(np.maximum(img, 0) * exposure_time)
     return counts / exposure_time + rng.normal(0, background_rms, img.shape)
+
+
+# ---------------------------------------------------------------------------
+# Substructure workload: EPL + SHEAR main deflector plus n_sub TNFW subhalos,
+# SERSIC_ELLIPSE source, 300x300 at 0.03"/pix, supersampling 3, 41x41 Moffat.
+SUBSTRUCTURE = dict(numPix=300, deltaPix=0.03, supersampling=3, psf_size=41, psf_fwhm=0.08, psf_beta=3.0,
+                    exposure_time=1000.0, background_rms=0.01, n_sub=1000, z_lens=0.5, z_source=1.5)
+
+# Flat LambdaCDM (H0 = 70 km/s/Mpc, Omega_m = 0.3) and physical constants
+_H0, _OM = 70.0, 0.3
+_C_KMS = 299792.458
+_RHO_CRIT0 = 2.77536627e11 * (_H0 / 100.0) ** 2  # M_sun / Mpc^3
+_C2_4PIG = 1.6624541e18  # c^2 / (4 pi G) in M_sun / Mpc
+_ARCSEC = np.pi / 180.0 / 3600.0
+
+
+def _comoving(z1, z2, n=4096):
+    """Comoving distance between redshifts z1 < z2 (Mpc), Simpson's rule."""
+    z = np.linspace(z1, z2, n + 1)
+    f = 1.0 / np.sqrt(_OM * (1 + z) ** 3 + 1 - _OM)
+    h = (z2 - z1) / n
+    return _C_KMS / _H0 * h / 3.0 * (f[0] + f[-1] + 4 * f[1:-1:2].sum() + 2 * f[2:-1:2].sum())
+
+
+def nfw_physical_to_angle(m200, c, z_lens, z_source):
+    """(Rs, alpha_Rs) in arcsec of an NFW halo of mass M200 (M_sun) and
+    concentration c: r200 from 200 x the critical density at z_lens,
+    rho0 = 200/3 rho_c c^3 / (ln(1+c) - c/(1+c)), and
+    alpha_Rs = 4 rho0 Rs^2 (1 + ln 1/2) / Sigma_crit, in angles."""
+    dd = _comoving(0.0, z_lens) / (1 + z_lens)
+    ds = _comoving(0.0, z_source) / (1 + z_source)
+    dds = _comoving(z_lens, z_source) / (1 + z_source)
+    sigma_crit = _C2_4PIG * ds / (dd * dds)
+    rho_c = _RHO_CRIT0 * (_OM * (1 + z_lens) ** 3 + 1 - _OM)
+    m200 = np.asarray(m200, dtype=np.float64)
+    c = np.asarray(c, dtype=np.float64)
+    r200 = (3.0 * m200 / (4.0 * np.pi * 200.0 * rho_c)) ** (1.0 / 3.0)
+    rs = r200 / c
+    rho0 = 200.0 / 3.0 * rho_c * c ** 3 / (np.log(1 + c) - c / (1 + c))
+    alpha_rs = rho0 * 4.0 * rs ** 2 * (1.0 + np.log(0.5)) / sigma_crit
+    return rs / dd / _ARCSEC, alpha_rs / dd / _ARCSEC
+
+
+def concentration(m200):
+    """Fixed concentration-mass relation c = 15 (M / 1e8 M_sun)^-0.1."""
+    return 15.0 * (np.asarray(m200, dtype=np.float64) / 1e8) ** -0.1
+
+
+def substructure_draw(draw, n_sub=None, cfg=None):
+    """One seeded substructure scene: (kwargs_lens main [EPL, SHEAR], subhalo
+    kwargs list [TNFW] * n_sub, kwargs_source).  log10 M ~ U[6, 9], positions
+    U[-3, 3]" squared, r_trunc = 5 Rs."""
+    cfg = SUBSTRUCTURE if cfg is None else cfg
+    n_sub = cfg["n_sub"] if n_sub is None else n_sub
+    rng = np.random.default_rng(seed(5, draw))
+    kwargs_lens = [
+        dict(theta_E=rng.uniform(0.8, 1.6), gamma=rng.uniform(1.8, 2.2), e1=rng.normal(0, 0.1),
+             e2=rng.normal(0, 0.1), center_x=0.0, center_y=0.0),
+        dict(gamma1=rng.normal(0, 0.03), gamma2=rng.normal(0, 0.03), ra_0=0.0, dec_0=0.0),
+    ]
+    kwargs_source = [
+        dict(amp=rng.uniform(1, 100), R_sersic=rng.uniform(0.1, 0.5), n_sersic=rng.uniform(1, 4),
+             e1=rng.normal(0, 0.1), e2=rng.normal(0, 0.1), center_x=rng.uniform(-0.1, 0.1),
+             center_y=rng.uniform(-0.1, 0.1)),
+    ]
+    m = 10.0 ** rng.uniform(6.0, 9.0, n_sub)
+    cx = rng.uniform(-3.0, 3.0, n_sub)
+    cy = rng.uniform(-3.0, 3.0, n_sub)
+    rs, alpha_rs = nfw_physical_to_angle(m, concentration(m), cfg["z_lens"], cfg["z_source"])
+    subs = [dict(Rs=float(rs[i]), alpha_Rs=float(alpha_rs[i]), r_trunc=float(5.0 * rs[i]),
+                 center_x=float(cx[i]), center_y=float(cy[i])) for i in range(n_sub)]
+    return kwargs_lens, subs, kwargs_source

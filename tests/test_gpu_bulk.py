@@ -99,3 +99,66 @@ def test_substructure_image(precision, tol):
     got = _np(im.image(lm.prepare_ray_shooting_kwargs(names, kl + subs, num_deflectors=256), ks))
     want = O.image(g, names, kl + subs, ["SERSIC_ELLIPSE"], ks, [], [], kernel=kernel)
     assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
+
+
+def _substructure_model(cfg, precision, data=None, kernel=None):
+    return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                      LensModelBulk(["EPL", "SHEAR", "TNFW"], precision=precision), LightModel(["SERSIC_ELLIPSE"]),
+                      kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-5)])
+def test_substructure_workload_vs_oracle(precision, tol):
+    """The substructure workload's scene (EPL + SHEAR + 1000 TNFW from the mass
+    function), on a 100x100 0.06"/pix s=2 grid covering the subhalo field:
+    image and logL against the C oracle."""
+    from oracle import c_oracle
+
+    cfg = dict(W.SUBSTRUCTURE, numPix=100, deltaPix=0.06, supersampling=2)
+    kernel = W.moffat_kernel(21, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    kl, subs, ks = W.substructure_draw(3)
+    names = ["EPL", "SHEAR"] + ["TNFW"] * len(subs)
+    g = W.grid_dict(cfg)
+    types = [1, 6] + [5] * len(subs) + [33]
+    im = _substructure_model(cfg, precision, kernel=kernel)
+    rs = im.LensModel.prepare_ray_shooting_kwargs(names, kl + subs, num_deflectors=1024)
+    params, n, m, keep = im.pack_bulk(rs, ks)
+    assert m.bulk_type == 5 and m.n_bulk == len(subs) and m.n_lens == 2  # single-type bulk loop
+    want = c_oracle.image(2 + len(subs), 1, 0, types, params, g, kernel=kernel)[0]
+    got = _np(im.image(rs, ks)).astype(np.float64)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
+    data = W.noisy(want, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(8))
+    im_d = _substructure_model(cfg, precision, data=data, kernel=kernel)
+    ll = float(im_d.likelihood_data_given_model(rs, ks))
+    ll_want = c_oracle.loglike(2 + len(subs), 1, 0, types, params, g, data, kernel=kernel,
+                               exposure_time=cfg["exposure_time"], background_rms=cfg["background_rms"])[0]
+    assert abs(ll - ll_want) / abs(ll_want) < (1e-10 if precision == "fp64" else 1e-6)
+
+
+def test_substructure_full_size_mixed_vs_fp64():
+    """Full substructure workload (300x300 s=3, 1000 subhalos, 41x41 PSF), two
+    realisations: the mixed form's images and logL against the FP64 form's
+    (whose formulas the reduced-grid test pins to the oracle)."""
+    import torch
+
+    cfg = W.SUBSTRUCTURE
+    kernel = W.moffat_kernel(cfg["psf_size"], cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    draws = [W.substructure_draw(d) for d in (0, 1)]
+    names = ["EPL", "SHEAR"] + ["TNFW"] * cfg["n_sub"]
+    im64 = _substructure_model(cfg, "fp64", kernel=kernel)
+    kl = W.stack_kwargs([d[0] + d[1] for d in draws])
+    ks = W.stack_kwargs([d[2] for d in draws])
+    rs = im64.LensModel.prepare_ray_shooting_kwargs(names, kl)
+    params, n, m, keep = im64.pack_bulk(rs, ks)
+    assert n == 2
+    pt = torch.as_tensor(params, device="cuda")
+    img64 = im64.image_batch(pt, model=m).cpu().numpy()
+    data = W.noisy(img64[0], cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(2))
+    im64d = _substructure_model(cfg, "fp64", data=data, kernel=kernel)
+    im32d = _substructure_model(cfg, "mixed", data=data, kernel=kernel)
+    img32 = im32d.image_batch(pt, model=m).double().cpu().numpy()
+    for i in range(2):
+        assert np.max(np.abs(img32[i] - img64[i])) / np.max(np.abs(img64[i])) < 1e-5
+    ll64 = im64d.likelihood_batch(pt, model=m).cpu().numpy()
+    ll32 = im32d.likelihood_batch(pt, model=m).cpu().numpy()
+    assert np.all(np.abs(ll32 - ll64) / np.abs(ll64) < 1e-6), (ll32, ll64)
