@@ -152,9 +152,9 @@ def build_substructure(batch, n_sub):
     data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
     im64 = model("fp64", data)
     im32 = model("mixed", data)
-    kl = W.stack_kwargs([d[0] + d[1] for d in draws])
+    lm = im64.LensModel
+    rsk = lm.stack_ray_shooting_kwargs([lm.prepare_ray_shooting_kwargs(names, d[0] + d[1]) for d in draws])
     ks = W.stack_kwargs([d[2] for d in draws])
-    rsk = im64.LensModel.prepare_ray_shooting_kwargs(names, kl)
     params_np, n, m, keep = im64.pack_bulk(rsk, ks)
     params = torch.as_tensor(params_np, device="cuda")
     return cfg, kernel, names, data, params, params_np, (m, keep), im64, im32

@@ -68,6 +68,19 @@ class LensModelBulk(object):
             all_kwargs[kwarg] = np.array(vals)
         return {"all_kwargs": all_kwargs, "index_list": np.array(index_list)}
 
+    @staticmethod
+    def stack_ray_shooting_kwargs(prepared):
+        """Stacks prepare_ray_shooting_kwargs results of B models with the same
+        index list into one with [B, n_deflectors] arrays (one batch)."""
+        index_list = np.asarray(prepared[0]["index_list"])
+        for r in prepared[1:]:
+            if not np.array_equal(np.asarray(r["index_list"]), index_list):
+                raise ValueError("stacked models must share the deflector index list")
+        keys = prepared[0]["all_kwargs"].keys()
+        all_kwargs = {k: np.stack([np.asarray(r["all_kwargs"][k], dtype=np.float64) for r in prepared])
+                      for k in keys}
+        return {"all_kwargs": all_kwargs, "index_list": index_list}
+
     # ------------------------------------------------------------ packing --
     def pack(self, ray_shooting_kwargs, max_inline=None):
         """(n_img, inline types, inline params [B][n_inline][8], bulk types, bulk params [B][n_bulk][8])."""

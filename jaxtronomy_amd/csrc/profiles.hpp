@@ -75,7 +75,10 @@ enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX,
               EPL_LOG2B,  // log2 b
               EPL_NF,     // series length of the float form (truncation below 1e-8)
               EPL_F2,     // f^2 = |z|^2
-              EPL_LNB };  // ln b
+              EPL_LNB,    // ln b
+              EPL_TM1,    // t - 1
+              EPL_PB };   // 2 b / (1 + q)
+static_assert(EPL_PB < kRec, "EPL record exceeds kRec");
 // NIE/SIE (nie.py:182-200, :280-295): cos, sin, q (clamped), q^2, s, b/sqrt(1-q^2),
 //          sqrt(1-q^2), q^2 s
 enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S };
@@ -83,7 +86,9 @@ enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S }
 enum : int { NFW_RS = 2, NFW_A, NFW_IRS };
 // TNFW    (tnfw.py:67-87, :226-248, :329-349): Rs, 4 rho0 Rs, tau, tau^2,
 //          tau^2/(tau^2+1)^2, tau pi + (tau^2-1) ln tau, (tau^2-1)/tau, 0.001 Rs, tau^2+1, 1/Rs
-enum : int { TN_RS = 2, TN_A, TN_TAU, TN_TAU2, TN_PRE, TN_C0, TN_C1, TN_RMIN, TN_T2P1, TN_IRS };
+enum : int { TN_RS = 2, TN_A, TN_TAU, TN_TAU2, TN_PRE, TN_C0, TN_C1, TN_RMIN, TN_T2P1, TN_IRS,
+              TN_K,      // pi (tau^2 + 1)^2 / 2
+              TN_T2M1 }; // tau^2 - 1
 // SHEAR   (shear.py:37-53): ra_0, dec_0 in the centre slots, gamma1, gamma2
 enum : int { SH_G1 = 2, SH_G2 };
 // SERSIC  (sersic_ellipse.py:62-93, sersic_utils.py:83-110, :239-267,
@@ -147,6 +152,8 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[EPL_LOG2B] = log2(b);
       rec[EPL_F2] = f * f;
       rec[EPL_LNB] = log(b);
+      rec[EPL_TM1] = t - 1.0;
+      rec[EPL_PB] = 2.0 * b / (1.0 + q);
       rec[EPL_NF] = (double)nmax;
       if (coef) {
         const double bb = t / 2.0, cc = 2.0 - bb;
@@ -181,7 +188,7 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
         int nf = nmax;
         if (wmax > 0.0) {
           double tail = cmax / (1.0 - wmax);
-          for (nf = 1; nf < nmax && tail >= 1e-8; ++nf) tail *= wmax;
+          for (nf = 1; nf < nmax && tail >= 1e-9; ++nf) tail *= wmax;
         }
         rec[EPL_NF] = (double)nf;
       }
@@ -239,6 +246,8 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[TN_RMIN] = 0.001 * Rs;
       rec[TN_T2P1] = tau2 + 1.0;
       rec[TN_IRS] = 1.0 / Rs;
+      rec[TN_K] = 0.5 * M_PI * (tau2 + 1.0) * (tau2 + 1.0);
+      rec[TN_T2M1] = tau2 - 1.0;
       break;
     }
     case LENSIM_SHEAR: {
@@ -367,27 +376,30 @@ __device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double 
   ay += fr * s + fi * c;
 }
 
-// EPL, float form: the same formulas with |z| = f used exactly, the series
-// truncated at EPL_NF terms (tail below 1e-8 of the sum), hardware
-// reciprocal / log2 / exp2, and R^2 in place of R (R = max(|Z|, 1e-9) <=>
-// R^2 = max(|Z|^2, 1e-18)).  Long series (nmax >= 100) run in double.
+// EPL, mixed form.  The geometry runs in double (rotation, Z, 1/|Z| by a
+// float rsqrt and one Newton step, the final products); float carries only
+// the parts whose size bounds their error:
+//  * delta = 2F1 - 1 = (S - 1 + z/2) / (1 - z/2), S - 1 = w sum_{n>=1} F_n w^(n-1),
+//    of size ~ f t / (4 - t), from the series truncated at EPL_NF terms
+//    (tail below 1e-9);
+//  * (b/R)^(t-1) = 1 + expm1((t - 1) ln(b/R)), |t - 1| small.
+// Then alpha = 2b/(1+q) (b/R)^(t-1) (Z/R) (1 + delta), so its relative error
+// is about eps (|delta| + |(t-1) ln(b/R)|) instead of several float ulps.
+// Long series (nmax >= 100) run in double.
 __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *__restrict__ coef_f,
                                              const double *__restrict__ coef_d, double x, double y, double &ax,
                                              double &ay) {
   const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
   const double cd = r.d[EPL_C], sd = r.d[EPL_S];
-  const float xr = float(dx * cd + dy * sd);
-  const float yr = float(-dx * sd + dy * cd);
-  const float q = r[EPL_Q], f = r[EPL_F];
-  const float Zr = q * xr, Zi = yr;
-  const float R2 = fmaf(Zr, Zr, Zi * Zi);
-  float Or, Oi;
+  const double Zrd = r.d[EPL_Q] * fma(dx, cd, dy * sd);
+  const double Zid = fma(-dx, sd, dy * cd);
+  const double R2d = fma(Zrd, Zrd, Zid * Zid);
+  const float Zr = float(Zrd), Zi = float(Zid), R2 = float(R2d);
+  float dr = 0.f, di = 0.f;  // delta (0 for f == 0, and at the centre where the reference's NaN becomes 0)
   const int nf = (int)r[EPL_NF];
-  if (nf == 1 || R2 == 0.f) {
-    Or = Zr;  // f == 0, or the centre where the reference's NaN becomes 0
-    Oi = Zi;
-  } else {
+  if (nf > 1 && R2 > 0.f) {
     const float inv = __builtin_amdgcn_rcpf(R2);
+    const float f = r[EPL_F];
     const float zr = -f * (Zr * Zr - Zi * Zi) * inv;
     const float zi = -f * (2.f * Zr * Zi) * inv;
     // w = z / (z - 2) with |z|^2 = f^2:  w = (f^2 - 2 z, -2 Im z) / (f^2 - 4 Re z + 4)
@@ -395,30 +407,38 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
     const float dd = __builtin_amdgcn_rcpf(fmaf(-4.f, zr, f2 + 4.f));
     const float wr = fmaf(-2.f, zr, f2) * dd;
     const float wi = -2.f * zi * dd;
-    float Sr, Si;
+    float Pr, Pi;
     if (r.d[EPL_NMAX] >= 100.0) {
-      double Srd, Sid;
-      real_poly_at_complex<double>(coef_d, (int)r.d[EPL_NMAX], double(wr), double(wi), Srd, Sid);
-      Sr = float(Srd);
-      Si = float(Sid);
+      double Prd, Pid;
+      real_poly_at_complex<double>(coef_d + 1, (int)r.d[EPL_NMAX] - 1, double(wr), double(wi), Prd, Pid);
+      Pr = float(Prd);
+      Pi = float(Pid);
     } else {
-      real_poly_at_complex<float>(coef_f, nf, wr, wi, Sr, Si);
+      real_poly_at_complex<float>(coef_f + 1, nf - 1, wr, wi, Pr, Pi);
     }
-    // 2F1 = S / (1 - z/2)
+    // numerator S - 1 + z/2 = w P + z/2, denominator 1 - z/2
+    const float nr = fmaf(wr, Pr, fmaf(-wi, Pi, 0.5f * zr));
+    const float ni = fmaf(wr, Pi, fmaf(wi, Pr, 0.5f * zi));
     const float er = fmaf(-0.5f, zr, 1.f), ei = -0.5f * zi;
     const float ed = __builtin_amdgcn_rcpf(fmaf(er, er, ei * ei));
-    const float hr = fmaf(Sr, er, Si * ei) * ed;
-    const float hi = fmaf(Si, er, -Sr * ei) * ed;
-    Or = fmaf(Zr, hr, -Zi * hi);
-    Oi = fmaf(Zr, hi, Zi * hr);
+    dr = fmaf(nr, er, ni * ei) * ed;
+    di = fmaf(ni, er, -nr * ei) * ed;
   }
-  const float R2c = fmaxf(R2, 1e-18f);
-  const float lg = fmaf(-0.5f, __builtin_amdgcn_logf(R2c), r[EPL_LOG2B]);  // log2(b / R)
-  const float scale = r[EPL_PREF] * __builtin_amdgcn_exp2f(r[EPL_T] * lg);
-  const float fr = scale * Or, fi = scale * Oi;
-  const float c = r[EPL_C], s = r[EPL_S];
-  ax += double(fmaf(fr, c, -fi * s));
-  ay += double(fmaf(fr, s, fi * c));
+  // Omega = Z (1 + delta)
+  const double Or = Zrd + double(fmaf(Zr, dr, -Zi * di));
+  const double Oi = Zid + double(fmaf(Zr, di, Zi * dr));
+  // 1/R with R = max(|Z|, 1e-9)
+  const double R2c = R2d > 1e-18 ? R2d : 1e-18;
+  const float R2cf = float(R2c);
+  double iR = double(__builtin_amdgcn_rsqf(R2cf));
+  iR = iR * fma(-0.5 * R2c, iR * iR, 1.5);
+  // (b/R)^(t-1)
+  const float lnbr = fmaf(-0.34657359f, __builtin_amdgcn_logf(R2cf), r[EPL_LNB]);  // ln b - ln(R^2)/2
+  const float em = ::expm1f(r[EPL_TM1] * lnbr);
+  const double scale = r.d[EPL_PB] * iR * (1.0 + double(em));
+  const double fr = scale * Or, fi = scale * Oi;
+  ax += fr * cd - fi * sd;
+  ay += fr * sd + fi * cd;
 }
 
 // NIE / SIE deflection, LensModel/Profiles/nie.py:86-107, :280-295, :333-342.
@@ -539,6 +559,29 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
     const double a = r.d[TN_A] * g / X2;
     ax += a * double(dx);
     ay += a * double(dy);
+    return;
+  }
+  if (sizeof(T) == sizeof(float) && X >= T(1.5)) {
+    // Outside the scale radius (2 X^2 + tau^2 - 1) F(X) and -pi sqrt(tau^2 + X^2)
+    // both grow like pi X and cancel to O(1).  With v = sqrt(X^2 - 1),
+    // w = sqrt(X^2 + tau^2) and F = (pi/2 - atan(1/v)) / v, their sum is
+    //   pi (tau^2 + 1)^2 / (2 v (v + w)^2) - (2 X^2 + tau^2 - 1) atan(1/v) / v
+    // (from (w - v)^2 = (tau^2 + 1)^2 / (v + w)^2), and
+    // L = ln(X / (tau + w)) = -asinh(tau / X), evaluated through log1p.
+    const float Xf = float(X), tau2 = float(r[TN_TAU2]);
+    const float X2 = Xf * Xf;
+    const float v = Fn<float>::sqrt_((Xf - 1.0f) * (Xf + 1.0f));
+    const float w = Fn<float>::sqrt_(tau2 + X2);
+    const float iv = Fn<float>::rcp(v);
+    const float vw = v + w;
+    const float A = float(r[TN_K]) * iv * Fn<float>::rcp(vw * vw) -
+                    (2.0f * X2 + float(r[TN_T2M1])) * ::atanf(iv) * iv;
+    const float z = float(r[TN_TAU]) * Fn<float>::rcp(Xf);
+    const float ash = ::log1pf(z + z * z * Fn<float>::rcp(1.0f + Fn<float>::sqrt_(1.0f + z * z)));
+    const float g = float(r[TN_PRE]) * (A + float(r[TN_C0]) - w * ash * float(r[TN_C1]));
+    const float a = float(r[TN_A]) * g * Fn<float>::rcp(X2);
+    ax += double(a * float(dx));
+    ay += double(a * float(dy));
     return;
   }
   const T tau = r[TN_TAU], tau2 = r[TN_TAU2];

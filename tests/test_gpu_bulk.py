@@ -146,9 +146,9 @@ def test_substructure_full_size_mixed_vs_fp64():
     draws = [W.substructure_draw(d) for d in (0, 1)]
     names = ["EPL", "SHEAR"] + ["TNFW"] * cfg["n_sub"]
     im64 = _substructure_model(cfg, "fp64", kernel=kernel)
-    kl = W.stack_kwargs([d[0] + d[1] for d in draws])
+    lm = im64.LensModel
+    rs = lm.stack_ray_shooting_kwargs([lm.prepare_ray_shooting_kwargs(names, d[0] + d[1]) for d in draws])
     ks = W.stack_kwargs([d[2] for d in draws])
-    rs = im64.LensModel.prepare_ray_shooting_kwargs(names, kl)
     params, n, m, keep = im64.pack_bulk(rs, ks)
     assert n == 2
     pt = torch.as_tensor(params, device="cuda")

@@ -65,8 +65,8 @@ def test_grad_vs_central_fd(precision, tol):
         scale = np.maximum(np.abs(fd), 1e-3 * np.abs(logl64.cpu().numpy()) / max(abs(params[0, k, i].item()), 0.1))
         err = np.max(np.abs(g[:, k, i] - fd) / scale)
         assert err < tol, (k, i, FREE[(k, i)], g[:, k, i], fd)
-    # slots beyond a profile's kwargs stay zero
-    assert np.all(g[:, 1, 2:] == 0) and np.all(g[:, 0, 6:] == 0)
+    # slots beyond a profile's kwargs stay zero (SHEAR has 4: gamma1, gamma2, ra_0, dec_0)
+    assert np.all(g[:, 1, 4:] == 0) and np.all(g[:, 0, 6:] == 0)
 
 
 def test_grad_vs_oracle_fd_small():
