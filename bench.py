@@ -169,6 +169,37 @@ def substructure_work(params_np, n_ray, n_sub):
     return F + n_img * n_ray * 36.0 * n_sub, S + n_img * n_ray * 7.0 * n_sub
 
 
+def serve_measure(im, params_np, data, capacity, seconds, fractions):
+    """Served load on the request server: Poisson arrivals of log-likelihood
+    requests at fractions of the batched capacity; achieved rate and latency
+    percentiles per offered rate."""
+    from jaxtronomy_amd.serving import LensServer, latency_percentiles, poisson_load
+
+    srv = LensServer(n_streams=4)
+    srv.register("epl180", im, max_batch=64, n_slots=2, n_lanes=3)
+    slot = srv.put_data("epl180", data)
+    slots = np.full(params_np.shape[0], slot)
+    srv.warmup("epl180", params_np[0])
+    srv.start()
+    points = []
+    try:
+        poisson_load(srv, "epl180", params_np, 0.2 * capacity, 0.2, slots=slots, seed=99)
+        for i, frac in enumerate(fractions):
+            rate = frac * capacity
+            b0, n0 = srv.batches, srv.requests_done
+            futs, wall = poisson_load(srv, "epl180", params_np, rate, seconds, slots=slots, seed=i)
+            pct = latency_percentiles(futs)
+            nb = srv.batches - b0
+            points.append({"offered_per_s": round(rate, 1), "achieved_per_s": round(len(futs) / wall, 1),
+                           "p50_ms": round(pct[50], 4), "p99_ms": round(pct[99], 4),
+                           "mean_batch": round((srv.requests_done - n0) / max(nb, 1), 2)})
+            del futs
+            log("bench: served %.0f/s offered -> %s" % (rate, points[-1]))
+    finally:
+        srv.stop()
+    return points
+
+
 def main_substructure(args):
     import torch
 
@@ -246,6 +277,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("epl180", "substructure"), default="epl180")
     ap.add_argument("--n-sub", type=int, default=1000)
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a design "
@@ -258,6 +292,15 @@ def main():
 
     backend.require_gpu()
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
+        return
     if args.workload == "substructure":
         return main_substructure(args)
     log("bench: building models")
@@ -316,6 +359,13 @@ def main():
                      "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
                      "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3)},
     }
+    if args.serve_seconds > 0:
+        log("bench: served load")
+        line["serving"] = {
+            "workload": "Poisson arrivals of single log-likelihood requests (same model and data as above) on the "
+                        "continuous-batching server: max batch 64, 3 graph lanes, 4 streams",
+            "points": serve_measure(im32, params_np, data, value, args.serve_seconds, (0.25, 0.5, 0.75)),
+        }
     if not args.no_cpu:
         log("bench: cpu baseline (%d images)" % args.cpu_sample)
         cb, ll_cpu = cpu_baseline(cfg, kernel, params_np, data, args.cpu_sample)

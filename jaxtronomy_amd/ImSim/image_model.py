@@ -219,9 +219,10 @@ class ImageModel(object):
         backend.check(rc, "lensim_image")
         return out
 
-    def _data_struct(self, data=None):
+    def _data_struct(self, data=None, noise_map=None):
         """lensim_data of the ImageData; ``data`` ([B, ny, nx] float64 device
-        tensor) gives each batch image its own observation instead."""
+        tensor) gives each batch image its own observation instead, and
+        ``noise_map`` (same shape) its own noise map."""
         buf = self.Data.device_buffers()
         if self._mask_dev is None:
             self._mask_dev = backend.to_device_f64(self.likelihood_mask.astype(np.float64))
@@ -233,20 +234,26 @@ class ImageModel(object):
         d.exposure_time = buf["expo_time"]
         d.background_rms = self.Data.background_rms
         if data is not None:
-            if data.dtype != torch.float64 or not data.is_contiguous() or \
-                    tuple(data.shape[-2:]) != (self._grid.ny, self._grid.nx):
-                raise ValueError("per-image data must be a contiguous float64 [B, %d, %d] tensor"
-                                 % (self._grid.ny, self._grid.nx))
-            if buf["noise"] is not None:
+            for t in (data, noise_map):
+                if t is not None and (t.dtype != torch.float64 or not t.is_contiguous() or
+                                      tuple(t.shape) != tuple(data.shape) or
+                                      tuple(t.shape[-2:]) != (self._grid.ny, self._grid.nx)):
+                    raise ValueError("per-image data / noise maps must be contiguous float64 [B, %d, %d] tensors"
+                                     % (self._grid.ny, self._grid.nx))
+            if buf["noise"] is not None and noise_map is None:
                 raise ValueError("per-image data with a noise map needs per-image noise maps")
             d.d_data = data.data_ptr()
+            d.d_noise_map = None if noise_map is None else noise_map.data_ptr()
             d.image_stride = self._grid.nx * self._grid.ny
+        elif noise_map is not None:
+            raise ValueError("per-image noise maps need per-image data")
         return d
 
-    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None, data=None):
+    def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None, data=None,
+                         noise_map=None):
         """log-likelihoods [B] (float64) of a packed device parameter block,
-        against the ImageData or, with ``data`` [B, ny, nx], one observation per
-        batch image."""
+        against the ImageData or, with ``data`` [B, ny, nx] (and optionally
+        ``noise_map`` [B, ny, nx]), one observation per batch image."""
         lib = backend.lib()
         m = self._model if model is None else model
         if n_img is None:
@@ -255,7 +262,7 @@ class ImageModel(object):
             raise ValueError("per-image data holds %d images for a batch of %d" % (data.shape[0], n_img))
         if logl is None:
             logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
-        d = self._data_struct(data)
+        d = self._data_struct(data, noise_map)
         ws = self._ws.get(self.workspace_bytes(n_img, m))
         rc = lib.lensim_loglike(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
                                 ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),

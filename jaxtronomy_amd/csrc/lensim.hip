@@ -615,35 +615,35 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // B fragments of the banded kernel rows, in the lane layout of
 // v_mfma_f32_16x16x32_f16 (lane l holds B[8 (l / 16) + j][l % 16], j < 8),
 // split into fp16 hi + lo of the kernel scaled by 2^sk (max |K| 2^sk in
-// [2^14, 2^15)).  One block; hdr[0] receives sk.
+// [2^14, 2^15)).  One thread per fragment; every block reduces the kernel
+// maximum itself (K is small), block 0 stores sk in hdr[0].
 __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
                                                             int *__restrict__ hdr, half8 *__restrict__ frag) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   double m = 0.0;
   for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(k[i]));
-  red[threadIdx.x] = m;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const double mx = red[0];
+  const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   const int sk = (mx > 0.0 && isfinite(mx)) ? 14 - ilogb(mx) : 0;
-  if (threadIdx.x == 0) hdr[0] = sk;
-  const int n_frag = ky * nks * 2 * 64;
-  for (int idx = threadIdx.x; idx < n_frag; idx += blockDim.x) {
-    const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
-    const int a = ak / nks, ks = ak - a * nks;
-    const int n = lane & 15, g = lane >> 4;
-    half8 v;
-    for (int j = 0; j < 8; ++j) {
-      const int c = 32 * ks + 8 * g + j - n;  // tap of the flipped kernel row a
-      const double w = (c >= 0 && c < kx) ? ldexp(k[(ky - 1 - a) * kx + (kx - 1 - c)], sk) : 0.0;
-      const _Float16 h = (_Float16)w;
-      v[j] = s == 0 ? h : (_Float16)(w - (double)h);
-    }
-    frag[idx] = v;
+  if (blockIdx.x == 0 && threadIdx.x == 0) hdr[0] = sk;
+  const double scale = ldexp(1.0, sk);  // exact power of two
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ky * nks * 2 * 64) return;
+  const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
+  const int a = ak / nks, ks = ak - a * nks;
+  const int n = lane & 15, g = lane >> 4;
+  const double *krow = k + size_t(ky - 1 - a) * kx;
+  half8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 32 * ks + 8 * g + j - n;  // tap of the flipped kernel row a
+    const double w = (c >= 0 && c < kx) ? krow[kx - 1 - c] * scale : 0.0;
+    const _Float16 h = (_Float16)w;
+    v[j] = s == 0 ? h : (_Float16)(w - (double)h);
   }
+  frag[idx] = v;
 }
 
 // Direct 'same' convolution on the matrix cores: block = 64 rows x 32 columns
@@ -1011,7 +1011,8 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
   half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
   if (mfma) {
-    psf_mfma_prep_kernel<<<1, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
+    const int n_frag = ky * mfma_nks(kx) * 2 * 64;
+    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
     rc = check_launch("psf_mfma_prep_kernel");
   } else {
     const int n = kyp * kx;

@@ -4,12 +4,13 @@ A *bucket* is one request class: an ImageModel configuration (grid,
 supersampling, PSF, profile lists, precision).  It owns
 
 * a pool of resident data slots: a log-likelihood request names the slot that
-  holds its observation, which stays on the device between requests;
+  holds its observation (and, for buckets registered with ``noise_maps``, its
+  noise map), which stays on the device between requests;
 * ``n_lanes`` lanes, each a full set of static device buffers for up to
-  ``max_batch`` parameter sets (parameters, slot indices, gathered data,
-  workspace, results, pinned host staging) and one captured HIP graph per
-  (request kind, padded batch size): the gather of the batch's observations
-  plus the library's launch sequence, replayed for every batch.
+  ``max_batch`` parameter sets (parameters, slot indices, gathered
+  observations, workspace, results, pinned host staging) and one captured HIP
+  graph per (request kind, padded batch size): the gather of the batch's
+  observations plus the library's launch sequence, replayed for every batch.
 
 The scheduler keeps up to ``n_streams`` batches in flight.  Whenever a stream
 and a lane of the oldest waiting bucket are free, it admits every request of
@@ -41,33 +42,30 @@ from . import backend
 
 
 class Future(object):
-    """Result handle of one request."""
+    """Result handle of one request (set by the scheduler thread)."""
 
-    __slots__ = ("value", "t_submit", "t_done", "_event")
+    __slots__ = ("value", "t_submit", "t_done", "_cond")
 
-    def __init__(self):
+    def __init__(self, cond, t_submit):
         self.value = None
-        self.t_submit = time.perf_counter()
+        self.t_submit = t_submit
         self.t_done = None
-        self._event = threading.Event()
+        self._cond = cond
 
     def done(self):
-        return self._event.is_set()
+        return self.t_done is not None
 
     def result(self, timeout=None):
-        if not self._event.wait(timeout):
-            raise TimeoutError("request not completed")
+        if self.t_done is None:
+            with self._cond:
+                if not self._cond.wait_for(self.done, timeout):
+                    raise TimeoutError("request not completed")
         return self.value
 
     @property
     def latency(self):
-        """Seconds from submit() to completion."""
+        """Seconds from arrival (submit time unless given) to completion."""
         return None if self.t_done is None else self.t_done - self.t_submit
-
-    def _set(self, value):
-        self.value = value
-        self.t_done = time.perf_counter()
-        self._event.set()
 
 
 def _pow2_at_least(n, cap):
@@ -80,17 +78,19 @@ def _pow2_at_least(n, cap):
 class _Lane(object):
     """One set of static buffers of a bucket (and the graphs captured on them)."""
 
-    def __init__(self, bucket):
-        b = bucket
+    def __init__(self, b):
         dev = backend.device()
         ny, nx = b.shape
         mb = b.max_batch
         self.busy = False
         self.params = torch.zeros((mb, b.n_prof, backend.NPARAM), dtype=torch.float64, device=dev)
         self.params_host = torch.zeros((mb, b.n_prof, backend.NPARAM), dtype=torch.float64).pin_memory()
+        self.params_np = self.params_host.numpy()
         self.slot_idx = torch.zeros(mb, dtype=torch.long, device=dev)
         self.slot_host = torch.zeros(mb, dtype=torch.long).pin_memory()
+        self.slot_np = self.slot_host.numpy()
         self.batch_data = torch.zeros((mb, ny, nx), dtype=torch.float64, device=dev)
+        self.batch_noise = torch.zeros((mb, ny, nx), dtype=torch.float64, device=dev) if b.noise_maps else None
         self.logl = torch.zeros(mb, dtype=torch.float64, device=dev)
         self.logl_host = torch.zeros(mb, dtype=torch.float64).pin_memory()
         self.images = torch.zeros((mb, ny, nx), dtype=b.dtype, device=dev)
@@ -99,14 +99,18 @@ class _Lane(object):
         self.data = backend.LensimData()
         ctypes.memmove(ctypes.byref(self.data), ctypes.byref(b.data_template), ctypes.sizeof(self.data))
         self.data.d_data = self.batch_data.data_ptr()
+        if self.batch_noise is not None:
+            self.data.d_noise_map = self.batch_noise.data_ptr()
         self.data.image_stride = nx * ny
         self.graphs = {}
-        self.nbytes = (self.params.numel() * 8 + self.slot_idx.numel() * 8 + self.batch_data.numel() * 8 +
-                       self.logl.numel() * 8 + self.images.numel() * self.images.element_size() + b.ws_bytes)
+        self.nbytes = sum(t.numel() * t.element_size() for t in (self.params, self.slot_idx, self.batch_data,
+                                                                  self.logl, self.images, self.ws))
+        if self.batch_noise is not None:
+            self.nbytes += self.batch_noise.numel() * 8
 
 
 class _Bucket(object):
-    def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs):
+    def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs, noise_maps):
         if image_model._bulk:
             raise NotImplementedError("the server takes ImageModels with a plain lens model list")
         self.key = key
@@ -115,14 +119,15 @@ class _Bucket(object):
         if not 1 <= self.max_batch <= 65535:
             raise ValueError("max_batch must be in [1, 65535]")
         self.use_graphs = use_graphs
+        self.noise_maps = bool(noise_maps)
         self.shape = (image_model._grid.ny, image_model._grid.nx)
         self.n_prof = len(image_model._names)
         self.dtype = torch.float64 if image_model._prec == backend.FP64 else torch.float32
         dev = backend.device()
         buf = image_model.Data.device_buffers()
-        if buf["noise"] is not None:
-            raise NotImplementedError("the server computes the variance from the model; the ImageData "
-                                      "must not carry a noise map")
+        if buf["noise"] is not None and not self.noise_maps:
+            raise ValueError("the ImageData carries a noise map: register the bucket with noise_maps=True and "
+                             "give each observation its noise map")
         self.mask = backend.to_device_f64(image_model.likelihood_mask.astype(np.float64))
         self.expo_map = buf["expo_map"]
         d = backend.LensimData()
@@ -133,9 +138,10 @@ class _Bucket(object):
         self.data_template = d
         self.ws_bytes = max(image_model.workspace_bytes(self.max_batch), 256)
         self.slots = torch.zeros((int(n_slots),) + self.shape, dtype=torch.float64, device=dev)
+        self.noise_slots = torch.ones_like(self.slots) if self.noise_maps else None
         self.free_slots = list(range(int(n_slots)))[::-1]
         self.lanes = [_Lane(self) for _ in range(int(n_lanes))]
-        self.nbytes = self.slots.numel() * 8 + sum(l.nbytes for l in self.lanes)
+        self.nbytes = self.slots.numel() * 8 * (2 if self.noise_maps else 1) + sum(l.nbytes for l in self.lanes)
 
     def free_lane(self):
         for lane in self.lanes:
@@ -149,7 +155,10 @@ class _Bucket(object):
         lib = backend.lib()
         st = backend.stream_handle()
         if kind == "loglike":
-            torch.index_select(self.slots, 0, lane.slot_idx[:n], out=lane.batch_data[:n])
+            idx = lane.slot_idx[:n]
+            torch.index_select(self.slots, 0, idx, out=lane.batch_data[:n])
+            if self.noise_maps:
+                torch.index_select(self.noise_slots, 0, idx, out=lane.batch_noise[:n])
             rc = lib.lensim_loglike(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
                                     ctypes.byref(lane.data), backend.ptr(lane.params), n, im._prec,
                                     backend.ptr(lane.logl), None, backend.ptr(lane.ws), lane.ws.numel(), st)
@@ -166,8 +175,8 @@ class _Bucket(object):
         returns the completion event (results are then in the lane's host buffers)."""
         npad = _pow2_at_least(n, self.max_batch)
         if npad > n:  # padding rows: copies of the first request, results ignored
-            lane.params_host[n:npad] = lane.params_host[0]
-            lane.slot_host[n:npad] = lane.slot_host[0]
+            lane.params_np[n:npad] = lane.params_np[0]
+            lane.slot_np[n:npad] = lane.slot_np[0]
         with torch.cuda.stream(stream):
             lane.params[:npad].copy_(lane.params_host[:npad], non_blocking=True)
             lane.slot_idx[:npad].copy_(lane.slot_host[:npad], non_blocking=True)
@@ -192,13 +201,26 @@ class _Bucket(object):
         return ev
 
 
+class _Block(object):
+    """Queued requests of one kind for one bucket (consumed from the front)."""
+
+    __slots__ = ("kind", "params", "slots", "futures", "pos")
+
+    def __init__(self, kind, params, slots, futures):
+        self.kind = kind
+        self.params = params
+        self.slots = slots
+        self.futures = futures
+        self.pos = 0
+
+
 class LensServer(object):
     """Continuous-batching server over registered request classes (buckets).
 
     ``register(key, image_model)`` sets up a bucket; ``put_data`` makes an
-    observation resident; ``submit`` queues a request and returns a Future;
-    ``step()`` / ``drain()`` drive the scheduler from the caller's thread and
-    ``start()`` / ``stop()`` from a background thread.
+    observation resident; ``submit`` / ``submit_many`` queue requests and return
+    Futures; ``step()`` / ``drain()`` drive the scheduler from the caller's
+    thread and ``start()`` / ``stop()`` from a background thread.
     """
 
     def __init__(self, n_streams=4, hbm_budget=64 << 30, use_graphs=True):
@@ -210,9 +232,10 @@ class LensServer(object):
         self.hbm_used = 0
         self.use_graphs = use_graphs
         self.buckets = {}
-        self.pending = collections.OrderedDict()  # key -> deque of (kind, params, slot, future)
+        self.pending = collections.OrderedDict()  # key -> deque of _Block
         self.inflight = []
         self.lock = threading.Lock()
+        self.cond = threading.Condition()
         self.batches = 0
         self.requests_done = 0
         self.batch_sizes = collections.Counter()
@@ -220,10 +243,10 @@ class LensServer(object):
         self._stop = False
 
     # ---------------------------------------------------------- admission --
-    def register(self, key, image_model, max_batch=64, n_slots=16, n_lanes=2):
+    def register(self, key, image_model, max_batch=64, n_slots=16, n_lanes=2, noise_maps=False):
         if key in self.buckets:
             raise ValueError("bucket %r is already registered" % (key,))
-        b = _Bucket(key, image_model, max_batch, n_slots, n_lanes, self.use_graphs)
+        b = _Bucket(key, image_model, max_batch, n_slots, n_lanes, self.use_graphs, noise_maps)
         if self.hbm_used + b.nbytes > self.hbm_budget:
             raise MemoryError("bucket %r needs %d B; %d of %d B already used" % (key, b.nbytes, self.hbm_used,
                                                                               self.hbm_budget))
@@ -231,18 +254,53 @@ class LensServer(object):
         self.buckets[key] = b
         return b
 
-    def put_data(self, key, image):
-        """Make an observation resident; returns its slot id, or None when all of
-        the bucket's slots are in use."""
+    def warmup(self, key, example_params, kinds=("loglike",)):
+        """Capture every lane's graphs (each request kind, each padded batch size)
+        ahead of load, with ``example_params`` [n_profiles, 8] as the
+        parameters, so that no capture stalls a batch under load."""
+        b = self.buckets[key]
+        ex = np.asarray(example_params, dtype=np.float64).reshape(b.n_prof, backend.NPARAM)
+        with self.lock:
+            if any(l.busy for l in b.lanes) or self.pending.get(key):
+                raise RuntimeError("warm up a bucket before submitting requests to it")
+        sizes = []
+        n = 1
+        while n < b.max_batch:
+            sizes.append(n)
+            n *= 2
+        sizes.append(b.max_batch)
+        stream = self.streams[0]
+        for lane in b.lanes:
+            lane.params_np[:] = ex
+            lane.slot_np[:] = 0
+            for kind in kinds:
+                for n in sizes:
+                    b.run(lane, kind, n, stream)
+        stream.synchronize()
+
+    def put_data(self, key, image, noise_map=None):
+        """Make an observation (and its noise map, for buckets registered with
+        noise_maps) resident; returns its slot id, or None when all of the
+        bucket's slots are in use."""
         b = self.buckets[key]
         img = np.asarray(image, dtype=np.float64)
         if img.shape != b.shape:
             raise ValueError("observation of shape %s for a %s bucket" % (img.shape, b.shape))
+        if b.noise_maps:
+            if noise_map is None:
+                raise ValueError("this bucket needs a noise map with each observation")
+            noise_map = np.asarray(noise_map, dtype=np.float64)
+            if noise_map.shape != b.shape:
+                raise ValueError("noise map of shape %s for a %s bucket" % (noise_map.shape, b.shape))
+        elif noise_map is not None:
+            raise ValueError("bucket %r was registered without noise maps" % (key,))
         with self.lock:
             if not b.free_slots:
                 return None
             slot = b.free_slots.pop()
         b.slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(img)))
+        if b.noise_maps:
+            b.noise_slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(noise_map)))
         torch.cuda.current_stream().synchronize()
         return slot
 
@@ -256,44 +314,82 @@ class LensServer(object):
         the ImageModel.pack layout; ``kind`` is "loglike" (needs ``slot``) or
         "image".  Returns a Future whose value is the float logL or the
         [ny, nx] image (a CPU tensor)."""
+        p = np.asarray(params, dtype=np.float64)[None]
+        return self.submit_many(key, p, kind, None if slot is None else [slot])[0]
+
+    def submit_many(self, key, params, kind="loglike", slots=None, t_arrival=None):
+        """Queue B requests at once: params [B, n_profiles, 8]; slots [B] for
+        log-likelihood requests; t_arrival optional [B] arrival times on the
+        time.perf_counter clock (latencies are measured from them)."""
         if kind not in ("loglike", "image"):
             raise ValueError("kind must be 'loglike' or 'image'")
         b = self.buckets[key]
+        params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, b.n_prof, backend.NPARAM)
+        n = params.shape[0]
         if kind == "loglike":
-            if slot is None or not 0 <= int(slot) < b.slots.shape[0]:
-                raise ValueError("a log-likelihood request needs a valid data slot")
-        p = np.asarray(params, dtype=np.float64).reshape(b.n_prof, backend.NPARAM)
-        fut = Future()
+            if slots is None:
+                raise ValueError("log-likelihood requests need data slots")
+            slots = np.asarray(slots, dtype=np.int64).reshape(n)
+            if np.any(slots < 0) or np.any(slots >= b.slots.shape[0]):
+                raise ValueError("data slot out of range")
+        else:
+            slots = np.zeros(n, dtype=np.int64)
+        now = time.perf_counter()
+        ts = [now] * n if t_arrival is None else list(np.asarray(t_arrival, dtype=np.float64))
+        futs = [Future(self.cond, t) for t in ts]
         with self.lock:
-            self.pending.setdefault(key, collections.deque()).append((kind, p, int(slot or 0), fut))
-        return fut
+            self.pending.setdefault(key, collections.deque()).append(_Block(kind, params, slots, futs))
+        return futs
 
     # ---------------------------------------------------------- scheduling --
     def _retire(self):
-        still = []
+        if not self.inflight:
+            return
+        still, done = [], []
         for item in self.inflight:
-            ev, b, lane, kind, reqs, si = item
-            if not ev.query():
-                still.append(item)
-                continue
+            (done if item[0].query() else still).append(item)
+        if not done:
+            return
+        self.inflight = still
+        now = time.perf_counter()
+        for ev, b, lane, kind, futs, si in done:
+            n = len(futs)
             if kind == "loglike":
-                vals = lane.logl_host[:len(reqs)].numpy().copy()
-                for i, r in enumerate(reqs):
-                    r[3]._set(float(vals[i]))
+                vals = lane.logl_host[:n].numpy().tolist()
             else:
-                imgs = lane.images_host[:len(reqs)].clone()
-                for i, r in enumerate(reqs):
-                    r[3]._set(imgs[i])
+                vals = list(lane.images_host[:n].clone().unbind(0))
+            for f, v in zip(futs, vals):
+                f.value = v
+                f.t_done = now
             lane.busy = False
             self.free_streams.append(si)
-            self.requests_done += len(reqs)
-        self.inflight = still
+            self.requests_done += n
+        with self.cond:
+            self.cond.notify_all()
+
+    def _take(self, q, max_n):
+        """Up to max_n requests of the first block's kind from the front of q:
+        (kind, params, slots, futures)."""
+        kind = q[0].kind
+        ps, ss, fs = [], [], []
+        need = max_n
+        while q and need > 0 and q[0].kind == kind:
+            blk = q[0]
+            k = min(need, len(blk.futures) - blk.pos)
+            ps.append(blk.params[blk.pos:blk.pos + k])
+            ss.append(blk.slots[blk.pos:blk.pos + k])
+            fs.extend(blk.futures[blk.pos:blk.pos + k])
+            blk.pos += k
+            need -= k
+            if blk.pos == len(blk.futures):
+                q.popleft()
+        return kind, ps, ss, fs
 
     def _dispatch(self):
         launched = False
         while self.free_streams:
             with self.lock:
-                pick = None
+                pick = lane = None
                 for key, q in self.pending.items():
                     if q:
                         lane = self.buckets[key].free_lane()
@@ -302,22 +398,18 @@ class LensServer(object):
                             break
                 if pick is None:
                     return launched
-                q = self.pending[pick]
                 b = self.buckets[pick]
-                kind = q[0][0]
-                reqs = []
-                while q and len(reqs) < b.max_batch and q[0][0] == kind:
-                    reqs.append(q.popleft())
+                kind, ps, ss, futs = self._take(self.pending[pick], b.max_batch)
                 self.pending.move_to_end(pick)  # round-robin between buckets
-            for i, r in enumerate(reqs):
-                lane.params_host[i].copy_(torch.from_numpy(r[1]))
-                lane.slot_host[i] = r[2]
+            n = len(futs)
+            np.concatenate(ps, axis=0, out=lane.params_np[:n])
+            np.concatenate(ss, axis=0, out=lane.slot_np[:n])
             si = self.free_streams.pop(0)
             lane.busy = True
-            ev = b.run(lane, kind, len(reqs), self.streams[si])
-            self.inflight.append((ev, b, lane, kind, reqs, si))
+            ev = b.run(lane, kind, n, self.streams[si])
+            self.inflight.append((ev, b, lane, kind, futs, si))
             self.batches += 1
-            self.batch_sizes[len(reqs)] += 1
+            self.batch_sizes[n] += 1
             launched = True
         return launched
 
@@ -373,3 +465,43 @@ def latency_percentiles(futures, qs=(50, 99)):
     if lat.size == 0:
         return {q: None for q in qs}
     return {q: float(np.percentile(lat, q)) for q in qs}
+
+
+def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None, seed=0, tick=50e-6):
+    """Offer requests to a running server as a Poisson process of ``rate``
+    requests/s for ``duration`` seconds, cycling through ``params`` [M, n, 8]
+    (and ``slots`` [M]); waits for completion and returns (futures, wall seconds
+    from the first arrival to the last completion)."""
+    import gc
+
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(rate * duration))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    m = params.shape[0]
+    futs = []
+    i = 0
+    # the cyclic garbage collector is paused while load is offered: the
+    # server's objects form no reference cycles, and a full collection over
+    # the outstanding requests would stall the loop for tens of milliseconds
+    gc.collect()
+    gc.disable()
+    t0 = time.perf_counter() + 1e-3
+    arrivals = t0 + np.cumsum(gaps)
+    try:
+        while i < n_req:
+            now = time.perf_counter()
+            j = int(np.searchsorted(arrivals, now, side="right"))
+            if j > i:
+                idx = np.arange(i, j) % m
+                futs.extend(server.submit_many(key, params[idx], kind,
+                                               None if slots is None else np.asarray(slots)[idx],
+                                               t_arrival=arrivals[i:j]))
+                i = j
+            else:
+                time.sleep(tick)
+        for f in futs:
+            f.result(timeout=120)
+    finally:
+        gc.enable()
+    wall = max(f.t_done for f in futs) - arrivals[0]
+    return futs, wall

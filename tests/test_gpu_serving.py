@@ -154,3 +154,46 @@ def test_graph_replays_with_eager_work_between():
         scratch = (scratch * 1.0001 + 1.0).sum() + scratch  # eager work between replays
         torch.cuda.synchronize()
     assert len(graph.buckets["a"].lanes[0].graphs) == 1
+
+
+def test_noise_map_slots():
+    """Observations with their own noise maps: server results equal the same
+    requests evaluated alone, and the noise-map variance is really used."""
+    im, cfg = _image_model("mixed", n=48, s=1, k=9)
+    srv = LensServer(n_streams=2)
+    srv.register("n", im, max_batch=8, n_slots=3, noise_maps=True)
+    obs = _observations(im, cfg, 3, 13)
+    rng = np.random.default_rng(3)
+    noise = [np.full((48, 48), 0.01) + 0.005 * rng.random((48, 48)) for _ in obs]
+    slots = [srv.put_data("n", o, noise_map=nm) for o, nm in zip(obs, noise)]
+    with pytest.raises(ValueError):
+        srv.put_data("n", obs[0])
+    params = _requests(im, 11, 5)
+    futs = srv.submit_many("n", params, "loglike", [slots[i % 3] for i in range(11)])
+    srv.drain()
+    for i, f in enumerate(futs):
+        d = torch.as_tensor(obs[i % 3][None], device="cuda").contiguous()
+        nm = torch.as_tensor(noise[i % 3][None], device="cuda").contiguous()
+        alone = im.likelihood_batch(torch.as_tensor(params[i][None], device="cuda"), data=d, noise_map=nm).item()
+        assert f.result() == alone
+        plain = im.likelihood_batch(torch.as_tensor(params[i][None], device="cuda"), data=d).item()
+        assert alone != plain
+
+
+def test_poisson_load_latency():
+    from jaxtronomy_amd.serving import poisson_load
+
+    im, cfg = _image_model("mixed", n=64, s=2, k=13)
+    srv = LensServer(n_streams=4)
+    srv.register("a", im, max_batch=32, n_slots=2, n_lanes=3)
+    slot = srv.put_data("a", _observations(im, cfg, 1, 2)[0])
+    params = _requests(im, 16, 3)
+    srv.start()
+    try:
+        futs, wall = poisson_load(srv, "a", params, rate=5000.0, duration=0.3, slots=[slot] * 16)
+    finally:
+        srv.stop()
+    assert len(futs) == 1500 and all(f.done() for f in futs)
+    p = latency_percentiles(futs)
+    assert 0 < p[50] <= p[99] < 1e4
+    assert srv.batches < len(futs)  # requests were batched
