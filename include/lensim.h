@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LENSIM_ABI_VERSION 3
+#define LENSIM_ABI_VERSION 4
 #define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
 #define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
 #define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
@@ -114,13 +114,27 @@ typedef struct {
   double pix2angle[4]; /* M00 M01 M10 M11 */
 } lensim_grid;
 
-/* PSF convolution (ImSim/Numerics/numerics.py:102-146). */
+/* PSF convolution (ImSim/Numerics/numerics.py:102-146).
+ * supersampled = 0: d_kernel is sampled on the image pixels and convolves the
+ *   s x s mean of the rays (PixelKernelConvolution / GaussianConvolution,
+ *   ImSim/Numerics/convolution.py:74-96, :217-236).
+ * supersampled = 1 (supersampling_convolution=True, numerics.py:103-116):
+ *   d_kernel is sampled on the supersampled grid and convolves the
+ *   (ny s) x (nx s) ray fluxes before the s x s mean
+ *   (SubgridKernelConvolution.re_size_convolve, convolution.py:154-166, or
+ *   GaussianConvolution.re_size_convolve, :238-252); an optional low-resolution
+ *   kernel (kx_low x ky_low, zero padding) then convolves the s x s mean of the
+ *   unconvolved rays and is added (the split kernel of convolution.py:117-131).
+ *   Ignored when the grid's supersampling is 1. */
 typedef struct {
-  int32_t type;           /* lensim_psf_type */
-  int32_t kx;             /* odd kernel width */
-  int32_t ky;             /* odd kernel height */
-  int32_t reserved;
-  const double *d_kernel; /* [ky][kx], float64 device memory */
+  int32_t type;               /* lensim_psf_type */
+  int32_t kx;                 /* odd kernel width */
+  int32_t ky;                 /* odd kernel height */
+  int32_t supersampled;       /* 0 or 1, see above */
+  const double *d_kernel;     /* [ky][kx], float64 device memory */
+  int32_t kx_low;             /* odd width of the low-resolution split kernel, or 0 for none */
+  int32_t ky_low;             /* odd height, or 0 */
+  const double *d_kernel_low; /* [ky_low][kx_low] float64 device memory, or NULL */
 } lensim_psf;
 
 /* Data and noise of one image for the likelihood
@@ -191,7 +205,7 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
  * test/test_ImSim/test_image_model.py:274-335; EPL's custom JVP epl.py:321-340).
  * Adjoint path: dlogL/dmodel (model-based variance included) -> correlation
  * with the PSF -> per-ray forward-mode derivatives of each profile.
- * Not supported: bulk deflector lists, the edge-padded GAUSSIAN PSF. */
+ * Not supported: bulk deflector lists, supersampled convolution. */
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                                   int32_t n_img, int32_t precision);
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,

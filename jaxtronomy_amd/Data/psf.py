@@ -3,7 +3,14 @@
 Follows the public behaviour of lenstronomy's ``PSF`` class, which the
 reference imports (Util/class_creator.py:7, ImSim/image_model.py:64): PIXEL
 kernels must be odd-sized and are normalised to unit sum; GAUSSIAN carries a
-FWHM and a truncation (in units of sigma, default 5).
+FWHM and a truncation (in units of sigma, default 5).  A PIXEL kernel given
+with point_source_supersampling_factor = s > 1 is sampled on the supersampled
+grid: it is kept (normalised) for supersampled convolution, and the regular
+kernel is its s x s block mean (degrade_kernel, odd s).
+
+lenstronomy itself is not available here; degrade_kernel and split_kernel are
+restated from its documented behaviour and pinned by their invariants (unit
+total flux, the hole under the supersampled cut, centring), not by its output.
 """
 
 import numpy as np
@@ -30,6 +37,10 @@ class PSF(object):
             k = np.asarray(kernel_point_source, dtype=np.float64)
             if k.ndim != 2 or k.shape[0] % 2 == 0 or k.shape[1] % 2 == 0:
                 raise ValueError("kernel needs to have odd axis number, not %s" % (k.shape,))
+            self._kernel_super = None
+            if point_source_supersampling_factor > 1:
+                self._kernel_super = k / np.sum(k) if kernel_point_source_normalisation else k
+                k = degrade_kernel(k, point_source_supersampling_factor)
             if kernel_point_source_normalisation:
                 k = k / np.sum(k)
             self._kernel = k
@@ -45,6 +56,19 @@ class PSF(object):
     @property
     def kernel_point_source(self):
         return self._kernel
+
+    def kernel_point_source_supersampled(self, supersampling_factor):
+        """The PIXEL kernel sampled on a grid supersampled by supersampling_factor:
+        the kernel given at that factor (point_source_supersampling_factor), or
+        the regular kernel itself for a factor of 1."""
+        if supersampling_factor == 1:
+            return self._kernel
+        if self.psf_type == "PIXEL" and getattr(self, "_kernel_super", None) is not None and \
+                self._point_source_supersampling_factor == supersampling_factor:
+            return self._kernel_super
+        raise NotImplementedError("a PIXEL kernel supersampled by %d must be given (kernel_point_source with "
+                                  "point_source_supersampling_factor=%d); interpolating a regular kernel onto a "
+                                  "finer grid is not supported" % (supersampling_factor, supersampling_factor))
 
     @property
     def fwhm(self):
@@ -88,3 +112,61 @@ def gaussian_pixel_kernel(fwhm, pixel_scale, truncation, supersampling_factor=1,
     c = 1.0 / (2 * np.pi * sigma_scaled ** 2)
     k = c * np.exp(-(xx ** 2 / sigma_scaled ** 2 + yy ** 2 / sigma_scaled ** 2) / 2.0)
     return k / np.sum(k)
+
+
+def degrade_kernel(kernel_super, degrading_factor):
+    """Kernel on a grid degrading_factor times coarser: zero-pad the (odd)
+    kernel symmetrically to numPix * factor pixels (numPix odd) and take the
+    mean of every factor x factor block (lenstronomy kernel_util.degrade_kernel,
+    odd factors)."""
+    if degrading_factor == 1:
+        return kernel_super
+    if degrading_factor % 2 == 0:
+        raise NotImplementedError("even degrading factors are not supported")
+    n_kernel = len(kernel_super)
+    num_pix = int(round(n_kernel / degrading_factor + 0.5))
+    if num_pix % 2 == 0:
+        num_pix += 1
+    n_high = num_pix * degrading_factor
+    padded = np.zeros((n_high, n_high))
+    i0 = (n_high - n_kernel) // 2
+    padded[i0:i0 + n_kernel, i0:i0 + n_kernel] = kernel_super
+    return padded.reshape(num_pix, degrading_factor, num_pix, degrading_factor).mean(axis=(1, 3))
+
+
+def split_kernel(kernel_super, supersampling_kernel_size, supersampling_factor, normalized=True):
+    """(low-resolution kernel with a hole, supersampled central cut) such that
+    convolving the supersampled image with the cut and the pooled image with
+    the holed kernel approximates the full supersampled convolution
+    (lenstronomy kernel_util.split_kernel, used at
+    ImSim/Numerics/convolution.py:117-131).  The cut is the central
+    supersampling_kernel_size * s (made odd) pixels; the holed kernel is the
+    supersampled kernel with that centre zeroed, degraded to the pixel grid and
+    rescaled so that both parts together carry unit flux."""
+    if supersampling_factor <= 1:
+        raise ValueError("To split a kernel, the supersampling_factor needs to be > 1, given %s"
+                         % supersampling_factor)
+    if supersampling_kernel_size % 2 == 0:
+        raise ValueError("supersampling_kernel_size needs to be an odd number, given %s" % supersampling_kernel_size)
+    kernel_super = np.asarray(kernel_super, dtype=np.float64)
+    n_super = len(kernel_super)
+    n_sub = supersampling_kernel_size * supersampling_factor
+    if n_sub % 2 == 0:
+        n_sub += 1
+    n_sub = min(n_sub, n_super)
+    n_min = (n_super - 1) // 2 - (n_sub - 1) // 2
+    n_max = (n_super - 1) // 2 + (n_sub - 1) // 2 + 1
+    hole = kernel_super.copy()
+    hole[n_min:n_max, n_min:n_max] = 0.0
+    hole_low = degrade_kernel(hole, supersampling_factor)
+    cut = kernel_super[n_min:n_max, n_min:n_max].copy()
+    if normalized:
+        flux_cut = np.sum(cut)
+        flux_hole = np.sum(hole_low)
+        if flux_hole > 0:
+            hole_low = hole_low * (1.0 - flux_cut) / flux_hole
+        else:
+            cut = cut / np.sum(cut)
+    else:
+        hole_low = hole_low / supersampling_factor ** 2
+    return hole_low, cut

@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from .. import backend, profiles
-from ..Data.psf import cut_psf, gaussian_pixel_kernel
+from ..Data.psf import cut_psf, gaussian_pixel_kernel, split_kernel
 from ..LensModel.lens_model import LensModel
 from ..LensModel.lens_model_bulk import LensModelBulk
 from ..LightModel.light_model import LightModel
@@ -87,6 +87,12 @@ class ImageModel(object):
             self._psf.type = backend.PSF_GAUSSIAN if self._psf_type == "GAUSSIAN" else backend.PSF_PIXEL
             self._psf.ky, self._psf.kx = self._kernel.shape
             self._psf.d_kernel = self._kernel_dev.data_ptr()
+            if self._ss:
+                self._psf.supersampled = 1
+                if self._kernel_low is not None:
+                    self._kernel_low_dev = backend.to_device_f64(self._kernel_low)
+                    self._psf.ky_low, self._psf.kx_low = self._kernel_low.shape
+                    self._psf.d_kernel_low = self._kernel_low_dev.data_ptr()
         self._mask_dev = None
         self._bulk = isinstance(self.LensModel, LensModelBulk)
         self._lens_types = [] if self._bulk else list(self.LensModel._types)
@@ -114,16 +120,32 @@ class ImageModel(object):
             raise ValueError("supersampling_factor must be >= 1")
         if supersampling_factor == 1:
             supersampling_convolution = False
-        if supersampling_convolution:
-            raise NotImplementedError("supersampling_convolution=True is not supported yet")
         if flux_evaluate_indexes is not None:
             raise NotImplementedError("flux_evaluate_indexes (sub-frame evaluation) is not supported yet")
         if convolution_type not in ("fft", "grid", "fft_static"):
             raise ValueError("convolution_type %s not supported!" % convolution_type)
         self._s = supersampling_factor
         self._psf_type = self.PSF.psf_type
+        self._ss = bool(supersampling_convolution) and self._psf_type != "NONE"
+        self._kernel_low = None
         pixel_width = self.Data.pixel_width
-        if self._psf_type == "PIXEL":
+        if self._ss and self._psf_type == "PIXEL":
+            # SubgridKernelConvolution (ImSim/Numerics/convolution.py:100-166, numerics.py:103-116)
+            kernel = np.asarray(self.PSF.kernel_point_source_supersampled(supersampling_factor), dtype=np.float64)
+            if convolution_kernel_size is not None:
+                size = convolution_kernel_size * supersampling_factor
+                if size % 2 == 0:
+                    size += 1
+                kernel = cut_psf(kernel, size)
+            if supersampling_kernel_size is not None:
+                self._kernel_low, kernel = split_kernel(kernel, supersampling_kernel_size, supersampling_factor)
+                self._kernel_low = np.ascontiguousarray(self._kernel_low)
+            self._kernel = np.ascontiguousarray(kernel)
+        elif self._ss and self._psf_type == "GAUSSIAN":
+            # GaussianConvolution with supersampling_convolution (convolution.py:169-252)
+            trunc = self.PSF.truncation if truncation_conv is None else truncation_conv
+            self._kernel = gaussian_pixel_kernel(self.PSF.fwhm, pixel_width, trunc, supersampling_factor, True)
+        elif self._psf_type == "PIXEL":
             kernel = np.asarray(self.PSF.kernel_point_source, dtype=np.float64)
             if convolution_kernel_size is not None:
                 size = convolution_kernel_size

@@ -15,7 +15,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 NPARAM = 8
 MAX_PROFILES = 64
 EPL_NMAX = 500
@@ -64,8 +64,11 @@ class LensimPsf(ctypes.Structure):
         ("type", ctypes.c_int32),
         ("kx", ctypes.c_int32),
         ("ky", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("supersampled", ctypes.c_int32),
         ("d_kernel", ctypes.c_void_p),
+        ("kx_low", ctypes.c_int32),
+        ("ky_low", ctypes.c_int32),
+        ("d_kernel_low", ctypes.c_void_p),
     ]
 
 

@@ -99,6 +99,9 @@ __host__ __device__ inline int type_of(const ModelArgs &m, int k) {
 struct Layout {
   size_t recd, recf, coefd, coeff, dcoefd, dcoeff, psf, lowres, partial, blkmax, bfrag;
   size_t gmodel, glow, gpart;  // gradient: dlogL/dmodel, dlogL/dlowres, per-block partial gradients
+  size_t gpad, gext;           // edge-padded PSF adjoint: padded dlogL/dmodel and its correlation
+  size_t hr, chr, lowu, lowc, psf_low;  // supersampled convolution: ray fluxes, their convolution,
+                                        // pooled unconvolved image and its low-resolution convolution
   size_t total;
   int n_tiles;
 };
@@ -139,7 +142,15 @@ int n_tiles_of(int nx, int ny) {
   return tx * ty;
 }
 
-Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision) {
+// Supersampled convolution plan (lensim_psf.supersampled on a grid with s > 1).
+struct SSPlan {
+  int on = 0;        // convolve on the supersampled grid
+  int s = 1;
+  int kxl = 0, kyl = 0;  // low-resolution split kernel (0: none)
+};
+
+Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision,
+              const SSPlan &ss = SSPlan()) {
   Layout L{};
   const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
   size_t off = 0;
@@ -162,10 +173,26 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   L.lowres = off;
   off = align256(off + size_t(n_img) * nx * ny * tsz);
   L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
-  L.partial = off;
-  off = align256(off + size_t(n_img) * L.n_tiles * 8);
-  L.blkmax = off;  // per render block max |value|, [n_img][ceil(nx ny / 256)]
-  off = align256(off + size_t(n_img) * ((size_t(nx) * ny + 255) / 256) * 4);
+  const size_t npix = size_t(nx) * ny, npix_r = ss.on ? npix * ss.s * ss.s : npix;  // rendered pixels
+  L.partial = off;  // per-tile (or per-block) chi^2 partials
+  off = align256(off + size_t(n_img) * (L.n_tiles > int((npix + 255) / 256) ? L.n_tiles : (npix + 255) / 256) * 8);
+  L.blkmax = off;  // per render block max |value|, [n_img][ceil(rendered pixels / 256)]
+  off = align256(off + size_t(n_img) * ((npix_r + 255) / 256) * 4);
+  L.hr = L.chr = L.lowu = L.lowc = L.psf_low = off;
+  if (ss.on) {
+    L.hr = off;
+    off = align256(off + size_t(n_img) * npix_r * tsz);
+    L.chr = off;
+    off = align256(off + size_t(n_img) * npix_r * tsz);
+    if (ss.kxl > 0) {
+      L.lowu = off;
+      off = align256(off + size_t(n_img) * npix * tsz);
+      L.lowc = off;
+      off = align256(off + size_t(n_img) * npix * tsz);
+      L.psf_low = off;
+      off = align256(off + size_t(ss.kxl) * kyp_of(ss.kyl) * 8);
+    }
+  }
   L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments
   if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(ky) * mfma_nks(kx) * 2 * 64 * 16);
   L.total = off;
@@ -173,7 +200,7 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
 }
 
 // Layout of lensim_loglike_grad: the image layout plus the gradient buffers.
-Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision) {
+Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, int edge) {
   Layout L = layout(m, nx, ny, kx, ky, n_img, precision);
   const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
   const size_t npix = size_t(nx) * ny;
@@ -184,6 +211,13 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
   off = align256(off + size_t(n_img) * npix * tsz);
   L.gpart = off;
   off = align256(off + size_t(n_img) * m.n_prof * ((npix + 255) / 256) * LENSIM_NPARAM * 8);
+  L.gpad = L.gext = off;
+  if (edge) {  // extended domain (ny + ky - 1) x (nx + kx - 1)
+    const size_t next = size_t(nx + kx - 1) * (ny + ky - 1);
+    off = align256(off + size_t(n_img) * next * tsz);
+    L.gext = off;
+    off = align256(off + size_t(n_img) * next * tsz);
+  }
   L.total = off;
   return L;
 }
@@ -984,6 +1018,28 @@ int psf_dims(const lensim_psf *psf, uint32_t flags, int &kx, int &ky, int &edge)
   return LENSIM_OK;
 }
 
+// Supersampled-convolution plan of a PSF on grid g (off when unconvolved,
+// without a PSF, or when the grid is not supersampled).
+int psf_ss(const lensim_psf *psf, const GridArgs &g, uint32_t flags, SSPlan &ss) {
+  ss = SSPlan();
+  const bool none = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
+  if (none || !psf->supersampled || g.s == 1) return LENSIM_OK;
+  if (psf->supersampled != 1) return fail(LENSIM_EINVAL, "psf.supersampled must be 0 or 1");
+  if (int64_t(g.nx) * g.ny * g.s * g.s > (int64_t(1) << 30)) return fail(LENSIM_EINVAL, "supersampled grid too large");
+  ss.on = 1;
+  ss.s = g.s;
+  if (psf->kx_low || psf->ky_low || psf->d_kernel_low) {
+    if (psf->type != LENSIM_PSF_PIXEL) return fail(LENSIM_EINVAL, "a split kernel needs a PIXEL psf");
+    if (psf->kx_low <= 0 || psf->ky_low <= 0 || !(psf->kx_low & 1) || !(psf->ky_low & 1))
+      return fail(LENSIM_EINVAL, "low-resolution kernel must have odd positive sizes (got %d x %d)", psf->ky_low,
+                  psf->kx_low);
+    if (!psf->d_kernel_low) return fail(LENSIM_EINVAL, "low-resolution kernel pointer is NULL");
+    ss.kxl = psf->kx_low;
+    ss.kyl = psf->ky_low;
+  }
+  return LENSIM_OK;
+}
+
 template <typename T>
 size_t conv_smem(int kx, int kyp) {
   return size_t((ConvTile<T>::TY + kyp - 1) | 1) * (ConvTile<T>::TX + kx - 1) * sizeof(T);
@@ -1090,6 +1146,221 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   return LENSIM_OK;
 }
 
+// s x s block means of [n_img][ny s][nx s] into [n_img][ny][nx]
+// (Util/image_util.py re_size).
+template <typename T>
+__global__ void pool_kernel(const T *__restrict__ in, int nx, int ny, int s, T *__restrict__ out) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= nx * ny) return;
+  const int img = blockIdx.y;
+  const int y = pix / nx, x = pix - y * nx;
+  const int nxh = nx * s;
+  const T *src = in + size_t(img) * nx * ny * s * s + size_t(y) * s * nxh + size_t(x) * s;
+  double acc = 0.0;
+  for (int sy = 0; sy < s; ++sy)
+    for (int sx = 0; sx < s; ++sx) acc += double(src[size_t(sy) * nxh + sx]);
+  out[size_t(img) * nx * ny + pix] = T(acc / (double(s) * double(s)));
+}
+
+// Final image of a supersampled convolution: the s x s mean of the convolved
+// ray fluxes plus the low-resolution part (if any), with the chi^2 of every
+// pixel reduced to one partial per block.
+template <typename T>
+__global__ __launch_bounds__(256) void combine_kernel(ConvArgs c, const T *__restrict__ chr, const T *__restrict__ lowc,
+                                                      int s, T *__restrict__ out, double *__restrict__ partial) {
+  const int img = blockIdx.y;
+  const int npix = c.nx * c.ny;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  double chi = 0.0;
+  if (pix < npix) {
+    const int y = pix / c.nx, x = pix - y * c.nx;
+    const int nxh = c.nx * s;
+    const T *src = chr + size_t(img) * npix * s * s + size_t(y) * s * nxh + size_t(x) * s;
+    double acc = 0.0;
+    for (int sy = 0; sy < s; ++sy)
+      for (int sx = 0; sx < s; ++sx) acc += double(src[size_t(sy) * nxh + sx]);
+    double v = acc / (double(s) * double(s));
+    if (lowc) v += double(lowc[size_t(img) * npix + pix]);
+    const T m = T(v);
+    if (out) out[size_t(img) * npix + pix] = m;
+    if (partial) chi = chi2_pixel<T>(c, pix, m, img);
+    if (c.gout) static_cast<T *>(c.gout)[size_t(img) * npix + pix] = T(dlogl_dm(c, pix, double(m), img));
+  }
+  if (partial) {
+    __shared__ double red[4];
+    for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+// Launches conv_kernel<T> ('same', zero or edge padding) of [n_img][ny][nx]
+// with a prepared transposed kernel, no chi^2.
+template <typename T>
+int launch_conv_plain(int nx, int ny, int kx, int ky, int edge, const T *in, const double *kf, T *out, int n_img,
+                      hipStream_t st) {
+  ConvArgs c{};
+  c.nx = nx;
+  c.ny = ny;
+  c.kx = kx;
+  c.ky = ky;
+  c.kyp = kyp_of(ky);
+  c.edge = edge;
+  c.n_tiles_x = (nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+  const int n_tiles = c.n_tiles_x * ((ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
+  const size_t smem = conv_smem<T>(kx, c.kyp);
+  if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+  }
+  conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, in, kf, out,
+                                                                                                 nullptr);
+  return check_launch("conv_kernel");
+}
+
+// Supersampled convolution (supersampling_convolution=True,
+// ImSim/Numerics/numerics.py:103-116, convolution.py:154-166, :238-252): the
+// rays are rendered as a (ny s) x (nx s) image (each ray its own pixel),
+// convolved there with the supersampled kernel, averaged over s x s, and the
+// low-resolution split kernel's convolution of the averaged unconvolved rays
+// is added.
+template <typename T>
+int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+                 const SSPlan &ss, const lensim_data *data, const double *params, int n_img, unsigned flags, T *out,
+                 double *logl, unsigned char *ws, const Layout &L, hipStream_t st) {
+  int rc = launch_prep(m, params, n_img, ws, L, st, false);
+  if (rc) return rc;
+  const int s = ss.s;
+  GridArgs gh = g;  // one pixel per ray: pixel (I, J) = sub-ray (I mod s, J mod s) of pixel (I / s, J / s)
+  const double o0 = -0.5 + 1.0 / (2.0 * s);
+  gh.nx = g.nx * s;
+  gh.ny = g.ny * s;
+  gh.s = 1;
+  gh.ra0 = g.ra0 + (g.m00 + g.m01) * o0;
+  gh.dec0 = g.dec0 + (g.m10 + g.m11) * o0;
+  gh.m00 = g.m00 / s;
+  gh.m01 = g.m01 / s;
+  gh.m10 = g.m10 / s;
+  gh.m11 = g.m11 / s;
+  gh.scale = g.scale * double(s) * double(s);  // pixel_width^2; the s x s mean follows the convolution
+  const bool mfma = use_mfma<T>(false, kx, ky);
+  const int kyp = kyp_of(ky);
+  double *kf = reinterpret_cast<double *>(ws + L.psf);
+  int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
+  half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
+  if (mfma) {
+    const int n_frag = ky * mfma_nks(kx) * 2 * 64;
+    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
+    rc = check_launch("psf_mfma_prep_kernel");
+  } else {
+    psf_prep_kernel<<<(kyp * kx + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, kyp, kf);
+    rc = check_launch("psf_prep_kernel");
+  }
+  if (rc) return rc;
+  const int npix = g.nx * g.ny, npix_h = gh.nx * gh.ny;
+  T *hr = reinterpret_cast<T *>(ws + L.hr);
+  T *chr = reinterpret_cast<T *>(ws + L.chr);
+  float *blkmax = mfma ? reinterpret_cast<float *>(ws + L.blkmax) : nullptr;
+  const double *recd = reinterpret_cast<const double *>(ws + L.recd);
+  const float *recf = reinterpret_cast<const float *>(ws + L.recf);
+  const double *coefd = reinterpret_cast<const double *>(ws + L.coefd);
+  const float *coeff = reinterpret_cast<const float *>(ws + L.coeff);
+  dim3 rgrid((npix_h + 255) / 256, n_img);
+  timing_mark(0, st);
+  if (m.has_halo)
+    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, gh, recd, recf, coefd, coeff, flags, hr, blkmax);
+  else
+    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, gh, recd, recf, coefd, coeff, flags, hr, blkmax);
+  if ((rc = check_launch("render_kernel"))) return rc;
+  timing_mark(1, st);
+  if (mfma) {
+    ConvArgs c{};
+    c.nx = gh.nx;
+    c.ny = gh.ny;
+    c.kx = kx;
+    c.ky = ky;
+    c.kyp = kyp;
+    c.edge = edge;
+    c.n_tiles_x = (gh.nx + kMfTX - 1) / kMfTX;
+    conv_mfma_kernel<<<dim3(mfma_tiles(gh.nx, gh.ny), n_img), 256, mfma_smem(kx, ky), st>>>(
+        c, mfma_nks(kx), mfma_ls(kx), reinterpret_cast<const float *>(hr), hdr, frag, blkmax, (npix_h + 255) / 256,
+        reinterpret_cast<float *>(chr), nullptr);
+    rc = check_launch("conv_mfma_kernel");
+  } else {
+    rc = launch_conv_plain<T>(gh.nx, gh.ny, kx, ky, edge, hr, kf, chr, n_img, st);
+  }
+  if (rc) return rc;
+  T *lowc = nullptr;
+  if (ss.kxl > 0) {
+    T *lowu = reinterpret_cast<T *>(ws + L.lowu);
+    lowc = reinterpret_cast<T *>(ws + L.lowc);
+    double *kfl = reinterpret_cast<double *>(ws + L.psf_low);
+    pool_kernel<T><<<dim3((npix + 255) / 256, n_img), 256, 0, st>>>(hr, g.nx, g.ny, s, lowu);
+    if ((rc = check_launch("pool_kernel"))) return rc;
+    const int kypl = kyp_of(ss.kyl);
+    psf_prep_kernel<<<(kypl * ss.kxl + 255) / 256, 256, 0, st>>>(psf->d_kernel_low, ss.kxl, ss.kyl, kypl, kfl);
+    if ((rc = check_launch("psf_prep_kernel (low)"))) return rc;
+    if ((rc = launch_conv_plain<T>(g.nx, g.ny, ss.kxl, ss.kyl, 0, lowu, kfl, lowc, n_img, st))) return rc;
+  }
+  timing_mark(2, st);
+  ConvArgs cc{};
+  cc.nx = g.nx;
+  cc.ny = g.ny;
+  double *partial = nullptr;
+  if (data) {
+    cc.data = data->d_data;
+    cc.noise = data->d_noise_map;
+    cc.expo = data->d_exposure_map;
+    cc.mask = data->d_mask;
+    cc.exposure_time = data->exposure_time;
+    cc.bkg2 = data->background_rms * data->background_rms;
+    cc.dstride = data->image_stride;
+    partial = reinterpret_cast<double *>(ws + L.partial);
+  }
+  const int nblk = (npix + 255) / 256;
+  combine_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(cc, chr, lowc, s, out, partial);
+  if ((rc = check_launch("combine_kernel"))) return rc;
+  if (data) {
+    logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, nblk, n_img, logl);
+    return check_launch("logl_kernel");
+  }
+  return LENSIM_OK;
+}
+
+// Zero-padded copy of [n_img][ny][nx] into [n_img][ny + 2 cy][nx + 2 cx].
+template <typename T>
+__global__ void pad_kernel(const T *__restrict__ in, int nx, int ny, int cx, int cy, T *__restrict__ out) {
+  const int nxe = nx + 2 * cx, nye = ny + 2 * cy;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nxe * nye) return;
+  const int img = blockIdx.y;
+  const int Y = idx / nxe, X = idx - Y * nxe;
+  const int y = Y - cy, x = X - cx;
+  const T v = (y >= 0 && y < ny && x >= 0 && x < nx) ? in[size_t(img) * nx * ny + size_t(y) * nx + x] : T(0);
+  out[size_t(img) * nxe * nye + idx] = v;
+}
+
+// Adjoint of edge clamping: pixel (y, x) receives every extended pixel
+// (Y, X) with clamp(Y - cy) = y and clamp(X - cx) = x.
+template <typename T>
+__global__ void fold_kernel(const T *__restrict__ ext, int nx, int ny, int cx, int cy, T *__restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nx * ny) return;
+  const int img = blockIdx.y;
+  const int y = idx / nx, x = idx - y * nx;
+  const int nxe = nx + 2 * cx, nye = ny + 2 * cy;
+  const int y0 = y == 0 ? 0 : y + cy, y1 = y == ny - 1 ? nye - 1 : y + cy;
+  const int x0 = x == 0 ? 0 : x + cx, x1 = x == nx - 1 ? nxe - 1 : x + cx;
+  const T *e = ext + size_t(img) * nxe * nye;
+  double s = 0.0;
+  for (int Y = y0; Y <= y1; ++Y)
+    for (int X = x0; X <= x1; ++X) s += double(e[size_t(Y) * nxe + X]);
+  out[size_t(img) * nx * ny + idx] = T(s);
+}
+
 // log-likelihood and its gradient: the forward pipeline writes dlogL/dmodel,
 // the adjoint of the PSF (a correlation with the kernel) gives dlogL/d(pooled
 // image), and grad_kernel accumulates d(ray flux)/d(kwargs) per profile.
@@ -1123,9 +1394,33 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
   }
   T *glow = reinterpret_cast<T *>(ws + L.glow);
-  conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, gmodel, kf, glow,
-                                                                                                 nullptr);
-  if ((rc = check_launch("conv_kernel (adjoint)"))) return rc;
+  if (!edge) {
+    conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, gmodel, kf, glow,
+                                                                                                   nullptr);
+    if ((rc = check_launch("conv_kernel (adjoint)"))) return rc;
+  } else {
+    // Edge padding: the forward pass is a valid correlation of the
+    // edge-extended image E[Y][X] = in[clamp(Y - cy)][clamp(X - cx)], so its
+    // adjoint is the zero-padded 'same' correlation of dlogL/dmodel over the
+    // extended domain, folded back onto the pixels each extended one copies.
+    const int cx = (kx - 1) / 2, cy = (ky - 1) / 2;
+    const int nxe = g.nx + 2 * cx, nye = g.ny + 2 * cy;
+    T *gpad = reinterpret_cast<T *>(ws + L.gpad);
+    T *gext = reinterpret_cast<T *>(ws + L.gext);
+    const int ne = nxe * nye;
+    pad_kernel<T><<<dim3((ne + 255) / 256, n_img), 256, 0, st>>>(gmodel, g.nx, g.ny, cx, cy, gpad);
+    if ((rc = check_launch("pad_kernel"))) return rc;
+    ConvArgs ce = c;
+    ce.nx = nxe;
+    ce.ny = nye;
+    ce.n_tiles_x = (nxe + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
+    const int n_tiles_e = ce.n_tiles_x * ((nye + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
+    conv_kernel<T><<<dim3(n_tiles_e, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(ce, gpad, kf,
+                                                                                                     gext, nullptr);
+    if ((rc = check_launch("conv_kernel (adjoint, extended)"))) return rc;
+    fold_kernel<T><<<dim3((g.nx * g.ny + 255) / 256, n_img), 256, 0, st>>>(gext, g.nx, g.ny, cx, cy, glow);
+    if ((rc = check_launch("fold_kernel"))) return rc;
+  }
   // per-profile gradients
   GradArgs ga;
   ga.params = params;
@@ -1200,13 +1495,16 @@ size_t lensim_workspace_size(const lensim_model *model, const lensim_grid *grid,
   ModelArgs m;
   if (model_args(model, m)) return 0;
   int nx = 1, ny = 1;
+  SSPlan ss;
   if (grid) {
     nx = grid->nx;
     ny = grid->ny;
+    GridArgs g;
+    if (grid_args(grid, g) || psf_ss(psf, g, 0, ss)) return 0;
   }
   int kx = 1, ky = 1, edge = 0;
   if (psf_dims(psf, 0, kx, ky, edge)) return 0;
-  return layout(m, nx, ny, kx, ky, n_img, precision).total;
+  return layout(m, nx, ny, kx, ky, n_img, precision, ss).total;
 }
 
 int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n_img, const double *d_x,
@@ -1300,10 +1598,20 @@ int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensi
   // kernel dims used by the layout: the real PSF unless unconvolved
   int lkx = 1, lky = 1, le = 0;
   psf_dims(psf, 0, lkx, lky, le);
-  const Layout L = layout(m, g.nx, g.ny, lkx, lky, n_img, precision);
+  SSPlan ss, ss_layout;
+  if ((rc = psf_ss(psf, g, flags, ss))) return rc;
+  if ((rc = psf_ss(psf, g, 0, ss_layout))) return rc;
+  const Layout L = layout(m, g.nx, g.ny, lkx, lky, n_img, precision, ss_layout);
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
+  if (ss.on) {
+    if (precision == LENSIM_FP64)
+      return run_image_ss<double>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
+                                  static_cast<double *>(d_out), nullptr, w, L, st);
+    return run_image_ss<float>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
+                               static_cast<float *>(d_out), nullptr, w, L, st);
+  }
   if (precision == LENSIM_FP64)
     return run_image<double>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags,
                              static_cast<double *>(d_out), nullptr, w, L, st);
@@ -1331,11 +1639,20 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
   if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
   if (!d_logl || (m.n_prof && !d_params)) return fail(LENSIM_EINVAL, "NULL buffer");
   if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
-  const Layout L = layout(m, g.nx, g.ny, kx, ky, n_img, precision);
+  SSPlan ss;
+  if ((rc = psf_ss(psf, g, 0, ss))) return rc;
+  const Layout L = layout(m, g.nx, g.ny, kx, ky, n_img, precision, ss);
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
   const unsigned flags = LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT;
+  if (ss.on) {
+    if (precision == LENSIM_FP64)
+      return run_image_ss<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
+                                  static_cast<double *>(d_model_out), d_logl, w, L, st);
+    return run_image_ss<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
+                               static_cast<float *>(d_model_out), d_logl, w, L, st);
+  }
   if (precision == LENSIM_FP64)
     return run_image<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags,
                              static_cast<double *>(d_model_out), d_logl, w, L, st);
@@ -1349,7 +1666,7 @@ size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *
   if (model_args(model, m) || !grid) return 0;
   int kx = 1, ky = 1, edge = 0;
   if (psf_dims(psf, 0, kx, ky, edge)) return 0;
-  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision).total;
+  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision, edge).total;
 }
 
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -1363,7 +1680,11 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if ((rc = grid_args(grid, g))) return rc;
   int kx, ky, edge;
   if ((rc = psf_dims(psf, 0, kx, ky, edge))) return rc;
-  if (edge) return fail(LENSIM_EUNSUPPORTED, "gradients with the edge-padded GAUSSIAN PSF are not supported");
+  {
+    SSPlan ss;
+    if ((rc = psf_ss(psf, g, 0, ss))) return rc;
+    if (ss.on) return fail(LENSIM_EUNSUPPORTED, "gradients with supersampled convolution are not supported");
+  }
   if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
   if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
     return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
@@ -1374,7 +1695,7 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");
   if (!d_logl || !d_grad || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
   if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
-  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision);
+  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge);
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);

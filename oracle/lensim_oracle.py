@@ -498,23 +498,36 @@ def moffat_kernel(size, fwhm_pix, beta):
 def image(grid, lens_model_list, kwargs_lens, source_list, kwargs_source,
           lens_light_list, kwargs_lens_light, psf_type="PIXEL", kernel=None,
           flux_scaling=1.0, alpha_scaling=1.0, unconvolved=False,
-          source_add=True, lens_light_add=True):
+          source_add=True, lens_light_add=True, supersampling_convolution=False, kernel_low=None):
     """ImageModel.image, ImSim/image_model.py:431-492.
 
     grid: dict(nx, ny, transform_pix2angle, ra_at_xy_0, dec_at_xy_0, s).
     Source light: rays -> source profiles -> x flux_scaling (:313-368) ->
     re_size_convolve (numerics.py:158-177); lens light: profiles on the grid ->
     re_size_convolve -> x flux_scaling (:370-393).
+    supersampling_convolution (s > 1): ``kernel`` is sampled on the supersampled
+    grid and convolves the ray fluxes before the s x s mean; ``kernel_low`` (a
+    split kernel's low-resolution part, or None) convolves the mean of the
+    unconvolved rays and is added (SubgridKernelConvolution.re_size_convolve,
+    ImSim/Numerics/convolution.py:154-166; GaussianConvolution.re_size_convolve,
+    :238-252).
     """
     nx, ny, s = grid["nx"], grid["ny"], grid["s"]
     M = np.asarray(grid["transform_pix2angle"], float)
     pixel_width = np.sqrt(abs(np.linalg.det(M)))
     x, y = subgrid_coordinates(nx, ny, M, grid["ra_at_xy_0"], grid["dec_at_xy_0"], s)
+    mode = "edge" if psf_type == "GAUSSIAN" else "zero"
 
     def re_size_convolve(flux):
-        img = re_size(flux.reshape(ny * s, nx * s), s)
+        hr = flux.reshape(ny * s, nx * s)
+        if supersampling_convolution and s > 1 and not unconvolved and psf_type != "NONE":
+            img = re_size(convolve_same(hr, kernel, mode=mode), s)
+            if kernel_low is not None:
+                img = img + convolve_same(re_size(hr, s), kernel_low, mode="zero")
+            return img * pixel_width ** 2
+        img = re_size(hr, s)
         if not unconvolved and psf_type != "NONE":
-            img = convolve_same(img, kernel, mode="edge" if psf_type == "GAUSSIAN" else "zero")
+            img = convolve_same(img, kernel, mode=mode)
         return img * pixel_width ** 2
 
     model = np.zeros((ny, nx))

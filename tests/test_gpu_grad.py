@@ -100,6 +100,49 @@ def test_grad_vs_oracle_fd_small():
                 assert abs(grads[j][name] - fd) <= 1e-5 * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-5), ("mixed", 1e-3)])
+def test_grad_gaussian_psf_vs_oracle_fd(precision, tol):
+    """GAUSSIAN PSF (edge-padded convolution, its adjoint folds the padding back
+    onto the border pixels): gradients against central differences of the CPU
+    oracle, with the source near the frame edge so the border matters."""
+    cfg = dict(W.EPL180, numPix=20, supersampling=2, deltaPix=0.15)
+    g = W.grid_dict(cfg)
+    psf = PSF("GAUSSIAN", fwhm=0.4, truncation=3)
+    kernel = O.gaussian_kernel(0.4, cfg["deltaPix"], 3)
+    assert kernel.shape[0] >= 7
+    kl, ks, kll = W.epl180_draw(4)
+    ks = [dict(ks[0], center_x=1.2, center_y=-1.1)]
+    kl = [dict(kl[0], center_x=0.6), kl[1]]
+    kll = [dict(kll[0], center_x=-1.3, center_y=1.35, R_sersic=0.4)]
+    m = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, psf_type="GAUSSIAN",
+                kernel=kernel)
+    data = W.noisy(m, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(6))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf, LensModel(["EPL", "SHEAR"]),
+                    LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+    kl = [dict(kl[0], theta_E=kl[0]["theta_E"] * 1.02), kl[1]]
+    logl, glens, gsrc, glight = im.likelihood_gradient(kl, ks, kll)
+
+    def ll(klx, ksx, kllx):
+        mm = O.image(g, ["EPL", "SHEAR"], klx, ["SERSIC_ELLIPSE"], ksx, ["SERSIC_ELLIPSE"], kllx,
+                     psf_type="GAUSSIAN", kernel=kernel)
+        return O.log_likelihood(mm, data, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"])
+
+    ll0 = ll(kl, ks, kll)
+    assert abs(float(logl) - ll0) < (1e-9 if precision == "fp64" else 1e-6) * abs(ll0)
+    for which, grads, kwl in ((0, glens, kl), (1, gsrc, ks), (2, glight, kll)):
+        for j, kw in enumerate(kwl):
+            for name in grads[j]:
+                h = 1e-6 * max(abs(kw[name]), 0.1)
+                lists = [[dict(d) for d in kl], [dict(d) for d in ks], [dict(d) for d in kll]]
+                lists[which][j][name] = kw[name] + h
+                up = ll(*lists)
+                lists[which][j][name] = kw[name] - h
+                dn = ll(*lists)
+                fd = (up - dn) / (2 * h)
+                assert abs(grads[j][name] - fd) <= tol * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
+
+
 def test_torch_autograd_backward():
     im, truth = _model("mixed", n=60, s=2, k=11)
     params = torch.as_tensor(_perturbed(im, truth, 3, 2), device="cuda").requires_grad_(True)

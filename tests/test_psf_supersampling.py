@@ -1,0 +1,103 @@
+"""Supersampled PSF helpers (restated lenstronomy kernel_util behaviour, no
+reference output available): invariants of degrade_kernel and split_kernel,
+and the oracle's supersampled convolution."""
+
+import numpy as np
+import pytest
+
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.psf import PSF, degrade_kernel, split_kernel
+from oracle import lensim_oracle as O
+
+
+def test_degrade_kernel_block_means_and_centring():
+    k = W.moffat_kernel(21, 6.0, 3.0)
+    d = degrade_kernel(k, 3)
+    # numPix = round(21 / 3 + 0.5) = 8 -> made odd: 9, the 21 pixels padded by 3 on each side
+    assert d.shape == (9, 9)
+    np.testing.assert_allclose(d.sum() * 9, k.sum(), rtol=1e-14)
+    np.testing.assert_allclose(d, d[::-1, ::-1], rtol=1e-14)  # centred: symmetric stays symmetric
+    assert np.unravel_index(np.argmax(d), d.shape) == (4, 4)
+    np.testing.assert_allclose(d[4, 4], k[9:12, 9:12].mean(), rtol=1e-14)
+    assert np.all(d[0, :] == 0) or np.all(d[0, :] < d[1, :])
+    # a kernel whose size is not a multiple of the factor is zero-padded symmetrically
+    k2 = W.moffat_kernel(19, 6.0, 3.0)
+    d2 = degrade_kernel(k2, 3)
+    assert d2.shape[0] % 2 == 1
+    np.testing.assert_allclose(d2.sum() * 9, k2.sum(), rtol=1e-14)
+    np.testing.assert_allclose(d2, d2.T, rtol=1e-14)
+    with pytest.raises(NotImplementedError):
+        degrade_kernel(k, 2)
+
+
+def test_split_kernel_invariants():
+    k = W.moffat_kernel(27, 6.0, 2.5)
+    k = k / k.sum()
+    low, cut = split_kernel(k, 3, 3)
+    # degraded grid: numPix = round(27 / 3 + 0.5) = 10 -> made odd: 11
+    assert cut.shape == (9, 9) and low.shape == (11, 11)
+    np.testing.assert_allclose(cut, k[9:18, 9:18], rtol=0)  # the central 3 x 3 pixels, supersampled
+    np.testing.assert_allclose(low.sum() + cut.sum(), 1.0, rtol=1e-14)  # unit flux together
+    assert np.all(low[4:7, 4:7] == 0)  # the hole under the cut
+    assert np.all(low[3, 3:8] > 0) and np.all(low[7, 3:8] > 0)
+    np.testing.assert_allclose(low, low.T, rtol=1e-14)
+    # a cut wider than the kernel keeps the whole kernel and an empty low part
+    low2, cut2 = split_kernel(k, 11, 3)
+    np.testing.assert_allclose(cut2, k, rtol=1e-14)
+    assert np.all(low2 == 0)
+    with pytest.raises(ValueError):
+        split_kernel(k, 4, 3)
+    with pytest.raises(ValueError):
+        split_kernel(k, 3, 1)
+
+
+def test_psf_supersampled_kernel():
+    k = W.moffat_kernel(21, 6.0, 3.0)
+    psf = PSF("PIXEL", kernel_point_source=k, point_source_supersampling_factor=3)
+    np.testing.assert_allclose(psf.kernel_point_source_supersampled(3), k / k.sum(), rtol=1e-14)
+    reg = psf.kernel_point_source
+    assert reg.shape == (9, 9)
+    np.testing.assert_allclose(reg.sum(), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(reg, degrade_kernel(k, 3) / degrade_kernel(k, 3).sum(), rtol=1e-14)
+    with pytest.raises(NotImplementedError):
+        psf.kernel_point_source_supersampled(5)
+
+
+def _grid(n=24, s=3, d=0.1):
+    cfg = dict(W.EPL180, numPix=n, supersampling=s, deltaPix=d)
+    return cfg, W.grid_dict(cfg)
+
+
+def test_oracle_supersampled_convolution_limits():
+    cfg, g = _grid(n=40, s=3, d=0.1)
+    kl, ks, kll = W.epl180_draw(1)
+    args = (g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll)
+    unconv = O.image(*args, unconvolved=True)
+    # a delta kernel on the supersampled grid leaves the image unconvolved
+    delta = np.zeros((9, 9))
+    delta[4, 4] = 1.0
+    np.testing.assert_allclose(O.image(*args, kernel=delta, supersampling_convolution=True), unconv, rtol=1e-13,
+                               atol=1e-15)
+    # independent path: render the rays as pixels of a grid 3x finer (origin
+    # moved to the first sub-pixel), convolve with scipy's FFT, block-average
+    from scipy import signal
+
+    kc = O.gaussian_kernel(0.1, 0.1 / 3, 3)
+    M = np.asarray(g["transform_pix2angle"])
+    o0 = -0.5 + 1.0 / 6.0
+    g_hr = dict(nx=120, ny=120, s=1, transform_pix2angle=M / 3, ra_at_xy_0=g["ra_at_xy_0"] + (M[0, 0] + M[0, 1]) * o0,
+                dec_at_xy_0=g["dec_at_xy_0"] + (M[1, 0] + M[1, 1]) * o0)
+    u_hr = O.image(g_hr, *args[1:], unconvolved=True) * 9.0  # x (pixel / sub-pixel area)
+    want = O.re_size(signal.fftconvolve(u_hr, kc, mode="same"), 3)
+    np.testing.assert_allclose(O.image(*args, kernel=kc, supersampling_convolution=True), want, rtol=1e-9,
+                               atol=1e-12 * np.abs(want).max())
+    k = W.moffat_kernel(27, 6.0, 3.0)
+    k = k / k.sum()
+    # split: the low part plus the cut together approximate the full supersampled convolution
+    low, cut = split_kernel(k, 3, 3)
+    full = O.image(*args, kernel=k, supersampling_convolution=True)
+    split = O.image(*args, kernel=cut, kernel_low=low, supersampling_convolution=True)
+    assert 0 < np.max(np.abs(split - full)) < 2e-2 * np.max(np.abs(full))
+    # and both differ from convolving the pooled image with the degraded kernel
+    plain = O.image(*args, kernel=degrade_kernel(k, 3) * 9)
+    assert np.max(np.abs(plain - full)) > np.max(np.abs(split - full))
