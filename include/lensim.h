@@ -205,7 +205,7 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
  * test/test_ImSim/test_image_model.py:274-335; EPL's custom JVP epl.py:321-340).
  * Adjoint path: dlogL/dmodel (model-based variance included) -> correlation
  * with the PSF -> per-ray forward-mode derivatives of each profile.
- * Not supported: bulk deflector lists, supersampled convolution. */
+ * Not supported: bulk deflector lists. */
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                                   int32_t n_img, int32_t precision);
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,

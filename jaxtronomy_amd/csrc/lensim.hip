@@ -100,6 +100,7 @@ struct Layout {
   size_t recd, recf, coefd, coeff, dcoefd, dcoeff, psf, lowres, partial, blkmax, bfrag;
   size_t gmodel, glow, gpart;  // gradient: dlogL/dmodel, dlogL/dlowres, per-block partial gradients
   size_t gpad, gext;           // edge-padded PSF adjoint: padded dlogL/dmodel and its correlation
+  size_t gchr, glu;            // supersampled convolution adjoint buffers
   size_t hr, chr, lowu, lowc, psf_low;  // supersampled convolution: ray fluxes, their convolution,
                                         // pooled unconvolved image and its low-resolution convolution
   size_t total;
@@ -199,24 +200,35 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   return L;
 }
 
-// Layout of lensim_loglike_grad: the image layout plus the gradient buffers.
-Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, int edge) {
-  Layout L = layout(m, nx, ny, kx, ky, n_img, precision);
+// Layout of lensim_loglike_grad: the image layout plus the gradient buffers
+// (on the ray grid when the convolution is supersampled).
+Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, int edge,
+                   const SSPlan &ss = SSPlan()) {
+  Layout L = layout(m, nx, ny, kx, ky, n_img, precision, ss);
   const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
   const size_t npix = size_t(nx) * ny;
+  const int cs = ss.on ? ss.s : 1;
+  const size_t npix_r = npix * cs * cs;  // pixels of the grid the convolution runs on
   size_t off = L.total;
   L.gmodel = off;
   off = align256(off + size_t(n_img) * npix * tsz);
   L.glow = off;
-  off = align256(off + size_t(n_img) * npix * tsz);
+  off = align256(off + size_t(n_img) * npix_r * tsz);
   L.gpart = off;
-  off = align256(off + size_t(n_img) * m.n_prof * ((npix + 255) / 256) * LENSIM_NPARAM * 8);
-  L.gpad = L.gext = off;
-  if (edge) {  // extended domain (ny + ky - 1) x (nx + kx - 1)
-    const size_t next = size_t(nx + kx - 1) * (ny + ky - 1);
+  off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 255) / 256) * LENSIM_NPARAM * 8);
+  L.gpad = L.gext = L.gchr = L.glu = off;
+  if (edge) {  // extended domain (ny cs + ky - 1) x (nx cs + kx - 1)
+    const size_t next = size_t(nx * cs + kx - 1) * (ny * cs + ky - 1);
+    L.gpad = off;
     off = align256(off + size_t(n_img) * next * tsz);
     L.gext = off;
     off = align256(off + size_t(n_img) * next * tsz);
+  }
+  if (ss.on) {
+    L.gchr = off;  // dlogL / d(convolved rays)
+    off = align256(off + size_t(n_img) * npix_r * tsz);
+    L.glu = off;   // dlogL / d(pooled unconvolved rays), split kernel
+    off = align256(off + size_t(n_img) * npix * tsz);
   }
   L.total = off;
   return L;
@@ -1221,20 +1233,12 @@ int launch_conv_plain(int nx, int ny, int kx, int ky, int edge, const T *in, con
   return check_launch("conv_kernel");
 }
 
-// Supersampled convolution (supersampling_convolution=True,
-// ImSim/Numerics/numerics.py:103-116, convolution.py:154-166, :238-252): the
-// rays are rendered as a (ny s) x (nx s) image (each ray its own pixel),
-// convolved there with the supersampled kernel, averaged over s x s, and the
-// low-resolution split kernel's convolution of the averaged unconvolved rays
-// is added.
-template <typename T>
-int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
-                 const SSPlan &ss, const lensim_data *data, const double *params, int n_img, unsigned flags, T *out,
-                 double *logl, unsigned char *ws, const Layout &L, hipStream_t st) {
-  int rc = launch_prep(m, params, n_img, ws, L, st, false);
-  if (rc) return rc;
-  const int s = ss.s;
-  GridArgs gh = g;  // one pixel per ray: pixel (I, J) = sub-ray (I mod s, J mod s) of pixel (I / s, J / s)
+// The rays of grid g as a grid of their own: pixel (I, J) is sub-ray
+// (I mod s, J mod s) of pixel (I / s, J / s) (make_subgrid offsets), with the
+// pixel_width^2 scale because the s x s mean follows the convolution.
+GridArgs ray_grid(const GridArgs &g) {
+  const int s = g.s;
+  GridArgs gh = g;
   const double o0 = -0.5 + 1.0 / (2.0 * s);
   gh.nx = g.nx * s;
   gh.ny = g.ny * s;
@@ -1245,7 +1249,37 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
   gh.m01 = g.m01 / s;
   gh.m10 = g.m10 / s;
   gh.m11 = g.m11 / s;
-  gh.scale = g.scale * double(s) * double(s);  // pixel_width^2; the s x s mean follows the convolution
+  gh.scale = g.scale * double(s) * double(s);
+  return gh;
+}
+
+// Adjoint of the s x s mean: out[J][I] (+)= in[J / s][I / s] / s^2.
+template <typename T>
+__global__ void unpool_kernel(const T *__restrict__ in, int nx, int ny, int s, int accumulate, T *__restrict__ out) {
+  const int nxh = nx * s, nyh = ny * s;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nxh * nyh) return;
+  const int img = blockIdx.y;
+  const int J = idx / nxh, I = idx - J * nxh;
+  const double v = double(in[size_t(img) * nx * ny + size_t(J / s) * nx + I / s]) / (double(s) * double(s));
+  T *o = out + size_t(img) * nxh * nyh + idx;
+  *o = accumulate ? T(double(*o) + v) : T(v);
+}
+
+// Supersampled convolution (supersampling_convolution=True,
+// ImSim/Numerics/numerics.py:103-116, convolution.py:154-166, :238-252): the
+// rays are rendered as a (ny s) x (nx s) image (each ray its own pixel),
+// convolved there with the supersampled kernel, averaged over s x s, and the
+// low-resolution split kernel's convolution of the averaged unconvolved rays
+// is added.
+template <typename T>
+int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+                 const SSPlan &ss, const lensim_data *data, const double *params, int n_img, unsigned flags, T *out,
+                 double *logl, unsigned char *ws, const Layout &L, hipStream_t st, T *gout = nullptr) {
+  int rc = launch_prep(m, params, n_img, ws, L, st, gout != nullptr);
+  if (rc) return rc;
+  const int s = ss.s;
+  const GridArgs gh = ray_grid(g);
   const bool mfma = use_mfma<T>(false, kx, ky);
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
@@ -1318,6 +1352,7 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
     cc.exposure_time = data->exposure_time;
     cc.bkg2 = data->background_rms * data->background_rms;
     cc.dstride = data->image_stride;
+    cc.gout = gout;
     partial = reinterpret_cast<double *>(ws + L.partial);
   }
   const int nblk = (npix + 255) / 256;
@@ -1361,65 +1396,69 @@ __global__ void fold_kernel(const T *__restrict__ ext, int nx, int ny, int cx, i
   out[size_t(img) * nx * ny + idx] = T(s);
 }
 
+// Adjoint of the 'same' convolution of [n_img][ny][nx] by a kernel prepared
+// UNflipped in kf (a correlation): zero padding directly; edge padding via the
+// extended domain.  The forward pass with edge padding is a valid correlation
+// of the edge-extended image E[Y][X] = in[clamp(Y - cy)][clamp(X - cx)], so its
+// adjoint is the zero-padded 'same' correlation over the extended domain,
+// folded back onto the pixels each extended one copies.
+template <typename T>
+int adjoint_conv(int nx, int ny, int kx, int ky, int edge, const T *gin, const double *kf, T *gout, T *gpad, T *gext,
+                 int n_img, hipStream_t st) {
+  if (!edge) return launch_conv_plain<T>(nx, ny, kx, ky, 0, gin, kf, gout, n_img, st);
+  const int cx = (kx - 1) / 2, cy = (ky - 1) / 2;
+  const int nxe = nx + 2 * cx, nye = ny + 2 * cy;
+  const int ne = nxe * nye;
+  pad_kernel<T><<<dim3((ne + 255) / 256, n_img), 256, 0, st>>>(gin, nx, ny, cx, cy, gpad);
+  int rc = check_launch("pad_kernel");
+  if (rc) return rc;
+  if ((rc = launch_conv_plain<T>(nxe, nye, kx, ky, 0, gpad, kf, gext, n_img, st))) return rc;
+  fold_kernel<T><<<dim3((nx * ny + 255) / 256, n_img), 256, 0, st>>>(gext, nx, ny, cx, cy, gout);
+  return check_launch("fold_kernel");
+}
+
 // log-likelihood and its gradient: the forward pipeline writes dlogL/dmodel,
-// the adjoint of the PSF (a correlation with the kernel) gives dlogL/d(pooled
-// image), and grad_kernel accumulates d(ray flux)/d(kwargs) per profile.
+// the adjoint of the PSF (a correlation with the kernel) gives dlogL/d(ray
+// image) -- through the s x s mean's adjoint first when the convolution is
+// supersampled -- and grad_kernel accumulates d(ray flux)/d(kwargs) per
+// profile over the grid the convolution saw.
 template <typename T>
 int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
-             const lensim_data *data, const double *params, int n_img, double *logl, double *grad,
+             const SSPlan &ss, const lensim_data *data, const double *params, int n_img, double *logl, double *grad,
              unsigned char *ws, const Layout &L, hipStream_t st) {
   T *gmodel = reinterpret_cast<T *>(ws + L.gmodel);
-  int rc = run_image<T>(m, g, psf, kx, ky, edge, data, params, n_img, LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT,
-                        nullptr, logl, ws, L, st, gmodel);
+  const unsigned flags = LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT;
+  int rc = ss.on ? run_image_ss<T>(m, g, psf, kx, ky, edge, ss, data, params, n_img, flags, nullptr, logl, ws, L, st,
+                                   gmodel)
+                 : run_image<T>(m, g, psf, kx, ky, edge, data, params, n_img, flags, nullptr, logl, ws, L, st, gmodel);
   if (rc) return rc;
-  // adjoint of the convolution (zero padding): correlate with the kernel
   const bool identity = !psf || psf->type == LENSIM_PSF_NONE;
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
   psf_prep_kernel<<<(kyp * kx + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kyp, kf, 0);
   if ((rc = check_launch("psf_prep_kernel"))) return rc;
-  ConvArgs c{};
-  c.nx = g.nx;
-  c.ny = g.ny;
-  c.kx = kx;
-  c.ky = ky;
-  c.kyp = kyp;
-  c.edge = 0;
-  c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
-  const int n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
-  const size_t smem = conv_smem<T>(kx, kyp);
-  if (smem > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-  }
   T *glow = reinterpret_cast<T *>(ws + L.glow);
-  if (!edge) {
-    conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, gmodel, kf, glow,
-                                                                                                   nullptr);
-    if ((rc = check_launch("conv_kernel (adjoint)"))) return rc;
+  T *gpad = reinterpret_cast<T *>(ws + L.gpad);
+  T *gext = reinterpret_cast<T *>(ws + L.gext);
+  const GridArgs gg = ss.on ? ray_grid(g) : g;  // the grid whose pixels the rays fill
+  if (!ss.on) {
+    if ((rc = adjoint_conv<T>(g.nx, g.ny, kx, ky, edge, gmodel, kf, glow, gpad, gext, n_img, st))) return rc;
   } else {
-    // Edge padding: the forward pass is a valid correlation of the
-    // edge-extended image E[Y][X] = in[clamp(Y - cy)][clamp(X - cx)], so its
-    // adjoint is the zero-padded 'same' correlation of dlogL/dmodel over the
-    // extended domain, folded back onto the pixels each extended one copies.
-    const int cx = (kx - 1) / 2, cy = (ky - 1) / 2;
-    const int nxe = g.nx + 2 * cx, nye = g.ny + 2 * cy;
-    T *gpad = reinterpret_cast<T *>(ws + L.gpad);
-    T *gext = reinterpret_cast<T *>(ws + L.gext);
-    const int ne = nxe * nye;
-    pad_kernel<T><<<dim3((ne + 255) / 256, n_img), 256, 0, st>>>(gmodel, g.nx, g.ny, cx, cy, gpad);
-    if ((rc = check_launch("pad_kernel"))) return rc;
-    ConvArgs ce = c;
-    ce.nx = nxe;
-    ce.ny = nye;
-    ce.n_tiles_x = (nxe + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
-    const int n_tiles_e = ce.n_tiles_x * ((nye + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
-    conv_kernel<T><<<dim3(n_tiles_e, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(ce, gpad, kf,
-                                                                                                     gext, nullptr);
-    if ((rc = check_launch("conv_kernel (adjoint, extended)"))) return rc;
-    fold_kernel<T><<<dim3((g.nx * g.ny + 255) / 256, n_img), 256, 0, st>>>(gext, g.nx, g.ny, cx, cy, glow);
-    if ((rc = check_launch("fold_kernel"))) return rc;
+    T *gchr = reinterpret_cast<T *>(ws + L.gchr);
+    const int nh = gg.nx * gg.ny;
+    unpool_kernel<T><<<dim3((nh + 255) / 256, n_img), 256, 0, st>>>(gmodel, g.nx, g.ny, ss.s, 0, gchr);
+    if ((rc = check_launch("unpool_kernel"))) return rc;
+    if ((rc = adjoint_conv<T>(gg.nx, gg.ny, kx, ky, edge, gchr, kf, glow, gpad, gext, n_img, st))) return rc;
+    if (ss.kxl > 0) {  // low-resolution part: adjoint of its convolution, then of the mean
+      const int kypl = kyp_of(ss.kyl);
+      double *kfl = reinterpret_cast<double *>(ws + L.psf_low);
+      psf_prep_kernel<<<(kypl * ss.kxl + 255) / 256, 256, 0, st>>>(psf->d_kernel_low, ss.kxl, ss.kyl, kypl, kfl, 0);
+      if ((rc = check_launch("psf_prep_kernel (low)"))) return rc;
+      T *glu = reinterpret_cast<T *>(ws + L.glu);
+      if ((rc = launch_conv_plain<T>(g.nx, g.ny, ss.kxl, ss.kyl, 0, gmodel, kfl, glu, n_img, st))) return rc;
+      unpool_kernel<T><<<dim3((nh + 255) / 256, n_img), 256, 0, st>>>(glu, g.nx, g.ny, ss.s, 1, glow);
+      if ((rc = check_launch("unpool_kernel"))) return rc;
+    }
   }
   // per-profile gradients
   GradArgs ga;
@@ -1430,14 +1469,14 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   ga.dcoeff = reinterpret_cast<const float *>(ws + L.dcoeff);
   ga.glow = glow;
   ga.gpart = reinterpret_cast<double *>(ws + L.gpart);
-  const int npix = g.nx * g.ny, nblk = (npix + 255) / 256;
+  const int npix = gg.nx * gg.ny, nblk = (npix + 255) / 256;
   dim3 grid(nblk, n_img, m.n_prof);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
   if (m.has_halo)
-    grad_kernel<T, true><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+    grad_kernel<T, true><<<grid, 256, 0, st>>>(m, gg, ga, recd, recf, ga.coefd, ga.coeff);
   else
-    grad_kernel<T, false><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+    grad_kernel<T, false><<<grid, 256, 0, st>>>(m, gg, ga, recd, recf, ga.coefd, ga.coeff);
   if ((rc = check_launch("grad_kernel"))) return rc;
   const int n = n_img * m.n_prof * LENSIM_NPARAM;
   grad_reduce_kernel<<<(n + 255) / 256, 256, 0, st>>>(ga.gpart, nblk, n, grad);
@@ -1666,7 +1705,10 @@ size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *
   if (model_args(model, m) || !grid) return 0;
   int kx = 1, ky = 1, edge = 0;
   if (psf_dims(psf, 0, kx, ky, edge)) return 0;
-  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision, edge).total;
+  GridArgs g;
+  SSPlan ss;
+  if (grid_args(grid, g) || psf_ss(psf, g, 0, ss)) return 0;
+  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision, edge, ss).total;
 }
 
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -1680,11 +1722,8 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if ((rc = grid_args(grid, g))) return rc;
   int kx, ky, edge;
   if ((rc = psf_dims(psf, 0, kx, ky, edge))) return rc;
-  {
-    SSPlan ss;
-    if ((rc = psf_ss(psf, g, 0, ss))) return rc;
-    if (ss.on) return fail(LENSIM_EUNSUPPORTED, "gradients with supersampled convolution are not supported");
-  }
+  SSPlan ss;
+  if ((rc = psf_ss(psf, g, 0, ss))) return rc;
   if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
   if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
     return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
@@ -1695,13 +1734,13 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");
   if (!d_logl || !d_grad || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
   if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
-  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge);
+  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge, ss);
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
   if (precision == LENSIM_FP64)
-    return run_grad<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, d_logl, d_grad, w, L, st);
-  return run_grad<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, d_logl, d_grad, w, L, st);
+    return run_grad<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+  return run_grad<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
 }
 
 }  // extern "C"
