@@ -103,7 +103,10 @@ typedef struct {
 
 /* Pixel grid + supersampling (ImSim/Numerics/grid.py:22-64): pixel (i, j)
  * (column i fastest) has centre ra = ra_at_xy_0 + M00 i + M01 j,
- * dec = dec_at_xy_0 + M10 i + M11 j; images are [ny][nx]. */
+ * dec = dec_at_xy_0 + M10 i + M11 j; images are [ny][nx].
+ * d_eval_mask (RegularGrid flux_evaluate_indexes, grid.py:47-58): when not
+ * NULL, only pixels with a non-zero mask byte shoot rays; the others have
+ * zero flux (before the convolution). */
 typedef struct {
   int32_t nx;
   int32_t ny;
@@ -111,7 +114,8 @@ typedef struct {
   int32_t reserved;
   double ra_at_xy_0;
   double dec_at_xy_0;
-  double pix2angle[4]; /* M00 M01 M10 M11 */
+  double pix2angle[4];         /* M00 M01 M10 M11 */
+  const uint8_t *d_eval_mask;  /* [ny][nx] device bytes, or NULL for every pixel */
 } lensim_grid;
 
 /* PSF convolution (ImSim/Numerics/numerics.py:102-146).
@@ -196,6 +200,17 @@ int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensi
 int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                    const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
                    double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream);
+
+/* log-likelihood of given model images against the data:
+ * -1/2 sum mask (model - data)^2 / (C_D(model) + |error|), as lensim_loglike
+ * computes it for its own models.
+ * Replaces ImageData.log_likelihood (Data/imaging_data.py:159-182).
+ * d_model: [n_img][ny][nx], float64 (precision LENSIM_FP64) or float32
+ * (LENSIM_MIXED); the sums run in float64.  Needs ws of
+ * lensim_data_loglike_workspace_size bytes. */
+size_t lensim_data_loglike_workspace_size(int32_t nx, int32_t ny, int32_t n_img);
+int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const void *d_model, int32_t n_img,
+                        int32_t precision, double *d_logl, void *ws, size_t ws_bytes, void *stream);
 
 /* log-likelihood and its gradient with respect to every kwarg of every
  * profile, for a batch: d_grad is [n_img][n_profiles][LENSIM_NPARAM] in the

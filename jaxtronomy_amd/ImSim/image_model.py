@@ -78,6 +78,15 @@ class ImageModel(object):
         M = np.asarray(data_class.transform_pix2angle, dtype=np.float64)
         for i, v in enumerate([M[0, 0], M[0, 1], M[1, 0], M[1, 1]]):
             self._grid.pix2angle[i] = float(v)
+        self._full_shape = (int(ny), int(nx))
+        self._eval_mask_dev = None
+        if self._frame is not None:
+            x_min, y_min, n = self._frame
+            self._grid.nx = self._grid.ny = n
+            ra, dec = data_class.map_pix2coord(x_min, y_min)
+            self._grid.ra_at_xy_0, self._grid.dec_at_xy_0 = float(ra), float(dec)
+            self._eval_mask_dev = torch.as_tensor(self._eval_mask, device=backend.device())
+            self._grid.d_eval_mask = self._eval_mask_dev.data_ptr()
         self._psf = backend.LensimPsf()
         self._kernel_dev = None
         if self._kernel is None:
@@ -120,8 +129,10 @@ class ImageModel(object):
             raise ValueError("supersampling_factor must be >= 1")
         if supersampling_factor == 1:
             supersampling_convolution = False
+        self._frame = None
+        self._eval_mask = None
         if flux_evaluate_indexes is not None:
-            raise NotImplementedError("flux_evaluate_indexes (sub-frame evaluation) is not supported yet")
+            self._init_sub_frame(np.asarray(flux_evaluate_indexes, dtype=bool))
         if convolution_type not in ("fft", "grid", "fft_static"):
             raise ValueError("convolution_type %s not supported!" % convolution_type)
         self._s = supersampling_factor
@@ -160,6 +171,30 @@ class ImageModel(object):
             self._kernel = None
         else:
             raise ValueError("psf_type %s not valid! Chose either NONE, GAUSSIAN or PIXEL." % self._psf_type)
+
+    def _init_sub_frame(self, mask):
+        """NumericsSubFrame (ImSim/Numerics/numerics_subframe.py:61-101, :135-191):
+        the model is computed on the smallest rectangle holding every True pixel
+        (rays shot only there, the convolution zero-padded at the rectangle's
+        edge) and the rest of the frame is zero.  As in the reference, the
+        rectangle's first-axis range [x_min, x_max] and second-axis range
+        [y_min, y_max] index the image as [x_min:x_max+1, y_min:y_max+1] while
+        the rectangle's grid starts at map_pix2coord(x_min, y_min), and the
+        rectangle must be square (the reference assigns an (ny_sub, nx_sub)
+        image into an (nx_sub, ny_sub) slot)."""
+        nx, ny = self.Data.num_pixel_axes
+        if mask.shape != (ny, nx) and mask.shape != (nx, ny):
+            raise ValueError("flux_evaluate_indexes has shape %s for a %dx%d image" % (mask.shape, ny, nx))
+        a, b = np.where(mask)
+        if a.size == 0:
+            raise ValueError("flux_evaluate_indexes selects no pixel")
+        x_min, x_max, y_min, y_max = int(a.min()), int(a.max()), int(b.min()), int(b.max())
+        n1, n2 = x_max - x_min + 1, y_max - y_min + 1
+        if n1 != n2:
+            raise ValueError("the sub-frame of flux_evaluate_indexes is %d x %d; the reference only handles square "
+                             "sub-frames" % (n1, n2))
+        self._frame = (x_min, y_min, n1)
+        self._eval_mask = np.ascontiguousarray(mask[x_min:x_max + 1, y_min:y_max + 1].astype(np.uint8))
 
     @property
     def kernel(self):
@@ -233,12 +268,19 @@ class ImageModel(object):
             n_img = params.shape[0]
         dtype = torch.float64 if self._prec == backend.FP64 else torch.float32
         if out is None:
-            out = torch.empty((n_img, self._grid.ny, self._grid.nx), dtype=dtype, device=params.device)
+            out = torch.empty((n_img,) + self._full_shape, dtype=dtype, device=params.device)
+        sub = out
+        if self._frame is not None:
+            sub = torch.empty((n_img, self._grid.ny, self._grid.nx), dtype=dtype, device=params.device)
         ws = self._ws.get(self.workspace_bytes(n_img, m))
         rc = lib.lensim_image(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
-                              backend.ptr(params), n_img, self._prec, flags, backend.ptr(out), backend.ptr(ws),
+                              backend.ptr(params), n_img, self._prec, flags, backend.ptr(sub), backend.ptr(ws),
                               ws.numel(), backend.stream_handle())
         backend.check(rc, "lensim_image")
+        if self._frame is not None:  # NumericsSubFrame._complete_frame (numerics_subframe.py:135-148)
+            x_min, y_min, n = self._frame
+            out.zero_()
+            out[:, x_min:x_min + n, y_min:y_min + n] = sub
         return out
 
     def _data_struct(self, data=None, noise_map=None):
@@ -285,6 +327,17 @@ class ImageModel(object):
         if logl is None:
             logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
         d = self._data_struct(data, noise_map)
+        if self._frame is not None:
+            # the model is zero outside the sub-frame, so the likelihood runs on
+            # the completed frame (ImageData.log_likelihood)
+            model_img = self.image_batch(params, n_img, model=m, out=model_out)
+            ny, nx = self._full_shape
+            nbytes = lib.lensim_data_loglike_workspace_size(nx, ny, n_img)
+            ws = self._ws.get(nbytes)
+            rc = lib.lensim_data_loglike(ctypes.byref(d), nx, ny, backend.ptr(model_img), n_img, self._prec,
+                                         backend.ptr(logl), backend.ptr(ws), ws.numel(), backend.stream_handle())
+            backend.check(rc, "lensim_data_loglike")
+            return logl
         ws = self._ws.get(self.workspace_bytes(n_img, m))
         rc = lib.lensim_loglike(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
                                 ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
@@ -298,6 +351,8 @@ class ImageModel(object):
         lib = backend.lib()
         if self._bulk:
             raise NotImplementedError("gradients of bulk deflector lists are not supported")
+        if self._frame is not None:
+            raise NotImplementedError("gradients with flux_evaluate_indexes sub-frames are not supported")
         if n_img is None:
             n_img = params.shape[0]
         params = params.contiguous()

@@ -56,6 +56,7 @@ class LensimGrid(ctypes.Structure):
         ("ra_at_xy_0", ctypes.c_double),
         ("dec_at_xy_0", ctypes.c_double),
         ("pix2angle", ctypes.c_double * 4),
+        ("d_eval_mask", ctypes.c_void_p),
     ]
 
 
@@ -98,6 +99,9 @@ _SIGNATURES = {
                                     ctypes.c_size_t, _P]),
     "lensim_loglike": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                       ctypes.c_size_t, _P]),
+    "lensim_data_loglike_workspace_size": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "lensim_data_loglike": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32,
+                                           _P, _P, ctypes.c_size_t, _P]),
     "lensim_grad_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                            ctypes.c_size_t, _P]),

@@ -405,7 +405,16 @@ struct GridArgs {
   int nx, ny, s;
   double ra0, dec0, m00, m01, m10, m11;
   double scale;  // pixel_width^2 / s^2
+  const uint8_t *mask;  // evaluation mask of the image pixels (NULL: all), [mask_ny][mask_nx]
+  int mask_nx, mask_s;  // pixel (i, j) of this grid is mask pixel (i / mask_s, j / mask_s)
 };
+
+// Whether grid pixel (i, j) is evaluated (RegularGrid flux_evaluate_indexes,
+// ImSim/Numerics/grid.py:47-58: the rays of other pixels are not shot and
+// their flux is zero).
+__device__ __forceinline__ bool evaluated(const GridArgs &g, int i, int j) {
+  return !g.mask || g.mask[size_t(j / g.mask_s) * g.mask_nx + i / g.mask_s] != 0;
+}
 
 template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
@@ -429,7 +438,8 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
   const int k_src0 = m.n_defl, k_src1 = m.n_defl + m.n_source, k_l1 = m.n_prof;
   double acc = 0.0;
-  for (int sy = 0; active && sy < g.s; ++sy) {
+  const bool eval = active && evaluated(g, i, j);
+  for (int sy = 0; eval && sy < g.s; ++sy) {
     const double oy = o0 + sy * inv_s;
     for (int sx = 0; sx < g.s; ++sx) {
       const double ox = o0 + sx * inv_s;
@@ -916,7 +926,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
   const double inv_s = 1.0 / g.s;
   const double o0 = -0.5 + 1.0 / (2.0 * g.s);
   // a lens profile only matters if a source is there to be lensed
-  const bool live = active && w != 0.0 && (k >= m.n_defl || m.n_source > 0);
+  const bool live = active && w != 0.0 && (k >= m.n_defl || m.n_source > 0) && evaluated(g, i, j);
   for (int sy = 0; live && sy < g.s; ++sy) {
     const double oy = o0 + sy * inv_s;
     for (int sx = 0; sx < g.s; ++sx) {
@@ -1009,6 +1019,9 @@ int grid_args(const lensim_grid *grid, GridArgs &g) {
   const double det = g.m00 * g.m11 - g.m01 * g.m10;
   const double pixel_width = sqrt(fabs(det));  // lenstronomy PixelGrid.pixel_width
   g.scale = pixel_width * pixel_width / (double(g.s) * double(g.s));
+  g.mask = grid->d_eval_mask;
+  g.mask_nx = g.nx;
+  g.mask_s = 1;
   return LENSIM_OK;
 }
 
@@ -1207,6 +1220,22 @@ __global__ __launch_bounds__(256) void combine_kernel(ConvArgs c, const T *__res
   }
 }
 
+// chi^2 of given model images, one partial per block of 256 pixels.
+template <typename T>
+__global__ __launch_bounds__(256) void data_chi2_kernel(ConvArgs c, const T *__restrict__ model,
+                                                        double *__restrict__ partial) {
+  const int img = blockIdx.y;
+  const int npix = c.nx * c.ny;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  double chi = 0.0;
+  if (pix < npix) chi = chi2_pixel<T>(c, pix, model[size_t(img) * npix + pix], img);
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 // Launches conv_kernel<T> ('same', zero or edge padding) of [n_img][ny][nx]
 // with a prepared transposed kernel, no chi^2.
 template <typename T>
@@ -1250,6 +1279,7 @@ GridArgs ray_grid(const GridArgs &g) {
   gh.m10 = g.m10 / s;
   gh.m11 = g.m11 / s;
   gh.scale = g.scale * double(s) * double(s);
+  gh.mask_s = g.mask_s * s;
   return gh;
 }
 
@@ -1697,6 +1727,52 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
                              static_cast<double *>(d_model_out), d_logl, w, L, st);
   return run_image<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<float *>(d_model_out),
                           d_logl, w, L, st);
+}
+
+size_t lensim_data_loglike_workspace_size(int32_t nx, int32_t ny, int32_t n_img) {
+  if (nx <= 0 || ny <= 0 || n_img < 0) return 0;
+  return align256(size_t(n_img) * ((size_t(nx) * ny + 255) / 256) * 8) + 256;
+}
+
+int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const void *d_model, int32_t n_img,
+                        int32_t precision, double *d_logl, void *ws, size_t ws_bytes, void *stream) {
+  if (nx <= 0 || ny <= 0) return fail(LENSIM_EINVAL, "image sizes must be positive (nx=%d ny=%d)", nx, ny);
+  if (int64_t(nx) * ny > (int64_t(1) << 30)) return fail(LENSIM_EINVAL, "image too large");
+  if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
+  if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
+    return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (data->image_stride != 0 && data->image_stride < int64_t(nx) * ny)
+    return fail(LENSIM_EINVAL, "image_stride %lld is smaller than an image", (long long)data->image_stride);
+  if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
+  if (n_img == 0) return LENSIM_OK;
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_model || !d_logl) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const size_t need = lensim_data_loglike_workspace_size(nx, ny, n_img);
+  if (ws_bytes < need) return fail(LENSIM_EWORKSPACE, "workspace of %zu B is smaller than the %zu B needed", ws_bytes,
+                                   need);
+  if (!ws || (reinterpret_cast<uintptr_t>(ws) & 255)) return fail(LENSIM_EINVAL, "workspace must be 256-byte aligned");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  ConvArgs c{};
+  c.nx = nx;
+  c.ny = ny;
+  c.data = data->d_data;
+  c.noise = data->d_noise_map;
+  c.expo = data->d_exposure_map;
+  c.mask = data->d_mask;
+  c.exposure_time = data->exposure_time;
+  c.bkg2 = data->background_rms * data->background_rms;
+  c.dstride = data->image_stride;
+  double *partial = static_cast<double *>(ws);
+  const int nblk = (nx * ny + 255) / 256;
+  if (precision == LENSIM_FP64)
+    data_chi2_kernel<double><<<dim3(nblk, n_img), 256, 0, st>>>(c, static_cast<const double *>(d_model), partial);
+  else
+    data_chi2_kernel<float><<<dim3(nblk, n_img), 256, 0, st>>>(c, static_cast<const float *>(d_model), partial);
+  int rc = check_launch("data_chi2_kernel");
+  if (rc) return rc;
+  logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, nblk, n_img, d_logl);
+  return check_launch("logl_kernel");
 }
 
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,

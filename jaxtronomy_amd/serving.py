@@ -113,6 +113,8 @@ class _Bucket(object):
     def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs, noise_maps):
         if image_model._bulk:
             raise NotImplementedError("the server takes ImageModels with a plain lens model list")
+        if image_model._frame is not None:
+            raise NotImplementedError("the server takes ImageModels without flux_evaluate_indexes sub-frames")
         self.key = key
         self.im = image_model
         self.max_batch = int(max_batch)

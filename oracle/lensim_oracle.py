@@ -498,7 +498,8 @@ def moffat_kernel(size, fwhm_pix, beta):
 def image(grid, lens_model_list, kwargs_lens, source_list, kwargs_source,
           lens_light_list, kwargs_lens_light, psf_type="PIXEL", kernel=None,
           flux_scaling=1.0, alpha_scaling=1.0, unconvolved=False,
-          source_add=True, lens_light_add=True, supersampling_convolution=False, kernel_low=None):
+          source_add=True, lens_light_add=True, supersampling_convolution=False, kernel_low=None,
+          flux_evaluate_indexes=None, _eval=None):
     """ImageModel.image, ImSim/image_model.py:431-492.
 
     grid: dict(nx, ny, transform_pix2angle, ra_at_xy_0, dec_at_xy_0, s).
@@ -511,14 +512,43 @@ def image(grid, lens_model_list, kwargs_lens, source_list, kwargs_source,
     unconvolved rays and is added (SubgridKernelConvolution.re_size_convolve,
     ImSim/Numerics/convolution.py:154-166; GaussianConvolution.re_size_convolve,
     :238-252).
+    flux_evaluate_indexes (bool [ny, nx] or None): NumericsSubFrame
+    (ImSim/Numerics/numerics_subframe.py:61-191) -- the model of the square
+    rectangle [x_min:x_max+1, y_min:y_max+1] around the True pixels, on a grid
+    starting at pixel (x_min, y_min) (map_pix2coord), with the rays of False
+    pixels not shot (RegularGrid, grid.py:47-58, :126-150), convolved inside
+    the rectangle and zero elsewhere.
     """
+    if flux_evaluate_indexes is not None:
+        mask = np.asarray(flux_evaluate_indexes, bool)
+        a, b = np.where(mask)
+        x_min, x_max, y_min, y_max = a.min(), a.max(), b.min(), b.max()
+        n = x_max - x_min + 1
+        assert y_max - y_min + 1 == n, "square sub-frames only"
+        M = np.asarray(grid["transform_pix2angle"], float)
+        sub = dict(grid, nx=n, ny=n, ra_at_xy_0=grid["ra_at_xy_0"] + M[0, 0] * x_min + M[0, 1] * y_min,
+                   dec_at_xy_0=grid["dec_at_xy_0"] + M[1, 0] * x_min + M[1, 1] * y_min)
+        sub_img = image(sub, lens_model_list, kwargs_lens, source_list, kwargs_source, lens_light_list,
+                        kwargs_lens_light, psf_type, kernel, flux_scaling, alpha_scaling, unconvolved, source_add,
+                        lens_light_add, supersampling_convolution, kernel_low, None,
+                        _eval=mask[x_min:x_max + 1, y_min:y_max + 1])
+        full = np.zeros((grid["ny"], grid["nx"]))
+        full[x_min:x_max + 1, y_min:y_max + 1] = sub_img
+        return full
     nx, ny, s = grid["nx"], grid["ny"], grid["s"]
     M = np.asarray(grid["transform_pix2angle"], float)
     pixel_width = np.sqrt(abs(np.linalg.det(M)))
     x, y = subgrid_coordinates(nx, ny, M, grid["ra_at_xy_0"], grid["dec_at_xy_0"], s)
     mode = "edge" if psf_type == "GAUSSIAN" else "zero"
 
+    if _eval is not None:  # rays of pixels outside the mask carry no flux
+        keep = np.repeat(np.repeat(np.asarray(_eval, bool), s, axis=0), s, axis=1).ravel()
+    else:
+        keep = None
+
     def re_size_convolve(flux):
+        if keep is not None:
+            flux = np.where(keep, flux, 0.0)
         hr = flux.reshape(ny * s, nx * s)
         if supersampling_convolution and s > 1 and not unconvolved and psf_type != "NONE":
             img = re_size(convolve_same(hr, kernel, mode=mode), s)

@@ -390,3 +390,42 @@ def test_supersampled_convolution(variant, precision, tol, ltol):
         plain = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll,
                         kernel=psf.kernel_point_source)
         assert img_err(plain, want) > 1e-4
+
+
+@pytest.mark.parametrize("psf_type", ["PIXEL", "GAUSSIAN"])
+@pytest.mark.parametrize("precision,tol,ltol", [("fp64", FP64_TOL, FP64_TOL), ("mixed", MIXED_IMG_TOL,
+                                                                                  MIXED_LOGL_TOL)])
+def test_flux_evaluate_indexes_sub_frame(psf_type, precision, tol, ltol):
+    """flux_evaluate_indexes: rays only in the mask's pixels, the model of the
+    square sub-frame around them convolved there, zero elsewhere; image and logL
+    (over the whole frame) against the oracle, with x_min != y_min."""
+    cfg = dict(W.EPL180, numPix=40, supersampling=2, deltaPix=0.1)
+    g = W.grid_dict(cfg)
+    kl, ks, kll = W.epl180_draw(8)
+    mask = np.zeros((40, 40), bool)
+    yy, xx = np.mgrid[0:40, 0:40]
+    mask[(yy - 17) ** 2 + (xx - 21) ** 2 < 13 ** 2] = True  # a disc: square bounding box 4..30 x 8..34
+    if psf_type == "PIXEL":
+        kernel = W.moffat_kernel(9, 2.0, 3.0)
+        psf = PSF("PIXEL", kernel_point_source=kernel)
+    else:
+        psf = PSF("GAUSSIAN", fwhm=0.25, truncation=3)
+        kernel = O.gaussian_kernel(0.25, cfg["deltaPix"], 3)
+    want = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, psf_type=psf_type,
+                   kernel=kernel, flux_evaluate_indexes=mask)
+    data = W.noisy(O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll,
+                           psf_type=psf_type, kernel=kernel), cfg["exposure_time"], cfg["background_rms"],
+                   np.random.default_rng(4))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf, LensModel(["EPL", "SHEAR"]),
+                    LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": mask}, precision=precision)
+    assert im._frame == (4, 8, 27)
+    got = _np(im.image(kl, ks, kll)).astype(np.float64)
+    assert got.shape == (40, 40)
+    assert img_err(got, want) < tol
+    assert np.all(got[:4] == 0)
+    ll = float(im.likelihood_data_given_model(kl, ks, kll))
+    ll_want = O.log_likelihood(want, data, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"])
+    assert abs(ll - ll_want) < ltol * abs(ll_want)
+    with pytest.raises(NotImplementedError):
+        im.likelihood_gradient(kl, ks, kll)

@@ -325,3 +325,34 @@ def test_image_point_mass_like_ring():
     npt.assert_allclose(img, img[::-1, ::-1], rtol=1e-12, atol=1e-15)
     ring = img[30, 30 + 20]  # r = 1.0 arcsec = theta_E: brightest
     assert ring == img.max()
+
+
+def test_sub_frame_semantics():
+    """NumericsSubFrame restated: outside the rectangle zero; deep inside equal to
+    the full-frame model; unconvolved masked pixels zero; all-True equals no mask;
+    the rectangle's grid starts at map_pix2coord(x_min, y_min) while it is placed
+    at [x_min:, y_min:] (the reference's index order)."""
+    from jaxtronomy_amd import workloads as W
+
+    cfg = dict(W.EPL180, numPix=24, supersampling=2, deltaPix=0.12)
+    g = W.grid_dict(cfg)
+    kl, ks, kll = W.epl180_draw(3)
+    kern = W.moffat_kernel(5, 1.5, 3.0)
+    args = (g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll)
+    full = O.image(*args, kernel=kern)
+    np.testing.assert_array_equal(O.image(*args, kernel=kern, flux_evaluate_indexes=np.ones((24, 24), bool)), full)
+    mask = np.zeros((24, 24), bool)
+    mask[4:20, 4:20] = True  # x_min = y_min: no transposition
+    mask[10, 11] = False
+    sub = O.image(*args, kernel=kern, flux_evaluate_indexes=mask)
+    assert np.all(sub[:4] == 0) and np.all(sub[:, 20:] == 0)
+    np.testing.assert_allclose(sub[6:8, 6:18], full[6:8, 6:18], rtol=1e-13)  # 2+ pixels from edge and hole
+    un = O.image(*args, unconvolved=True, flux_evaluate_indexes=mask)
+    assert un[10, 11] == 0 and un[10, 12] != 0
+    # x_min != y_min: local pixel (i, j) is full-grid pixel (col x_min + j, row y_min + i),
+    # written at [x_min + i, y_min + j]
+    mask2 = np.zeros((24, 24), bool)
+    mask2[2:12, 7:17] = True
+    un2 = O.image(*args, unconvolved=True, flux_evaluate_indexes=mask2)
+    un_full = O.image(*args, unconvolved=True)
+    np.testing.assert_allclose(un2[2:12, 7:17], un_full[7:17, 2:12], rtol=1e-13)
