@@ -212,6 +212,31 @@ size_t lensim_data_loglike_workspace_size(int32_t nx, int32_t ny, int32_t n_img)
 int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const void *d_model, int32_t n_img,
                         int32_t precision, double *d_logl, void *ws, size_t ws_bytes, void *stream);
 
+/* Linear amplitude solve: the amplitude of every source and lens-light
+ * profile (SERSIC, SERSIC_ELLIPSE, GAUSSIAN 'amp', in profile order) by
+ * weighted least squares against the data, then the likelihood of the solved
+ * model.  Replaces ImageLinearFit.image_linear_solve and
+ * likelihood_data_given_model (ImSim/image_linear_solve.py:98-181, :229-340)
+ * with linear_response_matrix (:362-455) and de_lens.get_param_WLS /
+ * marginalization_new (ImSim/de_lens.py:10-77):
+ *   A_l = the model image of light profile l alone at amp = 1 (x flux_scaling),
+ *   w = mask / C_D(data) (the data-based variance, Data.C_D),
+ *   M = A W A^T, R = A W d, amp = M^-1 R if cond(M) < 5 / eps else 0,
+ *   model = sum_l amp_l A_l inside the mask, 0 outside,
+ *   logL = -1/2 sum mask (model - d)^2 / C_D(model)
+ *          [+ marginalisation constant of M^-1 when source_marg].
+ * linear_prior <= 0 means none.  Outputs (device, per image of the batch):
+ * d_amp [n_img][n_lin] or NULL, d_logl [n_img], d_model_out [n_img][ny][nx]
+ * or NULL, d_cov [n_img][n_lin][n_lin] (M^-1, zero when unstable) or NULL,
+ * d_response_out [n_lin][n_img][ny][nx] (the A_l) or NULL; images in the
+ * compute type (float64 / float32).  n_lin = n_source + n_lens_light <= 8. */
+size_t lensim_linear_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                    int32_t n_img, int32_t precision);
+int lensim_linear_solve(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                        const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                        int32_t source_marg, double linear_prior, double *d_amp, double *d_logl, void *d_model_out,
+                        double *d_cov, void *d_response_out, void *ws, size_t ws_bytes, void *stream);
+
 /* log-likelihood and its gradient with respect to every kwarg of every
  * profile, for a batch: d_grad is [n_img][n_profiles][LENSIM_NPARAM] in the
  * parameter-block layout (slots beyond a profile's kwargs are zero).

@@ -102,6 +102,9 @@ _SIGNATURES = {
     "lensim_data_loglike_workspace_size": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "lensim_data_loglike": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32,
                                            _P, _P, ctypes.c_size_t, _P]),
+    "lensim_linear_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
+    "lensim_linear_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_double, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P]),
     "lensim_grad_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                            ctypes.c_size_t, _P]),

@@ -1513,6 +1513,341 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   return check_launch("grad_reduce_kernel");
 }
 
+// ------------------------------------------------- linear amplitude solve --
+// ImageLinearFit.image_linear_solve (ImSim/image_linear_solve.py:98-181): the
+// response of every linear light profile (its image at amp = 1,
+// linear_response_matrix :362-455), weighted least squares against the data
+// (de_lens.get_param_WLS, ImSim/de_lens.py:10-36) and the likelihood of the
+// solved model (:229-340).
+constexpr int kMaxLin = 8;  // linear amplitudes per model
+constexpr int kLinAcc = kMaxLin * (kMaxLin + 1) / 2 + kMaxLin;  // M (upper triangle) and R
+
+// params with the amplitude (slot 0) of every light profile set to 1
+__global__ void amp_one_kernel(const double *__restrict__ params, int n_img, int n_prof, int k_light0,
+                               double *__restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_img * n_prof * LENSIM_NPARAM) return;
+  const int q = idx % LENSIM_NPARAM, k = (idx / LENSIM_NPARAM) % n_prof;
+  out[idx] = (k >= k_light0 && q == 0) ? 1.0 : params[idx];
+}
+
+// Weights mask / C_D(data) (ImageModel.error_response, image_model.py:577-596:
+// Data.C_D, the variance estimated from the data) and per-block partial sums
+// of M = A W A^T (upper triangle) and R = A W d.  A is [n_lin][n_img][npix].
+template <typename T>
+__global__ __launch_bounds__(256) void normal_kernel(ConvArgs c, const T *__restrict__ A, int n_lin, int n_img,
+                                                     double *__restrict__ part) {
+  const int img = blockIdx.y;
+  const int npix = c.nx * c.ny;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kLinAcc];
+#pragma unroll
+  for (int q = 0; q < kLinAcc; ++q) acc[q] = 0.0;
+  if (p < npix) {
+    const size_t pd = p + size_t(img) * size_t(c.dstride);
+    const double d = c.data[pd];
+    double var;
+    if (c.noise) {
+      var = c.noise[pd] * c.noise[pd];
+    } else {
+      const double t = c.expo ? c.expo[p] : c.exposure_time;
+      var = (d >= 0.0 ? d : 0.0) / t + c.bkg2;
+    }
+    const double w = (c.mask ? c.mask[p] : 1.0) / var;
+    double a[kMaxLin];
+#pragma unroll
+    for (int l = 0; l < kMaxLin; ++l) a[l] = l < n_lin ? double(A[(size_t(l) * n_img + img) * npix + p]) : 0.0;
+    if (w != 0.0) {
+      int q = 0;
+#pragma unroll
+      for (int i = 0; i < kMaxLin; ++i) {
+#pragma unroll
+        for (int j = i; j < kMaxLin; ++j) acc[q++] = a[i] * w * a[j];
+      }
+#pragma unroll
+      for (int i = 0; i < kMaxLin; ++i) acc[q++] = a[i] * w * d;
+    }
+  }
+  __shared__ double red[4][kLinAcc];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < kLinAcc; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kLinAcc; q += blockDim.x)
+    part[(size_t(img) * gridDim.x + blockIdx.x) * kLinAcc + q] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+}
+
+// Eigenvalues of a symmetric n x n matrix (cyclic Jacobi rotations).
+__device__ void sym_eigenvalues(double (*a)[kMaxLin], int n, double *ev) {
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a[i][j] * a[i][j];
+    if (off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        if (a[p][q] == 0.0) continue;
+        const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) ev[i] = a[i][i];
+}
+
+// LU factorisation with partial pivoting in place; returns the permutation
+// parity (+1 / -1), 0 if a pivot is exactly zero.
+__device__ int lu_factor(double (*a)[kMaxLin], int n, int *piv) {
+  int parity = 1;
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(a[i][k]) > fabs(a[pr][k])) pr = i;
+    piv[k] = pr;
+    if (pr != k) {
+      parity = -parity;
+      for (int j = 0; j < n; ++j) {
+        const double t = a[k][j];
+        a[k][j] = a[pr][j];
+        a[pr][j] = t;
+      }
+    }
+    if (a[k][k] == 0.0) return 0;
+    for (int i = k + 1; i < n; ++i) {
+      a[i][k] /= a[k][k];
+      for (int j = k + 1; j < n; ++j) a[i][j] -= a[i][k] * a[k][j];
+    }
+  }
+  return parity;
+}
+
+__device__ void lu_solve(const double (*lu)[kMaxLin], const int *piv, int n, double *b) {
+  for (int k = 0; k < n; ++k) {
+    const double t = b[k];
+    b[k] = b[piv[k]];
+    b[piv[k]] = t;
+  }
+  for (int i = 1; i < n; ++i)
+    for (int j = 0; j < i; ++j) b[i] -= lu[i][j] * b[j];
+  for (int i = n - 1; i >= 0; --i) {
+    for (int j = i + 1; j < n; ++j) b[i] -= lu[i][j] * b[j];
+    b[i] /= lu[i][i];
+  }
+}
+
+// One thread per image: sums the partials, checks cond(M) < 5 / eps, solves
+// M B = R (B = 0 when unstable or not finite), optionally inverts M
+// (nan_to_num'ed, zero when unstable) and the marginalisation constant
+// (de_lens.marginalisation_const / marginalization_new, de_lens.py:40-77).
+__global__ void lin_solve_kernel(const double *__restrict__ part, int nblk, int n_lin, int n_img, int source_marg,
+                                 double d_prior, double *__restrict__ amp, double *__restrict__ cov,
+                                 double *__restrict__ marg) {
+  const int img = blockIdx.x * blockDim.x + threadIdx.x;
+  if (img >= n_img) return;
+  double acc[kLinAcc];
+  for (int q = 0; q < kLinAcc; ++q) acc[q] = 0.0;
+  for (int b = 0; b < nblk; ++b)
+    for (int q = 0; q < kLinAcc; ++q) acc[q] += part[(size_t(img) * nblk + b) * kLinAcc + q];
+  double M[kMaxLin][kMaxLin], W[kMaxLin][kMaxLin], R[kMaxLin];
+  int q = 0;
+  for (int i = 0; i < kMaxLin; ++i)
+    for (int j = i; j < kMaxLin; ++j) {
+      M[i][j] = M[j][i] = acc[q];
+      ++q;
+    }
+  for (int i = 0; i < kMaxLin; ++i) R[i] = acc[q++];
+  const int n = n_lin;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) W[i][j] = M[i][j];
+  double ev[kMaxLin];
+  sym_eigenvalues(W, n, ev);
+  double emax = 0.0, emin = INFINITY;
+  for (int i = 0; i < n; ++i) {
+    emax = fmax(emax, fabs(ev[i]));
+    emin = fmin(emin, fabs(ev[i]));
+  }
+  const double eps = 2.220446049250313e-16;
+  const bool stable = emin > 0.0 && emax / emin < 5.0 / eps;
+  int piv[kMaxLin];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) W[i][j] = M[i][j];
+  const int parity = lu_factor(W, n, piv);
+  double B[kMaxLin];
+  for (int i = 0; i < n; ++i) B[i] = R[i];
+  bool finite = stable && parity != 0;
+  if (finite) {
+    lu_solve(W, piv, n, B);
+    for (int i = 0; i < n; ++i) finite = finite && isfinite(B[i]);
+  }
+  for (int i = 0; i < n; ++i) amp[size_t(img) * n + i] = finite ? B[i] : 0.0;
+  if (cov) {
+    for (int j = 0; j < n; ++j) {
+      double e[kMaxLin];
+      for (int i = 0; i < n; ++i) e[i] = i == j ? 1.0 : 0.0;
+      if (stable && parity != 0) lu_solve(W, piv, n, e);
+      for (int i = 0; i < n; ++i) {
+        const double v = (stable && parity != 0 && isfinite(e[i])) ? e[i] : 0.0;
+        cov[(size_t(img) * n + i) * n + j] = v;
+      }
+    }
+  }
+  double mc = 0.0;
+  if (source_marg) {
+    if (!(stable && parity != 0)) {
+      mc = -1e15;  // slogdet / eig of a zero covariance
+    } else if (!(d_prior > 0.0)) {
+      // 1/2 log det(M^-1) = -1/2 sum log|U_ii| with the sign of det(M)
+      double logdet = 0.0;
+      int sign = parity;
+      for (int i = 0; i < n; ++i) {
+        logdet += log(fabs(W[i][i]));
+        if (W[i][i] < 0.0) sign = -sign;
+      }
+      mc = sign * (-logdet) / 2.0;
+    } else {
+      // eigenvalues of M^-1 = 1 / eigenvalues of M, clipped at d_prior^2
+      double ld = 0.0, sgn = 1.0;
+      for (int i = 0; i < n; ++i) {
+        const double v = 1.0 / ev[i];
+        sgn *= v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0);
+        ld += log(fmin(fabs(v), d_prior * d_prior));
+      }
+      const double log_det = ld * sgn;
+      mc = isnan(log_det) ? -1e15 : log_det / 2.0 + n / 2.0 * log(M_PI / 2.0) - n * log(d_prior);
+    }
+  }
+  marg[img] = mc;
+}
+
+// Solved model (zero outside the likelihood mask, array_masked2image) and its chi^2.
+template <typename T>
+__global__ __launch_bounds__(256) void lin_model_kernel(ConvArgs c, const T *__restrict__ A, const double *__restrict__ amp,
+                                                        int n_lin, int n_img, T *__restrict__ out,
+                                                        double *__restrict__ partial) {
+  const int img = blockIdx.y;
+  const int npix = c.nx * c.ny;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double chi = 0.0;
+  if (p < npix) {
+    double m = 0.0;
+    if (!c.mask || c.mask[p] != 0.0)
+      for (int l = 0; l < n_lin; ++l) m += amp[size_t(img) * n_lin + l] * double(A[(size_t(l) * n_img + img) * npix + p]);
+    const T mt = T(m);
+    if (out) out[size_t(img) * npix + p] = mt;
+    chi = chi2_pixel<T>(c, p, mt, img);
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void add_kernel(double *__restrict__ x, const double *__restrict__ y, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += y[i];
+}
+
+// Workspace of the linear solve: the image layout of one column, then the
+// columns A, the amp = 1 parameters, partials, amplitudes and constants.
+struct LinLayout {
+  Layout img;
+  size_t A, params1, part, chi, amp, marg, total;
+};
+
+LinLayout lin_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, const SSPlan &ss,
+                     int n_lin) {
+  LinLayout L;
+  L.img = layout(m, nx, ny, kx, ky, n_img, precision, ss);
+  const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
+  const size_t npix = size_t(nx) * ny, nblk = (npix + 255) / 256;
+  size_t off = L.img.total;
+  L.A = off;
+  off = align256(off + size_t(n_lin) * n_img * npix * tsz);
+  L.params1 = off;
+  off = align256(off + size_t(n_img) * m.n_prof * LENSIM_NPARAM * 8);
+  L.part = off;
+  off = align256(off + size_t(n_img) * nblk * kLinAcc * 8);
+  L.chi = off;
+  off = align256(off + size_t(n_img) * nblk * 8);
+  L.amp = off;
+  off = align256(off + size_t(n_img) * kMaxLin * 8);
+  L.marg = off;
+  off = align256(off + size_t(n_img) * 8);
+  L.total = off;
+  return L;
+}
+
+template <typename T>
+int run_linear(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+               const SSPlan &ss, const lensim_data *data, const double *params, int n_img, int source_marg,
+               double d_prior, double *amp_out, double *logl, T *model_out, double *cov, T *resp_out,
+               unsigned char *ws, const LinLayout &LL, hipStream_t st) {
+  const int n_lin = m.n_source + m.n_light;
+  const int npix = g.nx * g.ny, nblk = (npix + 255) / 256;
+  double *params1 = reinterpret_cast<double *>(ws + LL.params1);
+  const int ntot = n_img * m.n_prof * LENSIM_NPARAM;
+  amp_one_kernel<<<(ntot + 255) / 256, 256, 0, st>>>(params, n_img, m.n_prof, m.n_defl, params1);
+  int rc = check_launch("amp_one_kernel");
+  if (rc) return rc;
+  T *A = resp_out ? resp_out : reinterpret_cast<T *>(ws + LL.A);
+  // one column per linear profile: the model with every other light profile off
+  for (int l = 0; l < n_lin; ++l) {
+    ModelArgs mc = m;
+    for (int k = m.n_lens; k < m.n_lens + n_lin; ++k)
+      if (k - m.n_lens != l) mc.types[k] = LENSIM_NULL;
+    const unsigned flags = l < m.n_source ? LENSIM_IMAGE_SOURCE : LENSIM_IMAGE_LENS_LIGHT;
+    T *col = A + size_t(l) * n_img * npix;
+    rc = ss.on ? run_image_ss<T>(mc, g, psf, kx, ky, edge, ss, nullptr, params1, n_img, flags, col, nullptr, ws,
+                                 LL.img, st)
+               : run_image<T>(mc, g, psf, kx, ky, edge, nullptr, params1, n_img, flags, col, nullptr, ws, LL.img, st);
+    if (rc) return rc;
+  }
+  ConvArgs c{};
+  c.nx = g.nx;
+  c.ny = g.ny;
+  c.data = data->d_data;
+  c.noise = data->d_noise_map;
+  c.expo = data->d_exposure_map;
+  c.mask = data->d_mask;
+  c.exposure_time = data->exposure_time;
+  c.bkg2 = data->background_rms * data->background_rms;
+  c.dstride = data->image_stride;
+  double *part = reinterpret_cast<double *>(ws + LL.part);
+  normal_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(c, A, n_lin, n_img, part);
+  if ((rc = check_launch("normal_kernel"))) return rc;
+  double *amp = reinterpret_cast<double *>(ws + LL.amp);
+  double *marg = reinterpret_cast<double *>(ws + LL.marg);
+  lin_solve_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(part, nblk, n_lin, n_img, source_marg, d_prior, amp, cov, marg);
+  if ((rc = check_launch("lin_solve_kernel"))) return rc;
+  double *chi = reinterpret_cast<double *>(ws + LL.chi);
+  lin_model_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(c, A, amp, n_lin, n_img, model_out, chi);
+  if ((rc = check_launch("lin_model_kernel"))) return rc;
+  logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(chi, nblk, n_img, logl);
+  if ((rc = check_launch("logl_kernel"))) return rc;
+  add_kernel<<<(n_img + 255) / 256, 256, 0, st>>>(logl, marg, n_img);
+  if ((rc = check_launch("add_kernel"))) return rc;
+  if (amp_out) {
+    const hipError_t e = hipMemcpyAsync(amp_out, amp, size_t(n_img) * n_lin * 8, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(amp)");
+  }
+  return LENSIM_OK;
+}
+
 int check_ws(const Layout &L, void *ws, size_t ws_bytes) {
   if (L.total > ws_bytes)
     return fail(LENSIM_EWORKSPACE, "workspace of %zu B is smaller than the %zu B needed", ws_bytes, L.total);
@@ -1773,6 +2108,58 @@ int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const v
   if (rc) return rc;
   logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, nblk, n_img, d_logl);
   return check_launch("logl_kernel");
+}
+
+size_t lensim_linear_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                    int32_t n_img, int32_t precision) {
+  ModelArgs m;
+  if (model_args(model, m) || !grid) return 0;
+  GridArgs g;
+  SSPlan ss;
+  int kx = 1, ky = 1, edge = 0;
+  if (grid_args(grid, g) || psf_ss(psf, g, 0, ss) || psf_dims(psf, 0, kx, ky, edge)) return 0;
+  return lin_layout(m, g.nx, g.ny, kx, ky, n_img, precision, ss, m.n_source + m.n_light).total;
+}
+
+int lensim_linear_solve(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                        const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                        int32_t source_marg, double linear_prior, double *d_amp, double *d_logl, void *d_model_out,
+                        double *d_cov, void *d_response_out, void *ws, size_t ws_bytes, void *stream) {
+  ModelArgs m;
+  int rc = model_args(model, m);
+  if (rc) return rc;
+  GridArgs g;
+  if ((rc = grid_args(grid, g))) return rc;
+  int kx, ky, edge;
+  if ((rc = psf_dims(psf, 0, kx, ky, edge))) return rc;
+  SSPlan ss;
+  if ((rc = psf_ss(psf, g, 0, ss))) return rc;
+  const int n_lin = m.n_source + m.n_light;
+  if (n_lin == 0) return fail(LENSIM_EINVAL, "the linear solve needs at least one light profile");
+  if (n_lin > kMaxLin) return fail(LENSIM_EUNSUPPORTED, "at most %d linear amplitudes (got %d)", kMaxLin, n_lin);
+  if (!data || !data->d_data) return fail(LENSIM_EINVAL, "data image is NULL");
+  if (!data->d_noise_map && !data->d_exposure_map && !(data->exposure_time > 0.0))
+    return fail(LENSIM_EINVAL, "need a noise map, an exposure map or a positive exposure time");
+  if (data->image_stride != 0 && data->image_stride < int64_t(g.nx) * g.ny)
+    return fail(LENSIM_EINVAL, "image_stride %lld is smaller than an image", (long long)data->image_stride);
+  if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
+  if (n_img == 0) return LENSIM_OK;
+  if (n_img > 65535) return fail(LENSIM_EINVAL, "n_img %d exceeds 65535 per call", n_img);
+  if (!d_logl || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
+  const LinLayout LL = lin_layout(m, g.nx, g.ny, kx, ky, n_img, precision, ss, n_lin);
+  if (LL.total > ws_bytes)
+    return fail(LENSIM_EWORKSPACE, "workspace of %zu B is smaller than the %zu B needed", ws_bytes, LL.total);
+  if (!ws || (reinterpret_cast<uintptr_t>(ws) & 255)) return fail(LENSIM_EINVAL, "workspace must be 256-byte aligned");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned char *w = static_cast<unsigned char *>(ws);
+  if (precision == LENSIM_FP64)
+    return run_linear<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
+                              d_logl, static_cast<double *>(d_model_out), d_cov,
+                              static_cast<double *>(d_response_out), w, LL, st);
+  return run_linear<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
+                           d_logl, static_cast<float *>(d_model_out), d_cov, static_cast<float *>(d_response_out), w,
+                           LL, st);
 }
 
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,

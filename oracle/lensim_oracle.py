@@ -592,3 +592,74 @@ def log_likelihood(model, data, mask=None, background_rms=None, exposure_map=Non
     chi2 = chi2 / (c_d + np.abs(additional_error_map))
     chi2 = chi2 * mask
     return -np.sum(chi2, axis=(-2, -1)) / 2
+
+
+# ---------------------------------------------------------------------------
+# Linear amplitude solve (ImSim/image_linear_solve.py, ImSim/de_lens.py)
+
+EPSILON = np.finfo(float).eps
+
+
+def linear_solve(grid, lens_model_list, kwargs_lens, source_list, kwargs_source, lens_light_list, kwargs_lens_light,
+                 data, mask=None, background_rms=None, exposure_map=None, noise_map=None, psf_type="PIXEL",
+                 kernel=None, flux_scaling=1.0, source_marg=False, linear_prior=None, **image_kwargs):
+    """ImageLinearFit.image_linear_solve + likelihood_data_given_model
+    (ImSim/image_linear_solve.py:98-181, :229-340).
+
+    Response of each light profile at amp = 1 (linear_response_matrix
+    :362-455, LinearBasis.functions_split LightModel/linear_basis.py:25-92),
+    weights from the data-based variance (ImageModel.error_response
+    image_model.py:577-596 -> ImageData.C_D), get_param_WLS (de_lens.py:10-36),
+    model zero outside the mask (array_masked2image), logL with the model-based
+    variance, plus the marginalisation constant (de_lens.py:40-77).
+    Returns (logL, amplitudes, model image, M_inv or None).
+    """
+    data = np.asarray(data, float)
+    if mask is None:
+        mask = np.ones_like(data)
+    m1 = np.asarray(mask, bool).ravel()
+    cols = []
+    for i in range(len(source_list)):
+        kw = dict(kwargs_source[i], amp=1.0)
+        cols.append(image(grid, lens_model_list, kwargs_lens, [source_list[i]], [kw], [], [], psf_type=psf_type,
+                          kernel=kernel, flux_scaling=flux_scaling, **image_kwargs))
+    for i in range(len(lens_light_list)):
+        kw = dict(kwargs_lens_light[i], amp=1.0)
+        cols.append(image(grid, lens_model_list, kwargs_lens, [], [], [lens_light_list[i]], [kw], psf_type=psf_type,
+                          kernel=kernel, flux_scaling=flux_scaling, **image_kwargs))
+    A = np.nan_to_num(np.array([c.ravel()[m1] for c in cols]))  # (n_lin, n_masked)
+    if noise_map is not None:
+        c_d = np.asarray(noise_map, float) ** 2
+    else:
+        c_d = np.where(data >= 0, data, 0) / exposure_map + background_rms ** 2
+    w = 1.0 / c_d.ravel()[m1]
+    d = data.ravel()[m1]
+    M = A.dot((w * A).T)
+    R = A.dot(w * d)
+    stable = np.linalg.cond(M) < 5 / EPSILON
+    if stable:
+        B = np.linalg.solve(M, R)
+        if not np.all(np.isfinite(B)):
+            B = np.zeros(len(R))
+        M_inv = np.nan_to_num(np.linalg.inv(M), nan=0, posinf=0, neginf=0)
+    else:
+        B = np.zeros(len(R))
+        M_inv = np.zeros_like(M)
+    flat = np.zeros(data.size)
+    flat[m1] = A.T.dot(B)
+    model = flat.reshape(data.shape)
+    logL = log_likelihood(model, data, mask=mask, background_rms=background_rms, exposure_map=exposure_map,
+                          noise_map=noise_map)
+    if source_marg:
+        if linear_prior is None:
+            sign, log_det = np.linalg.slogdet(M_inv)
+            logL += -1e15 if sign == 0 else sign * log_det / 2
+        else:
+            v = np.linalg.eigvals(M_inv).real
+            v_abs = np.minimum(np.abs(v), linear_prior ** 2)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                log_det = np.sum(np.log(v_abs)) * np.prod(np.sign(v))
+            m = len(v)
+            logL += -1e15 if np.isnan(log_det) else (log_det / 2 + m / 2.0 * np.log(np.pi / 2.0)
+                                                      - m * np.log(linear_prior))
+    return logL, B, model, M_inv

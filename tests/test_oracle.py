@@ -356,3 +356,40 @@ def test_sub_frame_semantics():
     un2 = O.image(*args, unconvolved=True, flux_evaluate_indexes=mask2)
     un_full = O.image(*args, unconvolved=True)
     np.testing.assert_allclose(un2[2:12, 7:17], un_full[7:17, 2:12], rtol=1e-13)
+
+
+def test_linear_solve_recovers_amplitudes():
+    """get_param_WLS restated: noiseless data of known amplitudes are recovered
+    exactly; the solution equals an independent weighted lstsq; the
+    marginalisation constant is -1/2 log det M."""
+    from jaxtronomy_amd import workloads as W
+
+    cfg = dict(W.EPL180, numPix=30, supersampling=1, deltaPix=0.1)
+    g = W.grid_dict(cfg)
+    kl, ks, kll = W.epl180_draw(2)
+    kern = W.moffat_kernel(7, 1.5, 3.0)
+    clean = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kern)
+    mask = np.ones((30, 30))
+    mask[:3] = 0
+    args = (g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll)
+    logl, B, model, M_inv = O.linear_solve(*args, clean, mask=mask, background_rms=0.01, exposure_map=1000.0,
+                                           kernel=kern, source_marg=True)
+    npt.assert_allclose(B, [ks[0]["amp"], kll[0]["amp"]], rtol=1e-10)
+    npt.assert_allclose(model[3:], clean[3:], rtol=1e-9, atol=1e-12)
+    assert np.all(model[:3] == 0)
+    # noisy data: the WLS solution equals lstsq of the whitened system
+    noisy = W.noisy(clean, 1000.0, 0.01, np.random.default_rng(0))
+    logl, B, model, M_inv = O.linear_solve(*args, noisy, mask=mask, background_rms=0.01, exposure_map=1000.0,
+                                           kernel=kern, source_marg=True)
+    cols = [O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], [dict(ks[0], amp=1.0)], [], [], kernel=kern),
+            O.image(g, ["EPL", "SHEAR"], kl, [], [], ["SERSIC_ELLIPSE"], [dict(kll[0], amp=1.0)], kernel=kern)]
+    sel = mask.ravel() > 0
+    var = np.maximum(noisy, 0) / 1000.0 + 1e-4
+    sw = 1 / np.sqrt(var.ravel()[sel])
+    Aw = np.array([c.ravel()[sel] * sw for c in cols]).T
+    B_ls = np.linalg.lstsq(Aw, noisy.ravel()[sel] * sw, rcond=None)[0]
+    npt.assert_allclose(B, B_ls, rtol=1e-8)
+    M = Aw.T @ Aw
+    chi = O.log_likelihood(model, noisy, mask=mask, background_rms=0.01, exposure_map=1000.0)
+    npt.assert_allclose(logl - chi, -0.5 * np.linalg.slogdet(M)[1], rtol=1e-8)
+    npt.assert_allclose(M_inv, np.linalg.inv(M), rtol=1e-8)
