@@ -88,6 +88,18 @@ def time_steps(im, params, steps, warmup, backend, model=None):
                 conv_ms=conv_ms / max(calls, 1), logl=logl.cpu().numpy())
 
 
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy's readings), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def render_work(params_np, n_ray, lens_light=True):
     """Algorithmic flops and special-function evaluations of one render launch:
     per ray EPL 72 + 8 (N_terms - 1) flops and 6 specials (N_terms = the series
@@ -355,8 +367,9 @@ def main():
         "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4)},
         "roofline": {"bound": "valu", "kernel": "render_kernel<float>", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic("render_kernel_float_epl180_b64_pmc.txt"),
                      "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
+                     "algorithmic_bytes": 4 * n_ray // cfg["supersampling"] ** 2 * args.batch,
                      "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3)},
     }
     if args.serve_seconds > 0:

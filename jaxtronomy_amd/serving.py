@@ -172,6 +172,17 @@ class _Bucket(object):
                                   backend.ptr(lane.ws), lane.ws.numel(), st)
             backend.check(rc, "lensim_image")
 
+    def _batch(self, lane, kind, npad):
+        """One batch of npad rows: staging in, launches, results out (the pinned
+        host buffers are fixed, so a captured graph holds the copies too)."""
+        lane.params[:npad].copy_(lane.params_host[:npad], non_blocking=True)
+        lane.slot_idx[:npad].copy_(lane.slot_host[:npad], non_blocking=True)
+        self._launch(lane, kind, npad)
+        if kind == "loglike":
+            lane.logl_host[:npad].copy_(lane.logl[:npad], non_blocking=True)
+        else:
+            lane.images_host[:npad].copy_(lane.images[:npad], non_blocking=True)
+
     def run(self, lane, kind, n, stream):
         """Launch a batch whose first n rows of the lane's host staging are filled;
         returns the completion event (results are then in the lane's host buffers)."""
@@ -180,24 +191,19 @@ class _Bucket(object):
             lane.params_np[n:npad] = lane.params_np[0]
             lane.slot_np[n:npad] = lane.slot_np[0]
         with torch.cuda.stream(stream):
-            lane.params[:npad].copy_(lane.params_host[:npad], non_blocking=True)
-            lane.slot_idx[:npad].copy_(lane.slot_host[:npad], non_blocking=True)
             if not self.use_graphs:
-                self._launch(lane, kind, npad)
+                self._batch(lane, kind, npad)
             else:
                 g = lane.graphs.get((kind, npad))
                 if g is None:
-                    self._launch(lane, kind, npad)  # warm-up outside the capture
+                    self._batch(lane, kind, npad)  # warm-up outside the capture
                     stream.synchronize()
                     g = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(g, stream=stream):
-                        self._launch(lane, kind, npad)
+                        self._batch(lane, kind, npad)
                     lane.graphs[(kind, npad)] = g
-                g.replay()
-            if kind == "loglike":
-                lane.logl_host[:n].copy_(lane.logl[:n], non_blocking=True)
-            else:
-                lane.images_host[:n].copy_(lane.images[:n], non_blocking=True)
+                else:
+                    g.replay()
             ev = torch.cuda.Event()
             ev.record(stream)
         return ev

@@ -419,7 +419,7 @@ def test_flux_evaluate_indexes_sub_frame(psf_type, precision, tol, ltol):
     im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf, LensModel(["EPL", "SHEAR"]),
                     LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
                     kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": mask}, precision=precision)
-    assert im._frame == (4, 8, 27)
+    assert im._frame == (5, 9, 25)  # rows 5..29, columns 9..33
     got = _np(im.image(kl, ks, kll)).astype(np.float64)
     assert got.shape == (40, 40)
     assert img_err(got, want) < tol
