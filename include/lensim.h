@@ -83,6 +83,23 @@ typedef enum {
 #define LENSIM_IMAGE_LENS_LIGHT  2u  /* add the lens light (lens_light_add) */
 #define LENSIM_IMAGE_UNCONVOLVED 4u  /* skip the PSF (unconvolved=True) */
 
+/* Multi-plane ray tracing through redshift-ordered deflectors
+ * (MultiPlaneBulk.ray_shooting, LensModel/MultiPlane/multi_plane_bulk.py:152-212):
+ * starting from x = 0 with the ray angle alpha = theta, for every plane p
+ *   x += alpha * T_ij[p];  alpha -= alpha_red_p(x / T_z[p]) * r2p[p]
+ * where the last plane is the source (no deflector), and beta = x / T_z[last].
+ * The arrays come from MultiPlaneBulk.prepare_ray_shooting_kwargs
+ * (multi_plane_bulk.py:54-150). */
+typedef struct {
+  int32_t n_planes;          /* deflector planes + 1 (the source) */
+  int32_t reserved;
+  const int32_t *d_order;    /* [n_planes - 1] deflector index (inline lens, then bulk) of each plane */
+  const double *d_T_ij;      /* [n_img][n_planes] transverse comoving distance from the previous plane */
+  const double *d_T_z;       /* [n_img][n_planes] transverse comoving distance from z = 0 */
+  const double *d_r2p;       /* [n_img][n_planes] reduced-to-physical deflection factor */
+  int64_t stride;            /* elements between images in the three arrays (0: shared) */
+} lensim_multiplane;
+
 /* Model description: the profile lists of LensModel(lens_model_list),
  * LightModel(source list) and LightModel(lens light list)
  * (LensModel/lens_model.py:25-216, LightModel/light_model_base.py:63-228),
@@ -99,6 +116,7 @@ typedef struct {
   int32_t n_bulk;                     /* bulk deflectors after the n_lens inline ones */
   int32_t bulk_type;                  /* their common lensim_profile, or LENSIM_NULL if mixed */
   const int32_t *d_bulk_types;        /* [n_bulk] device array (read when bulk_type == LENSIM_NULL) */
+  const lensim_multiplane *multiplane; /* NULL: single plane; else every deflector sits on its own plane */
 } lensim_model;
 
 /* Pixel grid + supersampling (ImSim/Numerics/grid.py:22-64): pixel (i, j)

@@ -218,7 +218,8 @@ class ImageModel(object):
         ip = np.broadcast_to(ip, (n,) + ip.shape[1:])
         bp = np.broadcast_to(bp, (n,) + bp.shape[1:])
         params = np.ascontiguousarray(np.concatenate([ip, bp, light], axis=1))
-        m, keep = self.LensModel._model(it, bt, self._src_types, self._light_types, self._flux_scaling)
+        m, keep = self.LensModel._model(it, bt, self._src_types, self._light_types, self._flux_scaling,
+                                        ray_shooting_kwargs=kwargs_lens)
         return params, n, m, keep
 
     def pack(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None):

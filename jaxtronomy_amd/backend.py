@@ -44,6 +44,19 @@ class LensimModel(ctypes.Structure):
         ("n_bulk", ctypes.c_int32),
         ("bulk_type", ctypes.c_int32),
         ("d_bulk_types", ctypes.c_void_p),
+        ("multiplane", ctypes.c_void_p),
+    ]
+
+
+class LensimMultiplane(ctypes.Structure):
+    _fields_ = [
+        ("n_planes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("d_order", ctypes.c_void_p),
+        ("d_T_ij", ctypes.c_void_p),
+        ("d_T_z", ctypes.c_void_p),
+        ("d_r2p", ctypes.c_void_p),
+        ("stride", ctypes.c_int64),
     ]
 
 

@@ -88,6 +88,11 @@ struct ModelArgs {
   int32_t types[LENSIM_MAX_PROFILES];
   int32_t epl_index[LENSIM_MAX_PROFILES];
   double alpha_scaling, flux_scaling;
+  // multi-plane ray tracing (mp_n == 0: single plane)
+  int32_t mp_n;
+  const int32_t *mp_order;
+  const double *mp_tij, *mp_tz, *mp_r2p;
+  int64_t mp_stride;
 };
 
 __host__ __device__ inline int type_of(const ModelArgs &m, int k) {
@@ -280,6 +285,19 @@ int model_args(const lensim_model *model, ModelArgs &m) {
   }
   m.alpha_scaling = model->alpha_scaling;
   m.flux_scaling = model->flux_scaling;
+  if (const lensim_multiplane *mp = model->multiplane) {
+    if (mp->n_planes != m.n_defl + 1)
+      return fail(LENSIM_EINVAL, "multiplane.n_planes %d must be the %d deflectors + 1", mp->n_planes, m.n_defl);
+    if ((m.n_defl > 0 && !mp->d_order) || !mp->d_T_ij || !mp->d_T_z || !mp->d_r2p)
+      return fail(LENSIM_EINVAL, "multiplane arrays are NULL");
+    if (mp->stride != 0 && mp->stride < mp->n_planes) return fail(LENSIM_EINVAL, "multiplane stride too small");
+    m.mp_n = mp->n_planes;
+    m.mp_order = mp->d_order;
+    m.mp_tij = mp->d_T_ij;
+    m.mp_tz = mp->d_T_z;
+    m.mp_r2p = mp->d_r2p;
+    m.mp_stride = mp->stride;
+  }
   return LENSIM_OK;
 }
 
@@ -390,6 +408,55 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
   }
 }
 
+// One deflector k (inline lens k < n_lens with its series coefficients, or bulk).
+template <typename T, bool HALO>
+__device__ __forceinline__ void deflect_one(const ModelArgs &m, const RecPtrs<T> &rp, int img, int k, double x,
+                                            double y, double &ax, double &ay) {
+  const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
+  if (k < m.n_lens) {
+    const int e = m.epl_index[k];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    add_deflection<T, HALO>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+  } else {
+    const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
+    add_deflection<T, HALO>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+  }
+}
+
+// Source-plane position of the ray at angle (x, y): single plane
+// beta = theta - alpha_scaling * alpha, or the multi-plane recursion of
+// MultiPlaneBulk.ray_shooting (LensModel/MultiPlane/multi_plane_bulk.py:152-212).
+template <typename T, bool HALO>
+__device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
+                                          double &bx, double &by) {
+  if (m.mp_n == 0) {
+    double ax = 0.0, ay = 0.0;
+    deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
+    bx = x - ax * m.alpha_scaling;
+    by = y - ay * m.alpha_scaling;
+    return;
+  }
+  const size_t o = size_t(img) * size_t(m.mp_stride);
+  double angx = x, angy = y, px = 0.0, py = 0.0, tz = 1.0;
+  for (int p = 0; p < m.mp_n; ++p) {
+    const double dt = m.mp_tij[o + p];
+    tz = m.mp_tz[o + p];
+    px += angx * dt;
+    py += angy * dt;
+    if (p + 1 < m.mp_n) {
+      double ax = 0.0, ay = 0.0;
+      int k = m.mp_order[p];
+      k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
+      deflect_one<T, HALO>(m, rp, img, k, px / tz, py / tz, ax, ay);
+      const double f = m.mp_r2p[o + p];
+      angx -= ax * f;
+      angy -= ay * f;
+    }
+  }
+  bx = px / tz;
+  by = py / tz;
+}
+
 template <typename T>
 __device__ __forceinline__ double light_sum(const ModelArgs &m, const RecPtrs<T> &rp, int k0, int k1, double x,
                                             double y) {
@@ -447,10 +514,8 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
-        double ax = 0.0, ay = 0.0;
-        deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
-        const double bx = x - ax * m.alpha_scaling;
-        const double by = y - ay * m.alpha_scaling;
+        double bx, by;
+        trace_ray<T, HALO>(m, rp, img, x, y, bx, by);
         acc += light_sum<T>(m, rp, k_src0, k_src1, bx, by);
       }
       if (do_light) acc += light_sum<T>(m, rp, k_src1, k_l1, x, y);
@@ -485,11 +550,10 @@ __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n_ray) return;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
-  const double x = xs[i], y = ys[i];
-  double ax = 0.0, ay = 0.0;
-  deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
-  bx[size_t(img) * n_ray + i] = x - ax * m.alpha_scaling;
-  by[size_t(img) * n_ray + i] = y - ay * m.alpha_scaling;
+  double tx, ty;
+  trace_ray<T, HALO>(m, rp, img, xs[i], ys[i], tx, ty);
+  bx[size_t(img) * n_ray + i] = tx;
+  by[size_t(img) * n_ray + i] = ty;
 }
 
 template <typename T>
@@ -2181,6 +2245,7 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   int rc = model_args(model, m);
   if (rc) return rc;
   if (m.n_bulk) return fail(LENSIM_EUNSUPPORTED, "gradients of bulk deflector lists are not supported");
+  if (m.mp_n) return fail(LENSIM_EUNSUPPORTED, "gradients of multi-plane ray tracing are not supported");
   GridArgs g;
   if ((rc = grid_args(grid, g))) return rc;
   int kx, ky, edge;

@@ -319,7 +319,19 @@ __device__ __forceinline__ void real_poly_at_complex(const ST *__restrict__ c, i
   }
   const ST sw = ST(2) * wr, tw = wr * wr + wi * wi;
   ST b2 = ST(0), b1 = c[N - 1];
-  for (int k = N - 2; k >= 1; --k) {
+  int k = N - 2;
+  // four terms per step: the coefficient loads of a step are issued together
+  // (they do not depend on the recurrence) and the b1/b2 rotation unrolls away
+  for (; k >= 4; k -= 4) {
+    const ST c0 = c[k], c1 = c[k - 1], c2 = c[k - 2], c3 = c[k - 3];
+    const ST a0 = fma(sw, b1, fma(-tw, b2, c0));
+    const ST a1 = fma(sw, a0, fma(-tw, b1, c1));
+    const ST a2 = fma(sw, a1, fma(-tw, a0, c2));
+    const ST a3 = fma(sw, a2, fma(-tw, a1, c3));
+    b2 = a2;
+    b1 = a3;
+  }
+  for (; k >= 1; --k) {
     const ST b0 = fma(sw, b1, fma(-tw, b2, c[k]));
     b2 = b1;
     b1 = b0;

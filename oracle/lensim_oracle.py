@@ -663,3 +663,23 @@ def linear_solve(grid, lens_model_list, kwargs_lens, source_list, kwargs_source,
             logL += -1e15 if np.isnan(log_det) else (log_det / 2 + m / 2.0 * np.log(np.pi / 2.0)
                                                       - m * np.log(linear_prior))
     return logL, B, model, M_inv
+
+
+def multi_plane_ray_shooting(x, y, lens_model_list, kwargs_lens, T_ij_list, T_z_list, reduced2physical_factor):
+    """MultiPlaneBulk.ray_shooting, LensModel/MultiPlane/multi_plane_bulk.py:152-212:
+    the scan over redshift-sorted planes (the last entries of the distance lists
+    are the source plane and any NULL fillers)."""
+    ax = np.asarray(x, float).copy()
+    ay = np.asarray(y, float).copy()
+    px = np.zeros_like(ax)
+    py = np.zeros_like(ay)
+    n = len(lens_model_list)
+    for p in range(len(T_ij_list)):
+        px = px + ax * T_ij_list[p]
+        py = py + ay * T_ij_list[p]
+        if p < n:
+            func, names = LENS_FUNCS[lens_model_list[p]]
+            fx, fy = _call(func, names, px / T_z_list[p], py / T_z_list[p], kwargs_lens[p])
+            ax = ax - fx * reduced2physical_factor[p]
+            ay = ay - fy * reduced2physical_factor[p]
+    return px / T_z_list[-1], py / T_z_list[-1]

@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts():
-    assert ctypes.sizeof(backend.LensimModel) == 12 + 4 * 64 + 4 + 16 + 8 + 8  # padded to 8 before the doubles
+    assert ctypes.sizeof(backend.LensimModel) == 12 + 4 * 64 + 4 + 16 + 8 + 8 + 8  # padded to 8 before the doubles
+    assert ctypes.sizeof(backend.LensimMultiplane) == 8 + 4 * 8 + 8
     assert ctypes.sizeof(backend.LensimGrid) == 16 + 16 + 32 + 8
     assert ctypes.sizeof(backend.LensimPsf) == 40
     assert ctypes.sizeof(backend.LensimData) == 56

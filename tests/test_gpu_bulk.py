@@ -162,3 +162,57 @@ def test_substructure_full_size_mixed_vs_fp64():
     ll64 = im64d.likelihood_batch(pt, model=m).cpu().numpy()
     ll32 = im32d.likelihood_batch(pt, model=m).cpu().numpy()
     assert np.all(np.abs(ll32 - ll64) / np.abs(ll64) < 1e-6), (ll32, ll64)
+
+
+def _multiplane_scene(n_sub=40, seed=21):
+    from jaxtronomy_amd.Util.cosmo import FlatLambdaCDM
+
+    kl, subs, ks = W.substructure_draw(4, n_sub=n_sub)
+    rng = np.random.default_rng(seed)
+    z = [0.5, 0.5] + list(rng.uniform(0.2, 1.3, n_sub))
+    names = ["EPL", "SHEAR"] + ["TNFW"] * n_sub
+    # heavier subhalos so that the planes matter at the test's tolerance
+    subs = [dict(s, alpha_Rs=s["alpha_Rs"] * 20) for s in subs]
+    return names, kl + subs, ks, z, FlatLambdaCDM(70.0, 0.3)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-6)])
+def test_multi_plane_ray_shooting(precision, tol):
+    """MultiPlaneBulk.ray_shooting: EPL + SHEAR at z=0.5 and 40 TNFW between
+    z=0.2 and 1.3, source at z=1.5, against the oracle's plane scan."""
+    names, kw, _, z, cosmo = _multiplane_scene()
+    lm = LensModelBulk(["EPL", "SHEAR", "TNFW"], multi_plane=True, cosmo=cosmo, precision=precision)
+    rs = lm.prepare_ray_shooting_kwargs(names, kw, 1.5, z, num_deflectors=64)
+    order = np.argsort(np.array(z))
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-3, 3, 3000)
+    y = rng.uniform(-3, 3, 3000)
+    bx, by = lm.ray_shooting(x, y, rs)
+    wx, wy = O.multi_plane_ray_shooting(x, y, [names[i] for i in order], [kw[i] for i in order], rs["T_ij_list"],
+                                        rs["T_z_list"], rs["reduced2physical_factor"])
+    scale = max(np.max(np.abs(x - wx)), np.max(np.abs(y - wy)))
+    err = max(np.max(np.abs(_np(bx) - wx)), np.max(np.abs(_np(by) - wy))) / scale
+    assert err < tol, err
+    # and it differs from the single-plane sum
+    sx, sy = O.ray_shooting(x, y, names, kw)
+    assert np.max(np.abs(sx - wx)) > 100 * tol * scale
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-5)])
+def test_multi_plane_image(precision, tol):
+    names, kw, ks, z, cosmo = _multiplane_scene(n_sub=30)
+    cfg = dict(W.SUBSTRUCTURE, numPix=60, deltaPix=0.1, supersampling=2)
+    g = W.grid_dict(cfg)
+    kernel = W.moffat_kernel(11, 1.5, 3.0)
+    lm = LensModelBulk(["EPL", "SHEAR", "TNFW"], multi_plane=True, cosmo=cosmo, precision=precision)
+    im = ImageModel(ImageData(**W.data_kwargs(cfg)), PSF("PIXEL", kernel_point_source=kernel), lm,
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+    rs = lm.prepare_ray_shooting_kwargs(names, kw, 1.5, z)
+    got = _np(im.image(rs, ks)).astype(np.float64)
+    order = np.argsort(np.array(z))
+    x, y = O.subgrid_coordinates(60, 60, g["transform_pix2angle"], g["ra_at_xy_0"], g["dec_at_xy_0"], 2)
+    bx, by = O.multi_plane_ray_shooting(np.ravel(x), np.ravel(y), [names[i] for i in order], [kw[i] for i in order],
+                                        rs["T_ij_list"], rs["T_z_list"], rs["reduced2physical_factor"])
+    flux = O.surface_brightness(bx, by, ["SERSIC_ELLIPSE"], ks)
+    want = O.convolve_same(O.re_size(flux.reshape(120, 120), 2), kernel) * 0.1 ** 2
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
