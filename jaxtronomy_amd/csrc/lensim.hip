@@ -822,31 +822,44 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int
 #pragma unroll
     for (int j = 0; j < 4; ++j) dacc[t][j] = 0.0;
   const int rowbase = (16 * wave + li) * ls + 8 * lg;
-  for (int a = 0; a < c.ky; ++a) {
-    floatx4 acc[kMfTiles];
+  // one step per (kernel row a, K block ks); the next step's B fragments are
+  // loaded (from L2) before this step's MFMAs so their latency overlaps
+  const int n_it = c.ky * nks;
+  half8 bh = frag[0 * 64 + lane];
+  half8 bl = frag[1 * 64 + lane];
+  floatx4 acc[kMfTiles];
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < nks; ++ks) {
-      const half8 bh = frag[((a * nks + ks) * 2 + 0) * 64 + lane];
-      const half8 bl = frag[((a * nks + ks) * 2 + 1) * 64 + lane];
-      half8 ah[kMfTiles], al[kMfTiles];
+  for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  int a = 0, ks = 0;
+  for (int it = 0; it < n_it; ++it) {
+    const int itn = it + 1 < n_it ? it + 1 : it;
+    const half8 bhn = frag[(itn * 2 + 0) * 64 + lane];
+    const half8 bln = frag[(itn * 2 + 1) * 64 + lane];
+    half8 ah[kMfTiles], al[kMfTiles];
 #pragma unroll
-      for (int t = 0; t < kMfTiles; ++t) {
-        const int off = rowbase + a * ls + 16 * t + 32 * ks;
-        ah[t] = *reinterpret_cast<const half8 *>(lh + off);
-        al[t] = *reinterpret_cast<const half8 *>(ll + off);
-      }
-#pragma unroll
-      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bh, acc[t], 0, 0, 0);
-#pragma unroll
-      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bl, acc[t], 0, 0, 0);
-#pragma unroll
-      for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bh, acc[t], 0, 0, 0);
+    for (int t = 0; t < kMfTiles; ++t) {
+      const int off = rowbase + a * ls + 16 * t + 32 * ks;
+      ah[t] = *reinterpret_cast<const half8 *>(lh + off);
+      al[t] = *reinterpret_cast<const half8 *>(ll + off);
     }
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t)
+    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bh, acc[t], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
+    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bl, acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bh, acc[t], 0, 0, 0);
+    bh = bhn;
+    bl = bln;
+    if (++ks == nks) {  // a kernel row is complete: fold its float sums into double
+      ks = 0;
+      ++a;
+#pragma unroll
+      for (int t = 0; t < kMfTiles; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
+        acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
   }
   // lane l holds D[4 (l / 16) + j][l % 16] of each tile
   const double unscale = ldexp(1.0, -(sa + sk));

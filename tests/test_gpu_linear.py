@@ -93,3 +93,39 @@ def test_linear_solve_batch_equals_single_and_positive_flux():
                                                check_positive_flux=True)
     assert np.any(a < 0)
     assert abs(float(l_pos) - (float(l_plain) - 1e8)) < 1e-6 * 1e8
+
+
+def test_likelihood_module_batch():
+    """Sampler-facing batch: LikelihoodModule.logL of N parameter vectors equals
+    the per-model likelihoods, bounds give -1e18, and with the linear solver it
+    equals ImageLinearFit.likelihood_data_given_model."""
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.Sampling.likelihood import LikelihoodModule, Param
+
+    im, g, cfg, kernel, data, mask, (kl, ks, kll) = _setup("fp64")
+    fixed = [{"center_x": 0.0, "center_y": 0.0}, {"ra_0": 0.0, "dec_0": 0.0}]
+    plain = ImageModel(im.Data, im.PSF, im.LensModel, im.SourceModel, im.LensLightModel,
+                       kwargs_numerics={"supersampling_factor": 2}, likelihood_mask=mask)
+    p = Param(plain, kwargs_fixed_lens=fixed, kwargs_lower_lens=[{"theta_E": 0.5}, {}],
+              kwargs_upper_lens=[{"theta_E": 2.0}, {}])
+    lm = LikelihoodModule(plain, p, custom_logL_addition=lambda **kw: -kw["kwargs_lens"][0]["theta_E"])
+    draws = [W.epl180_draw(d) for d in (9, 12, 13, 14)]
+    args = np.stack([p.kwargs2args([dict(d[0][0], center_x=0.0, center_y=0.0), d[0][1]], d[1], d[2],
+                                   image_model=plain) for d in draws])
+    args[3, 0] = 5.0  # theta_E beyond its upper bound
+    logl = lm.logL(args)
+    for i in range(3):
+        kl_i = [dict(draws[i][0][0], center_x=0.0, center_y=0.0), draws[i][0][1]]
+        want = float(plain.likelihood_data_given_model(kl_i, draws[i][1], draws[i][2])) - kl_i[0]["theta_E"]
+        assert abs(logl[i] - want) <= 1e-12 * abs(want)
+    assert logl[3] == -1e18
+    # linear solver: the amplitudes are solved, not sampled
+    pl = Param(im, kwargs_fixed_lens=fixed, linear_solver=True)
+    lml = LikelihoodModule(im, pl, check_bounds=False)
+    args_l = np.stack([pl.kwargs2args([dict(d[0][0], center_x=0.0, center_y=0.0), d[0][1]], d[1], d[2],
+                                      image_model=im) for d in draws[:2]])
+    logl_l = lml.logL(args_l)
+    for i in range(2):
+        kl_i = [dict(draws[i][0][0], center_x=0.0, center_y=0.0), draws[i][0][1]]
+        want, _ = im.likelihood_data_given_model(kl_i, [dict(draws[i][1][0])], [dict(draws[i][2][0])])
+        assert abs(logl_l[i] - float(want)) <= 1e-12 * abs(float(want))
