@@ -560,17 +560,32 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
   T X = (sizeof(T) == sizeof(double)) ? R / r[TN_RS] : R * r[TN_IRS];
   X = X > T(0.001) ? X : T(0.001);
   if (sizeof(T) == sizeof(float) && X < T(1)) {
-    // Inside the scale radius the leading terms of g cancel (g ~ x^2 ln x while
-    // the terms grow like ln(1/x)); the float form evaluates them in double there.
-    const double Xd = double(X);
-    const double tau = r.d[TN_TAU], tau2 = r.d[TN_TAU2], X2 = Xd * Xd;
-    const double sqtx = ::sqrt(tau2 + X2);
-    const double L = ::log(Xd / (tau + sqtx));
-    const double g = r.d[TN_PRE] * ((r.d[TN_T2P1] + 2.0 * (X2 - 1.0)) * tnfw_F(Xd) + r.d[TN_C0] +
-                                    sqtx * (-M_PI + L * r.d[TN_C1]));
-    const double a = r.d[TN_A] * g / X2;
-    ax += a * double(dx);
-    ay += a * double(dy);
+    // Inside the scale radius the terms of g cancel to O(X^2 ln X).  With
+    // s = sqrt(1 - X^2), w = sqrt(tau^2 + X^2) and F = atanh(s)/s, the bracket
+    // of g equals X^2 C with
+    //   C = (tau^2 - 1) [ ln X (1 - tau^2 - X^2) / ((w s + tau) tau s)
+    //                     + ln(1 + s) / (s (1 + s)) - ln(tau + w) / (tau (tau + w))
+    //                     - k log1p(-k X^2) / (-k X^2) ]  + 2 F - pi / (tau + w),
+    //   k = (tau^2 + 1) / ((tau s + w)(tau + w)),
+    // (from w s - tau = X^2 (1 - tau^2 - X^2) / (w s + tau), tau s - w =
+    // -X^2 (tau^2 + 1) / (tau s + w), w - tau = X^2 / (w + tau)), so
+    // alpha = 4 rho0 Rs P C (dx, dy) without cancellation or 1 / X^2.
+    const float Xf = float(X), X2 = Xf * Xf;
+    const float tau = float(r[TN_TAU]), tau2 = float(r[TN_TAU2]);
+    const float sf = Fn<float>::sqrt_((1.0f - Xf) * (1.0f + Xf));
+    const float w = Fn<float>::sqrt_(tau2 + X2);
+    const float lnX = Xf > 0.5f ? ::log1pf(Xf - 1.0f) : ::logf(Xf);
+    const float k = float(r[TN_T2P1]) * Fn<float>::rcp((tau * sf + w) * (tau + w));
+    const float y = -k * X2;
+    const float l1 = fabsf(y) < 1e-4f ? 1.0f + y * (-0.5f + y * (1.0f / 3.0f)) : ::log1pf(y) / y;
+    const float itw = Fn<float>::rcp(tau + w);
+    const float C = float(r[TN_T2M1]) * (lnX * (1.0f - tau2 - X2) * Fn<float>::rcp((w * sf + tau) * tau * sf) +
+                                         ::log1pf(sf) * Fn<float>::rcp(sf * (1.0f + sf)) -
+                                         ::logf(tau + w) * itw / tau - l1 * k) +
+                    2.0f * nfw_h_f32(Xf) - float(M_PI) * itw;
+    const float a = float(r[TN_A]) * float(r[TN_PRE]) * C;  // 4 rho0 Rs g / X^2
+    ax += double(a * float(dx));
+    ay += double(a * float(dy));
     return;
   }
   if (sizeof(T) == sizeof(float) && X >= T(1.5)) {
