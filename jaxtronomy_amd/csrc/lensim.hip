@@ -128,9 +128,18 @@ int kyp_of(int ky) { return (ky + kWin - 1) / kWin * kWin; }
 // tile, with A = 16 image rows of the padded tile and B[k][n] = Kf[a][k - n]
 // (the band of kernel row a).  nks K-steps of 32 cover 16 + kx - 1 columns.
 constexpr int kMfWaves = 4;  // waves per block, each 16 output rows
-constexpr int kMfTiles = 2;  // 16-column tiles per wave
+constexpr int kMfTiles = 4;  // 16-column tiles per wave
 constexpr int kMfTY = 16 * kMfWaves, kMfTX = 16 * kMfTiles;
 int mfma_nks(int kx) { return (16 + kx - 1 + 31) / 32; }
+constexpr int kMfMaxNks = 4;  // kernels up to 113 columns
+// Kernel rows per unrolled loop body (and per float -> double fold): the B
+// fragments of that many rows (about 4 K-steps) are in flight at once.
+constexpr int mfma_unroll(int nks) { return (4 + nks - 1) / nks; }
+// B-fragment steps: ky padded to a multiple of the unroll with zero rows.
+int mfma_steps(int kx, int ky) {
+  const int nks = mfma_nks(kx), u = mfma_unroll(nks);
+  return (ky + u - 1) / u * u * nks;
+}
 int mfma_tw(int kx) { return 16 * (kMfTiles - 1) + 32 * mfma_nks(kx); }
 // LDS row stride in halves: >= tw and = 8 (mod 16), so the 8 rows read by
 // lanes 0..7 of a ds_read_b128 land on distinct banks
@@ -200,7 +209,7 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
     }
   }
   L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments
-  if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(ky) * mfma_nks(kx) * 2 * 64 * 16);
+  if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(mfma_steps(kx, ky)) * 2 * 64 * 16);
   L.total = off;
   return L;
 }
@@ -736,9 +745,11 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // v_mfma_f32_16x16x32_f16 (lane l holds B[8 (l / 16) + j][l % 16], j < 8),
 // split into fp16 hi + lo of the kernel scaled by 2^sk (max |K| 2^sk in
 // [2^14, 2^15)).  One thread per fragment; every block reduces the kernel
-// maximum itself (K is small), block 0 stores sk in hdr[0].
+// maximum itself (K is small), block 0 stores sk in hdr[0].  The fragments of
+// the padding steps (ky * nks up to n_steps) are zero.
 __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
-                                                            int *__restrict__ hdr, half8 *__restrict__ frag) {
+                                                            int n_steps, int *__restrict__ hdr,
+                                                            half8 *__restrict__ frag) {
   __shared__ double red[4];
   double m = 0.0;
   for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(k[i]));
@@ -750,8 +761,12 @@ __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__rest
   if (blockIdx.x == 0 && threadIdx.x == 0) hdr[0] = sk;
   const double scale = ldexp(1.0, sk);  // exact power of two
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ky * nks * 2 * 64) return;
+  if (idx >= n_steps * 2 * 64) return;
   const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
+  if (ak >= ky * nks) {
+    frag[idx] = half8{};
+    return;
+  }
   const int a = ak / nks, ks = ak - a * nks;
   const int n = lane & 15, g = lane >> 4;
   const double *krow = k + size_t(ky - 1 - a) * kx;
@@ -771,17 +786,30 @@ __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__rest
 // lo planes of the image scaled by 2^sa (sa from the per-image max); per
 // kernel row a and K-step, three MFMAs (hi.hi, hi.lo, lo.hi) accumulate in
 // float32, and each row's sum is added to a float64 accumulator.
-__global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int ls, const float *__restrict__ in,
+template <int NKS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void conv_mfma_kernel(
+    ConvArgs c, int ls, const float *__restrict__ in,
                                                         const int *__restrict__ hdr,
                                                         const half8 *__restrict__ frag,
                                                         const float *__restrict__ blkmax, int nblk,
                                                         float *__restrict__ out, double *__restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int img = blockIdx.y, tile_id = blockIdx.x;
+  // XCD-aware order: consecutive workgroups go to different XCDs (round
+  // robin over 8), so the workgroups one XCD receives are given consecutive
+  // (image, tile) indices and share their halos in that XCD's L2
+  int img = blockIdx.y, tile_id = blockIdx.x;
+  {
+    const int n_blk = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+    if ((n_blk & 7) == 0) {
+      const int logical = (lin & 7) * (n_blk >> 3) + (lin >> 3);
+      img = logical / gridDim.x;
+      tile_id = logical - img * gridDim.x;
+    }
+  }
   const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
   const int Y0 = tyi * kMfTY, X0 = txi * kMfTX;
   const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
-  const int TH = kMfTY + c.ky - 1, TW = 16 * (kMfTiles - 1) + 32 * nks;
+  const int TH = kMfTY + c.ky - 1, TW = 16 * (kMfTiles - 1) + 32 * NKS;
   _Float16 *lh = reinterpret_cast<_Float16 *>(smem_raw);
   _Float16 *ll = lh + TH * ls;
   const int tid = threadIdx.x;
@@ -798,69 +826,107 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs c, int nks, int
   const int sa = (mx > 0.f && isfinite(mx)) ? 14 - ilogbf(mx) : 0;
   const int sk = hdr[0];
   const float *src = in + size_t(img) * c.nx * c.ny;
-  for (int idx = tid; idx < TH * TW; idx += 256) {
-    const int ly = idx / TW, lx = idx - ly * TW;
-    int gy = Y0 - cy + ly, gx = X0 - cx + lx;
-    float v = 0.f;
-    if (c.edge) {
-      gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
-      gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
-      v = src[gy * c.nx + gx];
-    } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
-      v = src[gy * c.nx + gx];
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  const float scale_a = ldexpf(1.0f, sa);  // exact power of two
+  // The padded tile: each wave fills rows wave, wave + 4, ..., lanes along a
+  // row.  Every load reads a clamped (in-image) address and is zeroed after
+  // the fact outside the image, so a batch of kLdRows loads per lane is
+  // issued before the first one is waited for.
+  constexpr int kLdRows = 8;
+  for (int lx0 = 0; lx0 < TW; lx0 += 64) {
+    const int lx = lx0 + lane;
+    const int gx0 = X0 - cx + lx;
+    const int gx = gx0 < 0 ? 0 : (gx0 >= c.nx ? c.nx - 1 : gx0);
+    const bool col_ok = lx < TW && (c.edge || gx0 == gx);
+    for (int ly0 = wave; ly0 < TH; ly0 += kMfWaves * kLdRows) {
+      float v[kLdRows];
+#pragma unroll
+      for (int r = 0; r < kLdRows; ++r) {
+        const int gy0 = Y0 - cy + ly0 + kMfWaves * r;
+        const int gy = gy0 < 0 ? 0 : (gy0 >= c.ny ? c.ny - 1 : gy0);
+        const float x = src[size_t(gy) * c.nx + gx];
+        v[r] = (c.edge || gy0 == gy) ? x : 0.f;
+      }
+#pragma unroll
+      for (int r = 0; r < kLdRows; ++r) {
+        const int ly = ly0 + kMfWaves * r;
+        if (lx < TW && ly < TH) {
+          const float xs = (col_ok ? v[r] : 0.f) * scale_a;
+          const _Float16 h = (_Float16)xs;
+          lh[ly * ls + lx] = h;
+          ll[ly * ls + lx] = (_Float16)(xs - (float)h);
+        }
+      }
     }
-    const float xs = ldexpf(v, sa);
-    const _Float16 h = (_Float16)xs;
-    lh[ly * ls + lx] = h;
-    ll[ly * ls + lx] = (_Float16)(xs - (float)h);
   }
   __syncthreads();
-  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   double dacc[kMfTiles][4];
 #pragma unroll
   for (int t = 0; t < kMfTiles; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j) dacc[t][j] = 0.0;
   const int rowbase = (16 * wave + li) * ls + 8 * lg;
-  // one step per (kernel row a, K block ks); the next step's B fragments are
-  // loaded (from L2) before this step's MFMAs so their latency overlaps
-  const int n_it = c.ky * nks;
-  half8 bh = frag[0 * 64 + lane];
-  half8 bl = frag[1 * 64 + lane];
-  floatx4 acc[kMfTiles];
+  // One step per (kernel row a, K block ks).  The rows are padded with
+  // zero-B rows to a multiple of U; a body covers U rows (R = U * NKS steps)
+  // and ends with one float -> double fold.  Two register sets of B
+  // fragments (from L2) alternate between bodies: a set is refilled with the
+  // steps 2R ahead as soon as its body's MFMAs are issued, so its loads fly
+  // during the other set's body.  No load sits under a branch (the refills
+  // past the last step re-read the last fragments and are not used).
+  constexpr int U = mfma_unroll(NKS), R = U * NKS;
+  const int ky_pad = (c.ky + U - 1) / U * U, n_steps = ky_pad * NKS;
+  half8 bq[2][R][2];
+  auto refill = [&](half8 (&b)[R][2], int s0) {
 #pragma unroll
-  for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  int a = 0, ks = 0;
-  for (int it = 0; it < n_it; ++it) {
-    const int itn = it + 1 < n_it ? it + 1 : it;
-    const half8 bhn = frag[(itn * 2 + 0) * 64 + lane];
-    const half8 bln = frag[(itn * 2 + 1) * 64 + lane];
-    half8 ah[kMfTiles], al[kMfTiles];
-#pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) {
-      const int off = rowbase + a * ls + 16 * t + 32 * ks;
-      ah[t] = *reinterpret_cast<const half8 *>(lh + off);
-      al[t] = *reinterpret_cast<const half8 *>(ll + off);
+    for (int p = 0; p < R; ++p) {
+      const int q = s0 + p < n_steps ? s0 + p : n_steps - 1;
+      b[p][0] = frag[(q * 2 + 0) * 64 + lane];
+      b[p][1] = frag[(q * 2 + 1) * 64 + lane];
     }
+  };
+  auto body = [&](const half8 (&b)[R][2], int a0) {
+    floatx4 acc[kMfTiles];
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bh, acc[t], 0, 0, 0);
+    for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bl, acc[t], 0, 0, 0);
+    for (int u = 0; u < U; ++u) {
+      const int ar = a0 + u < c.ky ? a0 + u : c.ky - 1;  // padding rows re-read the last row (times zero B)
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bh, acc[t], 0, 0, 0);
-    bh = bhn;
-    bl = bln;
-    if (++ks == nks) {  // a kernel row is complete: fold its float sums into double
-      ks = 0;
-      ++a;
+      for (int ks = 0; ks < NKS; ++ks) {
+        const int p = u * NKS + ks;
+        half8 ah[kMfTiles], al[kMfTiles];
 #pragma unroll
-      for (int t = 0; t < kMfTiles; ++t) {
+        for (int t = 0; t < kMfTiles; ++t) {
+          const int off = rowbase + ar * ls + 16 * t + 32 * ks;
+          ah[t] = *reinterpret_cast<const half8 *>(lh + off);
+          al[t] = *reinterpret_cast<const half8 *>(ll + off);
+        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
-        acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < kMfTiles; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], b[p][0], acc[t], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < kMfTiles; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], b[p][1], acc[t], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < kMfTiles; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], b[p][0], acc[t], 0, 0, 0);
       }
     }
+#pragma unroll
+    for (int t = 0; t < kMfTiles; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
+  };
+  refill(bq[0], 0);
+  refill(bq[1], R);
+  int a0 = 0;
+  for (; a0 + 2 * U <= ky_pad; a0 += 2 * U) {
+    body(bq[0], a0);
+    refill(bq[0], (a0 + 2 * U) * NKS);
+    body(bq[1], a0 + U);
+    refill(bq[1], (a0 + 3 * U) * NKS);
   }
+  if (a0 < ky_pad) body(bq[0], a0);  // an odd number of bodies: the last one
   // lane l holds D[4 (l / 16) + j][l % 16] of each tile
   const double unscale = ldexp(1.0, -(sa + sk));
   double chi = 0.0;
@@ -1152,7 +1218,33 @@ size_t conv_smem(int kx, int kyp) {
 template <typename T>
 bool use_mfma(bool identity, int kx, int ky) {
   if (sizeof(T) != sizeof(float) || identity) return false;
-  return mfma_smem(kx, ky) <= 65536;
+  return mfma_nks(kx) <= kMfMaxNks && mfma_smem(kx, ky) <= kMaxLds;
+}
+
+template <int NKS>
+int launch_conv_mfma_n(const ConvArgs &c, dim3 grid, const float *in, const int *hdr, const half8 *frag,
+                       const float *blkmax, int nblk, float *out, double *partial, hipStream_t st) {
+  const size_t smem = mfma_smem(c.kx, c.ky);
+  if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_mfma_kernel<NKS>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_mfma_kernel)");
+  }
+  conv_mfma_kernel<NKS><<<grid, 256, smem, st>>>(c, mfma_ls(c.kx), in, hdr, frag, blkmax, nblk, out, partial);
+  return check_launch("conv_mfma_kernel");
+}
+
+// The MFMA convolution of n_img images (c.kx, c.ky, c.edge, c.n_tiles_x set).
+int launch_conv_mfma(const ConvArgs &c, int n_tiles, int n_img, const float *in, const int *hdr, const half8 *frag,
+                     const float *blkmax, int nblk, float *out, double *partial, hipStream_t st) {
+  const dim3 grid(n_tiles, n_img);
+  switch (mfma_nks(c.kx)) {
+    case 1: return launch_conv_mfma_n<1>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 2: return launch_conv_mfma_n<2>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 3: return launch_conv_mfma_n<3>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 4: return launch_conv_mfma_n<4>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    default: return fail(LENSIM_EUNSUPPORTED, "psf width %d exceeds the matrix-core convolution", c.kx);
+  }
 }
 
 // The image pipeline shared by lensim_image and lensim_loglike.
@@ -1169,8 +1261,9 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
   half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
   if (mfma) {
-    const int n_frag = ky * mfma_nks(kx) * 2 * 64;
-    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
+    const int n_frag = mfma_steps(kx, ky) * 2 * 64;
+    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx),
+                                                               mfma_steps(kx, ky), hdr, frag);
     rc = check_launch("psf_mfma_prep_kernel");
   } else {
     const int n = kyp * kx;
@@ -1217,12 +1310,8 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   if (mfma) {
     c.n_tiles_x = (g.nx + kMfTX - 1) / kMfTX;
     n_tiles = mfma_tiles(g.nx, g.ny);
-    dim3 cgrid(n_tiles, n_img);
-    conv_mfma_kernel<<<cgrid, 256, mfma_smem(kx, ky), st>>>(c, mfma_nks(kx), mfma_ls(kx),
-                                                             reinterpret_cast<const float *>(lowres), hdr, frag,
-                                                             blkmax, (npix + 255) / 256,
-                                                             reinterpret_cast<float *>(out), partial);
-    rc = check_launch("conv_mfma_kernel");
+    rc = launch_conv_mfma(c, n_tiles, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
+                          (npix + 255) / 256, reinterpret_cast<float *>(out), partial, st);
   } else {
     c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
     n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
@@ -1393,8 +1482,9 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
   int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
   half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
   if (mfma) {
-    const int n_frag = ky * mfma_nks(kx) * 2 * 64;
-    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx), hdr, frag);
+    const int n_frag = mfma_steps(kx, ky) * 2 * 64;
+    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx),
+                                                               mfma_steps(kx, ky), hdr, frag);
     rc = check_launch("psf_mfma_prep_kernel");
   } else {
     psf_prep_kernel<<<(kyp * kx + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, kyp, kf);
@@ -1426,10 +1516,8 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
     c.kyp = kyp;
     c.edge = edge;
     c.n_tiles_x = (gh.nx + kMfTX - 1) / kMfTX;
-    conv_mfma_kernel<<<dim3(mfma_tiles(gh.nx, gh.ny), n_img), 256, mfma_smem(kx, ky), st>>>(
-        c, mfma_nks(kx), mfma_ls(kx), reinterpret_cast<const float *>(hr), hdr, frag, blkmax, (npix_h + 255) / 256,
-        reinterpret_cast<float *>(chr), nullptr);
-    rc = check_launch("conv_mfma_kernel");
+    rc = launch_conv_mfma(c, mfma_tiles(gh.nx, gh.ny), n_img, reinterpret_cast<const float *>(hr), hdr, frag,
+                          blkmax, (npix_h + 255) / 256, reinterpret_cast<float *>(chr), nullptr, st);
   } else {
     rc = launch_conv_plain<T>(gh.nx, gh.ny, kx, ky, edge, hr, kf, chr, n_img, st);
   }

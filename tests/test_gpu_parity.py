@@ -306,7 +306,8 @@ def test_noise_map_likelihood_and_gaussian_psf():
 
 # ------------------------------------------------------------ convolution --
 
-@pytest.mark.parametrize("shape", [(3, 3), (5, 11), (11, 11), (17, 17), (21, 9), (41, 41), (61, 61)])
+@pytest.mark.parametrize("shape", [(3, 3), (5, 11), (11, 11), (17, 17), (21, 9), (41, 41), (61, 61), (15, 75),
+                                   (101, 101)])
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("mixed", 2e-6)])
 def test_pixel_psf_shapes_asymmetric(shape, precision, tol):
     """Asymmetric (orientation-sensitive) and non-square kernels, both forms;
