@@ -192,10 +192,10 @@ def serve_measure(im, params_np, data, capacity, seconds, fractions):
     slot = srv.put_data("epl180", data)
     slots = np.full(params_np.shape[0], slot)
     srv.warmup("epl180", params_np[0])
-    srv.start()
-    points = []
+    points = []  # no background thread: the load loop steps the scheduler itself
     try:
-        poisson_load(srv, "epl180", params_np, 0.2 * capacity, 0.2, slots=slots, seed=99)
+        # steady state first: the lowest offered rate for half a second, not measured
+        poisson_load(srv, "epl180", params_np, fractions[0] * capacity, 0.5, slots=slots, seed=99)
         for i, frac in enumerate(fractions):
             rate = frac * capacity
             b0, n0 = srv.batches, srv.requests_done
@@ -376,7 +376,8 @@ def main():
         log("bench: served load")
         line["serving"] = {
             "workload": "Poisson arrivals of single log-likelihood requests (same model and data as above) on the "
-                        "continuous-batching server: max batch 64, 3 graph lanes, 4 streams",
+                        "continuous-batching server: max batch 64, 3 graph lanes, 4 streams; arrivals and "
+                        "scheduler in one event loop",
             "points": serve_measure(im32, params_np, data, value, args.serve_seconds, (0.25, 0.5, 0.75)),
         }
     if not args.no_cpu:

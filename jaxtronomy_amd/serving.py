@@ -283,7 +283,8 @@ class LensServer(object):
             lane.slot_np[:] = 0
             for kind in kinds:
                 for n in sizes:
-                    b.run(lane, kind, n, stream)
+                    b.run(lane, kind, n, stream)  # capture
+                    b.run(lane, kind, n, stream)  # first replay (uploads the graph)
         stream.synchronize()
 
     def put_data(self, key, image, noise_map=None):
@@ -476,16 +477,27 @@ def latency_percentiles(futures, qs=(50, 99)):
 
 
 def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None, seed=0, tick=50e-6):
-    """Offer requests to a running server as a Poisson process of ``rate``
-    requests/s for ``duration`` seconds, cycling through ``params`` [M, n, 8]
-    (and ``slots`` [M]); waits for completion and returns (futures, wall seconds
-    from the first arrival to the last completion)."""
+    """Offer requests to a server as a Poisson process of ``rate`` requests/s
+    for ``duration`` seconds, cycling through ``params`` [M, n, 8] (and
+    ``slots`` [M]); waits for completion and returns (futures, wall seconds
+    from the first arrival to the last completion).
+
+    With the server's background thread running (``start``), arrivals are
+    submitted from this thread and the scheduler runs in its own.  Without it,
+    this thread is the server's event loop: it submits the arrivals that are
+    due and steps the scheduler in turn, never sleeping.  The second form
+    avoids two Python threads contending for the interpreter lock: a loop that
+    releases and retakes the lock every few tens of microseconds (as an
+    arrival loop does between arrivals) can keep a waiting thread from taking
+    it for tens of milliseconds."""
     import gc
 
+    driven = server._thread is None
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rate * duration))
-    gaps = rng.exponential(1.0 / rate, n_req)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))  # offsets, shifted to the clock below
     m = params.shape[0]
+    slots = None if slots is None else np.asarray(slots)
     futs = []
     i = 0
     # the cyclic garbage collector is paused while load is offered: the
@@ -493,20 +505,22 @@ def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None
     # the outstanding requests would stall the loop for tens of milliseconds
     gc.collect()
     gc.disable()
-    t0 = time.perf_counter() + 1e-3
-    arrivals = t0 + np.cumsum(gaps)
+    arrivals += time.perf_counter() + 1e-3
     try:
         while i < n_req:
             now = time.perf_counter()
             j = int(np.searchsorted(arrivals, now, side="right"))
             if j > i:
                 idx = np.arange(i, j) % m
-                futs.extend(server.submit_many(key, params[idx], kind,
-                                               None if slots is None else np.asarray(slots)[idx],
+                futs.extend(server.submit_many(key, params[idx], kind, None if slots is None else slots[idx],
                                                t_arrival=arrivals[i:j]))
                 i = j
-            else:
+            if driven:
+                server.step()
+            elif j <= i:
                 time.sleep(tick)
+        if driven:
+            server.drain()
         for f in futs:
             f.result(timeout=120)
     finally:

@@ -180,15 +180,21 @@ def test_noise_map_slots():
         assert alone != plain
 
 
-def test_poisson_load_latency():
+@pytest.mark.parametrize("threaded", [True, False])
+def test_poisson_load_latency(threaded):
+    """Both loop forms: the scheduler in its own thread, and the load loop
+    driving the scheduler itself (one event loop)."""
     from jaxtronomy_amd.serving import poisson_load
 
     im, cfg = _image_model("mixed", n=64, s=2, k=13)
     srv = LensServer(n_streams=4)
     srv.register("a", im, max_batch=32, n_slots=2, n_lanes=3)
-    slot = srv.put_data("a", _observations(im, cfg, 1, 2)[0])
+    obs = _observations(im, cfg, 1, 2)[0]
+    slot = srv.put_data("a", obs)
     params = _requests(im, 16, 3)
-    srv.start()
+    srv.warmup("a", params[0])
+    if threaded:
+        srv.start()
     try:
         futs, wall = poisson_load(srv, "a", params, rate=5000.0, duration=0.3, slots=[slot] * 16)
     finally:
@@ -197,3 +203,6 @@ def test_poisson_load_latency():
     p = latency_percentiles(futs)
     assert 0 < p[50] <= p[99] < 1e4
     assert srv.batches < len(futs)  # requests were batched
+    # the results equal the same requests evaluated alone
+    for i in range(16):
+        assert futs[i].value == _alone_logl(im, params[i], obs)
