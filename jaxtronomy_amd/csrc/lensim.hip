@@ -141,12 +141,12 @@ int mfma_steps(int kx, int ky) {
   return (ky + u - 1) / u * u * nks;
 }
 int mfma_tw(int kx) { return 16 * (kMfTiles - 1) + 32 * mfma_nks(kx); }
-// LDS row stride in halves: >= tw and = 8 (mod 16), so the 8 rows read by
-// lanes 0..7 of a ds_read_b128 land on distinct banks
-int mfma_ls(int kx) {
-  const int tw = mfma_tw(kx);
-  return tw + ((8 - tw % 16) + 16) % 16;
-}
+// LDS row stride in halves.  The A reads (ds_read_b128, lane l at row l % 16,
+// 16-byte column l / 16) are free of bank conflicts when the stride in dwords
+// is 8 (mod 16), as measured on gfx950 (strides of 4 or 12 (mod 16) dwords
+// conflict, 0 (mod 64) serialises); tw = 48 + 32 nks halves is 16 (mod 32),
+// so the unpadded width already has that stride.
+int mfma_ls(int kx) { return mfma_tw(kx); }
 size_t mfma_smem(int kx, int ky) { return size_t(kMfTY + ky - 1) * mfma_ls(kx) * 2 * sizeof(_Float16); }
 int mfma_tiles(int nx, int ny) { return ((nx + kMfTX - 1) / kMfTX) * ((ny + kMfTY - 1) / kMfTY); }
 
@@ -813,6 +813,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   _Float16 *lh = reinterpret_cast<_Float16 *>(smem_raw);
   _Float16 *ll = lh + TH * ls;
   const int tid = threadIdx.x;
+  const float *src = in + size_t(img) * c.nx * c.ny;
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  // The padded tile: each wave fills rows wave, wave + 4, ..., lanes along a
+  // row, in batches of kLdRows rows x kLdCols 64-column chunks.  Every load
+  // reads a clamped (in-image) address and is zeroed after the fact outside
+  // the image, so a whole batch is issued before the first load is waited
+  // for; the first batch is in flight while the image maximum is reduced.
+  constexpr int kLdRows = 16, kLdCols = (16 * (kMfTiles - 1) + 32 * NKS + 63) / 64;
+  float v[kLdRows][kLdCols];
+  auto load_batch = [&](int ly0) {
+#pragma unroll
+    for (int r = 0; r < kLdRows; ++r) {
+      const int gy0 = Y0 - cy + ly0 + kMfWaves * r;
+      const int gy = gy0 < 0 ? 0 : (gy0 >= c.ny ? c.ny - 1 : gy0);
+      const bool row_ok = c.edge || gy0 == gy;
+#pragma unroll
+      for (int q = 0; q < kLdCols; ++q) {
+        const int gx0 = X0 - cx + 64 * q + lane;
+        const int gx = gx0 < 0 ? 0 : (gx0 >= c.nx ? c.nx - 1 : gx0);
+        const float x = src[size_t(gy) * c.nx + gx];
+        v[r][q] = (row_ok && (c.edge || gx0 == gx)) ? x : 0.f;
+      }
+    }
+  };
+  load_batch(wave);
   // image max |value| from the render blocks' maxima
   __shared__ float smax[4];
   {
@@ -825,39 +850,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   const float mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
   const int sa = (mx > 0.f && isfinite(mx)) ? 14 - ilogbf(mx) : 0;
   const int sk = hdr[0];
-  const float *src = in + size_t(img) * c.nx * c.ny;
-  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const float scale_a = ldexpf(1.0f, sa);  // exact power of two
-  // The padded tile: each wave fills rows wave, wave + 4, ..., lanes along a
-  // row.  Every load reads a clamped (in-image) address and is zeroed after
-  // the fact outside the image, so a batch of kLdRows loads per lane is
-  // issued before the first one is waited for.
-  constexpr int kLdRows = 8;
-  for (int lx0 = 0; lx0 < TW; lx0 += 64) {
-    const int lx = lx0 + lane;
-    const int gx0 = X0 - cx + lx;
-    const int gx = gx0 < 0 ? 0 : (gx0 >= c.nx ? c.nx - 1 : gx0);
-    const bool col_ok = lx < TW && (c.edge || gx0 == gx);
-    for (int ly0 = wave; ly0 < TH; ly0 += kMfWaves * kLdRows) {
-      float v[kLdRows];
+  auto store_batch = [&](int ly0) {
 #pragma unroll
-      for (int r = 0; r < kLdRows; ++r) {
-        const int gy0 = Y0 - cy + ly0 + kMfWaves * r;
-        const int gy = gy0 < 0 ? 0 : (gy0 >= c.ny ? c.ny - 1 : gy0);
-        const float x = src[size_t(gy) * c.nx + gx];
-        v[r] = (c.edge || gy0 == gy) ? x : 0.f;
-      }
+    for (int r = 0; r < kLdRows; ++r) {
+      const int ly = ly0 + kMfWaves * r;
 #pragma unroll
-      for (int r = 0; r < kLdRows; ++r) {
-        const int ly = ly0 + kMfWaves * r;
+      for (int q = 0; q < kLdCols; ++q) {
+        const int lx = 64 * q + lane;
         if (lx < TW && ly < TH) {
-          const float xs = (col_ok ? v[r] : 0.f) * scale_a;
+          const float xs = v[r][q] * scale_a;
           const _Float16 h = (_Float16)xs;
           lh[ly * ls + lx] = h;
           ll[ly * ls + lx] = (_Float16)(xs - (float)h);
         }
       }
     }
+  };
+  store_batch(wave);
+  for (int ly0 = wave + kMfWaves * kLdRows; ly0 < TH; ly0 += kMfWaves * kLdRows) {
+    load_batch(ly0);
+    store_batch(ly0);
   }
   __syncthreads();
   double dacc[kMfTiles][4];
