@@ -278,6 +278,192 @@ def main_substructure(args):
     print(json.dumps(line), flush=True)
 
 
+GRAD_FREE = [(0, 0), (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (2, 3), (2, 4),
+             (3, 0), (3, 1), (3, 2), (3, 3), (3, 4)]  # EPL 6, SHEAR 2, source 5, lens light 5 (centres fixed)
+
+
+def main_grad(args):
+    """Gradient config: dlogL/dtheta of 18 parameters at 256 perturbed points,
+    120x120 grid, supersampling 2, 21x21 Moffat PSF, mock data from a seeded
+    truth; mixed form timed, FP64 form beside it."""
+    import torch
+
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model import LensModel
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+
+    cfg = dict(W.EPL180, numPix=120, supersampling=2)
+    kernel = W.moffat_kernel(21, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    truth = W.epl180_draw(0)
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+
+    clean = model("fp64").image(*truth).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(3))
+    im64, im32 = model("fp64", data), model("mixed", data)
+    base, _ = im64.pack(*truth)
+    rng = np.random.default_rng(W.seed(3, 0))
+    p = np.repeat(base, args.batch, axis=0)
+    for (k, i) in GRAD_FREE:  # truth + N(0, 0.02 x scale) on every free parameter
+        p[:, k, i] += rng.normal(0, 0.02, args.batch) * max(abs(base[0, k, i]), 0.1)
+    params = torch.as_tensor(p, device="cuda")
+
+    def timed(im, steps, warmup):
+        for _ in range(warmup):
+            im.likelihood_and_grad_batch(params)
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
+            ll, g = im.likelihood_and_grad_batch(params)
+            evs[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(steps)])
+        return per, ll.cpu().numpy(), g.cpu().numpy()
+
+    log("bench: timing mixed gradients (%d steps)" % args.steps)
+    per32, ll32, g32 = timed(im32, args.steps, args.warmup)
+    log("bench: timing fp64 gradients (%d steps)" % args.fp64_steps)
+    per64, ll64, g64 = timed(im64, args.fp64_steps, max(1, args.warmup // 2))
+    scale = np.array([np.max(np.abs(g64[:, k, i])) for (k, i) in GRAD_FREE])
+    err = np.array([np.max(np.abs(g32[:, k, i] - g64[:, k, i])) for (k, i) in GRAD_FREE]) / scale
+    ms32, ms64 = float(np.sum(per32)) / args.steps, float(np.sum(per64)) / args.fp64_steps
+    line = {
+        "metric": "gradient evaluations/s (logL + 18-parameter gradient), 120^2 ss2, 21x21 PSF, mixed precision",
+        "value": round(args.batch / (ms32 / 1e3), 1),
+        "unit": "gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "EPL+SHEAR, SERSIC_ELLIPSE source + lens light, 120x120 @0.04\", supersampling 2, "
+                               "21x21 Moffat PSF, mock data from a seeded truth, points = truth + N(0, 0.02 scale)",
+                   "batch": args.batch, "free_parameters": len(GRAD_FREE)},
+        "p50_ms": round(float(np.percentile(per32, 50)), 4),
+        "p99_ms": round(float(np.percentile(per32, 99)), 4),
+        "fp64": {"value": round(args.batch / (ms64 / 1e3), 1), "ms_per_step": round(ms64, 4)},
+        "accuracy": {"mixed_grad_max_err_rel_to_param_scale_vs_fp64": float(np.max(err)),
+                     "mixed_logl_max_rel_vs_fp64": float(np.max(np.abs(ll32 - ll64) / np.abs(ll64)))},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def main_serve_mix(args):
+    """Serving mix: Poisson arrivals of log-likelihood (80%) and simulate (20%)
+    requests over 12 buckets spanning grids {64, 100, 180, 240}^2, supersampling
+    {1, 3, 5}, deflectors {SIE+SHEAR, EPL+SHEAR, NFW+SIE} and PSFs 11-41, each
+    request naming one of its bucket's resident observations and noise maps."""
+    import torch
+
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model import LensModel
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+    from jaxtronomy_amd.serving import LensServer, latency_percentiles, poisson_load_mix
+
+    m = W.SERVE_MIX
+    combos = W.mix_buckets(12)
+    n_obs, n_req_rows, max_batch = args.mix_slots, 16, 32
+    srv = LensServer(n_streams=4)
+    streams, cost, names, rays = [], [], [], []
+    log("bench: registering %d buckets (%d resident observations each)" % (len(combos), n_obs))
+    for bi, (n, s, defl, k) in enumerate(combos):
+        cfg = dict(numPix=n, deltaPix=m["deltaPix"], supersampling=s, exposure_time=m["exposure_time"],
+                   background_rms=m["background_rms"])
+        kernel = W.moffat_kernel(k, m["psf_fwhm"] / m["deltaPix"], m["psf_beta"])
+        im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(list(defl)), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        kwargs_numerics={"supersampling_factor": s}, precision="mixed")
+        key = "b%d" % bi
+        srv.register(key, im, max_batch=max_batch, n_slots=n_obs, n_lanes=2, noise_maps=True)
+        rows = np.stack([im.pack(*W.mix_draw(defl, 100 * bi + d))[0][0] for d in range(n_req_rows)])
+        truth = np.stack([im.pack(*W.mix_draw(defl, 100 * bi + 50 + d))[0][0] for d in range(4)])
+        clean = im.image_batch(torch.as_tensor(truth, device="cuda")).double().cpu().numpy()
+        slots = []
+        rng = np.random.default_rng(W.seed(4, 2000 + bi))
+        for o in range(n_obs):
+            c = clean[o % 4]
+            obs = W.noisy(c, cfg["exposure_time"], cfg["background_rms"], rng)
+            noise = np.sqrt(cfg["background_rms"] ** 2 + np.maximum(c, 0) / cfg["exposure_time"])
+            slots.append(srv.put_data(key, obs, noise))
+        srv.warmup(key, rows[0], kinds=("loglike", "image"))
+        # cost of a full batch on this bucket (for the mix's capacity estimate)
+        b = srv.buckets[key]
+        lane = b.lanes[0]
+        lane.params_np[:] = rows[np.arange(max_batch) % n_req_rows]
+        lane.slot_np[:] = 0
+        st = srv.streams[0]
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(st)
+        for _ in range(5):
+            b.run(lane, "loglike", max_batch, st)
+        t1.record(st)
+        st.synchronize()
+        cost.append(t0.elapsed_time(t1) / 5 / max_batch / 1e3)
+        # log-likelihood requests cycle through every resident observation
+        streams.append(dict(key=key, params=rows[np.arange(n_obs) % n_req_rows], slots=np.array(slots),
+                            kind="loglike", weight=0.8))
+        streams.append(dict(key=key, params=rows, slots=None, kind="image", weight=0.2))
+        names.append("%d^2 s%d %s K%d" % (n, s, "+".join(defl), k))
+        rays.append((n * s) ** 2)
+    capacity = 1.0 / float(np.mean(cost))  # requests/s when every request is batched at max_batch
+    log("bench: mix capacity estimate %.0f requests/s" % capacity)
+    points = []
+    poisson_load_mix(srv, streams, 0.3 * capacity, 0.5, seed=99)  # steady state first, not measured
+    for i, frac in enumerate((0.5, 0.8, 2.0)):
+        b0, n0 = srv.batches, srv.requests_done
+        futs, wall, peak, _ = poisson_load_mix(srv, streams, frac * capacity, args.serve_seconds, seed=i)
+        kinds = np.array([0 if isinstance(f.value, float) else 1 for f in futs])
+        pall = latency_percentiles(futs)
+        pll = latency_percentiles([f for f, k in zip(futs, kinds) if k == 0])
+        pim = latency_percentiles([f for f, k in zip(futs, kinds) if k == 1])
+        points.append({"offered_per_s": round(frac * capacity, 1), "achieved_per_s": round(len(futs) / wall, 1),
+                       "p50_ms": round(pall[50], 4), "p99_ms": round(pall[99], 4),
+                       "loglike_p99_ms": round(pll[99], 4), "image_p99_ms": round(pim[99], 4),
+                       "peak_in_flight": int(peak),
+                       "mean_batch": round((srv.requests_done - n0) / max(srv.batches - b0, 1), 2)})
+        log("bench: mix offered %.0f/s -> %s" % (frac * capacity, points[-1]))
+        del futs
+    srv.stop()
+    sat = points[-1]["achieved_per_s"]
+    line = {
+        "metric": "requests/s + p50/p99 latency, serving mix (grids 64-240^2, ss 1/3/5, 3 deflector sets, PSF 11-41)",
+        "value": sat,
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": len(points),
+        "warmup": 1,
+        "ms_per_step": round(1e3 / sat, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Poisson arrivals, 80% log-likelihood / 20% simulate, uniform over 12 buckets: " +
+                               "; ".join(names),
+                   "resident_observations": n_obs * len(combos), "max_batch": max_batch,
+                   "capacity_estimate_per_s": round(capacity, 1)},
+        "p50_ms": points[-1]["p50_ms"],
+        "p99_ms": points[-1]["p99_ms"],
+        "points": points,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,7 +473,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("epl180", "substructure"), default="epl180")
+    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix"), default="epl180")
+    ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
     ap.add_argument("--n-sub", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -315,6 +502,10 @@ def main():
         return
     if args.workload == "substructure":
         return main_substructure(args)
+    if args.workload == "grad":
+        return main_grad(args)
+    if args.workload == "serve-mix":
+        return main_serve_mix(args)
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2

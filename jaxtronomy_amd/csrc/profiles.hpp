@@ -95,10 +95,12 @@ enum : int { SH_G1 = 2, SH_G2 };
 //          param_util.py:103-124): amp, the product-average matrix
 //          ((1-e1), -e2; -e2, (1+e1))/norm, max(R_sersic, 1e-4), 1/n, b_n, 1/R_sersic_
 enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS,
-              SE_C1,      // invn / 2            (frac^(1/n) = exp2(C1 log2 R^2 + C2))
-              SE_C2,      // -invn log2 R_sersic_
+              SE_C1,      // invn / 2            (frac^(1/n) = exp2(C1 log2(R^2 IRS2)))
+              SE_IRS2,    // 1 / R_sersic_^2
               SE_R2MAX,   // (1000 R_sersic_)^2  (the 1000 R_sersic window on R^2)
-              SE_LNRS };  // ln R_sersic_
+              SE_LNRS,    // ln R_sersic_
+              SE_BNLO };  // b_n - float(b_n): the float form carries b_n as a head + tail pair
+static_assert(SE_BNLO < kRec, "SERSIC record exceeds kRec");
 // GAUSSIAN light (gaussian.py:27-42): amp/(2 pi sigma^2), 1/sigma^2
 enum : int { GA_C = 2, GA_IS2 };
 
@@ -277,9 +279,10 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[SE_BN] = fmax(1.9992 * n - 0.3271, 0.00001);
       rec[SE_IRS] = 1.0 / Rs_;
       rec[SE_C1] = 0.5 / n;
-      rec[SE_C2] = -log2(Rs_) / n;
+      rec[SE_IRS2] = 1.0 / (Rs_ * Rs_);
       rec[SE_R2MAX] = (1000.0 * Rs_) * (1000.0 * Rs_);
       rec[SE_LNRS] = log(Rs_);
+      rec[SE_BNLO] = rec[SE_BN] - double(float(rec[SE_BN]));
       break;
     }
     case LENSIM_GAUSSIAN: {
@@ -388,6 +391,26 @@ __device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double 
   ay += fr * s + fi * c;
 }
 
+// expm1(x) in float: for |x| < 0.5 the Taylor series to x^9 (truncation below
+// 3e-10 |x|), else 2^(x log2 e) - 1 with the product split into head and
+// rounding error (the subtraction then loses less than two bits).  Only the
+// finite arguments of the EPL power law reach it.
+__device__ __forceinline__ float expm1_f32(float x) {
+  float p = fmaf(x, 2.7557319224e-6f, 2.4801587302e-5f);
+  p = fmaf(x, p, 1.9841269841e-4f);
+  p = fmaf(x, p, 1.3888888889e-3f);
+  p = fmaf(x, p, 8.3333333333e-3f);
+  p = fmaf(x, p, 4.1666666667e-2f);
+  p = fmaf(x, p, 1.6666666667e-1f);
+  p = fmaf(x, p, 0.5f);
+  const float small = fmaf(x * x, p, x);
+  const float L = 1.4426950408889634f;
+  const float hi = x * L;
+  const float lo = fmaf(x, L, -hi) + x * 1.925963033500011e-8f;  // log2(e) - L
+  const float large = fmaf(__builtin_amdgcn_exp2f(hi), fmaf(lo, 0.6931471805599453f, 1.f), -1.f);
+  return fabsf(x) < 0.5f ? small : large;
+}
+
 // EPL, mixed form.  The geometry runs in double (rotation, Z, 1/|Z| by a
 // float rsqrt and one Newton step, the final products); float carries only
 // the parts whose size bounds their error:
@@ -446,7 +469,7 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
   iR = iR * fma(-0.5 * R2c, iR * iR, 1.5);
   // (b/R)^(t-1)
   const float lnbr = fmaf(-0.34657359f, __builtin_amdgcn_logf(R2cf), r[EPL_LNB]);  // ln b - ln(R^2)/2
-  const float em = ::expm1f(r[EPL_TM1] * lnbr);
+  const float em = expm1_f32(r[EPL_TM1] * lnbr);
   const double scale = r.d[EPL_PB] * iR * (1.0 + double(em));
   const double fr = scale * Or, fi = scale * Oi;
   ax += fr * cd - fi * sd;
@@ -678,21 +701,42 @@ __device__ __forceinline__ double sersic_kernel(double R2, const Rec<double> &r)
   return v;
 }
 
-// exp(e) in float to ~2 ulp: exp2 of e log2(e) split into a float head and the
-// rounding error of the product, 2^hi (1 + lo ln 2).
-__device__ __forceinline__ float exp_accurate_f32(float e) {
+// exp(e + e_lo) in float to ~2 ulp (|e_lo| << ulp(e)): exp2 of e log2(e) split
+// into a float head and the rounding error of the product, plus e_lo log2(e),
+// as 2^hi (1 + lo ln 2).
+__device__ __forceinline__ float exp_accurate_f32(float e, float e_lo = 0.f) {
   const float L = 1.4426950408889634f;
   const float hi = e * L;
-  const float lo = fmaf(e, L, -hi) + e * 1.925963033500011e-8f;  // log2(e) - L
+  float lo = fmaf(e, L, -hi) + e * 1.925963033500011e-8f;  // log2(e) - L
+  lo = fmaf(e_lo, L, lo);
   return __builtin_amdgcn_exp2f(hi) * fmaf(lo, 0.6931471805599453f, 1.f);
 }
 
-// Float form on R^2: frac^(1/n) = exp2(C1 log2 R^2 + C2), window R^2 <= R2MAX.
-__device__ __forceinline__ float sersic_kernel(float R2, const Rec<float> &r) {
-  const float R2c = fmaxf(1e-8f, R2);
-  const float pw = __builtin_amdgcn_exp2f(fmaf(r[SE_C1], __builtin_amdgcn_logf(R2c), r[SE_C2]));
-  const float e = r[SE_BN] * (1.f - pw);
-  return (R2c <= r[SE_R2MAX]) ? exp_accurate_f32(e) : 0.f;
+// Float form.  Avoided on purpose, because each would bias every value of a
+// profile with one sign (the image sums many of them, so a bias of 1e-7 moves
+// a log-likelihood of a poor fit by about 1e-6 relative):
+//  * float copies of the ellipse matrix, 1/R_sersic^2 and C1: the elliptical
+//    R^2, the ratio u = R^2 / R_sersic^2 and C1 log2 u are formed in double;
+//  * v_log_f32 of a large |log2 u|: gfx950's v_log_f32 reads about 0.45 ulp
+//    low on average, so the logarithm is split as exponent + log2 of the
+//    mantissa in [0.5, 1), and only the small second term is approximated;
+//  * a float b_n: the exponent b_n (1 - frac^(1/n)) takes b_n as a head + tail
+//    pair with the head product's rounding error.
+// frac^(1/n) = exp2(C1 log2 u) and exp of the exponent are then float
+// (v_exp_f32 is unbiased to 0.03 ulp).  Window: R^2 <= (1000 R_sersic)^2.
+__device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Rec<float> &r) {
+  const double xe = fma(r.d[SE_M00], dx, r.d[SE_M01] * dy);
+  const double ye = fma(r.d[SE_M01], dx, r.d[SE_M11] * dy);
+  const double R2 = fmax(1e-8, fma(xe, xe, ye * ye));
+  const float u = float(R2 * r.d[SE_IRS2]);
+  const float m = __builtin_amdgcn_frexp_mantf(u);  // u = m 2^ex, m in [0.5, 1)
+  const int ex = __builtin_amdgcn_frexp_expf(u);
+  const double L = double(ex) + double(__builtin_amdgcn_logf(m));
+  const float pw = __builtin_amdgcn_exp2f(float(r.d[SE_C1] * L));
+  const float t = 1.f - pw;
+  const float e = r[SE_BN] * t;
+  const float e_lo = fmaf(r[SE_BN], t, -e) + r[SE_BNLO] * t;
+  return (R2 <= r.d[SE_R2MAX]) ? exp_accurate_f32(e, e_lo) : 0.f;
 }
 
 // One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
@@ -704,9 +748,14 @@ __device__ __forceinline__ double light_value(int type, const Rec<T> &r, double 
   switch (type) {
     case LENSIM_SERSIC:
     case LENSIM_SERSIC_ELLIPSE: {
-      const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
-      const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
-      return double(r[SE_AMP] * sersic_kernel(xe * xe + ye * ye, r));
+      if constexpr (sizeof(T) == sizeof(float)) {
+        // the amplitude multiplies in double (a float amp would bias the profile by its rounding)
+        return r.d[SE_AMP] * double(sersic_value_f32(dxd, dyd, r));
+      } else {
+        const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
+        const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
+        return r.d[SE_AMP] * sersic_kernel(xe * xe + ye * ye, r);
+      }
     }
     case LENSIM_GAUSSIAN: {
       const T r2 = dx * dx * r[GA_IS2] + dy * dy * r[GA_IS2];

@@ -240,7 +240,7 @@ class LensServer(object):
         self.hbm_used = 0
         self.use_graphs = use_graphs
         self.buckets = {}
-        self.pending = collections.OrderedDict()  # key -> deque of _Block
+        self.pending = collections.OrderedDict()  # (key, kind) -> deque of _Block
         self.inflight = []
         self.lock = threading.Lock()
         self.cond = threading.Condition()
@@ -269,7 +269,7 @@ class LensServer(object):
         b = self.buckets[key]
         ex = np.asarray(example_params, dtype=np.float64).reshape(b.n_prof, backend.NPARAM)
         with self.lock:
-            if any(l.busy for l in b.lanes) or self.pending.get(key):
+            if any(l.busy for l in b.lanes) or any(self.pending.get((key, k)) for k in ("loglike", "image")):
                 raise RuntimeError("warm up a bucket before submitting requests to it")
         sizes = []
         n = 1
@@ -347,7 +347,7 @@ class LensServer(object):
         ts = [now] * n if t_arrival is None else list(np.asarray(t_arrival, dtype=np.float64))
         futs = [Future(self.cond, t) for t in ts]
         with self.lock:
-            self.pending.setdefault(key, collections.deque()).append(_Block(kind, params, slots, futs))
+            self.pending.setdefault((key, kind), collections.deque()).append(_Block(kind, params, slots, futs))
         return futs
 
     # ---------------------------------------------------------- scheduling --
@@ -377,7 +377,7 @@ class LensServer(object):
             self.cond.notify_all()
 
     def _take(self, q, max_n):
-        """Up to max_n requests of the first block's kind from the front of q:
+        """Up to max_n requests from the front of q, one (bucket, kind) queue:
         (kind, params, slots, futures)."""
         kind = q[0].kind
         ps, ss, fs = [], [], []
@@ -399,17 +399,17 @@ class LensServer(object):
         while self.free_streams:
             with self.lock:
                 pick = lane = None
-                for key, q in self.pending.items():
+                for qk, q in self.pending.items():
                     if q:
-                        lane = self.buckets[key].free_lane()
+                        lane = self.buckets[qk[0]].free_lane()
                         if lane is not None:
-                            pick = key
+                            pick = qk
                             break
                 if pick is None:
                     return launched
-                b = self.buckets[pick]
+                b = self.buckets[pick[0]]
                 kind, ps, ss, futs = self._take(self.pending[pick], b.max_batch)
-                self.pending.move_to_end(pick)  # round-robin between buckets
+                self.pending.move_to_end(pick)  # round-robin between (bucket, kind) queues
             n = len(futs)
             np.concatenate(ps, axis=0, out=lane.params_np[:n])
             np.concatenate(ss, axis=0, out=lane.slot_np[:n])
@@ -480,7 +480,23 @@ def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None
     """Offer requests to a server as a Poisson process of ``rate`` requests/s
     for ``duration`` seconds, cycling through ``params`` [M, n, 8] (and
     ``slots`` [M]); waits for completion and returns (futures, wall seconds
-    from the first arrival to the last completion).
+    from the first arrival to the last completion).  See ``poisson_load_mix``
+    for the two loop forms."""
+    futs, wall, _, _ = poisson_load_mix(server, [dict(key=key, params=params, slots=slots, kind=kind)], rate,
+                                        duration, seed=seed, tick=tick)
+    return futs, wall
+
+
+def poisson_load_mix(server, streams, rate, duration, seed=0, tick=50e-6):
+    """Poisson arrivals of ``rate`` requests/s for ``duration`` seconds over
+    several request streams.  Each stream is a dict with ``key`` (bucket),
+    ``params`` [M, n, 8], ``slots`` [M] (log-likelihood requests) and ``kind``,
+    and optionally ``weight``; every arrival picks a stream with probability
+    proportional to its weight and takes that stream's next parameter row.
+    Waits for completion; returns (futures in arrival order, wall seconds from
+    the first arrival to the last completion, the largest number of requests
+    in flight (submitted, not yet completed), (stream index, row) arrays of
+    the arrivals).
 
     With the server's background thread running (``start``), arrivals are
     submitted from this thread and the scheduler runs in its own.  Without it,
@@ -496,10 +512,17 @@ def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rate * duration))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))  # offsets, shifted to the clock below
-    m = params.shape[0]
-    slots = None if slots is None else np.asarray(slots)
-    futs = []
+    w = np.array([float(s.get("weight", 1.0)) for s in streams])
+    which = rng.choice(len(streams), size=n_req, p=w / w.sum())
+    row = np.zeros(n_req, dtype=np.int64)  # each stream cycles through its own rows
+    for k, s in enumerate(streams):
+        sel = np.nonzero(which == k)[0]
+        row[sel] = np.arange(sel.size) % s["params"].shape[0]
+    slots = [None if s.get("slots") is None else np.asarray(s["slots"]) for s in streams]
+    futs = [None] * n_req
     i = 0
+    done0 = server.requests_done
+    peak = 0
     # the cyclic garbage collector is paused while load is offered: the
     # server's objects form no reference cycles, and a full collection over
     # the outstanding requests would stall the loop for tens of milliseconds
@@ -511,10 +534,17 @@ def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None
             now = time.perf_counter()
             j = int(np.searchsorted(arrivals, now, side="right"))
             if j > i:
-                idx = np.arange(i, j) % m
-                futs.extend(server.submit_many(key, params[idx], kind, None if slots is None else slots[idx],
-                                               t_arrival=arrivals[i:j]))
+                due = np.arange(i, j)
+                for k in np.unique(which[due]):
+                    idx = due[which[due] == k]
+                    s = streams[k]
+                    r = row[idx]
+                    got = server.submit_many(s["key"], s["params"][r], s.get("kind", "loglike"),
+                                             None if slots[k] is None else slots[k][r], t_arrival=arrivals[idx])
+                    for q, f in zip(idx, got):
+                        futs[q] = f
                 i = j
+                peak = max(peak, i - (server.requests_done - done0))
             if driven:
                 server.step()
             elif j <= i:
@@ -526,4 +556,4 @@ def poisson_load(server, key, params, rate, duration, kind="loglike", slots=None
     finally:
         gc.enable()
     wall = max(f.t_done for f in futs) - arrivals[0]
-    return futs, wall
+    return futs, wall, peak, (which, row)

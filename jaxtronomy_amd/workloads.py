@@ -66,6 +66,57 @@ def epl180_draw(draw):
     return kwargs_lens, kwargs_source, kwargs_lens_light
 
 
+# Serving mix: grids, supersampling factors, deflector sets and PSF sizes
+# that requests are drawn from.
+SERVE_MIX = dict(grids=(64, 100, 180, 240), supersampling=(1, 3, 5),
+                 deflectors=(("SIE", "SHEAR"), ("EPL", "SHEAR"), ("NFW", "SIE")), psf_sizes=(11, 21, 31, 41),
+                 deltaPix=0.04, psf_fwhm=0.08, psf_beta=3.0, exposure_time=1000.0, background_rms=0.01)
+
+
+def mix_buckets(n, seed_=0):
+    """n (grid, s, deflectors, psf size) combinations of the serving mix: the
+    first 12 take every (grid, s) pair once, the deflector sets in turn (four
+    each) and seeded PSF sizes."""
+    rng = np.random.default_rng(seed(4, seed_))
+    m = SERVE_MIX
+    ng, ns, nd = len(m["grids"]), len(m["supersampling"]), len(m["deflectors"])
+    out = []
+    for i in range(n):
+        g = m["grids"][i % ng]
+        s = m["supersampling"][(i // ng) % ns]
+        d = m["deflectors"][(i + i // ng) % nd]
+        out.append((g, s, d, int(rng.choice(m["psf_sizes"]))))
+    return out
+
+
+def mix_draw(deflectors, draw):
+    """One seeded scene for a deflector set of the serving mix: (kwargs_lens,
+    kwargs_source, kwargs_lens_light) with one SERSIC_ELLIPSE source and lens
+    light."""
+    rng = np.random.default_rng(seed(4, 1000 + draw))
+    kl = []
+    for t in deflectors:
+        if t in ("SIE", "EPL"):
+            k = dict(theta_E=rng.uniform(0.8, 1.6), e1=rng.normal(0, 0.1), e2=rng.normal(0, 0.1),
+                     center_x=rng.normal(0, 0.02), center_y=rng.normal(0, 0.02))
+            if t == "EPL":
+                k["gamma"] = rng.uniform(1.8, 2.2)
+            kl.append(k)
+        elif t == "SHEAR":
+            kl.append(dict(gamma1=rng.normal(0, 0.03), gamma2=rng.normal(0, 0.03), ra_0=0.0, dec_0=0.0))
+        elif t == "NFW":
+            kl.append(dict(Rs=rng.uniform(2.0, 6.0), alpha_Rs=rng.uniform(0.3, 1.0), center_x=rng.normal(0, 0.05),
+                           center_y=rng.normal(0, 0.05)))
+        else:
+            raise ValueError(t)
+    ks = [dict(amp=rng.uniform(1, 100), R_sersic=rng.uniform(0.1, 0.5), n_sersic=rng.uniform(1, 4),
+               e1=rng.normal(0, 0.1), e2=rng.normal(0, 0.1), center_x=rng.uniform(-0.1, 0.1),
+               center_y=rng.uniform(-0.1, 0.1))]
+    kll = [dict(amp=rng.uniform(1, 100), R_sersic=rng.uniform(0.5, 1.5), n_sersic=rng.uniform(2, 5),
+                e1=rng.normal(0, 0.1), e2=rng.normal(0, 0.1), center_x=0.0, center_y=0.0)]
+    return kl, ks, kll
+
+
 def stack_kwargs(kwargs_lists):
     """List (over draws) of kwargs lists -> one kwargs list with array values."""
     out = []

@@ -206,3 +206,51 @@ def test_poisson_load_latency(threaded):
     # the results equal the same requests evaluated alone
     for i in range(16):
         assert futs[i].value == _alone_logl(im, params[i], obs)
+
+
+def test_serving_mix_buckets_equal_alone():
+    """Poisson arrivals over buckets of different grids, supersampling factors,
+    deflector sets and PSF sizes (log-likelihood with resident noise maps, and
+    simulate requests): every result equals the same request alone."""
+    from jaxtronomy_amd.serving import poisson_load_mix
+
+    m = W.SERVE_MIX
+    combos = [(64, 3, ("SIE", "SHEAR"), 11), (100, 1, ("NFW", "SIE"), 21), (48, 5, ("EPL", "SHEAR"), 31)]
+    srv = LensServer(n_streams=4)
+    streams, ref = [], {}
+    for bi, (n, s, defl, k) in enumerate(combos):
+        cfg = dict(numPix=n, deltaPix=m["deltaPix"], supersampling=s, exposure_time=m["exposure_time"],
+                   background_rms=m["background_rms"])
+        kernel = W.moffat_kernel(k, m["psf_fwhm"] / m["deltaPix"], m["psf_beta"])
+        im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(list(defl)), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        kwargs_numerics={"supersampling_factor": s}, precision="mixed")
+        key = "b%d" % bi
+        srv.register(key, im, max_batch=8, n_slots=3, n_lanes=2, noise_maps=True)
+        rows = np.stack([im.pack(*W.mix_draw(defl, 10 * bi + d))[0][0] for d in range(6)])
+        clean = im.image_batch(torch.as_tensor(rows[:3], device="cuda")).double().cpu().numpy()
+        rng = np.random.default_rng(bi)
+        obs, noise, slots = [], [], []
+        for o in range(3):
+            obs.append(W.noisy(clean[o], cfg["exposure_time"], cfg["background_rms"], rng))
+            noise.append(np.sqrt(cfg["background_rms"] ** 2 + np.maximum(clean[o], 0) / cfg["exposure_time"]))
+            slots.append(srv.put_data(key, obs[-1], noise[-1]))
+        srv.warmup(key, rows[0], kinds=("loglike", "image"))
+        streams.append(dict(key=key, params=rows, slots=np.array(slots)[np.arange(6) % 3], kind="loglike"))
+        streams.append(dict(key=key, params=rows, kind="image", weight=0.5))
+        ref[key] = (im, rows, obs, noise)
+    futs, wall, peak, (which, row) = poisson_load_mix(srv, streams, rate=3000.0, duration=0.2, seed=5)
+    srv.stop()
+    assert len(futs) == 600 and all(f.done() for f in futs) and peak >= 1
+    assert len(srv.batch_sizes) > 1 and len(set(which.tolist())) == len(streams)
+    for f, st, r in zip(futs, which, row):
+        s = streams[st]
+        im, rows, obs, noise = ref[s["key"]]
+        p = torch.as_tensor(rows[r][None], device="cuda")
+        if s["kind"] == "loglike":
+            o = r % 3  # slots were filled in order 0, 1, 2 and cycle with the rows
+            want = im.likelihood_batch(p, data=torch.as_tensor(obs[o][None], device="cuda"),
+                                       noise_map=torch.as_tensor(noise[o][None], device="cuda")).item()
+            assert f.value == want
+        else:
+            assert torch.equal(f.value, im.image_batch(p).cpu()[0])
