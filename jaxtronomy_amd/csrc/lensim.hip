@@ -317,11 +317,10 @@ __device__ inline bool bulk_type_ok_dev(int t) {
          t == LENSIM_SHEAR;
 }
 
-__global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int n_img, double *__restrict__ recd,
-                            float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
-                            double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n_img * m.n_prof) return;
+// Record (and EPL coefficients) of (image, profile) idx < n_img * n_prof.
+__device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, int idx, double *__restrict__ recd,
+                         float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
+                         double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
   const int img = idx / m.n_prof, k = idx - img * m.n_prof;
   int type = type_of(m, k);
   if (k >= m.n_lens && k < m.n_defl && !bulk_type_ok_dev(type)) type = LENSIM_NULL;
@@ -350,13 +349,18 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
   }
 }
 
+__global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int n_img, double *__restrict__ recd,
+                            float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
+                            double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n_img * m.n_prof) prep_one(m, params, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
+}
+
 // Flip the kernel (so the convolution runs as a correlation) and store it
 // transposed, column c of the flipped kernel contiguous: kt[c * kyp + a] =
 // K[ky-1-a][kx-1-c], zero for a >= ky.
-__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out,
-                                int flip = 1) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= kyp * kx) return;
+__device__ void psf_prep_one(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out,
+                             int flip, int idx) {
   const int c = idx / kyp, a = idx - c * kyp;
   if (!k) {  // identity kernel (no PSF / unconvolved)
     out[idx] = (a == 0 && c == 0) ? 1.0 : 0.0;
@@ -365,6 +369,12 @@ __global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, in
   // flip = 0 stores the kernel itself: the convolution then correlates with K,
   // which is the adjoint of convolving with K (zero padding)
   out[idx] = (a < ky) ? (flip ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : k[a * kx + c]) : 0.0;
+}
+
+__global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out,
+                                int flip = 1) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < kyp * kx) psf_prep_one(k, kx, ky, kyp, out, flip, idx);
 }
 
 template <typename T>
@@ -747,9 +757,9 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // [2^14, 2^15)).  One thread per fragment; every block reduces the kernel
 // maximum itself (K is small), block 0 stores sk in hdr[0].  The fragments of
 // the padding steps (ky * nks up to n_steps) are zero.
-__global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
-                                                            int n_steps, int *__restrict__ hdr,
-                                                            half8 *__restrict__ frag) {
+// One 256-thread block (index b) of the fragment preparation.
+__device__ void psf_mfma_prep_block(const double *__restrict__ k, int kx, int ky, int nks, int n_steps,
+                                    int *__restrict__ hdr, half8 *__restrict__ frag, int b) {
   __shared__ double red[4];
   double m = 0.0;
   for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(k[i]));
@@ -758,9 +768,9 @@ __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__rest
   __syncthreads();
   const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   const int sk = (mx > 0.0 && isfinite(mx)) ? 14 - ilogb(mx) : 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) hdr[0] = sk;
+  if (b == 0 && threadIdx.x == 0) hdr[0] = sk;
   const double scale = ldexp(1.0, sk);  // exact power of two
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = b * 256 + threadIdx.x;
   if (idx >= n_steps * 2 * 64) return;
   const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
   if (ak >= ky * nks) {
@@ -779,6 +789,50 @@ __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__rest
     v[j] = s == 0 ? h : (_Float16)(w - (double)h);
   }
   frag[idx] = v;
+}
+
+__global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
+                                                            int n_steps, int *__restrict__ hdr,
+                                                            half8 *__restrict__ frag) {
+  psf_mfma_prep_block(k, kx, ky, nks, n_steps, hdr, frag, blockIdx.x);
+}
+
+// The PSF preparation of a call, folded into the record preparation's launch
+// (the two are independent): mode 1 the flipped transposed kernel, 2 the MFMA
+// fragments.
+struct PsfJob {
+  int mode;
+  const double *k;
+  int kx, ky, kyp, nks, n_steps;
+  double *kt;
+  int *hdr;
+  half8 *frag;
+};
+
+int psf_job_blocks(const PsfJob &pj) {
+  if (pj.mode == 1) return (pj.kyp * pj.kx + 255) / 256;
+  if (pj.mode == 2) return (pj.n_steps * 2 * 64 + 255) / 256;
+  return 0;
+}
+
+// Blocks [0, nb_prep): one thread per (image, profile) record; the rest: the PSF job.
+__global__ __launch_bounds__(256) void prep_psf_kernel(ModelArgs m, const double *__restrict__ params, int n_img,
+                                                       double *__restrict__ recd, float *__restrict__ recf,
+                                                       double *__restrict__ coefd, float *__restrict__ coeff,
+                                                       double *__restrict__ dcoefd, float *__restrict__ dcoeff,
+                                                       PsfJob pj, int nb_prep) {
+  if (int(blockIdx.x) < nb_prep) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < n_img * m.n_prof) prep_one(m, params, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
+    return;
+  }
+  const int b = blockIdx.x - nb_prep;
+  if (pj.mode == 2) {
+    psf_mfma_prep_block(pj.k, pj.kx, pj.ky, pj.nks, pj.n_steps, pj.hdr, pj.frag, b);
+  } else {
+    const int idx = b * 256 + threadIdx.x;
+    if (idx < pj.kyp * pj.kx) psf_prep_one(pj.k, pj.kx, pj.ky, pj.kyp, pj.kt, 1, idx);
+  }
 }
 
 // Direct 'same' convolution on the matrix cores: block = 64 rows x 32 columns
@@ -1265,24 +1319,34 @@ template <typename T>
 int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
               const lensim_data *data, const double *params, int n_img, unsigned flags, T *out, double *logl,
               unsigned char *ws, const Layout &L, hipStream_t st, T *gout = nullptr) {
-  int rc = launch_prep(m, params, n_img, ws, L, st, gout != nullptr);
-  if (rc) return rc;
   const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
   const bool mfma = use_mfma<T>(identity, kx, ky);
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
   int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
   half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
+  // records and the PSF preparation in one launch
+  PsfJob pj{};
+  pj.mode = mfma ? 2 : 1;
+  pj.k = identity ? nullptr : psf->d_kernel;
+  pj.kx = kx;
+  pj.ky = ky;
+  pj.kyp = kyp;
+  pj.kt = kf;
+  pj.hdr = hdr;
+  pj.frag = frag;
   if (mfma) {
-    const int n_frag = mfma_steps(kx, ky) * 2 * 64;
-    psf_mfma_prep_kernel<<<(n_frag + 255) / 256, 256, 0, st>>>(psf->d_kernel, kx, ky, mfma_nks(kx),
-                                                               mfma_steps(kx, ky), hdr, frag);
-    rc = check_launch("psf_mfma_prep_kernel");
-  } else {
-    const int n = kyp * kx;
-    psf_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(identity ? nullptr : psf->d_kernel, kx, ky, kyp, kf);
-    rc = check_launch("psf_prep_kernel");
+    pj.nks = mfma_nks(kx);
+    pj.n_steps = mfma_steps(kx, ky);
   }
+  const int nb_prep = (n_img * m.n_prof + 255) / 256;
+  const bool dc = gout != nullptr;
+  prep_psf_kernel<<<nb_prep + psf_job_blocks(pj), 256, 0, st>>>(
+      m, params, n_img, reinterpret_cast<double *>(ws + L.recd), reinterpret_cast<float *>(ws + L.recf),
+      reinterpret_cast<double *>(ws + L.coefd), reinterpret_cast<float *>(ws + L.coeff),
+      dc ? reinterpret_cast<double *>(ws + L.dcoefd) : nullptr, dc ? reinterpret_cast<float *>(ws + L.dcoeff) : nullptr,
+      pj, nb_prep);
+  int rc = check_launch("prep_psf_kernel");
   if (rc) return rc;
   T *lowres = reinterpret_cast<T *>(ws + L.lowres);
   float *blkmax = mfma ? reinterpret_cast<float *>(ws + L.blkmax) : nullptr;

@@ -163,8 +163,16 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
         coef[0] = 1.0;
         if (nmax > 1) coef[1] = fm1;
         double cmax = fmax(1.0, fabs(fm1));
+        // the divisor 1/(cc + n - 1) does not depend on the recurrence: it is a
+        // reciprocal refined to full precision by two Newton steps, which the
+        // unrolled loop computes ahead, so the serial chain is two FMAs per n
+#pragma unroll 4
         for (int n = 2; n < nmax; ++n) {
-          const double fn = ((n - 1.0) * fm2 - (2.0 * bb - cc) * fm1) / (cc + n - 1.0);
+          const double d = cc + n - 1.0;
+          double r = __builtin_amdgcn_rcp(d);
+          r = fma(r, fma(-d, r, 1.0), r);
+          r = fma(r, fma(-d, r, 1.0), r);
+          const double fn = fma(n - 1.0, fm2, -(2.0 * bb - cc) * fm1) * r;
           coef[n] = fn;
           cmax = fmax(cmax, fabs(fn));
           fm2 = fm1;
