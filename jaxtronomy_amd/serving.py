@@ -17,8 +17,10 @@ and a lane of the oldest waiting bucket are free, it admits every request of
 that bucket that has arrived so far (up to ``max_batch``), so requests that
 arrive while other batches run join the next batch, and batches of different
 buckets (or lanes of one bucket) overlap on the device.  Results are copied to
-pinned host memory on the lane's stream before its completion event, so
-retiring a batch reads host memory only.
+pinned host memory on the lane's stream before its completion event (log-
+likelihoods to the lane's staging, images to a pinned buffer of the batch
+that the results view), so retiring a batch reads host memory only.  A
+captured batch is launched with hipGraphLaunch on its stream directly.
 
 Each image of a batch is computed from its own parameters (and observation)
 alone, so a request's result does not depend on the requests it was batched
@@ -39,6 +41,26 @@ import numpy as np
 import torch
 
 from . import backend
+
+_HIP = None
+
+
+def _hip():
+    """The HIP runtime already loaded by torch: graph launches and async copies
+    straight from the scheduler, without torch's stream context per batch."""
+    global _HIP
+    if _HIP is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipGraphLaunch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.hipGraphLaunch.restype = ctypes.c_int
+        h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.c_void_p]
+        h.hipMemcpyAsync.restype = ctypes.c_int
+        _HIP = h
+    return _HIP
+
+
+_D2H = 2  # hipMemcpyDeviceToHost
 
 
 class Future(object):
@@ -94,7 +116,6 @@ class _Lane(object):
         self.logl = torch.zeros(mb, dtype=torch.float64, device=dev)
         self.logl_host = torch.zeros(mb, dtype=torch.float64).pin_memory()
         self.images = torch.zeros((mb, ny, nx), dtype=b.dtype, device=dev)
-        self.images_host = torch.zeros((mb, ny, nx), dtype=b.dtype).pin_memory()
         self.ws = torch.empty(b.ws_bytes, dtype=torch.uint8, device=dev)
         self.data = backend.LensimData()
         ctypes.memmove(ctypes.byref(self.data), ctypes.byref(b.data_template), ctypes.sizeof(self.data))
@@ -173,40 +194,51 @@ class _Bucket(object):
             backend.check(rc, "lensim_image")
 
     def _batch(self, lane, kind, npad):
-        """One batch of npad rows: staging in, launches, results out (the pinned
-        host buffers are fixed, so a captured graph holds the copies too)."""
+        """One batch of npad rows: staging in, launches and, for log-likelihoods,
+        the results out (the pinned host buffers are fixed, so a captured graph
+        holds those copies too).  Images go out per batch (``run``)."""
         lane.params[:npad].copy_(lane.params_host[:npad], non_blocking=True)
         lane.slot_idx[:npad].copy_(lane.slot_host[:npad], non_blocking=True)
         self._launch(lane, kind, npad)
         if kind == "loglike":
             lane.logl_host[:npad].copy_(lane.logl[:npad], non_blocking=True)
-        else:
-            lane.images_host[:npad].copy_(lane.images[:npad], non_blocking=True)
 
     def run(self, lane, kind, n, stream):
         """Launch a batch whose first n rows of the lane's host staging are filled;
-        returns the completion event (results are then in the lane's host buffers)."""
+        returns (completion event, pinned host images [n, ny, nx] or None).  A
+        captured batch is one graph launch on the stream; simulate results then
+        go to a pinned buffer of their own, which the results view, so the lane
+        can take its next batch at once."""
         npad = _pow2_at_least(n, self.max_batch)
         if npad > n:  # padding rows: copies of the first request, results ignored
             lane.params_np[n:npad] = lane.params_np[0]
             lane.slot_np[n:npad] = lane.slot_np[0]
-        with torch.cuda.stream(stream):
-            if not self.use_graphs:
-                self._batch(lane, kind, npad)
-            else:
-                g = lane.graphs.get((kind, npad))
-                if g is None:
-                    self._batch(lane, kind, npad)  # warm-up outside the capture
+        ent = lane.graphs.get((kind, npad)) if self.use_graphs else None
+        if ent is not None:
+            rc = _hip().hipGraphLaunch(ctypes.c_void_p(ent[1]), ctypes.c_void_p(stream.cuda_stream))
+            if rc != 0:
+                raise RuntimeError("hipGraphLaunch failed (%d)" % rc)
+        else:
+            with torch.cuda.stream(stream):
+                self._batch(lane, kind, npad)  # eager, or the warm-up launch before a capture
+                if self.use_graphs:
                     stream.synchronize()
                     g = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(g, stream=stream):
                         self._batch(lane, kind, npad)
-                    lane.graphs[(kind, npad)] = g
-                else:
-                    g.replay()
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        return ev
+                    lane.graphs[(kind, npad)] = (g, g.raw_cuda_graph_exec())
+        images = None
+        if kind == "image":
+            # padded size: few distinct sizes keep torch's pinned-memory cache warm
+            images = torch.empty((npad,) + self.shape, dtype=self.dtype, pin_memory=True)[:n]
+            rc = _hip().hipMemcpyAsync(ctypes.c_void_p(images.data_ptr()), ctypes.c_void_p(lane.images.data_ptr()),
+                                       images.numel() * images.element_size(), _D2H,
+                                       ctypes.c_void_p(stream.cuda_stream))
+            if rc != 0:
+                raise RuntimeError("hipMemcpyAsync failed (%d)" % rc)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return ev, images
 
 
 class _Block(object):
@@ -339,7 +371,7 @@ class LensServer(object):
             if slots is None:
                 raise ValueError("log-likelihood requests need data slots")
             slots = np.asarray(slots, dtype=np.int64).reshape(n)
-            if np.any(slots < 0) or np.any(slots >= b.slots.shape[0]):
+            if n and (int(slots.min()) < 0 or int(slots.max()) >= b.slots.shape[0]):
                 raise ValueError("data slot out of range")
         else:
             slots = np.zeros(n, dtype=np.int64)
@@ -361,12 +393,12 @@ class LensServer(object):
             return
         self.inflight = still
         now = time.perf_counter()
-        for ev, b, lane, kind, futs, si in done:
+        for ev, images, b, lane, kind, futs, si in done:
             n = len(futs)
             if kind == "loglike":
                 vals = lane.logl_host[:n].numpy().tolist()
             else:
-                vals = list(lane.images_host[:n].clone().unbind(0))
+                vals = list(images.unbind(0))  # views of the batch's own pinned buffer
             for f, v in zip(futs, vals):
                 f.value = v
                 f.t_done = now
@@ -415,8 +447,8 @@ class LensServer(object):
             np.concatenate(ss, axis=0, out=lane.slot_np[:n])
             si = self.free_streams.pop(0)
             lane.busy = True
-            ev = b.run(lane, kind, n, self.streams[si])
-            self.inflight.append((ev, b, lane, kind, futs, si))
+            ev, images = b.run(lane, kind, n, self.streams[si])
+            self.inflight.append((ev, images, b, lane, kind, futs, si))
             self.batches += 1
             self.batch_sizes[n] += 1
             launched = True
