@@ -317,35 +317,34 @@ __device__ inline bool bulk_type_ok_dev(int t) {
          t == LENSIM_SHEAR;
 }
 
-// Record (and EPL coefficients) of (image, profile) idx < n_img * n_prof.
-__device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, int idx, double *__restrict__ recd,
-                         float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
-                         double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
-  const int img = idx / m.n_prof, k = idx - img * m.n_prof;
+// Record (and EPL coefficients) number idx < n_img * n_prof, profile-major
+// (idx = k * n_img + img), so the lanes of a wave share one profile type and
+// take one branch of prepare_record; records are stored [img][k].
+__device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, int n_img, int idx,
+                         double *__restrict__ recd, float *__restrict__ recf, double *__restrict__ coefd,
+                         float *__restrict__ coeff, double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
+  const int k = idx / n_img, img = idx - k * n_img;
+  const int ri = img * m.n_prof + k;
   int type = type_of(m, k);
   if (k >= m.n_lens && k < m.n_defl && !bulk_type_ok_dev(type)) type = LENSIM_NULL;
-  const double *p = params + size_t(idx) * LENSIM_NPARAM;
+  const double *p = params + size_t(ri) * LENSIM_NPARAM;
   double rec[kRec];
-  double *coef = nullptr;
+  double *coef = nullptr, *dcoef = nullptr;
+  float *cf = nullptr, *dcf = nullptr;
   const int e = k < m.n_lens ? m.epl_index[k] : -1;
-  double *dcoef = nullptr;
   if (e >= 0) {
-    coef = coefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
-    if (dcoefd) dcoef = dcoefd + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
-  }
-  prepare_record(type, p, rec, coef, dcoef);
-  for (int i = 0; i < kRec; ++i) {
-    recd[size_t(idx) * kRec + i] = rec[i];
-    recf[size_t(idx) * kRec + i] = float(rec[i]);
-  }
-  if (coef) {
-    const int nmax = (int)rec[EPL_NMAX];
-    float *cf = coeff + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
-    for (int n = 0; n < nmax; ++n) cf[n] = float(coef[n]);
-    if (dcoef) {
-      float *dcf = dcoeff + (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
-      for (int n = 0; n < nmax; ++n) dcf[n] = float(dcoef[n]);
+    const size_t co = (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+    coef = coefd + co;
+    cf = coeff + co;
+    if (dcoefd) {
+      dcoef = dcoefd + co;
+      dcf = dcoeff + co;
     }
+  }
+  prepare_record(type, p, rec, coef, dcoef, cf, dcf);
+  for (int i = 0; i < kRec; ++i) {
+    recd[size_t(ri) * kRec + i] = rec[i];
+    recf[size_t(ri) * kRec + i] = float(rec[i]);
   }
 }
 
@@ -353,7 +352,7 @@ __global__ void prep_kernel(ModelArgs m, const double *__restrict__ params, int 
                             float *__restrict__ recf, double *__restrict__ coefd, float *__restrict__ coeff,
                             double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < n_img * m.n_prof) prep_one(m, params, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
+  if (idx < n_img * m.n_prof) prep_one(m, params, n_img, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
 }
 
 // Flip the kernel (so the convolution runs as a correlation) and store it
@@ -823,7 +822,7 @@ __global__ __launch_bounds__(256) void prep_psf_kernel(ModelArgs m, const double
                                                        PsfJob pj, int nb_prep) {
   if (int(blockIdx.x) < nb_prep) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx < n_img * m.n_prof) prep_one(m, params, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
+    if (idx < n_img * m.n_prof) prep_one(m, params, n_img, idx, recd, recf, coefd, coeff, dcoefd, dcoeff);
     return;
   }
   const int b = blockIdx.x - nb_prep;

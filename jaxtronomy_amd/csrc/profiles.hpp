@@ -128,9 +128,10 @@ __host__ __device__ inline void ellipticity2phi_q(double e1, double e2, double &
 
 // Turns one kwargs slot p[LENSIM_NPARAM] into the derived record rec[kRec].
 // For EPL also writes the series coefficients F_n = 2F1(-n, t/2; 2-t/2; 2),
-// n < nmax, from Gauss' contiguous recurrence (Util/hyp2f1_util.py:156-166).
+// n < nmax, from Gauss' contiguous recurrence (Util/hyp2f1_util.py:156-166),
+// their float copies (coef_f) and, for gradients, dF_n/dt (dcoef, dcoef_f).
 __device__ inline void prepare_record(int type, const double *p, double *rec, double *coef,
-                                      double *dcoef = nullptr) {
+                                      double *dcoef = nullptr, float *coef_f = nullptr, float *dcoef_f = nullptr) {
   for (int i = 0; i < kRec; ++i) rec[i] = 0.0;
   switch (type) {
     case LENSIM_EPL: {
@@ -158,10 +159,23 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[EPL_PB] = 2.0 * b / (1.0 + q);
       rec[EPL_NF] = (double)nmax;
       if (coef) {
-        const double bb = t / 2.0, cc = 2.0 - bb;
+        // F_n by the recurrence, with float copies and (for gradients) dF_n/dt
+        // written as they are formed: nothing is read back from memory
+        const double bb = t / 2.0, cc = 2.0 - bb, k2 = 2.0 * bb - cc;
         double fm2 = 1.0, fm1 = 1.0 - 2.0 * bb / cc;
+        // d F_n / d t from the same recurrence: F_n = N_n / D_n with
+        // N_n = (n-1) F_{n-2} - (1.5 t - 2) F_{n-1}, D_n = 1 + n - t/2
+        double dm2 = 0.0, dm1 = -2.0 / (cc * cc);  // d/dt (1 - t / (2 - t/2))
         coef[0] = 1.0;
-        if (nmax > 1) coef[1] = fm1;
+        if (coef_f) coef_f[0] = 1.f;
+        if (dcoef) dcoef[0] = 0.0;
+        if (dcoef_f) dcoef_f[0] = 0.f;
+        if (nmax > 1) {
+          coef[1] = fm1;
+          if (coef_f) coef_f[1] = float(fm1);
+          if (dcoef) dcoef[1] = dm1;
+          if (dcoef_f) dcoef_f[1] = float(dm1);
+        }
         double cmax = fmax(1.0, fabs(fm1));
         // the divisor 1/(cc + n - 1) does not depend on the recurrence: it is a
         // reciprocal refined to full precision by two Newton steps, which the
@@ -172,28 +186,22 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
           double r = __builtin_amdgcn_rcp(d);
           r = fma(r, fma(-d, r, 1.0), r);
           r = fma(r, fma(-d, r, 1.0), r);
-          const double fn = fma(n - 1.0, fm2, -(2.0 * bb - cc) * fm1) * r;
+          const double fn = fma(n - 1.0, fm2, -k2 * fm1) * r;
           coef[n] = fn;
+          if (coef_f) coef_f[n] = float(fn);
+          if (dcoef) {
+            const double dn = ((n - 1.0) * dm2 - 1.5 * fm1 - k2 * dm1) * r + 0.5 * fn * r;
+            dcoef[n] = dn;
+            if (dcoef_f) dcoef_f[n] = float(dn);
+            dm2 = dm1;
+            dm1 = dn;
+          }
           cmax = fmax(cmax, fabs(fn));
           fm2 = fm1;
           fm1 = fn;
         }
         // |w| <= f / (2 - f) for |z| = f, so the terms n >= N add at most
         // max|F_n| wmax^N / (1 - wmax) to a sum of size ~1
-        if (dcoef) {
-          // d F_n / d t from the same recurrence: F_n = N_n / D_n with
-          // N_n = (n-1) F_{n-2} - (1.5 t - 2) F_{n-1}, D_n = 1 + n - t/2
-          double dm2 = 0.0, dm1 = -2.0 / (cc * cc);  // d/dt (1 - t / (2 - t/2))
-          dcoef[0] = 0.0;
-          if (nmax > 1) dcoef[1] = dm1;
-          for (int n = 2; n < nmax; ++n) {
-            const double Dn = cc + n - 1.0;
-            const double dn = ((n - 1.0) * dm2 - 1.5 * coef[n - 1] - (2.0 * bb - cc) * dm1) / Dn + 0.5 * coef[n] / Dn;
-            dcoef[n] = dn;
-            dm2 = dm1;
-            dm1 = dn;
-          }
-        }
         const double wmax = f / (2.0 - f);
         int nf = nmax;
         if (wmax > 0.0) {
