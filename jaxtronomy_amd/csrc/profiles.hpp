@@ -72,13 +72,18 @@ __device__ __forceinline__ T nan_to_1e10(T v) {
 // ------------------------------------------------------ record layouts ----
 // EPL     (epl.py:109-124): cos phi, sin phi, q, b, t, f, 2/(1+q), nmax
 enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX,
-              EPL_LOG2B,  // log2 b
+              EPL_NZ,     // float form: terms of the z-series of delta (0: the w-series)
               EPL_NF,     // series length of the float form (truncation below 1e-8)
               EPL_F2,     // f^2 = |z|^2
               EPL_LNB,    // ln b
               EPL_TM1,    // t - 1
               EPL_PB };   // 2 b / (1 + q)
 static_assert(EPL_PB < kRec, "EPL record exceeds kRec");
+// The float form's z-series coefficients g_1..g_NZ sit at this offset of an
+// EPL's float coefficient array, past the w-series' F_n (nmax <= 50 whenever
+// the z-series is chosen).
+constexpr int kZMax = 24, kZOff = LENSIM_EPL_NMAX - kZMax;
+static_assert(kZOff >= 50, "z-series coefficients overlap the w-series");
 // NIE/SIE (nie.py:182-200, :280-295): cos, sin, q (clamped), q^2, s, b/sqrt(1-q^2),
 //          sqrt(1-q^2), q^2 s
 enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S };
@@ -152,7 +157,7 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[EPL_F] = f;
       rec[EPL_PREF] = 2.0 / (1.0 + q);
       rec[EPL_NMAX] = (double)nmax;
-      rec[EPL_LOG2B] = log2(b);
+
       rec[EPL_F2] = f * f;
       rec[EPL_LNB] = log(b);
       rec[EPL_TM1] = t - 1.0;
@@ -209,6 +214,24 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
           for (nf = 1; nf < nmax && tail >= 1e-9; ++nf) tail *= wmax;
         }
         rec[EPL_NF] = (double)nf;
+        // float form, small f: delta = 2F1 - 1 = sum_{n>=1} g_n z^n with
+        // g_n = prod_{j<n} (b + j)/(c + j) (b = t/2, c = 2 - t/2, so 0 < g_n < 1
+        // for 0 < t < 2), |z| = f: the terms past N add at most f^(N+1)/(1-f)
+        int nz = 0;
+        if (coef_f && f > 0.0 && t > 0.0 && t < 2.0) {
+          double tail = f * f / (1.0 - f);
+          int n = 1;
+          for (; n <= kZMax && tail >= 1e-9; ++n) tail *= f;
+          if (tail < 1e-9) {  // the loop leaves tail = f^(n+1)/(1-f): n terms suffice
+            nz = n;
+            double g = 1.0;
+            for (int j = 1; j <= nz; ++j) {
+              g *= (bb + j - 1.0) / (cc + j - 1.0);
+              coef_f[kZOff + j - 1] = float(g);
+            }
+          }
+        }
+        rec[EPL_NZ] = (double)nz;
       }
       break;
     }
@@ -430,13 +453,15 @@ __device__ __forceinline__ float expm1_f32(float x) {
 // EPL, mixed form.  The geometry runs in double (rotation, Z, 1/|Z| by a
 // float rsqrt and one Newton step, the final products); float carries only
 // the parts whose size bounds their error:
-//  * delta = 2F1 - 1 = (S - 1 + z/2) / (1 - z/2), S - 1 = w sum_{n>=1} F_n w^(n-1),
-//    of size ~ f t / (4 - t), from the series truncated at EPL_NF terms
-//    (tail below 1e-9);
+//  * delta = 2F1 - 1, of size ~ f t / (4 - t): for f < 0.44 the z-series
+//    sum_{n>=1} g_n z^n (EPL_NZ terms, tail below 1e-9; z = -f (Z/|Z|)^2 from
+//    the rsqrt, no division), otherwise the reference's w-series form
+//    (S - 1 + z/2) / (1 - z/2), S - 1 = w sum_{n>=1} F_n w^(n-1), truncated at
+//    EPL_NF terms;
 //  * (b/R)^(t-1) = 1 + expm1((t - 1) ln(b/R)), |t - 1| small.
 // Then alpha = 2b/(1+q) (b/R)^(t-1) (Z/R) (1 + delta), so its relative error
 // is about eps (|delta| + |(t-1) ln(b/R)|) instead of several float ulps.
-// Long series (nmax >= 100) run in double.
+// Long series (nmax >= 100, f >= 0.73) form delta in double.
 __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *__restrict__ coef_f,
                                              const double *__restrict__ coef_d, double x, double y, double &ax,
                                              double &ay) {
@@ -447,8 +472,42 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
   const double R2d = fma(Zrd, Zrd, Zid * Zid);
   const float Zr = float(Zrd), Zi = float(Zid), R2 = float(R2d);
   float dr = 0.f, di = 0.f;  // delta (0 for f == 0, and at the centre where the reference's NaN becomes 0)
-  const int nf = (int)r[EPL_NF];
-  if (nf > 1 && R2 > 0.f) {
+  const int nf = (int)r[EPL_NF], nz = (int)r[EPL_NZ];
+  // 1/|Z|, unclamped: the direction Z/|Z| enters 2F1 however small |Z| is
+  // (float R^2 holds |Z| down to 1e-19; 0 at the centre, where delta = 0)
+  const float iRs = R2 > 0.f ? __builtin_amdgcn_rsqf(R2) : 0.f;
+  if (nz > 0) {
+    // z = -f (Z/|Z|)^2 from the rsqrt (no division); delta = z P(z) with the
+    // real coefficients g_1..g_nz
+    const float f = r[EPL_F];
+    const float ur = Zr * iRs, ui = Zi * iRs;
+    const float zr = -f * ((ur - ui) * (ur + ui));
+    const float zi = -2.f * f * (ur * ui);
+    float Pr, Pi;
+    real_poly_at_complex<float>(coef_f + kZOff, nz, zr, zi, Pr, Pi);
+    dr = fmaf(zr, Pr, -zi * Pi);
+    di = fmaf(zr, Pi, zi * Pr);
+  } else if (r.d[EPL_NMAX] >= 100.0) {
+    // f >= 0.73: |w| approaches f/(2-f) ~ 1 and the long series amplifies any
+    // error of w, so delta is formed in double throughout (uniform per image)
+    if (R2d > 0.0) {
+      const double f = r.d[EPL_F], f2 = f * f;
+      const double inv = 1.0 / R2d;
+      const double zr = -f * (Zrd * Zrd - Zid * Zid) * inv;
+      const double zi = -f * (2.0 * Zrd * Zid) * inv;
+      const double dd = 1.0 / fma(-4.0, zr, f2 + 4.0);
+      const double wr = fma(-2.0, zr, f2) * dd;
+      const double wi = -2.0 * zi * dd;
+      double Pr, Pi;
+      real_poly_at_complex<double>(coef_d + 1, (int)r.d[EPL_NMAX] - 1, wr, wi, Pr, Pi);
+      const double nr = fma(wr, Pr, fma(-wi, Pi, 0.5 * zr));
+      const double ni = fma(wr, Pi, fma(wi, Pr, 0.5 * zi));
+      const double er = fma(-0.5, zr, 1.0), ei = -0.5 * zi;
+      const double ed = 1.0 / fma(er, er, ei * ei);
+      dr = float(fma(nr, er, ni * ei) * ed);
+      di = float(fma(ni, er, -nr * ei) * ed);
+    }
+  } else if (nf > 1 && R2 > 0.f) {
     const float inv = __builtin_amdgcn_rcpf(R2);
     const float f = r[EPL_F];
     const float zr = -f * (Zr * Zr - Zi * Zi) * inv;
@@ -459,14 +518,7 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
     const float wr = fmaf(-2.f, zr, f2) * dd;
     const float wi = -2.f * zi * dd;
     float Pr, Pi;
-    if (r.d[EPL_NMAX] >= 100.0) {
-      double Prd, Pid;
-      real_poly_at_complex<double>(coef_d + 1, (int)r.d[EPL_NMAX] - 1, double(wr), double(wi), Prd, Pid);
-      Pr = float(Prd);
-      Pi = float(Pid);
-    } else {
-      real_poly_at_complex<float>(coef_f + 1, nf - 1, wr, wi, Pr, Pi);
-    }
+    real_poly_at_complex<float>(coef_f + 1, nf - 1, wr, wi, Pr, Pi);
     // numerator S - 1 + z/2 = w P + z/2, denominator 1 - z/2
     const float nr = fmaf(wr, Pr, fmaf(-wi, Pi, 0.5f * zr));
     const float ni = fmaf(wr, Pi, fmaf(wi, Pr, 0.5f * zi));
@@ -478,11 +530,11 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
   // Omega = Z (1 + delta)
   const double Or = Zrd + double(fmaf(Zr, dr, -Zi * di));
   const double Oi = Zid + double(fmaf(Zr, di, Zi * dr));
-  // 1/R with R = max(|Z|, 1e-9)
+  // 1/R with R = max(|Z|, 1e-9): one Newton step on the rsqrt, or 1e9 below the floor
   const double R2c = R2d > 1e-18 ? R2d : 1e-18;
   const float R2cf = float(R2c);
-  double iR = double(__builtin_amdgcn_rsqf(R2cf));
-  iR = iR * fma(-0.5 * R2c, iR * iR, 1.5);
+  double iR = double(iRs);
+  iR = R2d > 1e-18 ? iR * fma(-0.5 * R2c, iR * iR, 1.5) : 1e9;
   // (b/R)^(t-1)
   const float lnbr = fmaf(-0.34657359f, __builtin_amdgcn_logf(R2cf), r[EPL_LNB]);  // ln b - ln(R^2)/2
   const float em = expm1_f32(r[EPL_TM1] * lnbr);

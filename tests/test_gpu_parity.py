@@ -107,6 +107,25 @@ def test_epl_q_sweep_batched():
         npt.assert_allclose(_np(by[i]), y - ay, rtol=FP64_TOL, atol=1e-12)
 
 
+@pytest.mark.parametrize("gamma", [1.5, 1.9, 2.3])
+def test_epl_q_sweep_mixed(gamma):
+    """The mixed form over the same axis-ratio sweep: its float correction
+    delta = 2F1 - 1 switches from the z-series (f < 0.44) to the w-series and
+    to the double long series (nmax >= 100) along the sweep; the deflection
+    stays within 2e-6 of the deflection scale throughout, including points
+    on the axes and next to the centre (|Z| = 1e-12 takes the direction of Z
+    unclamped)."""
+    y = np.concatenate([np.linspace(-10, 10, 101), [1e-12, 0.0]])
+    x = np.concatenate([np.ones(101), [0.0, 1e-12]])
+    qs = np.linspace(0.03, 1, 200)[::-1]
+    c = (1 - qs) / (1 + qs)
+    kw = dict(theta_E=1.7, gamma=gamma, e1=c, e2=np.zeros_like(c), center_x=0.0, center_y=0.0)
+    bx, by = LensModel(["EPL"], precision="mixed").ray_shooting(x, y, [kw])
+    for i in range(200):
+        ax, ay = O.epl_derivatives(x, y, 1.7, gamma, c[i], 0.0)
+        assert defl_err(bx[i], by[i], x, y, ax, ay) < 2e-6, (i, c[i])
+
+
 def test_ray_shooting_profile_sum_and_k():
     x, y = _points(1)
     names = ["EPL", "SHEAR", "NFW"]
