@@ -128,8 +128,8 @@ int kyp_of(int ky) { return (ky + kWin - 1) / kWin * kWin; }
 // tile, with A = 16 image rows of the padded tile and B[k][n] = Kf[a][k - n]
 // (the band of kernel row a).  nks K-steps of 32 cover 16 + kx - 1 columns.
 constexpr int kMfWaves = 4;  // waves per block, each 16 output rows
-constexpr int kMfTiles = 4;  // 16-column tiles per wave
-constexpr int kMfTY = 16 * kMfWaves, kMfTX = 16 * kMfTiles;
+constexpr int kMfTY = 16 * kMfWaves;
+// 16-column tiles per wave: TL in [2, 4], chosen per call (mfma_plan)
 int mfma_nks(int kx) { return (16 + kx - 1 + 31) / 32; }
 constexpr int kMfMaxNks = 4;  // kernels up to 113 columns
 // Kernel rows per unrolled loop body (and per float -> double fold): the B
@@ -140,15 +140,57 @@ int mfma_steps(int kx, int ky) {
   const int nks = mfma_nks(kx), u = mfma_unroll(nks);
   return (ky + u - 1) / u * u * nks;
 }
-int mfma_tw(int kx) { return 16 * (kMfTiles - 1) + 32 * mfma_nks(kx); }
+int mfma_tw(int kx, int tl) { return 16 * (tl - 1) + 32 * mfma_nks(kx); }
 // LDS row stride in halves.  The A reads (ds_read_b128, lane l at row l % 16,
 // 16-byte column l / 16) are free of bank conflicts when the stride in dwords
 // is 8 (mod 16), as measured on gfx950 (strides of 4 or 12 (mod 16) dwords
-// conflict, 0 (mod 64) serialises); tw = 48 + 32 nks halves is 16 (mod 32),
-// so the unpadded width already has that stride.
-int mfma_ls(int kx) { return mfma_tw(kx); }
-size_t mfma_smem(int kx, int ky) { return size_t(kMfTY + ky - 1) * mfma_ls(kx) * 2 * sizeof(_Float16); }
-int mfma_tiles(int nx, int ny) { return ((nx + kMfTX - 1) / kMfTX) * ((ny + kMfTY - 1) / kMfTY); }
+// conflict, 0 (mod 64) serialises).  tw = 16 (tl - 1) + 32 nks halves: for
+// even tl it is 16 (mod 32) and unpadded; for tl = 3 it is padded by 16.
+int mfma_ls(int kx, int tl) {
+  const int tw = mfma_tw(kx, tl);
+  return tw % 32 == 16 ? tw : tw + 16;
+}
+size_t mfma_smem(int kx, int ky, int tl) {
+  return size_t(kMfTY + ky - 1) * mfma_ls(kx, tl) * 2 * sizeof(_Float16);
+}
+
+// Blocks of 64 rows x 16 tl columns.  All of a call's blocks are resident at
+// once on this workload's sizes, so the time is that of the busiest CU:
+// ceil(blocks / CUs) blocks of tl wave-tiles.  tl is the one in [2, 4] with
+// the least such load (ties to the wider tile, which re-reads fewer A
+// fragments): for 180^2 x 64 images, 768 blocks of 48 columns (3 per CU)
+// against 576 of 64 (2.25 per CU, so 3 on some).
+struct MfmaPlan {
+  int tl, n_tiles_x, n_tiles;
+};
+int cu_count() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      n = v;
+    else
+      n = 256;
+  }
+  return n;
+}
+MfmaPlan mfma_plan(int nx, int ny, int n_img, int kx, int ky) {
+  MfmaPlan best{0, 0, 0};
+  long best_load = 0;
+  const int cus = cu_count();
+  for (int tl = 4; tl >= 2; --tl) {
+    if (mfma_smem(kx, ky, tl) > kMaxLds) continue;
+    const int ntx = (nx + 16 * tl - 1) / (16 * tl), nt = ntx * ((ny + kMfTY - 1) / kMfTY);
+    const long blocks = long(nt) * n_img;
+    const long load = (blocks + cus - 1) / cus * tl;
+    if (best.tl == 0 || load < best_load) {
+      best = MfmaPlan{tl, ntx, nt};
+      best_load = load;
+    }
+  }
+  return best;
+}
 
 template <typename T>
 int n_tiles_of(int nx, int ny) {
@@ -839,7 +881,7 @@ __global__ __launch_bounds__(256) void prep_psf_kernel(ModelArgs m, const double
 // lo planes of the image scaled by 2^sa (sa from the per-image max); per
 // kernel row a and K-step, three MFMAs (hi.hi, hi.lo, lo.hi) accumulate in
 // float32, and each row's sum is added to a float64 accumulator.
-template <int NKS>
+template <int NKS, int TL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void conv_mfma_kernel(
     ConvArgs c, int ls, const float *__restrict__ in,
                                                         const int *__restrict__ hdr,
@@ -860,9 +902,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     }
   }
   const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
-  const int Y0 = tyi * kMfTY, X0 = txi * kMfTX;
+  const int Y0 = tyi * kMfTY, X0 = txi * 16 * TL;
   const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
-  const int TH = kMfTY + c.ky - 1, TW = 16 * (kMfTiles - 1) + 32 * NKS;
+  const int TH = kMfTY + c.ky - 1, TW = 16 * (TL - 1) + 32 * NKS;
   _Float16 *lh = reinterpret_cast<_Float16 *>(smem_raw);
   _Float16 *ll = lh + TH * ls;
   const int tid = threadIdx.x;
@@ -873,7 +915,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   // reads a clamped (in-image) address and is zeroed after the fact outside
   // the image, so a whole batch is issued before the first load is waited
   // for; the first batch is in flight while the image maximum is reduced.
-  constexpr int kLdRows = 16, kLdCols = (16 * (kMfTiles - 1) + 32 * NKS + 63) / 64;
+  constexpr int kLdRows = 16, kLdCols = (16 * (TL - 1) + 32 * NKS + 63) / 64;
   float v[kLdRows][kLdCols];
   auto load_batch = [&](int ly0) {
 #pragma unroll
@@ -926,9 +968,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     store_batch(ly0);
   }
   __syncthreads();
-  double dacc[kMfTiles][4];
+  double dacc[TL][4];
 #pragma unroll
-  for (int t = 0; t < kMfTiles; ++t)
+  for (int t = 0; t < TL; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j) dacc[t][j] = 0.0;
   const int rowbase = (16 * wave + li) * ls + 8 * lg;
@@ -951,35 +993,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     }
   };
   auto body = [&](const half8 (&b)[R][2], int a0) {
-    floatx4 acc[kMfTiles];
+    floatx4 acc[TL];
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < TL; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int ar = a0 + u < c.ky ? a0 + u : c.ky - 1;  // padding rows re-read the last row (times zero B)
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
         const int p = u * NKS + ks;
-        half8 ah[kMfTiles], al[kMfTiles];
+        half8 ah[TL], al[TL];
 #pragma unroll
-        for (int t = 0; t < kMfTiles; ++t) {
+        for (int t = 0; t < TL; ++t) {
           const int off = rowbase + ar * ls + 16 * t + 32 * ks;
           ah[t] = *reinterpret_cast<const half8 *>(lh + off);
           al[t] = *reinterpret_cast<const half8 *>(ll + off);
         }
 #pragma unroll
-        for (int t = 0; t < kMfTiles; ++t)
+        for (int t = 0; t < TL; ++t)
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], b[p][0], acc[t], 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < kMfTiles; ++t)
+        for (int t = 0; t < TL; ++t)
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], b[p][1], acc[t], 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < kMfTiles; ++t)
+        for (int t = 0; t < TL; ++t)
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], b[p][0], acc[t], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (int t = 0; t < kMfTiles; ++t)
+    for (int t = 0; t < TL; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
   };
@@ -997,7 +1039,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   const double unscale = ldexp(1.0, -(sa + sk));
   double chi = 0.0;
 #pragma unroll
-  for (int t = 0; t < kMfTiles; ++t) {
+  for (int t = 0; t < TL; ++t) {
     const int x = X0 + 16 * t + li;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -1284,31 +1326,46 @@ size_t conv_smem(int kx, int kyp) {
 template <typename T>
 bool use_mfma(bool identity, int kx, int ky) {
   if (sizeof(T) != sizeof(float) || identity) return false;
-  return mfma_nks(kx) <= kMfMaxNks && mfma_smem(kx, ky) <= kMaxLds;
+  return mfma_nks(kx) <= kMfMaxNks && mfma_smem(kx, ky, 2) <= kMaxLds;
 }
 
-template <int NKS>
+template <int NKS, int TL>
 int launch_conv_mfma_n(const ConvArgs &c, dim3 grid, const float *in, const int *hdr, const half8 *frag,
                        const float *blkmax, int nblk, float *out, double *partial, hipStream_t st) {
-  const size_t smem = mfma_smem(c.kx, c.ky);
+  const size_t smem = mfma_smem(c.kx, c.ky, TL);
   if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_mfma_kernel<NKS>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_mfma_kernel<NKS, TL>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_mfma_kernel)");
   }
-  conv_mfma_kernel<NKS><<<grid, 256, smem, st>>>(c, mfma_ls(c.kx), in, hdr, frag, blkmax, nblk, out, partial);
+  conv_mfma_kernel<NKS, TL><<<grid, 256, smem, st>>>(c, mfma_ls(c.kx, TL), in, hdr, frag, blkmax, nblk, out,
+                                                      partial);
   return check_launch("conv_mfma_kernel");
 }
 
-// The MFMA convolution of n_img images (c.kx, c.ky, c.edge, c.n_tiles_x set).
-int launch_conv_mfma(const ConvArgs &c, int n_tiles, int n_img, const float *in, const int *hdr, const half8 *frag,
+template <int NKS>
+int launch_conv_mfma_tl(const ConvArgs &c, int tl, dim3 grid, const float *in, const int *hdr, const half8 *frag,
+                        const float *blkmax, int nblk, float *out, double *partial, hipStream_t st) {
+  switch (tl) {
+    case 2: return launch_conv_mfma_n<NKS, 2>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 3: return launch_conv_mfma_n<NKS, 3>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 4: return launch_conv_mfma_n<NKS, 4>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    default: return fail(LENSIM_EINVAL, "matrix-core tile width %d", tl);
+  }
+}
+
+// The MFMA convolution of n_img images by plan pl (c.kx, c.ky, c.edge set;
+// c.n_tiles_x is set here).
+int launch_conv_mfma(ConvArgs c, const MfmaPlan &pl, int n_img, const float *in, const int *hdr, const half8 *frag,
                      const float *blkmax, int nblk, float *out, double *partial, hipStream_t st) {
-  const dim3 grid(n_tiles, n_img);
+  if (pl.tl == 0) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d exceeds the matrix-core tile's LDS", c.ky, c.kx);
+  c.n_tiles_x = pl.n_tiles_x;
+  const dim3 grid(pl.n_tiles, n_img);
   switch (mfma_nks(c.kx)) {
-    case 1: return launch_conv_mfma_n<1>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
-    case 2: return launch_conv_mfma_n<2>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
-    case 3: return launch_conv_mfma_n<3>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
-    case 4: return launch_conv_mfma_n<4>(c, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 1: return launch_conv_mfma_tl<1>(c, pl.tl, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 2: return launch_conv_mfma_tl<2>(c, pl.tl, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 3: return launch_conv_mfma_tl<3>(c, pl.tl, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
+    case 4: return launch_conv_mfma_tl<4>(c, pl.tl, grid, in, hdr, frag, blkmax, nblk, out, partial, st);
     default: return fail(LENSIM_EUNSUPPORTED, "psf width %d exceeds the matrix-core convolution", c.kx);
   }
 }
@@ -1384,9 +1441,9 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   }
   int n_tiles;
   if (mfma) {
-    c.n_tiles_x = (g.nx + kMfTX - 1) / kMfTX;
-    n_tiles = mfma_tiles(g.nx, g.ny);
-    rc = launch_conv_mfma(c, n_tiles, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
+    const MfmaPlan pl = mfma_plan(g.nx, g.ny, n_img, kx, ky);
+    n_tiles = pl.n_tiles;
+    rc = launch_conv_mfma(c, pl, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
                           (npix + 255) / 256, reinterpret_cast<float *>(out), partial, st);
   } else {
     c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
@@ -1591,9 +1648,8 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
     c.ky = ky;
     c.kyp = kyp;
     c.edge = edge;
-    c.n_tiles_x = (gh.nx + kMfTX - 1) / kMfTX;
-    rc = launch_conv_mfma(c, mfma_tiles(gh.nx, gh.ny), n_img, reinterpret_cast<const float *>(hr), hdr, frag,
-                          blkmax, (npix_h + 255) / 256, reinterpret_cast<float *>(chr), nullptr, st);
+    rc = launch_conv_mfma(c, mfma_plan(gh.nx, gh.ny, n_img, kx, ky), n_img, reinterpret_cast<const float *>(hr),
+                          hdr, frag, blkmax, (npix_h + 255) / 256, reinterpret_cast<float *>(chr), nullptr, st);
   } else {
     rc = launch_conv_plain<T>(gh.nx, gh.ny, kx, ky, edge, hr, kf, chr, n_img, st);
   }

@@ -408,6 +408,28 @@ def test_pixel_psf_shapes_asymmetric(shape, precision, tol):
     assert img_err(got, want) < tol
 
 
+@pytest.mark.parametrize("n,batch", [(70, 2), (180, 64), (256, 64)])
+def test_mfma_tile_widths(n, batch):
+    """The matrix-core convolution picks its tile width (32, 48 or 64 columns)
+    from the image size and batch so that blocks spread evenly over the CUs;
+    these sizes take each width.  Every image of the mixed form stays within
+    the mixed image bar of the FP64 form (itself pinned to the oracle)."""
+    cfg = dict(W.EPL180, numPix=n, supersampling=1)
+    kernel = W.moffat_kernel(41, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    draws = [W.epl180_draw(d) for d in range(batch)]
+    kl = W.stack_kwargs([d[0] for d in draws])
+    ks = W.stack_kwargs([d[1] for d in draws])
+    kll = W.stack_kwargs([d[2] for d in draws])
+    out = {}
+    for precision in ("fp64", "mixed"):
+        im = ImageModel(ImageData(**W.data_kwargs(cfg)), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        precision=precision)
+        out[precision] = _np(im.image(kl, ks, kll)).astype(np.float64)
+    for i in range(batch):
+        assert img_err(out["mixed"][i], out["fp64"][i]) < MIXED_IMG_TOL, i
+
+
 def test_mixed_gaussian_psf_edge_padding_and_signed_kernel():
     cfg = W.SMALL
     g = W.grid_dict(cfg)
