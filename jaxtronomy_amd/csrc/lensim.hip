@@ -599,6 +599,133 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   }
 }
 
+// The render for the common model family, its profile list fixed at compile
+// time: one main deflector (MAIN: EPL, or SIE / NIE) and an optional SHEAR, in
+// that order, then NSRC Sersic sources and NLL Sersic lens lights; one plane,
+// no bulk list.  Per profile it is render_kernel's arithmetic (the device
+// functions are shared); what goes is the runtime walk over the profile list,
+// its type switches and the scalar state they keep (render_kernel spills SGPRs
+// to VGPR lanes for it).
+template <typename T, int MAIN, bool SHEAR, int NSRC, int NLL>
+__global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
+                                                          const float *__restrict__ recf,
+                                                          const double *__restrict__ coefd,
+                                                          const float *__restrict__ coeff, unsigned flags,
+                                                          T *__restrict__ out, float *__restrict__ blkmax) {
+  const int img = blockIdx.y;
+  const int npix = g.nx * g.ny;
+  const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = pix0 < npix;
+  const int pix = active ? pix0 : npix - 1;  // inactive lanes stay for the wave reduction
+  const int i = pix % g.nx, j = pix / g.nx;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  constexpr int KS = SHEAR ? 2 : 1;  // first source record
+  const Rec<T> rm{rp.recd, rp.rect};
+  const Rec<T> rsh{rp.recd + kRec, rp.rect + kRec};
+  const size_t co = size_t(img) * m.n_epl * LENSIM_EPL_NMAX;  // the main EPL's coefficients (epl_index 0)
+  const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
+  const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
+  const double inv_s = 1.0 / g.s;
+  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && NSRC > 0;
+  const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && NLL > 0;
+  const double asc = m.alpha_scaling;
+  double acc = 0.0;
+  const bool eval = active && evaluated(g, i, j);
+  for (int sy = 0; eval && sy < g.s; ++sy) {
+    const double oy = o0 + sy * inv_s;
+    for (int sx = 0; sx < g.s; ++sx) {
+      const double ox = o0 + sx * inv_s;
+      // make_subgrid offsets (ImSim/Numerics/grid.py:60-62)
+      const double x = ra_c + g.m00 * ox + g.m01 * oy;
+      const double y = dec_c + g.m10 * ox + g.m11 * oy;
+      if (do_src) {
+        double ax = 0.0, ay = 0.0;
+        if constexpr (MAIN == LENSIM_EPL) {
+          if constexpr (sizeof(T) == sizeof(float))
+            epl_defl_f32(rm, rp.coeft + co, rp.coefd + co, x, y, ax, ay);
+          else
+            epl_defl_f64(rm, rp.coefd + co, x, y, ax, ay);
+        } else {
+          nie_defl<T>(rm, x, y, ax, ay);
+        }
+        if constexpr (SHEAR) shear_defl(rsh.d, x, y, ax, ay);
+        const double bx = x - ax * asc, by = y - ay * asc;
+#pragma unroll
+        for (int k = 0; k < NSRC; ++k)
+          acc += sersic_light<T>(Rec<T>{rp.recd + (KS + k) * kRec, rp.rect + (KS + k) * kRec}, bx, by);
+      }
+      if (do_light) {
+#pragma unroll
+        for (int k = 0; k < NLL; ++k)
+          acc += sersic_light<T>(Rec<T>{rp.recd + (KS + NSRC + k) * kRec, rp.rect + (KS + NSRC + k) * kRec}, x, y);
+      }
+    }
+  }
+  const T v = T(acc * (g.scale * m.flux_scaling));
+  if (active) out[size_t(img) * npix + pix] = v;
+  if (blkmax) {
+    __shared__ float wmax[256 / 64];
+    float mv = fabsf(float(v));
+    for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float b = wmax[0];
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
+      blkmax[size_t(img) * gridDim.x + blockIdx.x] = b;
+    }
+  }
+}
+
+int check_launch(const char *where);
+
+// Launches render_fast_kernel when the model is one of its compile-time lists
+// (EPL or SIE / NIE, optional SHEAR, one Sersic source, none or one Sersic lens
+// light), else render_kernel.
+template <typename T>
+int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, const float *recf, const double *coefd,
+                  const float *coeff, unsigned flags, T *out, float *blkmax, int n_img, hipStream_t st) {
+  const dim3 rgrid((g.nx * g.ny + 255) / 256, n_img);
+  auto sersic = [](int t) { return t == LENSIM_SERSIC || t == LENSIM_SERSIC_ELLIPSE; };
+  bool fast = m.mp_n == 0 && m.n_bulk == 0 && !m.has_halo && (m.n_lens == 1 || m.n_lens == 2) &&
+              m.n_source == 1 && (m.n_light == 0 || m.n_light == 1);
+  int main_t = 0;
+  bool shear = false;
+  if (fast) {
+    main_t = m.types[0];
+    shear = m.n_lens == 2;
+    fast = (main_t == LENSIM_EPL || main_t == LENSIM_SIE || main_t == LENSIM_NIE) &&
+           (!shear || m.types[1] == LENSIM_SHEAR) && (main_t != LENSIM_EPL || m.epl_index[0] == 0);
+    for (int k = m.n_lens; fast && k < m.n_lens + m.n_source + m.n_light; ++k) fast = sersic(m.types[k]);
+  }
+  if (!fast) {
+    if (m.has_halo)
+      render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax);
+    else
+      render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax);
+    return check_launch("render_kernel");
+  }
+  const bool epl = main_t == LENSIM_EPL, ll = m.n_light == 1;
+#define LENSIM_FAST(M, S, L)                                                                                   \
+  render_fast_kernel<T, M, S, 1, L><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax)
+  if (epl) {
+    if (shear) {
+      if (ll) LENSIM_FAST(LENSIM_EPL, true, 1); else LENSIM_FAST(LENSIM_EPL, true, 0);
+    } else {
+      if (ll) LENSIM_FAST(LENSIM_EPL, false, 1); else LENSIM_FAST(LENSIM_EPL, false, 0);
+    }
+  } else {
+    if (shear) {
+      if (ll) LENSIM_FAST(LENSIM_NIE, true, 1); else LENSIM_FAST(LENSIM_NIE, true, 0);
+    } else {
+      if (ll) LENSIM_FAST(LENSIM_NIE, false, 1); else LENSIM_FAST(LENSIM_NIE, false, 0);
+    }
+  }
+#undef LENSIM_FAST
+  return check_launch("render_fast_kernel");
+}
+
 template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double *__restrict__ recd,
                                                        const float *__restrict__ recf,
@@ -1408,16 +1535,11 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   float *blkmax = mfma ? reinterpret_cast<float *>(ws + L.blkmax) : nullptr;
   const int npix = g.nx * g.ny;
   timing_mark(0, st);
-  dim3 rgrid((npix + 255) / 256, n_img);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
   const double *coefd = reinterpret_cast<const double *>(ws + L.coefd);
   const float *coeff = reinterpret_cast<const float *>(ws + L.coeff);
-  if (m.has_halo)
-    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres, blkmax);
-  else
-    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, lowres, blkmax);
-  rc = check_launch("render_kernel");
+  rc = launch_render<T>(m, g, recd, recf, coefd, coeff, flags, lowres, blkmax, n_img, st);
   if (rc) return rc;
   timing_mark(1, st);
   ConvArgs c{};
@@ -1632,13 +1754,9 @@ int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, i
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
   const double *coefd = reinterpret_cast<const double *>(ws + L.coefd);
   const float *coeff = reinterpret_cast<const float *>(ws + L.coeff);
-  dim3 rgrid((npix_h + 255) / 256, n_img);
   timing_mark(0, st);
-  if (m.has_halo)
-    render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, gh, recd, recf, coefd, coeff, flags, hr, blkmax);
-  else
-    render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, gh, recd, recf, coefd, coeff, flags, hr, blkmax);
-  if ((rc = check_launch("render_kernel"))) return rc;
+  rc = launch_render<T>(m, gh, recd, recf, coefd, coeff, flags, hr, blkmax, n_img, st);
+  if (rc) return rc;
   timing_mark(1, st);
   if (mfma) {
     ConvArgs c{};

@@ -809,22 +809,29 @@ __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Re
 
 // One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
 // sersic.py:53-73) and GAUSSIAN (gaussian.py:27-42).
+// SERSIC / SERSIC_ELLIPSE at (x, y) (the circular SERSIC's record has e1 = e2 = 0).
+template <typename T>
+__device__ __forceinline__ double sersic_light(const Rec<T> &r, double x, double y) {
+  const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
+  if constexpr (sizeof(T) == sizeof(float)) {
+    // the amplitude multiplies in double (a float amp would bias the profile by its rounding)
+    return r.d[SE_AMP] * double(sersic_value_f32(dxd, dyd, r));
+  } else {
+    const T dx = T(dxd), dy = T(dyd);
+    const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
+    const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
+    return r.d[SE_AMP] * sersic_kernel(xe * xe + ye * ye, r);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ double light_value(int type, const Rec<T> &r, double x, double y) {
   const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
   const T dx = T(dxd), dy = T(dyd);
   switch (type) {
     case LENSIM_SERSIC:
-    case LENSIM_SERSIC_ELLIPSE: {
-      if constexpr (sizeof(T) == sizeof(float)) {
-        // the amplitude multiplies in double (a float amp would bias the profile by its rounding)
-        return r.d[SE_AMP] * double(sersic_value_f32(dxd, dyd, r));
-      } else {
-        const T xe = r[SE_M00] * dx + r[SE_M01] * dy;
-        const T ye = r[SE_M01] * dx + r[SE_M11] * dy;
-        return r.d[SE_AMP] * sersic_kernel(xe * xe + ye * ye, r);
-      }
-    }
+    case LENSIM_SERSIC_ELLIPSE:
+      return sersic_light<T>(r, x, y);
     case LENSIM_GAUSSIAN: {
       const T r2 = dx * dx * r[GA_IS2] + dy * dy * r[GA_IS2];
       return double(r[GA_C] * Fn<T>::exp_(-r2 / T(2)));
