@@ -556,9 +556,10 @@ def main():
         "mixed_over_fp64": round(value / value64, 2),
         "accuracy": {"mixed_image_max_rel_vs_fp64": img_rel, "mixed_logl_max_rel_vs_fp64": ll_rel},
         "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4)},
-        "roofline": {"bound": "valu", "kernel": "render_kernel<float>", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "kernel": "render_fast_kernel<float, EPL, SHEAR, 1, 1>",
+                     "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic("render_kernel_float_epl180_b64_pmc.txt"),
+                     "traffic": measured_traffic("render_fast_float_epl180_b64_pmc.txt"),
                      "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
                      "algorithmic_bytes": 4 * n_ray // cfg["supersampling"] ** 2 * args.batch,
                      "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3)},
