@@ -1,18 +1,22 @@
 // liblensim: HIP kernels for gfx950 and the C ABI declared in include/lensim.h.
 //
 // Pipeline of one batched image / likelihood call (all on one stream):
-//   prep_kernel     kwargs slots -> per-(image, profile) records (+ EPL series
-//                   coefficients), float64 and float32 copies
-//   psf_prep_kernel flip the PSF, pad its width to a multiple of the register
-//                   window, convert to the compute type
-//   render_kernel   one thread per pixel: s x s sub-rays generated from the grid
-//                   description, ray shooting through every deflector, source
-//                   light at the deflected position and lens light at the ray,
-//                   mean over the sub-pixel block, x pixel_width^2 x flux_scaling
-//   conv_kernel     LDS-tiled direct 'same' convolution (zero or edge padding)
-//                   with a register sliding window; optionally fused with the
-//                   per-pixel chi^2 and a block reduction
-//   logl_kernel     fixed-order sum of the per-tile chi^2 partials
+//   prep_psf_kernel  kwargs slots -> per-(image, profile) records (+ EPL series
+//                    coefficients), float64 and float32 copies; in the same
+//                    launch the PSF preparation: the flipped kernel for the
+//                    direct convolution, or the fp16 hi/lo B fragments of the
+//                    matrix-core convolution
+//   render_kernel /  one thread per pixel: s x s sub-rays generated from the
+//   render_fast_...  grid description, ray shooting through every deflector,
+//                    source light at the deflected position and lens light at
+//                    the ray, mean over the sub-pixel block, x pixel_width^2 x
+//                    flux_scaling (the fast kernel for compile-time profile
+//                    lists of the common model family)
+//   conv_kernel /    'same' convolution (zero or edge padding): LDS-tiled
+//   conv_mfma_kernel direct stencil in the compute type, or fp16 hi/lo split
+//                    products on the matrix cores (mixed form); optionally
+//                    fused with the per-pixel chi^2 and a block reduction
+//   logl_kernel      fixed-order sum of the per-tile chi^2 partials
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
@@ -1003,11 +1007,12 @@ __global__ __launch_bounds__(256) void prep_psf_kernel(ModelArgs m, const double
   }
 }
 
-// Direct 'same' convolution on the matrix cores: block = 64 rows x 32 columns
-// (4 waves x 2 tiles of 16 x 16).  The padded tile sits in LDS as fp16 hi and
-// lo planes of the image scaled by 2^sa (sa from the per-image max); per
-// kernel row a and K-step, three MFMAs (hi.hi, hi.lo, lo.hi) accumulate in
-// float32, and each row's sum is added to a float64 accumulator.
+// Direct 'same' convolution on the matrix cores: block = 64 rows x 16 TL
+// columns (4 waves x TL tiles of 16 x 16; NKS K-steps of 32 per kernel row).
+// The padded tile sits in LDS as fp16 hi and lo planes of the image scaled by
+// 2^sa (sa from the per-image max); per kernel row a and K-step, three MFMAs
+// (hi.hi, hi.lo, lo.hi) accumulate in float32, and the float sums are folded
+// into float64 accumulators once per unrolled body of rows.
 template <int NKS, int TL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void conv_mfma_kernel(
     ConvArgs c, int ls, const float *__restrict__ in,
