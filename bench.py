@@ -54,6 +54,8 @@ def build_models(batch):
     data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
     im64 = model("fp64", data)
     im32 = model("mixed", data)
+    # second instances (own workspaces) for the second batch in flight
+    im64.lane_twin, im32.lane_twin = model("fp64", data), model("mixed", data)
     kl = W.stack_kwargs([d[0] for d in draws])
     ks = W.stack_kwargs([d[1] for d in draws])
     kll = W.stack_kwargs([d[2] for d in draws])
@@ -86,6 +88,45 @@ def time_steps(im, params, steps, warmup, backend, model=None):
     total_ms = evs[0].elapsed_time(evs[-1])
     return dict(total_ms=total_ms, wall_s=wall, per_step=per_step, render_ms=render_ms / max(calls, 1),
                 conv_ms=conv_ms / max(calls, 1), logl=logl.cpu().numpy())
+
+
+def time_in_flight(im, params, steps, warmup, backend):
+    """Throughput with two batches in flight: step i runs on stream i % 2 with
+    its own workspace (``im`` and ``im.lane_twin``), as the request server
+    keeps several batches in flight on its streams.  One batch's render
+    (vector ALU) then overlaps the other's MFMA convolution.  Returns the total
+    time of ``steps`` steps, the mean render and convolution durations seen
+    on their streams while the two batches overlap, and the log-likelihoods
+    of the last step of each lane."""
+    import torch
+
+    lanes = [im, im.lane_twin]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    logls = [torch.empty(params.shape[0], dtype=torch.float64, device=params.device) for _ in lanes]
+    main = torch.cuda.current_stream()
+
+    def issue(n):
+        for s in streams:
+            s.wait_stream(main)
+        for i in range(n):
+            with torch.cuda.stream(streams[i % 2]):
+                lanes[i % 2].likelihood_batch(params, logl=logls[i % 2])
+        for s in streams:
+            main.wait_stream(s)
+
+    issue(warmup)
+    torch.cuda.synchronize()
+    backend.timing_read()  # drop earlier records
+    backend.timing_enable(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    issue(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    backend.timing_enable(False)
+    render_ms, conv_ms, calls = backend.timing_read()
+    return dict(total_ms=e0.elapsed_time(e1), render_ms=render_ms / max(calls, 1), conv_ms=conv_ms / max(calls, 1),
+                logl=[x.cpu().numpy() for x in logls])
 
 
 def measured_traffic(name):
@@ -471,7 +512,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix"), default="epl180")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
@@ -510,10 +551,16 @@ def main():
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
 
-    log("bench: timing mixed precision (%d steps)" % args.steps)
+    log("bench: timing mixed precision, one batch at a time (%d steps)" % args.steps)
     mixed = time_steps(im32, params, args.steps, args.warmup, backend)
-    log("bench: timing fp64 (%d steps)" % args.fp64_steps)
+    log("bench: timing fp64, one batch at a time (%d steps)" % args.fp64_steps)
     fp64 = time_steps(im64, params, args.fp64_steps, max(2, args.warmup // 2), backend)
+    log("bench: timing mixed precision, two batches in flight (%d steps)" % args.steps)
+    mixed_fl = time_in_flight(im32, params, args.steps, args.warmup, backend)
+    log("bench: timing fp64, two batches in flight (%d steps)" % args.fp64_steps)
+    fp64_fl = time_in_flight(im64, params, args.fp64_steps, max(2, args.warmup // 2), backend)
+    fl_diff = max(float(np.max(np.abs(x - mixed["logl"]))) for x in mixed_fl["logl"])
+    fl_diff64 = max(float(np.max(np.abs(x - fp64["logl"]))) for x in fp64_fl["logl"])
 
     # accuracy of the mixed form against the FP64 form of the same batch (the
     # FP64 form itself is held to 1e-10 of the oracle by tests/test_gpu_parity.py)
@@ -522,8 +569,10 @@ def main():
     img32 = im32.image_batch(params[:4]).double().cpu().numpy()
     img_rel = float(max(np.max(np.abs(img32[i] - img64[i])) / np.max(np.abs(img64[i])) for i in range(4)))
 
-    value = args.batch * args.steps / (mixed["total_ms"] / 1e3)
-    value64 = args.batch * args.fp64_steps / (fp64["total_ms"] / 1e3)
+    value = args.batch * args.steps / (mixed_fl["total_ms"] / 1e3)
+    value64 = args.batch * args.fp64_steps / (fp64_fl["total_ms"] / 1e3)
+    alone = args.batch * args.steps / (mixed["total_ms"] / 1e3)
+    alone64 = args.batch * args.fp64_steps / (fp64["total_ms"] / 1e3)
     F, S = render_work(params_np, n_ray)
     achieved = (F + 8 * S) / (mixed["render_ms"] / 1e3) / 1e12
     achieved64 = (F + 20 * S) / (fp64["render_ms"] / 1e3) / 1e12
@@ -535,7 +584,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["total_ms"] / args.steps, 4),
+        "ms_per_step": round(mixed_fl["total_ms"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -546,23 +595,33 @@ def main():
                         "supersampling 3, 41x41 Moffat PSF, image + chi2 logL",
             "batch": args.batch,
             "precision": "mixed: f32 profiles/convolution, f64 coordinates and chi2 sums",
+            "in_flight": "two batches, on two streams with a workspace each (value, ms_per_step); "
+                         "p50/p99 and one_batch_at_a_time: one batch alone",
         },
         "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
         "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4),
-        "fp64": {"value": round(value64, 1), "ms_per_step": round(fp64["total_ms"] / args.fp64_steps, 4),
+        "one_batch_at_a_time": {"value": round(alone, 1), "ms_per_step": round(mixed["total_ms"] / args.steps, 4)},
+        "fp64": {"value": round(value64, 1), "ms_per_step": round(fp64_fl["total_ms"] / args.fp64_steps, 4),
+                 "one_batch_at_a_time": {"value": round(alone64, 1),
+                                         "ms_per_step": round(fp64["total_ms"] / args.fp64_steps, 4)},
                  "p99_ms": round(float(np.percentile(fp64["per_step"], 99)), 4),
                  "render_ms": round(fp64["render_ms"], 4), "conv_ms": round(fp64["conv_ms"], 4),
                  "render_tflops_equiv": round(achieved64, 3)},
         "mixed_over_fp64": round(value / value64, 2),
-        "accuracy": {"mixed_image_max_rel_vs_fp64": img_rel, "mixed_logl_max_rel_vs_fp64": ll_rel},
-        "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4)},
+        "accuracy": {"mixed_image_max_rel_vs_fp64": img_rel, "mixed_logl_max_rel_vs_fp64": ll_rel,
+                     "in_flight_logl_max_abs_diff_vs_alone": fl_diff, "in_flight_fp64_logl_max_abs_diff": fl_diff64},
+        "kernels_ms": {"render": round(mixed["render_ms"], 4), "conv": round(mixed["conv_ms"], 4),
+                       "in_flight_render": round(mixed_fl["render_ms"], 4),
+                       "in_flight_conv": round(mixed_fl["conv_ms"], 4)},
         "roofline": {"bound": "valu", "kernel": "render_fast_kernel<float, EPL, SHEAR, 1, 1>",
                      "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": measured_traffic("render_fast_float_epl180_b64_pmc.txt"),
                      "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
                      "algorithmic_bytes": 4 * n_ray // cfg["supersampling"] ** 2 * args.batch,
-                     "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3)},
+                     "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3),
+                     "duration": "mean render launch of the one-batch-at-a-time phase, HIP events on its stream",
+                     "in_flight_achieved": round((F + 8 * S) / (mixed_fl["render_ms"] / 1e3) / 1e12, 3)},
     }
     if args.serve_seconds > 0:
         log("bench: served load")
