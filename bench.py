@@ -30,7 +30,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_models(batch):
+def build_models(batch, in_flight=3):
     import torch
 
     from jaxtronomy_amd import workloads as W
@@ -54,8 +54,9 @@ def build_models(batch):
     data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
     im64 = model("fp64", data)
     im32 = model("mixed", data)
-    # second instances (own workspaces) for the second batch in flight
-    im64.lane_twin, im32.lane_twin = model("fp64", data), model("mixed", data)
+    # further instances (own workspaces) for the other batches in flight
+    im64.twins = [model("fp64", data) for _ in range(in_flight - 1)]
+    im32.twins = [model("mixed", data) for _ in range(in_flight - 1)]
     kl = W.stack_kwargs([d[0] for d in draws])
     ks = W.stack_kwargs([d[1] for d in draws])
     kll = W.stack_kwargs([d[2] for d in draws])
@@ -91,17 +92,18 @@ def time_steps(im, params, steps, warmup, backend, model=None):
 
 
 def time_in_flight(im, params, steps, warmup, backend):
-    """Throughput with two batches in flight: step i runs on stream i % 2 with
-    its own workspace (``im`` and ``im.lane_twin``), as the request server
-    keeps several batches in flight on its streams.  One batch's render
-    (vector ALU) then overlaps the other's MFMA convolution.  Returns the total
-    time of ``steps`` steps, the mean render and convolution durations seen
-    on their streams while the two batches overlap, and the log-likelihoods
-    of the last step of each lane."""
+    """Throughput with k = 1 + len(im.twins) batches in flight: step i runs on
+    stream i % k with its own workspace (``im`` and its twins, instances of
+    the same model), as the request server keeps several batches in flight on
+    its streams.  One batch's render (vector ALU) then overlaps another's MFMA
+    convolution.  Returns the total time of ``steps`` steps, the mean render
+    and convolution durations seen on their streams while batches overlap, and
+    the log-likelihoods of the last step of each lane."""
     import torch
 
-    lanes = [im, im.lane_twin]
-    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    lanes = [im] + list(im.twins)
+    k = len(lanes)
+    streams = [torch.cuda.Stream() for _ in lanes]
     logls = [torch.empty(params.shape[0], dtype=torch.float64, device=params.device) for _ in lanes]
     main = torch.cuda.current_stream()
 
@@ -109,8 +111,8 @@ def time_in_flight(im, params, steps, warmup, backend):
         for s in streams:
             s.wait_stream(main)
         for i in range(n):
-            with torch.cuda.stream(streams[i % 2]):
-                lanes[i % 2].likelihood_batch(params, logl=logls[i % 2])
+            with torch.cuda.stream(streams[i % k]):
+                lanes[i % k].likelihood_batch(params, logl=logls[i % k])
         for s in streams:
             main.wait_stream(s)
 
@@ -508,13 +510,15 @@ def main_serve_mix(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=100, help="untimed steps before each timed phase (the GPU's "
+                    "clocks settle over the first ~15 ms of sustained load)")
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=60)
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix"), default="epl180")
+    ap.add_argument("--in-flight", type=int, default=3, help="batches in flight for the headline value (epl180)")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
     ap.add_argument("--n-sub", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -548,16 +552,16 @@ def main():
     if args.workload == "serve-mix":
         return main_serve_mix(args)
     log("bench: building models")
-    cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch)
+    cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch, args.in_flight)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
 
     log("bench: timing mixed precision, one batch at a time (%d steps)" % args.steps)
     mixed = time_steps(im32, params, args.steps, args.warmup, backend)
+    log("bench: timing mixed precision, %d batches in flight (%d steps)" % (args.in_flight, args.steps))
+    mixed_fl = time_in_flight(im32, params, args.steps, args.warmup, backend)
     log("bench: timing fp64, one batch at a time (%d steps)" % args.fp64_steps)
     fp64 = time_steps(im64, params, args.fp64_steps, max(2, args.warmup // 2), backend)
-    log("bench: timing mixed precision, two batches in flight (%d steps)" % args.steps)
-    mixed_fl = time_in_flight(im32, params, args.steps, args.warmup, backend)
-    log("bench: timing fp64, two batches in flight (%d steps)" % args.fp64_steps)
+    log("bench: timing fp64, %d batches in flight (%d steps)" % (args.in_flight, args.fp64_steps))
     fp64_fl = time_in_flight(im64, params, args.fp64_steps, max(2, args.warmup // 2), backend)
     fl_diff = max(float(np.max(np.abs(x - mixed["logl"]))) for x in mixed_fl["logl"])
     fl_diff64 = max(float(np.max(np.abs(x - fp64["logl"]))) for x in fp64_fl["logl"])
@@ -595,8 +599,8 @@ def main():
                         "supersampling 3, 41x41 Moffat PSF, image + chi2 logL",
             "batch": args.batch,
             "precision": "mixed: f32 profiles/convolution, f64 coordinates and chi2 sums",
-            "in_flight": "two batches, on two streams with a workspace each (value, ms_per_step); "
-                         "p50/p99 and one_batch_at_a_time: one batch alone",
+            "in_flight": "%d batches, on %d streams with a workspace each (value, ms_per_step); "
+                         "p50/p99 and one_batch_at_a_time: one batch alone" % (args.in_flight, args.in_flight),
         },
         "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
         "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4),
