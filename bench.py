@@ -534,7 +534,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=100, help="untimed steps before each timed phase (the GPU's "
                     "clocks settle over the first ~15 ms of sustained load)")
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=None,
+                    help="batch per step (default: 64 epl180, 16 substructure, 256 grad, as each workload is specified)")
     ap.add_argument("--fp64-steps", type=int, default=60)
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
@@ -546,6 +547,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference for HBM counters)")
     args = ap.parse_args()
+    if args.batch is None:
+        args.batch = {"substructure": 16, "grad": 256}.get(args.workload, 64)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a design "
               "note in DESIGN.md)" % args.gpus, file=sys.stderr)

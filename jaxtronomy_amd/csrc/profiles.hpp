@@ -561,6 +561,34 @@ __device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, do
   ay += double(fx * s + fy * c);
 }
 
+// atan(t) for 0 <= t <= 0.9 as t + t^3 P(t^2), P of degree 7 fitted to atan by
+// iteratively reweighted least squares on the relative error: 1.3 ulp in float
+// over the interval, with no range reduction, division or branch.
+__device__ __forceinline__ float atan_unit_f32(float t) {
+  const float s = t * t;
+  float p = 0.00463163447f;
+  p = fmaf(p, s, -0.0220576132f);
+  p = fmaf(p, s, 0.0508678642f);
+  p = fmaf(p, s, -0.0808524268f);
+  p = fmaf(p, s, 0.108694514f);
+  p = fmaf(p, s, -0.142524425f);
+  p = fmaf(p, s, 0.199977465f);
+  p = fmaf(p, s, -0.333332811f);
+  return fmaf(t, p * s, t);
+}
+
+// asinh(z) for 0 <= z <= 0.5 as z + z^3 Q(z^2), Q of degree 4 fitted the same
+// way: 1 ulp in float.
+__device__ __forceinline__ float asinh_small_f32(float z) {
+  const float s = z * z;
+  float p = -0.0132715709f;
+  p = fmaf(p, s, 0.0281910547f);
+  p = fmaf(p, s, -0.0443894712f);
+  p = fmaf(p, s, 0.0749869748f);
+  p = fmaf(p, s, -0.166666449f);
+  return fmaf(z, p * s, z);
+}
+
 // h(X) = acosh(1/X)/sqrt(1-X^2) (X<1), acos(1/X)/sqrt(X^2-1) (X>1), 1 at X=1.
 // It equals sum_k v^k/(2k+1) with v = 1 - X^2; near X = 1 the float form uses
 // that series because both closed branches cancel there.
@@ -583,6 +611,12 @@ __device__ __forceinline__ float nfw_h_f32(float X) {
     return __logf((1.0f + sv) / X) / sv;  // acosh(1/X) = ln((1 + sqrt(1-X^2))/X)
   }
   const float sv = __builtin_amdgcn_sqrtf(-v);
+  if (X >= 1.5f) {
+    // atan(sv) = pi/2 - atan(1/sv) with 1/sv <= 0.895 (no cancellation: the
+    // result is at least 0.84)
+    const float isv = __builtin_amdgcn_rcpf(sv);
+    return (1.57079633f - atan_unit_f32(isv)) * isv;
+  }
   return ::atanf(sv) / sv;  // acos(1/X) = atan(sqrt(X^2-1))
 }
 
@@ -685,19 +719,23 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
     // w = sqrt(X^2 + tau^2) and F = (pi/2 - atan(1/v)) / v, their sum is
     //   pi (tau^2 + 1)^2 / (2 v (v + w)^2) - (2 X^2 + tau^2 - 1) atan(1/v) / v
     // (from (w - v)^2 = (tau^2 + 1)^2 / (v + w)^2), and
-    // L = ln(X / (tau + w)) = -asinh(tau / X), evaluated through log1p.
-    const float Xf = float(X), tau2 = float(r[TN_TAU2]);
+    // L = ln(X / (tau + w)) = -asinh(z), z = tau / X: a series for z <= 0.5
+    // (where the logarithm's argument nears 1), else ln((tau + w) / X) with its
+    // argument at least 1.6.  1/v <= 0.895 here, inside atan_unit_f32's range.
+    const float Xf = float(X), tau2 = float(r[TN_TAU2]), tau = float(r[TN_TAU]);
     const float X2 = Xf * Xf;
     const float v = Fn<float>::sqrt_((Xf - 1.0f) * (Xf + 1.0f));
     const float w = Fn<float>::sqrt_(tau2 + X2);
     const float iv = Fn<float>::rcp(v);
     const float vw = v + w;
     const float A = float(r[TN_K]) * iv * Fn<float>::rcp(vw * vw) -
-                    (2.0f * X2 + float(r[TN_T2M1])) * ::atanf(iv) * iv;
-    const float z = float(r[TN_TAU]) * Fn<float>::rcp(Xf);
-    const float ash = ::log1pf(z + z * z * Fn<float>::rcp(1.0f + Fn<float>::sqrt_(1.0f + z * z)));
+                    (2.0f * X2 + float(r[TN_T2M1])) * atan_unit_f32(iv) * iv;
+    const float iX = Fn<float>::rcp(Xf);
+    const float z = tau * iX;
+    const float ash = z <= 0.5f ? asinh_small_f32(z)
+                                : 0.693147181f * __builtin_amdgcn_logf((tau + w) * iX);
     const float g = float(r[TN_PRE]) * (A + float(r[TN_C0]) - w * ash * float(r[TN_C1]));
-    const float a = float(r[TN_A]) * g * Fn<float>::rcp(X2);
+    const float a = float(r[TN_A]) * g * (iX * iX);
     ax += double(a * float(dx));
     ay += double(a * float(dy));
     return;
