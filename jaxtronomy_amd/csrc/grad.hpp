@@ -70,6 +70,52 @@ __device__ __forceinline__ void series_g(const T *__restrict__ c, const T *__res
   pi = b1 * wi;
 }
 
+// Clenshaw form of sum_{k<n} a(k) w^k for real coefficients a(k) at complex w
+// (sw = 2 Re w, tw = |w|^2), as series_g's value.
+template <typename T, typename A>
+__device__ __forceinline__ void clenshaw_c(A a, int n, T wr, T wi, T sw, T tw, T &pr, T &pi) {
+  if (n <= 0) {
+    pr = pi = T(0);
+    return;
+  }
+  if (n == 1) {
+    pr = a(0);
+    pi = T(0);
+    return;
+  }
+  T b2 = T(0), b1 = a(n - 1);
+  for (int k = n - 2; k >= 1; --k) {
+    const T b0 = sw * b1 - tw * b2 + a(k);
+    b2 = b1;
+    b1 = b0;
+  }
+  pr = b1 * wr - tw * b2 + a(0);
+  pi = b1 * wi;
+}
+
+// The series in duals without dual arithmetic inside the loop: the sum is
+// holomorphic in w, so its tangents are S'(w) dw + U(w) dt with
+// S'(w) = sum (k+1) c_{k+1} w^k and U(w) = sum dc_k w^k, three scalar
+// recurrences instead of one recurrence on N + 1 components.
+template <typename T, int N>
+__device__ __forceinline__ void series_g(const T *__restrict__ c, const T *__restrict__ dc, int n,
+                                         const Dual<T, N> &t, const Dual<T, N> &wr, const Dual<T, N> &wi,
+                                         Dual<T, N> &pr, Dual<T, N> &pi) {
+  const T wrv = wr.v, wiv = wi.v;
+  const T sw = T(2) * wrv, tw = wrv * wrv + wiv * wiv;
+  T sr, si, dr, di, ur, ui;
+  clenshaw_c<T>([&](int k) { return c[k]; }, n, wrv, wiv, sw, tw, sr, si);
+  clenshaw_c<T>([&](int k) { return T(k + 1) * c[k + 1]; }, n - 1, wrv, wiv, sw, tw, dr, di);
+  clenshaw_c<T>([&](int k) { return dc[k]; }, n, wrv, wiv, sw, tw, ur, ui);
+  pr.v = sr;
+  pi.v = si;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    pr.d[i] = dr * wr.d[i] - di * wi.d[i] + ur * t.d[i];
+    pi.d[i] = dr * wi.d[i] + di * wr.d[i] + ui * t.d[i];
+  }
+}
+
 // EPL, LensModel/Profiles/epl.py:109-124, :178-200, :364-394.  p: theta_E,
 // gamma, e1, e2, center_x, center_y.  nser: series length (from the value of f).
 template <typename T, typename S>
@@ -231,6 +277,60 @@ __device__ __forceinline__ S sersic_g(const S &R, const S &R_sersic, const S &n)
   const S frac = R_ / Rs_;
   if (!(val(frac) <= T(1000))) return lift(T(0), R);
   return exp(-bn * (exp(log(frac) / n) - T(1)));
+}
+
+// SERSIC_ELLIPSE at (x, y) with its derivatives in closed form: returns the
+// flux, writes d flux / d(amp, R_sersic, n_sersic, e1, e2, center_x,
+// center_y) to gp and d flux / d(x, y) to gx, gy.  Same expression, floors
+// and window as light_g (sersic_ellipse.py:62-93, param_util.py:103-124,
+// sersic_utils.py:239-267); a floor that binds passes no derivative, as
+// fmax does for duals.  With u = frac^(1/n) and E = exp(-b_n (u - 1)):
+//   d flux / d ln(frac) = -amp E b_n u / n,
+//   d flux / dn = -(d flux / d ln frac) ln(frac) / n - amp E (u - 1) db_n/dn.
+template <typename T>
+__device__ __forceinline__ T sersic_ellipse_grad(const double *p, T x, T y, T *gp, T &gx, T &gy) {
+  const T amp = T(p[0]), p1 = T(p[1]), n = T(p[2]), e1 = T(p[3]), e2 = T(p[4]);
+  const T dx = x - T(p[5]), dy = y - T(p[6]);
+  const T Rs = p1 >= T(0) ? p1 : T(0);
+  const T v = T(1) - e1 * e1 - e2 * e2;
+  const T sq = sqrt(v >= T(0) ? v : -v);
+  const bool nfree = sq >= T(0.000001);
+  const T norm = nfree ? sq : T(0.000001);
+  const T inorm = T(1) / norm;
+  const T xe = ((T(1) - e1) * dx - e2 * dy) * inorm;
+  const T ye = ((T(1) + e1) * dy - e2 * dx) * inorm;
+  const T R = sqrt(xe * xe + ye * ye);
+  const bool rfree = R >= T(0.0001), sfree = Rs >= T(0.0001);
+  const T R_ = rfree ? R : T(0.0001);
+  const T Rs_ = sfree ? Rs : T(0.0001);
+  const T bn0 = T(1.9992) * n - T(0.3271);
+  const bool bfree = bn0 >= T(0.00001);
+  const T bn = bfree ? bn0 : T(0.00001);
+  const T frac = R_ / Rs_;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) gp[i] = T(0);
+  gx = gy = T(0);
+  if (!(frac <= T(1000))) return T(0);
+  const T lf = log(frac);
+  const T u = exp(lf / n);
+  const T E = exp(-bn * (u - T(1)));
+  const T c = -amp * E * bn * u / n;  // d flux / d ln(frac)
+  gp[0] = E;
+  gp[1] = (sfree && p1 >= T(0)) ? -c / Rs_ : T(0);
+  gp[2] = -c * lf / n - (bfree ? amp * E * (u - T(1)) * T(1.9992) : T(0));
+  // d flux / d(xe, ye) through R = |(xe, ye)|
+  const T gR = rfree ? c / (R_ * R) : T(0);
+  const T gxe = gR * xe, gye = gR * ye;
+  // d norm / d e_i = -sign(v) e_i / sq where the floor does not bind
+  const T sg = v >= T(0) ? T(1) : T(-1);
+  const T dn1 = nfree ? -sg * e1 / sq : T(0), dn2 = nfree ? -sg * e2 / sq : T(0);
+  gp[3] = (gxe * (-dx - xe * dn1) + gye * (dy - ye * dn1)) * inorm;
+  gp[4] = (gxe * (-dy - xe * dn2) + gye * (-dx - ye * dn2)) * inorm;
+  gx = (gxe * (T(1) - e1) - gye * e2) * inorm;
+  gy = (gye * (T(1) + e1) - gxe * e2) * inorm;
+  gp[5] = -gx;
+  gp[6] = -gy;
+  return amp * E;
 }
 
 // One light profile from its raw slot at (x, y).

@@ -1228,6 +1228,25 @@ struct GradArgs {
   double *gpart;          // [n_img][n_prof][nblk][8]
 };
 
+// A light profile's flux and its derivatives in its own kwargs (the N seeds
+// of ps) at (x, y): closed form for SERSIC_ELLIPSE, duals otherwise.
+template <typename T, int N>
+__device__ __forceinline__ gm::Dual<T, N> light_grad(int type, const gm::Dual<T, N> *ps, const double *pk, T x,
+                                                     T y) {
+  using S = gm::Dual<T, N>;
+  if constexpr (N == 7) {
+    if (type == LENSIM_SERSIC_ELLIPSE) {
+      T gp[7], gx, gy;
+      S r;
+      r.v = gm::sersic_ellipse_grad<T>(pk, x, y, gp, gx, gy);
+#pragma unroll
+      for (int i = 0; i < 7; ++i) r.d[i] = gp[i];
+      return r;
+    }
+  }
+  return gm::light_g<T, S>(type, ps, S(x), S(y));
+}
+
 // Contribution of one ray to d flux / d(kwargs of profile k), with the
 // profile's own kwargs as N dual seeds; w = dlogL / d(ray flux).
 template <typename T, int N, bool HALO>
@@ -1262,23 +1281,37 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
     const S xs = S(T(x)), ys = S(T(y));
     gm::defl_g<T, S>(type, ps, c, dc, nser, xs, ys, ax, ay);
     const S bx = xs - ax * T(m.alpha_scaling), by = ys - ay * T(m.alpha_scaling);
-    flux = S(T(0));
+    // the sources' gradient in beta (two seeds), chained into this deflector's
+    // kwargs: d flux = dS/dbeta_x d beta_x + dS/dbeta_y d beta_y
+    using S2 = gm::Dual<T, 2>;
+    const S2 bx2 = S2::seed(bx.v, 0), by2 = S2::seed(by.v, 1);
+    S2 f2 = S2(T(0));
     for (int j = m.n_defl; j < m.n_defl + m.n_source; ++j) {
       const double *pj = ga.params + (size_t(img) * m.n_prof + j) * LENSIM_NPARAM;
-      S pl[LENSIM_NPARAM];
+      const int tj = m.types[j - m.n_bulk];
+      if (tj == LENSIM_SERSIC_ELLIPSE) {
+        T gp[7], gx, gy;
+        f2.v += gm::sersic_ellipse_grad<T>(pj, bx.v, by.v, gp, gx, gy);
+        f2.d[0] += gx;
+        f2.d[1] += gy;
+        continue;
+      }
+      S2 pl[LENSIM_NPARAM];
 #pragma unroll
-      for (int i = 0; i < LENSIM_NPARAM; ++i) pl[i] = S(T(pj[i]));
-      flux = flux + gm::light_g<T, S>(m.types[j - m.n_bulk], pl, bx, by);
+      for (int i = 0; i < LENSIM_NPARAM; ++i) pl[i] = S2(T(pj[i]));
+      f2 = f2 + gm::light_g<T, S2>(tj, pl, bx2, by2);
     }
+    flux.v = f2.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) flux.d[i] = f2.d[0] * bx.d[i] + f2.d[1] * by.d[i];
   } else if (k < m.n_defl + m.n_source) {
     // d/d(source kwargs) at the deflected position
     double ax = 0.0, ay = 0.0;
     deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
-    const S bx = S(T(x - ax * m.alpha_scaling)), by = S(T(y - ay * m.alpha_scaling));
-    flux = gm::light_g<T, S>(type, ps, bx, by);
+    flux = light_grad<T, N>(type, ps, pk, T(x - ax * m.alpha_scaling), T(y - ay * m.alpha_scaling));
   } else {
     // d/d(lens light kwargs) at the ray
-    flux = gm::light_g<T, S>(type, ps, S(T(x)), S(T(y)));
+    flux = light_grad<T, N>(type, ps, pk, T(x), T(y));
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) acc[i] += w * double(flux.d[i]);
