@@ -866,7 +866,9 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
       ++cc;
     }
     // next step's weights and window are requested before this step's FMAs
-    // (the wait for them lands after the FMAs)
+    // (the compiler still waits for them before the FMAs: the weights are
+    // scalar loads, which complete out of order, so any wait on the shared
+    // counter is a wait for all of them; latency is hidden across waves)
     double wn[kWin];
     T nb[2 * kWin - 1];
     if (t + 1 < nsteps) {
