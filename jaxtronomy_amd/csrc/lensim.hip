@@ -48,6 +48,7 @@ int hip_fail(hipError_t e, const char *where) {
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 constexpr int kWin = 8;  // register window (outputs per thread) of the convolution
+constexpr int kGradPix = 4;  // pixels per thread of the gradient kernel
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
 // ---- optional kernel timing (lensim_timing_*) ----
@@ -275,7 +276,8 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
   L.glow = off;
   off = align256(off + size_t(n_img) * npix_r * tsz);
   L.gpart = off;
-  off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 255) / 256) * LENSIM_NPARAM * 8);
+  off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 256 * kGradPix - 1) / (256 * kGradPix)) *
+                           LENSIM_NPARAM * 8);
   L.gpad = L.gext = L.gchr = L.glu = off;
   if (edge) {  // extended domain (ny cs + ky - 1) x (nx cs + kx - 1)
     const size_t next = size_t(nx * cs + kx - 1) * (ny * cs + ky - 1);
@@ -1249,6 +1251,32 @@ __device__ __forceinline__ gm::Dual<T, N> light_grad(int type, const gm::Dual<T,
   return gm::light_g<T, S>(type, ps, S(x), S(y));
 }
 
+// Sum of the source light profiles at beta and its gradient in beta (the
+// tangents d[0], d[1]): closed form for SERSIC_ELLIPSE, two dual seeds
+// otherwise.
+template <typename T>
+__device__ __forceinline__ gm::Dual<T, 2> sources_at(const ModelArgs &m, const GradArgs &ga, int img, T bx, T by) {
+  using S2 = gm::Dual<T, 2>;
+  const S2 bx2 = S2::seed(bx, 0), by2 = S2::seed(by, 1);
+  S2 f2 = S2(T(0));
+  for (int j = m.n_defl; j < m.n_defl + m.n_source; ++j) {
+    const double *pj = ga.params + (size_t(img) * m.n_prof + j) * LENSIM_NPARAM;
+    const int tj = m.types[j - m.n_bulk];
+    if (tj == LENSIM_SERSIC_ELLIPSE) {
+      T gp[7], gx, gy;
+      f2.v += gm::sersic_ellipse_grad<T>(pj, bx, by, gp, gx, gy);
+      f2.d[0] += gx;
+      f2.d[1] += gy;
+      continue;
+    }
+    S2 pl[LENSIM_NPARAM];
+#pragma unroll
+    for (int i = 0; i < LENSIM_NPARAM; ++i) pl[i] = S2(T(pj[i]));
+    f2 = f2 + gm::light_g<T, S2>(tj, pl, bx2, by2);
+  }
+  return f2;
+}
+
 // Contribution of one ray to d flux / d(kwargs of profile k), with the
 // profile's own kwargs as N dual seeds; w = dlogL / d(ray flux).
 template <typename T, int N, bool HALO>
@@ -1283,26 +1311,9 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
     const S xs = S(T(x)), ys = S(T(y));
     gm::defl_g<T, S>(type, ps, c, dc, nser, xs, ys, ax, ay);
     const S bx = xs - ax * T(m.alpha_scaling), by = ys - ay * T(m.alpha_scaling);
-    // the sources' gradient in beta (two seeds), chained into this deflector's
-    // kwargs: d flux = dS/dbeta_x d beta_x + dS/dbeta_y d beta_y
-    using S2 = gm::Dual<T, 2>;
-    const S2 bx2 = S2::seed(bx.v, 0), by2 = S2::seed(by.v, 1);
-    S2 f2 = S2(T(0));
-    for (int j = m.n_defl; j < m.n_defl + m.n_source; ++j) {
-      const double *pj = ga.params + (size_t(img) * m.n_prof + j) * LENSIM_NPARAM;
-      const int tj = m.types[j - m.n_bulk];
-      if (tj == LENSIM_SERSIC_ELLIPSE) {
-        T gp[7], gx, gy;
-        f2.v += gm::sersic_ellipse_grad<T>(pj, bx.v, by.v, gp, gx, gy);
-        f2.d[0] += gx;
-        f2.d[1] += gy;
-        continue;
-      }
-      S2 pl[LENSIM_NPARAM];
-#pragma unroll
-      for (int i = 0; i < LENSIM_NPARAM; ++i) pl[i] = S2(T(pj[i]));
-      f2 = f2 + gm::light_g<T, S2>(tj, pl, bx2, by2);
-    }
+    // the sources' gradient in beta, chained into this deflector's kwargs:
+    // d flux = dS/dbeta_x d beta_x + dS/dbeta_y d beta_y
+    const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bx.v, by.v);
     flux.v = f2.v;
 #pragma unroll
     for (int i = 0; i < N; ++i) flux.d[i] = f2.d[0] * bx.d[i] + f2.d[1] * by.d[i];
@@ -1319,26 +1330,107 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
   for (int i = 0; i < N; ++i) acc[i] += w * double(flux.d[i]);
 }
 
-// One block = 256 pixels x one image x one profile (blockIdx.z).
+// Contribution of one ray to d flux / d(kwargs of the EPL deflector k).
+// Three dual seeds carry gamma, e1 and e2; theta_E and the centre are in
+// closed form: alpha scales as b^t with b = theta_E sqrt(q), so
+// d alpha / d theta_E = t alpha / theta_E, and d alpha / d center = -(the
+// Hessian), written as the reference's EPLMajorAxis.hessian
+// (LensModel/Profiles/epl.py:398-440, and the rotation of :203-232):
+// kappa = (2 - t)/2 (b/R)^t and the shear of its eq. (17) from alpha itself.
+template <typename T, bool HALO>
+__device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+                                             int k, double x, double y, double w, double *acc) {
+  using S = gm::Dual<T, 3>;
+  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form
+  for (int j = 0; j < m.n_lens; ++j) {
+    if (j == k) continue;
+    const int ej = m.epl_index[j];
+    const size_t coj = (ej >= 0) ? (size_t(img) * m.n_epl + ej) * LENSIM_EPL_NMAX : 0;
+    add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
+                            rp.coeft + coj, x, y, axp, ayp);
+  }
+  S ps[LENSIM_NPARAM];
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = (i >= 1 && i <= 3) ? S::seed(T(pk[i]), i - 1) : S(T(pk[i]));
+  const int e = m.epl_index[k];
+  const size_t co = (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
+  const int nser = int(sizeof(T) == sizeof(float) ? rp.recd[k * kRec + EPL_NF] : rp.recd[k * kRec + EPL_NMAX]);
+  const T *c = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.coeff + co)
+                                          : reinterpret_cast<const T *>(ga.coefd + co);
+  const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
+                                           : reinterpret_cast<const T *>(ga.dcoefd + co);
+  S ax = S(T(0)), ay = S(T(0));
+  gm::defl_g<T, S>(LENSIM_EPL, ps, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
+  const T a_x = ax.v, a_y = ay.v;
+  const T thetaE = T(pk[0]), t = T(pk[1]) - T(1);
+  T dax[6], day[6];
+  const T dth = thetaE != T(0) ? t / thetaE : T(0);
+  dax[0] = a_x * dth;
+  day[0] = a_y * dth;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    dax[i + 1] = ax.d[i];
+    day[i + 1] = ay.d[i];
+  }
+  // Hessian in the major-axis frame, rotated back
+  T phi, q;
+  gm::phi_q<T>(T(pk[2]), T(pk[3]), phi, q);
+  const T b = thetaE * gm::sqrt(q);
+  const T cs = gm::cos(phi), sn = gm::sin(phi);
+  const T dx = T(x) - T(pk[4]), dy = T(y) - T(pk[5]);
+  const T xr = dx * cs + dy * sn, yr = dy * cs - dx * sn;
+  const T r = gm::sqrt(xr * xr + yr * yr);
+  T fxx = T(0), fxy = T(0), fyy = T(0);
+  if (r > T(0)) {
+    T R = gm::sqrt(q * q * xr * xr + yr * yr);
+    R = R >= T(0.00000001) ? R : T(0.00000001);
+    const T kappa = (T(2) - t) / T(2) * gm::exp(t * gm::log(b / R));
+    const T ct = xr / r, st = yr / r;
+    const T c2 = ct * ct * T(2) - T(1), s2 = st * ct * T(2);
+    const T fr = a_x * cs + a_y * sn, fi = a_y * cs - a_x * sn;  // alpha in the major-axis frame
+    const T g1 = (T(1) - t) * (fr * ct - fi * st) / r - kappa * c2;
+    const T g2 = (T(1) - t) * (fi * ct + fr * st) / r - kappa * s2;
+    const T c2p = cs * cs - sn * sn, s2p = T(2) * sn * cs;
+    const T G1 = c2p * g1 - s2p * g2, G2 = s2p * g1 + c2p * g2;
+    fxx = kappa + G1;
+    fyy = kappa - G1;
+    fxy = G2;
+  }
+  dax[4] = -fxx;
+  day[4] = -fxy;
+  dax[5] = -fxy;
+  day[5] = -fyy;
+  const T as = T(m.alpha_scaling);
+  const T bxv = T(x) - (T(axp) + a_x) * as, byv = T(y) - (T(ayp) + a_y) * as;
+  const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxv, byv);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) acc[i] += w * double(-as * (f2.d[0] * dax[i] + f2.d[1] * day[i]));
+}
+
+// One block = kGradPix x 256 pixels x one image x one profile (blockIdx.z);
+// each thread takes kGradPix pixels 256 apart, so the block reduction and the
+// per-thread set-up are shared by kGradPix pixels' rays.
 template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
                                                    const double *__restrict__ recd, const float *__restrict__ recf,
                                                    const double *__restrict__ coefd, const float *__restrict__ coeff) {
   const int img = blockIdx.y, k = blockIdx.z;
   const int npix = g.nx * g.ny;
-  const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const int type = type_of(m, k);
+  double acc[LENSIM_NPARAM];
+#pragma unroll
+  for (int q = 0; q < LENSIM_NPARAM; ++q) acc[q] = 0.0;
+  for (int pp = 0; pp < kGradPix; ++pp) {
+  const int pix0 = (blockIdx.x * kGradPix + pp) * blockDim.x + threadIdx.x;
   const bool active = pix0 < npix;
   const int pix = active ? pix0 : npix - 1;
   const int i = pix % g.nx, j = pix / g.nx;
-  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
-  const int type = type_of(m, k);
   // the pixel's flux is scale * flux_scaling * (sum over its rays)
   const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
                                 m.flux_scaling
                           : 0.0;
-  double acc[LENSIM_NPARAM];
-#pragma unroll
-  for (int q = 0; q < LENSIM_NPARAM; ++q) acc[q] = 0.0;
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
   const double inv_s = 1.0 / g.s;
@@ -1352,7 +1444,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       switch (type) {
-        case LENSIM_EPL: ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
+        case LENSIM_EPL: epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc); break;
         case LENSIM_SIE: ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
         case LENSIM_NIE: ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
         case LENSIM_NFW: if constexpr (HALO) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
@@ -1364,6 +1456,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
         default: break;
       }
     }
+  }
   }
   // block sum of the 8 accumulators in a fixed order
   __shared__ double red[256 / 64][LENSIM_NPARAM];
@@ -1954,7 +2047,7 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   ga.dcoeff = reinterpret_cast<const float *>(ws + L.dcoeff);
   ga.glow = glow;
   ga.gpart = reinterpret_cast<double *>(ws + L.gpart);
-  const int npix = gg.nx * gg.ny, nblk = (npix + 255) / 256;
+  const int npix = gg.nx * gg.ny, nblk = (npix + 256 * kGradPix - 1) / (256 * kGradPix);
   dim3 grid(nblk, n_img, m.n_prof);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
