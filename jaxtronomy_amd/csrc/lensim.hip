@@ -605,14 +605,33 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   }
 }
 
-// The render for the common model family, its profile list fixed at compile
-// time: one main deflector (MAIN: EPL, or SIE / NIE) and an optional SHEAR, in
-// that order, then NSRC Sersic sources and NLL Sersic lens lights; one plane,
-// no bulk list.  Per profile it is render_kernel's arithmetic (the device
-// functions are shared); what goes is the runtime walk over the profile list,
-// its type switches and the scalar state they keep (render_kernel spills SGPRs
-// to VGPR lanes for it).
-template <typename T, int MAIN, bool SHEAR, int NSRC, int NLL>
+// One deflector of a type fixed at compile time (EPL's series coefficients
+// at coef_t / coef_d; SIE records are NIE records).
+template <typename T, int TYPE>
+__device__ __forceinline__ void defl_fixed(const Rec<T> &r, const T *coef_t, const double *coef_d, double x,
+                                           double y, double &ax, double &ay) {
+  if constexpr (TYPE == LENSIM_EPL) {
+    if constexpr (sizeof(T) == sizeof(float))
+      epl_defl_f32(r, coef_t, coef_d, x, y, ax, ay);
+    else
+      epl_defl_f64(r, coef_d, x, y, ax, ay);
+  } else if constexpr (TYPE == LENSIM_NIE) {
+    nie_defl<T>(r, x, y, ax, ay);
+  } else if constexpr (TYPE == LENSIM_NFW) {
+    nfw_defl<T>(r, x, y, ax, ay);
+  } else if constexpr (TYPE == LENSIM_SHEAR) {
+    shear_defl(r.d, x, y, ax, ay);
+  }
+}
+
+// The render for the common model families, the profile list fixed at compile
+// time: deflector D0 (EPL, SIE / NIE or NFW) and optionally D1 (SHEAR, SIE /
+// NIE or NFW; 0: none), in that order, then NSRC Sersic sources and NLL Sersic
+// lens lights; one plane, no bulk list.  Per profile it is render_kernel's
+// arithmetic (the device functions are shared); what goes is the runtime walk
+// over the profile list, its type switches and the scalar state they keep
+// (render_kernel spills SGPRs to VGPR lanes for it).
+template <typename T, int D0, int D1, int NSRC, int NLL>
 __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
                                                           const float *__restrict__ recf,
                                                           const double *__restrict__ coefd,
@@ -625,10 +644,10 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   const int pix = active ? pix0 : npix - 1;  // inactive lanes stay for the wave reduction
   const int i = pix % g.nx, j = pix / g.nx;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
-  constexpr int KS = SHEAR ? 2 : 1;  // first source record
-  const Rec<T> rm{rp.recd, rp.rect};
-  const Rec<T> rsh{rp.recd + kRec, rp.rect + kRec};
-  const size_t co = size_t(img) * m.n_epl * LENSIM_EPL_NMAX;  // the main EPL's coefficients (epl_index 0)
+  constexpr int KS = D1 ? 2 : 1;  // first source record
+  const Rec<T> r0{rp.recd, rp.rect};
+  const Rec<T> r1{rp.recd + kRec, rp.rect + kRec};
+  const size_t co = size_t(img) * m.n_epl * LENSIM_EPL_NMAX;  // D0's coefficients when it is the EPL (epl_index 0)
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
   const double inv_s = 1.0 / g.s;
@@ -647,15 +666,8 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
         double ax = 0.0, ay = 0.0;
-        if constexpr (MAIN == LENSIM_EPL) {
-          if constexpr (sizeof(T) == sizeof(float))
-            epl_defl_f32(rm, rp.coeft + co, rp.coefd + co, x, y, ax, ay);
-          else
-            epl_defl_f64(rm, rp.coefd + co, x, y, ax, ay);
-        } else {
-          nie_defl<T>(rm, x, y, ax, ay);
-        }
-        if constexpr (SHEAR) shear_defl(rsh.d, x, y, ax, ay);
+        defl_fixed<T, D0>(r0, rp.coeft + co, rp.coefd + co, x, y, ax, ay);
+        if constexpr (D1 != 0) defl_fixed<T, D1>(r1, nullptr, nullptr, x, y, ax, ay);
         const double bx = x - ax * asc, by = y - ay * asc;
 #pragma unroll
         for (int k = 0; k < NSRC; ++k)
@@ -687,22 +699,24 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
 int check_launch(const char *where);
 
 // Launches render_fast_kernel when the model is one of its compile-time lists
-// (EPL or SIE / NIE, optional SHEAR, one Sersic source, none or one Sersic lens
-// light), else render_kernel.
+// (EPL, SIE / NIE each with or without SHEAR after it, NFW + SIE / NIE in
+// either order; one Sersic source, none or one Sersic lens light), else
+// render_kernel.
 template <typename T>
 int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, const float *recf, const double *coefd,
                   const float *coeff, unsigned flags, T *out, float *blkmax, int n_img, hipStream_t st) {
   const dim3 rgrid((g.nx * g.ny + 255) / 256, n_img);
   auto sersic = [](int t) { return t == LENSIM_SERSIC || t == LENSIM_SERSIC_ELLIPSE; };
-  bool fast = m.mp_n == 0 && m.n_bulk == 0 && !m.has_halo && (m.n_lens == 1 || m.n_lens == 2) &&
-              m.n_source == 1 && (m.n_light == 0 || m.n_light == 1);
-  int main_t = 0;
-  bool shear = false;
+  auto nie = [](int t) { return t == LENSIM_SIE ? LENSIM_NIE : t; };
+  bool fast = m.mp_n == 0 && m.n_bulk == 0 && (m.n_lens == 1 || m.n_lens == 2) && m.n_source == 1 &&
+              (m.n_light == 0 || m.n_light == 1);
+  int d0 = 0, d1 = 0;
   if (fast) {
-    main_t = m.types[0];
-    shear = m.n_lens == 2;
-    fast = (main_t == LENSIM_EPL || main_t == LENSIM_SIE || main_t == LENSIM_NIE) &&
-           (!shear || m.types[1] == LENSIM_SHEAR) && (main_t != LENSIM_EPL || m.epl_index[0] == 0);
+    d0 = nie(m.types[0]);
+    d1 = m.n_lens == 2 ? nie(m.types[1]) : 0;
+    const bool pair = ((d0 == LENSIM_EPL || d0 == LENSIM_NIE) && (d1 == 0 || d1 == LENSIM_SHEAR)) ||
+                      (d0 == LENSIM_NFW && d1 == LENSIM_NIE) || (d0 == LENSIM_NIE && d1 == LENSIM_NFW);
+    fast = pair && (d0 != LENSIM_EPL || m.epl_index[0] == 0);
     for (int k = m.n_lens; fast && k < m.n_lens + m.n_source + m.n_light; ++k) fast = sersic(m.types[k]);
   }
   if (!fast) {
@@ -712,22 +726,20 @@ int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, con
       render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax);
     return check_launch("render_kernel");
   }
-  const bool epl = main_t == LENSIM_EPL, ll = m.n_light == 1;
-#define LENSIM_FAST(M, S, L)                                                                                   \
-  render_fast_kernel<T, M, S, 1, L><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax)
-  if (epl) {
-    if (shear) {
-      if (ll) LENSIM_FAST(LENSIM_EPL, true, 1); else LENSIM_FAST(LENSIM_EPL, true, 0);
-    } else {
-      if (ll) LENSIM_FAST(LENSIM_EPL, false, 1); else LENSIM_FAST(LENSIM_EPL, false, 0);
-    }
-  } else {
-    if (shear) {
-      if (ll) LENSIM_FAST(LENSIM_NIE, true, 1); else LENSIM_FAST(LENSIM_NIE, true, 0);
-    } else {
-      if (ll) LENSIM_FAST(LENSIM_NIE, false, 1); else LENSIM_FAST(LENSIM_NIE, false, 0);
-    }
-  }
+  const bool ll = m.n_light == 1;
+#define LENSIM_FAST(A, B)                                                                                       \
+  do {                                                                                                          \
+    if (ll)                                                                                                     \
+      render_fast_kernel<T, A, B, 1, 1><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax); \
+    else                                                                                                        \
+      render_fast_kernel<T, A, B, 1, 0><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax); \
+  } while (0)
+  if (d0 == LENSIM_EPL && d1 == LENSIM_SHEAR) LENSIM_FAST(LENSIM_EPL, LENSIM_SHEAR);
+  else if (d0 == LENSIM_EPL) LENSIM_FAST(LENSIM_EPL, 0);
+  else if (d0 == LENSIM_NIE && d1 == LENSIM_SHEAR) LENSIM_FAST(LENSIM_NIE, LENSIM_SHEAR);
+  else if (d0 == LENSIM_NIE && d1 == LENSIM_NFW) LENSIM_FAST(LENSIM_NIE, LENSIM_NFW);
+  else if (d0 == LENSIM_NIE) LENSIM_FAST(LENSIM_NIE, 0);
+  else LENSIM_FAST(LENSIM_NFW, LENSIM_NIE);
 #undef LENSIM_FAST
   return check_launch("render_fast_kernel");
 }

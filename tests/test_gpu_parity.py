@@ -533,3 +533,35 @@ def test_flux_evaluate_indexes_sub_frame(psf_type, precision, tol, ltol):
     assert abs(ll - ll_want) < ltol * abs(ll_want)
     with pytest.raises(NotImplementedError):
         im.likelihood_gradient(kl, ks, kll)
+
+
+_PAIR_KWARGS = {
+    "NFW": dict(Rs=0.8, alpha_Rs=0.6, center_x=0.1, center_y=-0.05),
+    "SIE": dict(theta_E=1.1, e1=0.05, e2=-0.03, center_x=0.0, center_y=0.02),
+    "NIE": dict(theta_E=1.0, e1=-0.04, e2=0.06, s_scale=0.05, center_x=0.01, center_y=0.0),
+    "EPL": dict(theta_E=1.2, gamma=2.1, e1=0.08, e2=-0.02, center_x=0.0, center_y=0.0),
+    "SHEAR": dict(gamma1=0.03, gamma2=-0.02),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lenses", [("NFW", "SIE"), ("SIE", "NFW"), ("SIE",), ("EPL",), ("NIE", "SHEAR")])
+@pytest.mark.parametrize("lens_light", [False, True])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("mixed", MIXED_IMG_TOL)])
+def test_fast_render_deflector_lists(lenses, lens_light, precision, tol):
+    """The deflector lists the compile-time render serves (one or two
+    deflectors: EPL or SIE / NIE with or without SHEAR, NFW with SIE in either
+    order), with and without lens light, against the oracle's image."""
+    cfg = W.SMALL
+    kernel = O.gaussian_kernel(cfg["psf_fwhm"], cfg["deltaPix"], 6)
+    kl = [dict(_PAIR_KWARGS[n]) for n in lenses]
+    ks = [dict(amp=20.0, R_sersic=0.3, n_sersic=2.5, e1=0.1, e2=-0.05, center_x=0.05, center_y=0.1)]
+    kll = [dict(amp=8.0, R_sersic=0.8, n_sersic=3.5, e1=-0.05, e2=0.02, center_x=0.0, center_y=0.0)]
+    im = ImageModel(ImageData(**W.data_kwargs(cfg)), PSF("PIXEL", kernel_point_source=kernel), LensModel(list(lenses)),
+                    LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"] if lens_light else []),
+                    kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+    got = _np(im.image(kl, ks, kll if lens_light else None))
+    g = dict(W.grid_dict(cfg), s=2)
+    want = O.image(g, list(lenses), kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"] if lens_light else [],
+                   kll if lens_light else [], kernel=kernel)
+    assert img_err(got, want) < tol, (lenses, lens_light, precision, img_err(got, want))
