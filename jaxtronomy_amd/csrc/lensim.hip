@@ -665,7 +665,8 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
-        double ax = 0.0, ay = 0.0;
+        // -0.0 + d is d for every d, so the first deflector's sum folds away
+        double ax = -0.0, ay = -0.0;
         defl_fixed<T, D0>(r0, rp.coeft + co, rp.coefd + co, x, y, ax, ay);
         if constexpr (D1 != 0) defl_fixed<T, D1>(r1, nullptr, nullptr, x, y, ax, ay);
         const double bx = x - ax * asc, by = y - ay * asc;

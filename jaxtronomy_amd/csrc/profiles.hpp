@@ -443,6 +443,10 @@ __device__ __forceinline__ float expm1_f32(float x) {
   p = fmaf(x, p, 1.6666666667e-1f);
   p = fmaf(x, p, 0.5f);
   const float small = fmaf(x * x, p, x);
+  // the EPL's argument (t - 1) ln(b/R) is below 0.5 in magnitude for most
+  // rays: a wave whose lanes all are skips the exp2 form (a uniform branch)
+  if (__all(fabsf(x) < 0.5f)) return small;
+  asm volatile("");  // keeps the exp2 form behind the branch (else it is speculated and selected)
   const float L = 1.4426950408889634f;
   const float hi = x * L;
   const float lo = fmaf(x, L, -hi) + x * 1.925963033500011e-8f;  // log2(e) - L
