@@ -1058,7 +1058,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   _Float16 *ll = lh + TH * ls;
   const int tid = threadIdx.x;
   const float *src = in + size_t(img) * c.nx * c.ny;
-  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  // the wave index is the same on every lane: read it as a scalar, so the
+  // tile rows a wave loads are addressed with scalar, not vector, arithmetic
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, li = lane & 15, lg = lane >> 4;
   // The padded tile: each wave fills rows wave, wave + 4, ..., lanes along a
   // row, in batches of kLdRows rows x kLdCols 64-column chunks.  Every load
   // reads a clamped (in-image) address and is zeroed after the fact outside
