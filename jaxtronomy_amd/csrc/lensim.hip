@@ -1066,7 +1066,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   // reads a clamped (in-image) address and is zeroed after the fact outside
   // the image, so a whole batch is issued before the first load is waited
   // for; the first batch is in flight while the image maximum is reduced.
-  constexpr int kLdRows = 16, kLdCols = (16 * (TL - 1) + 32 * NKS + 63) / 64;
+  constexpr int kLdRows = 8, kLdCols = (16 * (TL - 1) + 32 * NKS + 63) / 64;
   float v[kLdRows][kLdCols];
   auto load_batch = [&](int ly0) {
 #pragma unroll
