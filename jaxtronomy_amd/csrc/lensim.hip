@@ -1423,17 +1423,20 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   for (int i = 0; i < 6; ++i) acc[i] += w * double(-as * (f2.d[0] * dax[i] + f2.d[1] * day[i]));
 }
 
-// One block = kGradPix x 256 pixels x one image x one profile (blockIdx.z);
-// each thread takes kGradPix pixels 256 apart, so the block reduction and the
-// per-thread set-up are shared by kGradPix pixels' rays.
-template <typename T, bool HALO>
+// One block = kGradPix x 256 pixels x one image of profile k, whose type
+// TYPE is fixed at compile time (one launch per profile: each type's kernel
+// keeps only its own registers and code); each thread takes kGradPix pixels
+// 256 apart, so the block reduction and the per-thread set-up are shared by
+// kGradPix pixels' rays.
+template <typename T, bool HALO, int TYPE>
 __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
                                                    const double *__restrict__ recd, const float *__restrict__ recf,
-                                                   const double *__restrict__ coefd, const float *__restrict__ coeff) {
-  const int img = blockIdx.y, k = blockIdx.z;
+                                                   const double *__restrict__ coefd, const float *__restrict__ coeff,
+                                                   int k) {
+  const int img = blockIdx.y;
   const int npix = g.nx * g.ny;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
-  const int type = type_of(m, k);
+  constexpr int type = TYPE;
   double acc[LENSIM_NPARAM];
 #pragma unroll
   for (int q = 0; q < LENSIM_NPARAM; ++q) acc[q] = 0.0;
@@ -1458,18 +1461,15 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       const double ox = o0 + sx * inv_s;
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
-      switch (type) {
-        case LENSIM_EPL: epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc); break;
-        case LENSIM_SIE: ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_NIE: ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_NFW: if constexpr (HALO) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_TNFW: if constexpr (HALO) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_SHEAR: ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_SERSIC: ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_SERSIC_ELLIPSE: ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        case LENSIM_GAUSSIAN: ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc); break;
-        default: break;
-      }
+      if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc);
+      else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_NFW) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_TNFW) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_SERSIC) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_SERSIC_ELLIPSE) ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_GAUSSIAN) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
     }
   }
   }
@@ -1487,6 +1487,33 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
     for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[wv][threadIdx.x];
     ga.gpart[((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + threadIdx.x] = v;
   }
+}
+
+// grad_kernel for profile k of the model (its type read on the host).
+template <typename T, bool HALO>
+int launch_grad_profile(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, const double *recd,
+                        const float *recf, int nblk, int n_img, int k, hipStream_t st) {
+  const dim3 grid(nblk, n_img);
+#define LENSIM_GRAD(TY)                                                                                     \
+  grad_kernel<T, HALO, TY><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff, k)
+  switch (type_of(m, k)) {
+    case LENSIM_EPL: LENSIM_GRAD(LENSIM_EPL); break;
+    case LENSIM_SIE: LENSIM_GRAD(LENSIM_SIE); break;
+    case LENSIM_NIE: LENSIM_GRAD(LENSIM_NIE); break;
+    case LENSIM_NFW:  // present only in a model with halos (HALO)
+      if constexpr (HALO) LENSIM_GRAD(LENSIM_NFW); else LENSIM_GRAD(LENSIM_NULL);
+      break;
+    case LENSIM_TNFW:
+      if constexpr (HALO) LENSIM_GRAD(LENSIM_TNFW); else LENSIM_GRAD(LENSIM_NULL);
+      break;
+    case LENSIM_SHEAR: LENSIM_GRAD(LENSIM_SHEAR); break;
+    case LENSIM_SERSIC: LENSIM_GRAD(LENSIM_SERSIC); break;
+    case LENSIM_SERSIC_ELLIPSE: LENSIM_GRAD(LENSIM_SERSIC_ELLIPSE); break;
+    case LENSIM_GAUSSIAN: LENSIM_GRAD(LENSIM_GAUSSIAN); break;
+    default: LENSIM_GRAD(LENSIM_NULL); break;  // no kwargs: the kernel writes zero partials
+  }
+#undef LENSIM_GRAD
+  return check_launch("grad_kernel");
 }
 
 __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, int n, double *__restrict__ grad) {
@@ -2063,14 +2090,13 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   ga.glow = glow;
   ga.gpart = reinterpret_cast<double *>(ws + L.gpart);
   const int npix = gg.nx * gg.ny, nblk = (npix + 256 * kGradPix - 1) / (256 * kGradPix);
-  dim3 grid(nblk, n_img, m.n_prof);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
-  if (m.has_halo)
-    grad_kernel<T, true><<<grid, 256, 0, st>>>(m, gg, ga, recd, recf, ga.coefd, ga.coeff);
-  else
-    grad_kernel<T, false><<<grid, 256, 0, st>>>(m, gg, ga, recd, recf, ga.coefd, ga.coeff);
-  if ((rc = check_launch("grad_kernel"))) return rc;
+  for (int k = 0; k < m.n_prof; ++k) {
+    rc = m.has_halo ? launch_grad_profile<T, true>(m, gg, ga, recd, recf, nblk, n_img, k, st)
+                    : launch_grad_profile<T, false>(m, gg, ga, recd, recf, nblk, n_img, k, st);
+    if (rc) return rc;
+  }
   const int n = n_img * m.n_prof * LENSIM_NPARAM;
   grad_reduce_kernel<<<(n + 255) / 256, 256, 0, st>>>(ga.gpart, nblk, n, grad);
   return check_launch("grad_reduce_kernel");
