@@ -736,8 +736,13 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
                     (2.0f * X2 + float(r[TN_T2M1])) * atan_unit_f32(iv) * iv;
     const float iX = Fn<float>::rcp(Xf);
     const float z = tau * iX;
-    const float ash = z <= 0.5f ? asinh_small_f32(z)
-                                : 0.693147181f * __builtin_amdgcn_logf((tau + w) * iX);
+    float ash = asinh_small_f32(z);
+    // rays far from the subhalo (z <= 0.5) are the common case: a wave whose
+    // lanes all are skips the logarithm (a uniform branch)
+    if (!__all(z <= 0.5f)) {
+      asm volatile("");  // keeps the logarithm behind the branch
+      if (z > 0.5f) ash = 0.693147181f * __builtin_amdgcn_logf((tau + w) * iX);
+    }
     const float g = float(r[TN_PRE]) * (A + float(r[TN_C0]) - w * ash * float(r[TN_C1]));
     const float a = float(r[TN_A]) * g * (iX * iX);
     ax += double(a * float(dx));

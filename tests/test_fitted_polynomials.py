@@ -50,4 +50,4 @@ def test_far_field_ranges_fit():
     assert 1.0 / np.sqrt(1.5 ** 2 - 1.0) <= 0.9
     text = open(SRC).read()
     assert "X >= T(1.5)" in text and "X >= 1.5f" in text
-    assert "z <= 0.5f ? asinh_small_f32(z)" in text
+    assert "float ash = asinh_small_f32(z);" in text and "if (z > 0.5f) ash = " in text
