@@ -826,11 +826,11 @@ __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T mode
 // Block = TX columns x (TY / kWin) row groups; each thread computes kWin
 // consecutive rows of one column from a register window that slides down the
 // LDS tile, so consecutive lanes read consecutive LDS words.
-template <typename T>
+template <typename T, int TY>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in,
                                                    const double *__restrict__ kt, T *__restrict__ out,
                                                    double *__restrict__ partial) {
-  constexpr int TY = ConvTile<T>::TY, TX = ConvTile<T>::TX, NT = TX * TY / kWin;
+  constexpr int TX = ConvTile<T>::TX, NT = TX * TY / kWin;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);
   const int img = blockIdx.y;
@@ -1619,8 +1619,44 @@ int psf_ss(const lensim_psf *psf, const GridArgs &g, uint32_t flags, SSPlan &ss)
 }
 
 template <typename T>
-size_t conv_smem(int kx, int kyp) {
-  return size_t((ConvTile<T>::TY + kyp - 1) | 1) * (ConvTile<T>::TX + kx - 1) * sizeof(T);
+size_t conv_smem(int kx, int kyp, int ty = ConvTile<T>::TY) {
+  return size_t((ty + kyp - 1) | 1) * (ConvTile<T>::TX + kx - 1) * sizeof(T);
+}
+
+// conv_kernel over n_img images (c.n_tiles_x is set here, the tile count goes
+// to *n_tiles): float64 runs 64-row tiles where their LDS fits (four waves a
+// block, a smaller share of halo rows; 32 rows otherwise), float 32-row tiles.
+template <typename T>
+int launch_conv_kernel(ConvArgs c, int n_img, const T *in, const double *kf, T *out, double *partial,
+                       int *n_tiles, hipStream_t st) {
+  constexpr int TX = ConvTile<T>::TX;
+  int ty = ConvTile<T>::TY;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    if (conv_smem<T>(c.kx, c.kyp, 64) <= kMaxLds) ty = 64;
+  }
+  c.n_tiles_x = (c.nx + TX - 1) / TX;
+  *n_tiles = c.n_tiles_x * ((c.ny + ty - 1) / ty);
+  const size_t smem = conv_smem<T>(c.kx, c.kyp, ty);
+  if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", c.ky, c.kx, smem);
+  const dim3 grid(*n_tiles, n_img);
+  if constexpr (sizeof(T) == sizeof(double)) {
+    if (ty == 64) {
+      if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, 64>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+      }
+      conv_kernel<T, 64><<<grid, TX * 64 / kWin, smem, st>>>(c, in, kf, out, partial);
+      return check_launch("conv_kernel");
+    }
+  }
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, ConvTile<T>::TY>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+  }
+  conv_kernel<T, ConvTile<T>::TY><<<grid, TX * ConvTile<T>::TY / kWin, smem, st>>>(c, in, kf, out, partial);
+  return check_launch("conv_kernel");
 }
 
 // Whether the mixed form convolves on the matrix cores (any real PSF whose
@@ -1743,19 +1779,7 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
     rc = launch_conv_mfma(c, pl, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
                           (npix + 255) / 256, reinterpret_cast<float *>(out), partial, st);
   } else {
-    c.n_tiles_x = (g.nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
-    n_tiles = c.n_tiles_x * ((g.ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
-    const size_t smem = conv_smem<T>(kx, kyp);
-    if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
-    if (smem > 65536) {
-      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-      if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-    }
-    dim3 cgrid(n_tiles, n_img);
-    const int threads = ConvTile<T>::TX * ConvTile<T>::TY / kWin;
-    conv_kernel<T><<<cgrid, threads, smem, st>>>(c, lowres, kf, out, partial);
-    rc = check_launch("conv_kernel");
+    rc = launch_conv_kernel<T>(c, n_img, lowres, kf, out, partial, &n_tiles, st);
   }
   if (rc) return rc;
   timing_mark(2, st);
@@ -1844,18 +1868,8 @@ int launch_conv_plain(int nx, int ny, int kx, int ky, int edge, const T *in, con
   c.ky = ky;
   c.kyp = kyp_of(ky);
   c.edge = edge;
-  c.n_tiles_x = (nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
-  const int n_tiles = c.n_tiles_x * ((ny + ConvTile<T>::TY - 1) / ConvTile<T>::TY);
-  const size_t smem = conv_smem<T>(kx, c.kyp);
-  if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", ky, kx, smem);
-  if (smem > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-  }
-  conv_kernel<T><<<dim3(n_tiles, n_img), ConvTile<T>::TX * ConvTile<T>::TY / kWin, smem, st>>>(c, in, kf, out,
-                                                                                                 nullptr);
-  return check_launch("conv_kernel");
+  int n_tiles = 0;
+  return launch_conv_kernel<T>(c, n_img, in, kf, out, nullptr, &n_tiles, st);
 }
 
 // The rays of grid g as a grid of their own: pixel (I, J) is sub-ray
