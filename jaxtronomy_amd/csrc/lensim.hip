@@ -1624,31 +1624,27 @@ size_t conv_smem(int kx, int kyp, int ty = ConvTile<T>::TY) {
 }
 
 // conv_kernel over n_img images (c.n_tiles_x is set here, the tile count goes
-// to *n_tiles): float64 runs 64-row tiles where their LDS fits (four waves a
-// block, a smaller share of halo rows; 32 rows otherwise), float 32-row tiles.
+// to *n_tiles): 64-row tiles where their LDS fits (four waves a block, a
+// smaller share of halo rows), 32-row tiles otherwise.
 template <typename T>
 int launch_conv_kernel(ConvArgs c, int n_img, const T *in, const double *kf, T *out, double *partial,
                        int *n_tiles, hipStream_t st) {
   constexpr int TX = ConvTile<T>::TX;
   int ty = ConvTile<T>::TY;
-  if constexpr (sizeof(T) == sizeof(double)) {
-    if (conv_smem<T>(c.kx, c.kyp, 64) <= kMaxLds) ty = 64;
-  }
+  if (conv_smem<T>(c.kx, c.kyp, 64) <= kMaxLds) ty = 64;
   c.n_tiles_x = (c.nx + TX - 1) / TX;
   *n_tiles = c.n_tiles_x * ((c.ny + ty - 1) / ty);
   const size_t smem = conv_smem<T>(c.kx, c.kyp, ty);
   if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", c.ky, c.kx, smem);
   const dim3 grid(*n_tiles, n_img);
-  if constexpr (sizeof(T) == sizeof(double)) {
-    if (ty == 64) {
-      if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, 64>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-      }
-      conv_kernel<T, 64><<<grid, TX * 64 / kWin, smem, st>>>(c, in, kf, out, partial);
-      return check_launch("conv_kernel");
+  if (ty == 64) {
+    if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, 64>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+      if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
     }
+    conv_kernel<T, 64><<<grid, TX * 64 / kWin, smem, st>>>(c, in, kf, out, partial);
+    return check_launch("conv_kernel");
   }
   if (smem > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, ConvTile<T>::TY>),
