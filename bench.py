@@ -528,6 +528,135 @@ def main_serve_mix(args):
     print(json.dumps(line), flush=True)
 
 
+def main_sie60(args):
+    """The small scene (SIE+SHEAR, SERSIC_ELLIPSE source, 60x60 at 0.05"/pix,
+    supersampling 1, 11x11 Gaussian PSF of FWHM 0.1", batch 1).  One image is
+    far too little work to fill the GPU, so its figure is latency: a single
+    log-likelihood call with data resident, as the caller sees it (host clock
+    from the call to its completion), issued eagerly and as a replay of a
+    captured HIP graph, in both precision forms; the C FP64 oracle's time for
+    the same call on one host thread is reported beside it."""
+    import torch
+
+    from jaxtronomy_amd import backend
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model import LensModel
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+
+    cfg = W.SMALL
+    kl, ks = W.small_draw(0)
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)),
+                          PSF("GAUSSIAN", fwhm=cfg["psf_fwhm"], truncation=cfg["psf_truncation"]),
+                          LensModel(["SIE", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": cfg["supersampling"]}, precision=precision)
+
+    truth = model("fp64").image(kl, ks)
+    data = W.noisy(truth.cpu().numpy(), cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(1))
+    backend.timing_enable(False)  # the library's own event records synchronise; none inside a graph
+    forms, logls = {}, {}
+    for precision in ("fp64", "mixed"):
+        im = model(precision, data)
+        params_np, _ = im.pack(kl, ks)
+        params = torch.as_tensor(params_np, device="cuda")
+        logl = torch.empty(1, dtype=torch.float64, device="cuda")
+        st = torch.cuda.Stream()
+        torch.cuda.synchronize()
+
+        def call():
+            im.likelihood_batch(params, logl=logl)
+
+        rec = {}
+        with torch.cuda.stream(st):
+            for _ in range(args.warmup):
+                call()
+            st.synchronize()
+            host = np.empty(args.steps)
+            for i in range(args.steps):
+                t0 = time.perf_counter()
+                call()
+                st.synchronize()
+                host[i] = time.perf_counter() - t0
+            eager_ll = float(logl.item())
+        rec["eager"] = {"p50_ms": round(1e3 * float(np.percentile(host, 50)), 4),
+                        "p99_ms": round(1e3 * float(np.percentile(host, 99)), 4)}
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            call()
+        with torch.cuda.stream(st):
+            for _ in range(args.warmup):
+                g.replay()
+            st.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            dev = np.empty(args.steps)
+            for i in range(args.steps):
+                t0 = time.perf_counter()
+                e0.record(st)
+                g.replay()
+                e1.record(st)
+                st.synchronize()
+                host[i] = time.perf_counter() - t0
+                dev[i] = e0.elapsed_time(e1)
+            graph_ll = float(logl.item())
+        if graph_ll != eager_ll:
+            raise RuntimeError("graph replay logL %r differs from the eager call's %r" % (graph_ll, eager_ll))
+        rec["graph"] = {"p50_ms": round(1e3 * float(np.percentile(host, 50)), 4),
+                        "p99_ms": round(1e3 * float(np.percentile(host, 99)), 4),
+                        "device_p50_ms": round(float(np.percentile(dev, 50)), 4)}
+        forms[precision] = rec
+        logls[precision] = eager_ll
+        log("bench: %s %s" % (precision, rec))
+        del g
+    line = {
+        "metric": "latency of one log-likelihood call (p50, graph replay, FP64), SIE+SHEAR+Sersic 60^2 ss1",
+        "value": forms["fp64"]["graph"]["p50_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": forms["fp64"]["graph"]["p50_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "SIE+SHEAR deflector, SERSIC_ELLIPSE source, 60x60 @0.05\", supersampling 1, "
+                               "11x11 Gaussian PSF (FWHM 0.1\", truncation 6), image + chi2 logL, batch 1",
+                   "timing": "host clock from the call to its completion on the stream; device_p50_ms: HIP events "
+                             "around the graph replay"},
+        "p50_ms": forms["fp64"]["graph"]["p50_ms"],
+        "p99_ms": forms["fp64"]["graph"]["p99_ms"],
+        "fp64": forms["fp64"],
+        "mixed": forms["mixed"],
+        "accuracy": {"mixed_logl_rel_vs_fp64": abs(logls["mixed"] - logls["fp64"]) / abs(logls["fp64"])},
+    }
+    if not args.no_cpu:
+        from oracle import c_oracle
+
+        g = W.grid_dict(cfg)
+        im = model("fp64", data)
+        p = im.pack(kl, ks)[0]
+        c_oracle.lib()
+        run = lambda: c_oracle.loglike(2, 1, 0, [2, 6, 33], p, g, data, kernel=im.kernel, psf_type=2,
+                                       exposure_time=cfg["exposure_time"], background_rms=cfg["background_rms"],
+                                       n_threads=1)
+        ll_cpu = run()[0]
+        n = 200
+        t0 = time.perf_counter()
+        for _ in range(n):
+            run()
+        dt = (time.perf_counter() - t0) / n
+        line["cpu_baseline"] = {"value": round(1e3 * dt, 4), "unit": "ms", "cores": 1, "kind": "port",
+                                "sample": "%d calls of the same log-likelihood, C FP64 restatement of the reference "
+                                          "algorithm, one thread, on %s" % (n, cpu_model())}
+        line["accuracy"]["fp64_logl_rel_vs_cpu_oracle"] = abs(logls["fp64"] - ll_cpu) / abs(ll_cpu)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -539,7 +668,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=60)
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix"), default="epl180")
+    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60"), default="epl180")
     ap.add_argument("--in-flight", type=int, default=3, help="batches in flight for the headline value (epl180)")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
     ap.add_argument("--n-sub", type=int, default=1000)
@@ -575,6 +704,8 @@ def main():
         return main_grad(args)
     if args.workload == "serve-mix":
         return main_serve_mix(args)
+    if args.workload == "sie60":
+        return main_sie60(args)
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch, args.in_flight)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
