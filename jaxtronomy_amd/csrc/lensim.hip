@@ -616,9 +616,9 @@ __device__ __forceinline__ void defl_fixed(const Rec<T> &r, const T *coef_t, con
     else
       epl_defl_f64(r, coef_d, x, y, ax, ay);
   } else if constexpr (TYPE == LENSIM_NIE) {
-    nie_defl<T>(r, x, y, ax, ay);
+    nie_defl<T, true>(r, x, y, ax, ay);
   } else if constexpr (TYPE == LENSIM_NFW) {
-    nfw_defl<T>(r, x, y, ax, ay);
+    nfw_defl<T, true>(r, x, y, ax, ay);
   } else if constexpr (TYPE == LENSIM_SHEAR) {
     shear_defl(r.d, x, y, ax, ay);
   }
@@ -826,7 +826,11 @@ __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T mode
 // Block = TX columns x (TY / kWin) row groups; each thread computes kWin
 // consecutive rows of one column from a register window that slides down the
 // LDS tile, so consecutive lanes read consecutive LDS words.
-template <typename T, int TY>
+// LW: the weights are staged in LDS behind the tile.  For launches of fewer
+// blocks than CUs: there each step's scalar loads of the weights would miss
+// the caches in turn (the kernel was just written by another launch), one
+// memory latency per step.
+template <typename T, int TY, bool LW>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restrict__ in,
                                                    const double *__restrict__ kt, T *__restrict__ out,
                                                    double *__restrict__ partial) {
@@ -843,20 +847,40 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   const int THp = TH | 1;  // column stride of the column-major tile: odd, so lanes hit distinct banks
   const T *src = in + size_t(img) * c.nx * c.ny;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < TH * TW; idx += NT) {
-    const int ly = idx / TW, lx = idx - ly * TW;
-    int gy = y0 - cy + ly, gx = x0 - cx + lx;
-    T v = T(0);
-    if (ly >= TY + c.ky - 1) {
-      // rows that only meet the zero padding of the kernel height
-    } else if (c.edge) {
-      gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
-      gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
-      v = src[gy * c.nx + gx];
-    } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
-      v = src[gy * c.nx + gx];
+  // kLd elements per thread and pass: all their loads are issued before the
+  // first LDS store waits, so a launch of few blocks (LW) pays one memory
+  // latency per pass rather than one per element.  Large launches hide that
+  // latency across blocks and keep one element per pass (eight measured
+  // slower for the float adjoint convolution of the gradient).
+  constexpr int kLd = LW ? 8 : 1;
+  for (int base = tid; base < TH * TW; base += NT * kLd) {
+    T v[kLd];
+    int at[kLd];
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int idx = base + u * NT;
+      const int ly = idx / TW, lx = idx - ly * TW;
+      int gy = y0 - cy + ly, gx = x0 - cx + lx;
+      v[u] = T(0);
+      at[u] = idx < TH * TW ? lx * THp + ly : -1;
+      if (idx >= TH * TW || ly >= TY + c.ky - 1) {
+        // past the tile, or rows that only meet the zero padding of the kernel height
+      } else if (c.edge) {
+        gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
+        gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
+        v[u] = src[gy * c.nx + gx];
+      } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
+        v[u] = src[gy * c.nx + gx];
+      }
     }
-    tile[lx * THp + ly] = v;
+#pragma unroll
+    for (int u = 0; u < kLd; ++u)
+      if (at[u] >= 0) tile[at[u]] = v[u];
+  }
+  const int nch = kyp / kWin, nsteps = c.kx * nch;
+  double *wl = reinterpret_cast<double *>(smem_raw + ((size_t(THp) * TW * sizeof(T) + 7) & ~size_t(7)));
+  if constexpr (LW) {
+    for (int idx = tid; idx < nsteps * kWin; idx += NT) wl[idx] = kt[idx];
   }
   __syncthreads();
   const int lx = tid % TX, tg = tid / TX;
@@ -866,14 +890,13 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   for (int r = 0; r < kWin; ++r) acc[r] = 0.0;
   // steps t = (kernel column cc, row chunk ch) flattened; the window of step
   // t + 1 is loaded while step t is computed
-  const int nch = kyp / kWin, nsteps = c.kx * nch;
   const T *colbase = tile + lx * THp + tg * kWin;
   double buf[2 * kWin - 1];
 #pragma unroll
   for (int q = 0; q < 2 * kWin - 1; ++q) buf[q] = double(colbase[q]);
   double w[kWin];
 #pragma unroll
-  for (int jj = 0; jj < kWin; ++jj) w[jj] = kt[jj];
+  for (int jj = 0; jj < kWin; ++jj) w[jj] = LW ? wl[jj] : kt[jj];
   int cc = 0, ch = 0;
   for (int t = 0; t < nsteps; ++t) {
     if (++ch == nch) {
@@ -887,9 +910,14 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
     double wn[kWin];
     T nb[2 * kWin - 1];
     if (t + 1 < nsteps) {
-      const double *kw = kt + (t + 1) * kWin;  // = kt + cc * kyp + ch * kWin
+      if constexpr (LW) {
 #pragma unroll
-      for (int jj = 0; jj < kWin; ++jj) wn[jj] = kw[jj];
+        for (int jj = 0; jj < kWin; ++jj) wn[jj] = wl[(t + 1) * kWin + jj];
+      } else {
+        const double *kw = kt + (t + 1) * kWin;  // = kt + cc * kyp + ch * kWin
+#pragma unroll
+        for (int jj = 0; jj < kWin; ++jj) wn[jj] = kw[jj];
+      }
       const T *col = colbase + cc * THp + ch * kWin;
 #pragma unroll
       for (int q = 0; q < 2 * kWin - 1; ++q) nb[q] = col[q];
@@ -1626,6 +1654,18 @@ size_t conv_smem(int kx, int kyp, int ty = ConvTile<T>::TY) {
 // conv_kernel over n_img images (c.n_tiles_x is set here, the tile count goes
 // to *n_tiles): 64-row tiles where their LDS fits (four waves a block, a
 // smaller share of halo rows), 32-row tiles otherwise.
+template <typename T, int TY, bool LW>
+int launch_conv_tile(const ConvArgs &c, dim3 grid, size_t smem, const T *in, const double *kf, T *out,
+                     double *partial, hipStream_t st) {
+  if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, TY, LW>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
+  }
+  conv_kernel<T, TY, LW><<<grid, ConvTile<T>::TX * TY / kWin, smem, st>>>(c, in, kf, out, partial);
+  return check_launch("conv_kernel");
+}
+
 template <typename T>
 int launch_conv_kernel(ConvArgs c, int n_img, const T *in, const double *kf, T *out, double *partial,
                        int *n_tiles, hipStream_t st) {
@@ -1637,22 +1677,14 @@ int launch_conv_kernel(ConvArgs c, int n_img, const T *in, const double *kf, T *
   const size_t smem = conv_smem<T>(c.kx, c.kyp, ty);
   if (smem > kMaxLds) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d needs %zu B of LDS per tile", c.ky, c.kx, smem);
   const dim3 grid(*n_tiles, n_img);
-  if (ty == 64) {
-    if (smem > 65536) {  // dynamic LDS beyond 64 KB needs the kernel attribute raised first
-      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, 64>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-      if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-    }
-    conv_kernel<T, 64><<<grid, TX * 64 / kWin, smem, st>>>(c, in, kf, out, partial);
-    return check_launch("conv_kernel");
-  }
-  if (smem > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<T, ConvTile<T>::TY>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_kernel)");
-  }
-  conv_kernel<T, ConvTile<T>::TY><<<grid, TX * ConvTile<T>::TY / kWin, smem, st>>>(c, in, kf, out, partial);
-  return check_launch("conv_kernel");
+  // weights in LDS when the launch has fewer blocks than the GPU has CUs
+  const size_t smem_lw = ((smem + 7) & ~size_t(7)) + size_t(c.kx) * c.kyp * sizeof(double);
+  const bool lw = size_t(*n_tiles) * n_img < 256 && smem_lw <= kMaxLds;
+  if (ty == 64)
+    return lw ? launch_conv_tile<T, 64, true>(c, grid, smem_lw, in, kf, out, partial, st)
+              : launch_conv_tile<T, 64, false>(c, grid, smem, in, kf, out, partial, st);
+  return lw ? launch_conv_tile<T, ConvTile<T>::TY, true>(c, grid, smem_lw, in, kf, out, partial, st)
+            : launch_conv_tile<T, ConvTile<T>::TY, false>(c, grid, smem, in, kf, out, partial, st);
 }
 
 // Whether the mixed form convolves on the matrix cores (any real PSF whose

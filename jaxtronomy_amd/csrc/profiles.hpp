@@ -549,24 +549,40 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
 }
 
 // NIE / SIE deflection, LensModel/Profiles/nie.py:86-107, :280-295, :333-342.
-// Both forms run it in double.  A float form (double constants, float psi,
-// quotients, atan and atanh) was measured: its one-to-two-ulp random errors
-// of alpha, summed over the rays of a compact cuspy source (Sersic n ~ 4,
-// R_sersic 0.17") behind an NFW + SIE lens, moved the log-likelihood by
-// 1.8e-6 relative, against the mixed form's 1e-6.
-template <typename T>
+// DBL (the fast render kernels, whose profile list is fixed at compile time):
+// the float form runs the FP64 form's arithmetic instead.  Its float
+// atan / atanh leave one-to-two-ulp random errors in alpha, which a compact
+// cuspy source (Sersic n ~ 4, R_sersic 0.17") behind an NFW + SIE lens turned
+// into a log-likelihood error of 1.8e-6 relative.  Kernels that switch over
+// the profile type at run time (bulk lists, multi-plane, gradients) keep the
+// float form: the double library atan / atanh would raise every such
+// kernel's registers (occupancy 4 -> 2 waves per SIMD) whatever its model.
+template <typename T, bool DBL = false>
 __device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
   const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
   const double cd = r.d[NIE_C], sd = r.d[NIE_S];
-  const double xr = dx * cd + dy * sd;
-  const double yr = -dx * sd + dy * cd;
-  const double ss = r.d[NIE_SS];
-  const double psi = ::sqrt(r.d[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
-  const double sq = r.d[NIE_SQ], bsq = r.d[NIE_BSQ];
-  const double fx = bsq * ::atan(sq * xr / (psi + ss));
-  const double fy = bsq * ::atanh(sq * yr / (psi + r.d[NIE_Q2S]));
-  ax += fx * cd - fy * sd;
-  ay += fx * sd + fy * cd;
+  if constexpr (DBL || sizeof(T) == sizeof(double)) {
+    const double xr = dx * cd + dy * sd;
+    const double yr = -dx * sd + dy * cd;
+    const double ss = r.d[NIE_SS];
+    const double psi = ::sqrt(r.d[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
+    const double sq = r.d[NIE_SQ], bsq = r.d[NIE_BSQ];
+    const double fx = bsq * ::atan(sq * xr / (psi + ss));
+    const double fy = bsq * ::atanh(sq * yr / (psi + r.d[NIE_Q2S]));
+    ax += fx * cd - fy * sd;
+    ay += fx * sd + fy * cd;
+  } else {
+    const T xr = T(dx * cd + dy * sd);
+    const T yr = T(-dx * sd + dy * cd);
+    const T ss = r[NIE_SS];
+    const T psi = Fn<T>::sqrt_(r[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
+    const T sq = r[NIE_SQ], bsq = r[NIE_BSQ];
+    const T fx = bsq * Fn<T>::atan_(sq * xr / (psi + ss));
+    const T fy = bsq * Fn<T>::atanh_(sq * yr / (psi + r[NIE_Q2S]));
+    const T c = r[NIE_C], s = r[NIE_S];
+    ax += double(fx * c - fy * s);
+    ay += double(fx * s + fy * c);
+  }
 }
 
 // atan(t) for 0 <= t <= 0.9 as t + t^3 P(t^2), P of degree 7 fitted to atan by
@@ -636,30 +652,44 @@ __device__ __forceinline__ double nfw_g(double X) {
   return ::log(X / 2.0) + 1.0 / ::sqrt(X * X - 1.0) * ::acos(1.0 / X);
 }
 
+// Float form of g.  For X < 0.8 the closed form ln(X/2) + h(X) cancels (both
+// terms grow like ln(2/X) while g ~ X^2 ln(2/X) / 2), so it is rewritten with
+// s = sqrt(1 - X^2), u = 1 - s = X^2 / (1 + s):
+//   g = (u / s) ln(2 / X) + log1p(-u / 2) / s,
+// which has no cancellation there.
+__device__ __forceinline__ float nfw_g(float X) {
+  X = (X < 0.000001f) ? 0.000001f : X;
+  if (X < 0.8f) {
+    const float s = __builtin_amdgcn_sqrtf((1.0f - X) * (1.0f + X));
+    const float u = X * X / (1.0f + s);
+    return (u * ::logf(2.0f / X) + ::log1pf(-0.5f * u)) / s;
+  }
+  return ::logf(0.5f * X) + nfw_h_f32(X);
+}
+
 // NFW deflection, LensModel/Profiles/nfw.py:86-105 with nfw_alpha :257-279.
-template <typename T>
+// DBL (fast render kernels): the float form runs in double too.  The float
+// g(X) = ln(X/2) + h(X) cancels around X ~ 1 (0.31 at X = 1 from terms of 0.7
+// and 1), where an NFW main deflector's Einstein ring sits; with a compact
+// cuspy source that moved the log-likelihood by 2.4e-6 relative.
+template <typename T, bool DBL = false>
 __device__ __forceinline__ void nfw_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
   const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
-  if constexpr (sizeof(T) == sizeof(float)) {
-    // Mixed form: in double.  The float g(X) = ln(X/2) + h(X) cancels
-    // around X ~ 1 (0.31 at X = 1 from terms of 0.7 and 1), where an NFW main
-    // deflector's Einstein ring sits; its few-ulp errors, summed over the
-    // rays of a compact cuspy source (Sersic n ~ 4, R_sersic 0.17"), moved
-    // the log-likelihood by 2.4e-6 relative.  This is the reference's
-    // formula with the floors of the FP64 form.
-    double R = sqrt(fma(dxd, dxd, dyd * dyd));
+  if constexpr (DBL || sizeof(T) == sizeof(double)) {
+    const double dx = dxd, dy = dyd;
+    double R = ::sqrt(dx * dx + dy * dy);
     R = R > 0.00000001 ? R : 0.00000001;
+    // X = R / Rs by division: g(X) has an exact X == 1 branch and cancels
+    // near it, so X must round as the reference's does.
     const double X = R / r.d[NFW_RS];
     const double a = r.d[NFW_A] * nfw_g(X) / (X * X);
-    ax += a * dxd;
-    ay += a * dyd;
+    ax += a * dx;
+    ay += a * dy;
   } else {
     const T dx = T(dxd), dy = T(dyd);
     T R = Fn<T>::sqrt_(dx * dx + dy * dy);
     R = R > T(0.00000001) ? R : T(0.00000001);
-    // X = R / Rs by division: g(X) has an exact X == 1 branch and cancels
-    // near it, so X must round as the reference's does.
-    const T X = R / r[NFW_RS];
+    const T X = R * r[NFW_IRS];
     const T g = nfw_g(X);
     const T a = r[NFW_A] * g / (X * X);
     ax += double(a * dx);
