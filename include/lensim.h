@@ -5,8 +5,9 @@
  * One shared library, plain pointers and sizes, no torch types.  Every
  * buffer argument named d_* is DEVICE memory on the current HIP device;
  * every call is asynchronous on the given HIP stream (NULL = default stream)
- * and allocates nothing: scratch memory comes in through `ws` (query its
- * size with lensim_workspace_size).  All entry points return 0 on success or
+ * and allocates no device memory: scratch memory comes in through `ws`
+ * (query its size with lensim_workspace_size).  The graph entry points
+ * create HIP graph objects.  All entry points return 0 on success or
  * a negative lensim_status; lensim_last_error() returns a message for the
  * last failure on the calling thread.
  *
@@ -39,7 +40,7 @@
 extern "C" {
 #endif
 
-#define LENSIM_ABI_VERSION 4
+#define LENSIM_ABI_VERSION 5
 #define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
 #define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
 #define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
@@ -269,6 +270,27 @@ size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                         const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
                         double *d_logl, double *d_grad, void *ws, size_t ws_bytes, void *stream);
+
+/* Graph capture of a log-likelihood call, for callers that replay one batch
+ * shape many times (a sampler's likelihood, a request server's lanes) and
+ * want one launch per call.  Replaces, for those callers, the jit-compiled
+ * and cached ImageModel.likelihood_data_given_model that the reference's
+ * samplers call (Sampling/sampler.py:215-251, Sampling/likelihood.py:290-319).
+ * lensim_graph_capture_loglike captures the launches lensim_loglike makes
+ * with these arguments on `stream` (not the NULL stream) into a HIP graph and
+ * instantiates it; nothing runs.  lensim_graph_launch replays it on a stream:
+ * it reads the same device buffers (parameters, data, workspace) and writes
+ * the same outputs, so a caller updates d_params (or the data) in place
+ * between launches.  Launches of one graph must be ordered (they share the
+ * workspace).  The model, grid, psf and data structs are read only at
+ * capture.  lensim_graph_destroy frees the graph. */
+typedef struct lensim_graph_s *lensim_graph;
+int lensim_graph_capture_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                 const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                                 double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream,
+                                 lensim_graph *graph);
+int lensim_graph_launch(lensim_graph graph, void *stream);
+int lensim_graph_destroy(lensim_graph graph);
 
 /* Kernel timing for benchmarks (no counterpart in the reference).  While
  * enabled, every lensim_image / lensim_loglike call records HIP events on its

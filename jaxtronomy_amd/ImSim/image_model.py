@@ -107,7 +107,9 @@ class ImageModel(object):
         self._lens_types = [] if self._bulk else list(self.LensModel._types)
         self._src_types = list(self.SourceModel._types)
         self._light_types = list(self.LensLightModel._types)
-        self._names = ([] if self._bulk else list(self.LensModel.param_name_list)) + \
+        # slot layouts of every profile (see profiles.lens_layout): kwarg names,
+        # or (name, value) for slots a lens profile's options fix
+        self._names = ([] if self._bulk else list(self.LensModel._layouts)) + \
             list(self.SourceModel.param_name_list) + list(self.LensLightModel.param_name_list)
         self._model = backend.make_model(self._lens_types, self._src_types, self._light_types,
                                          alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
@@ -299,17 +301,19 @@ class ImageModel(object):
         d.exposure_time = buf["expo_time"]
         d.background_rms = self.Data.background_rms
         if data is not None:
+            # observations cover the whole frame (with flux_evaluate_indexes the
+            # likelihood runs on the completed frame, not on the sub-frame)
+            ny, nx = self._full_shape
             for t in (data, noise_map):
                 if t is not None and (t.dtype != torch.float64 or not t.is_contiguous() or
-                                      tuple(t.shape) != tuple(data.shape) or
-                                      tuple(t.shape[-2:]) != (self._grid.ny, self._grid.nx)):
+                                      tuple(t.shape) != tuple(data.shape) or tuple(t.shape[-2:]) != (ny, nx)):
                     raise ValueError("per-image data / noise maps must be contiguous float64 [B, %d, %d] tensors"
-                                     % (self._grid.ny, self._grid.nx))
+                                     % (ny, nx))
             if buf["noise"] is not None and noise_map is None:
                 raise ValueError("per-image data with a noise map needs per-image noise maps")
             d.d_data = data.data_ptr()
             d.d_noise_map = None if noise_map is None else noise_map.data_ptr()
-            d.image_stride = self._grid.nx * self._grid.ny
+            d.image_stride = nx * ny
         elif noise_map is not None:
             raise ValueError("per-image noise maps need per-image data")
         return d
@@ -387,8 +391,8 @@ class ImageModel(object):
         for types in (self._lens_types, self._src_types, self._light_types):
             lst = []
             for _ in types:
-                names = self._names[k]
-                lst.append({nm: (g[:, k, i] if n_img > 1 else float(g[0, k, i])) for i, nm in enumerate(names)})
+                lst.append({nm: (g[:, k, i] if n_img > 1 else float(g[0, k, i]))
+                            for i, nm in profiles.kwarg_slots(self._names[k])})
                 k += 1
             out.append(lst)
         return (logl if n_img > 1 else logl[0]), out[0], out[1], out[2]

@@ -3,6 +3,10 @@
 Mirrors the reference's ``LensModel`` (LensModel/lens_model.py:25-252) for the
 single-plane profiles EPL, SIE, NIE, NFW, TNFW and SHEAR.  ``ray_shooting``
 runs ``lensim_raytrace`` (one HIP kernel over rays x parameter sets).
+``profile_kwargs_list`` carries the profiles' constructor options
+(profiles.lens_layout: SIE's NIE / s_scale / gamma, NFW's interpol), and a
+source redshift other than the convention's scales the deflections by
+alpha_scaling (lens_model.py:176-195).
 Parameter values may be scalars (one model) or 1-D arrays of length B (a batch
 of B models evaluated at the same coordinates, like ``jax.vmap`` over kwargs);
 a batch adds a leading axis of length B to the outputs.
@@ -11,6 +15,7 @@ a batch adds a leading axis of length B to the outputs.
 import torch
 
 from .. import backend, profiles
+from ..Util.cosmo import Background
 
 
 class LensModel(object):
@@ -30,24 +35,36 @@ class LensModel(object):
             raise NotImplementedError("multi-plane ray shooting is not part of this library")
         if observed_convention_index is not None:
             raise NotImplementedError("observed_convention_index needs multi-plane lensing")
-        if z_source is not None and z_source_convention is not None and z_source != z_source_convention:
-            raise NotImplementedError("alpha_scaling from a cosmology (z_source != z_source_convention) "
-                                      "is not supported; pass alpha_scaling to the kwargs instead")
+        if cosmology_model != "FlatLambdaCDM" and cosmo is None:
+            raise ValueError("only FlatLambdaCDM cosmologies are available")
         if precision not in backend.PRECISIONS:
             raise ValueError("precision must be one of %s" % list(backend.PRECISIONS))
         self.lens_model_list = list(lens_model_list)
         self.z_lens = z_lens
         self.z_source = z_source
+        self._z_source_convention = z_source_convention
         self.redshift_list = lens_redshift_list
         self.multi_plane = False
         self.type = "SinglePlane"
         self.precision = precision
+        if profile_kwargs_list is None:
+            profile_kwargs_list = [{} for _ in self.lens_model_list]
+        if len(profile_kwargs_list) != len(self.lens_model_list):
+            raise ValueError("profile_kwargs_list has %d entries for %d lens models"
+                             % (len(profile_kwargs_list), len(self.lens_model_list)))
+        self.profile_kwargs_list = profile_kwargs_list
         self.alpha_scaling = 1.0
+        if z_source is not None and z_source_convention is not None and z_source != z_source_convention:
+            if z_lens is None:
+                raise ValueError("a lens redshift needs to provided when z_source != z_source_convention")
+            self.alpha_scaling = beta_double_source_plane(Background(cosmo), z_lens, z_source, z_source_convention)
         self._types = []
+        self._layouts = []
         self.param_name_list = []
-        for name in self.lens_model_list:
-            code, names = profiles.lens_type(name)
+        for name, pk in zip(self.lens_model_list, profile_kwargs_list):
+            code, layout, names = profiles.lens_layout(name, pk)
             self._types.append(code)
+            self._layouts.append(layout)
             self.param_name_list.append(names)
         self._ws = backend.Workspace()
 
@@ -70,7 +87,7 @@ class LensModel(object):
         idx = profiles.select(k, len(self._types))
         kwl = [kwargs[i] for i in idx]
         n_img = profiles.batch_size([kwargs])
-        params = profiles.pack([self.param_name_list[i] for i in idx], kwl, n_img)
+        params = profiles.pack([self._layouts[i] for i in idx], kwl, n_img)
         model = backend.make_model([self._types[i] for i in idx], [], [], alpha_scaling=self.alpha_scaling)
         return model, params, n_img
 
@@ -113,6 +130,18 @@ class LensModel(object):
         xt = backend.to_device_f64(x)
         yt = backend.to_device_f64(y)
         return xt - bx, yt - by
+
+
+def beta_double_source_plane(background, z_lens, z_source_1, z_source_2):
+    """Ratio of the reduced deflections for sources at z_source_1 and at
+    z_source_2 (lenstronomy LensCosmo.beta_double_source_plane, as called at
+    LensModel/lens_model.py:180-189): D_ds1 / D_s1 * D_s2 / D_ds2, angular
+    diameter distances of the background cosmology."""
+    ds1 = background.d_xy(0.0, z_source_1)
+    dds1 = background.d_xy(z_lens, z_source_1)
+    ds2 = background.d_xy(0.0, z_source_2)
+    dds2 = background.d_xy(z_lens, z_source_2)
+    return float(dds1 / ds1 * ds2 / dds2)
 
 
 def ctypes_ref(struct):

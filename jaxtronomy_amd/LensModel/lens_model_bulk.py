@@ -43,6 +43,17 @@ class LensModelBulk(object):
             if name not in _BULK_OK and name != "EPL":
                 raise ValueError("%s is not supported in LensModelBulk" % name)
         self.unique_lens_model_list = list(unique_lens_model_list)
+        if profile_kwargs_list is not None:
+            # the constructor options of each unique profile (single_plane_bulk.py:30-36): the ones that
+            # keep the profile's default evaluation are accepted, the others raise
+            if len(profile_kwargs_list) != len(self.unique_lens_model_list):
+                raise ValueError("profile_kwargs_list has %d entries for %d unique lens models"
+                                 % (len(profile_kwargs_list), len(self.unique_lens_model_list)))
+            for name, pk in zip(self.unique_lens_model_list, profile_kwargs_list):
+                code, layout, names = profiles.lens_layout(name, pk)
+                if code != profiles.LENS_PROFILES[name][0] or layout != names:
+                    raise NotImplementedError("profile options %s of %s are not supported in LensModelBulk"
+                                              % (pk, name))
         self.param_name_list = [profiles.LENS_PROFILES[n][1] for n in self.unique_lens_model_list]
         self.multi_plane = bool(multi_plane)
         self.type = "MultiPlaneBulk" if self.multi_plane else "SinglePlaneBulk"

@@ -2,9 +2,12 @@
 
 Mirrors the reference's sampling entry points for the imaging likelihood:
 ``LikelihoodModule.logL`` (Sampling/likelihood.py:290-373: parameter vector
--> kwargs, hard bounds -> -1e18, the image likelihood with NaN -> -1e15 as in
-ImageLikelihood.logL, Sampling/Likelihoods/image_likelihood.py:57-95, plus an
-optional custom addition) and ``prepare_logL_func`` (Sampling/sampler.py:215-251),
+-> kwargs, hard bounds -> -1e18; otherwise the custom addition, the Gaussian
+priors (lenstronomy's PriorLikelihood, likelihood.py:153-172, :353-356) and
+the image likelihood with NaN -> -1e15 as in ImageLikelihood.logL,
+Sampling/Likelihoods/image_likelihood.py:57-95, summed in that order, and the
+sum's NaN -> 1e-18, +-inf -> +-max float, likelihood.py:370) and
+``prepare_logL_func`` (Sampling/sampler.py:215-251),
 which vectorises it over a set of walkers / particles.  Here the whole set is
 one batched device call: the parameter vectors are cast into the packed
 parameter block on the host in one vectorised step and evaluated by
@@ -24,6 +27,28 @@ from .. import backend, profiles
 
 BOUND_PENALTY = -1e18  # likelihood.py:312-315
 NAN_LOGL = -1e15       # image_likelihood.py:93
+NAN_TOTAL = 1e-18      # likelihood.py:370
+
+
+def _gaussian_priors(prior_list, group, image_model, packed):
+    """sum over [index, param_name, mean, sigma] entries of -(value - mean)^2 / sigma^2 / 2
+    (lenstronomy PriorLikelihood's Gaussian priors) for each packed row [N]."""
+    out = np.zeros(packed.shape[0])
+    if not prior_list:
+        return out
+    offset = [0, len(image_model._lens_types), len(image_model._lens_types) + len(image_model._src_types)][group]
+    n_group = [len(image_model._lens_types), len(image_model._src_types), len(image_model._light_types)][group]
+    for entry in prior_list:
+        index, name, mean, sigma = entry
+        if not 0 <= int(index) < n_group:
+            raise ValueError("prior index %d out of range for %d profiles" % (index, n_group))
+        k = offset + int(index)
+        slots = dict((nm, q) for q, nm in profiles.kwarg_slots(image_model._names[k]))
+        if name not in slots:
+            raise ValueError("prior on %s, which profile %d does not have" % (name, index))
+        value = packed[:, k, slots[name]]
+        out -= (value - float(mean)) ** 2 / float(sigma) ** 2 / 2.0
+    return out
 
 
 class Param(object):
@@ -49,8 +74,10 @@ class Param(object):
             lw = lower[group] or [{} for _ in types]
             up = upper[group] or [{} for _ in types]
             for i, _ in enumerate(types):
-                names = image_model._names[k]
-                for q, nm in enumerate(names):
+                for q, nm in enumerate(image_model._names[k]):
+                    if not isinstance(nm, str):  # a slot fixed by the profile's options
+                        self._base[k, q] = float(nm[1])
+                        continue
                     if nm in fx[i]:
                         self._base[k, q] = float(fx[i][nm])
                         continue
@@ -90,7 +117,7 @@ class Param(object):
         for types in (image_model._lens_types, image_model._src_types, image_model._light_types):
             lst = []
             for _ in types:
-                lst.append({nm: float(params_row[k, q]) for q, nm in enumerate(image_model._names[k])})
+                lst.append({nm: float(params_row[k, q]) for q, nm in profiles.kwarg_slots(image_model._names[k])})
                 k += 1
             out.append(lst)
         return {"kwargs_lens": out[0], "kwargs_source": out[1], "kwargs_lens_light": out[2], "kwargs_ps": None,
@@ -111,7 +138,15 @@ class LikelihoodModule(object):
     """Batched LikelihoodModule.logL for the imaging likelihood."""
 
     def __init__(self, image_model, param, check_bounds=True, source_marg=False, linear_prior=None,
-                 check_positive_flux=False, custom_logL_addition=None):
+                 check_positive_flux=False, custom_logL_addition=None, prior_lens=None, prior_source=None,
+                 prior_lens_light=None, prior_ps=None, prior_special=None, prior_extinction=None, **kwargs):
+        if prior_ps or prior_special or prior_extinction:
+            raise NotImplementedError("priors on point sources, special or extinction parameters: those models "
+                                      "are not part of this library")
+        for key, value in kwargs.items():
+            if value is not None and value is not False:
+                raise NotImplementedError("LikelihoodModule option %s is not supported" % key)
+        self._priors = [(prior_lens, 0), (prior_source, 1), (prior_lens_light, 2)]
         self.imSim = image_model
         self.param = param
         self._check_bounds = check_bounds
@@ -135,13 +170,23 @@ class LikelihoodModule(object):
                 logl = torch.where(torch.any(r["amp"] < 0, dim=1), logl - 1e8, logl)
         else:
             logl = self.imSim.likelihood_batch(params)
-        logl = torch.nan_to_num(logl, nan=NAN_LOGL).cpu().numpy()
-        if self._custom is not None:  # custom_logL_addition(**kwargs_return), likelihood.py:349-351
-            logl = logl + np.array([self._custom(**self.param.params2kwargs(row, self.imSim)) for row in packed])
+        # ImageLikelihood.logL: NaN -> -1e15, +-inf -> +-max float
+        image = torch.nan_to_num(logl, nan=NAN_LOGL).cpu().numpy()
+        # LikelihoodModule.log_likelihood (likelihood.py:346-371): custom addition, priors, image, in that order
+        total = np.zeros(args.shape[0])
+        if self._custom is not None:
+            total = total + np.array([self._custom(**self.param.params2kwargs(row, self.imSim)) for row in packed],
+                                     dtype=np.float64)
+        prior = np.zeros(args.shape[0])
+        for prior_list, group in self._priors:
+            prior += _gaussian_priors(prior_list, group, self.imSim, packed)
+        total = total + prior
+        total = total + image
+        total = np.nan_to_num(total, nan=NAN_TOTAL)
         if self._check_bounds:
             hit = np.any((args < self._lower) | (args > self._upper), axis=1)
-            logl = np.where(hit, BOUND_PENALTY, logl)
-        return logl
+            total = np.where(hit, BOUND_PENALTY, total)
+        return total
 
     def __call__(self, a):
         return self.logL(a)

@@ -15,7 +15,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 NPARAM = 8
 MAX_PROFILES = 64
 EPL_NMAX = 500
@@ -121,6 +121,10 @@ _SIGNATURES = {
     "lensim_grad_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                            ctypes.c_size_t, _P]),
+    "lensim_graph_capture_loglike": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
+                                                    _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_void_p)]),
+    "lensim_graph_launch": (ctypes.c_int, [_P, _P]),
+    "lensim_graph_destroy": (ctypes.c_int, [_P]),
     "lensim_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "lensim_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int64)]),

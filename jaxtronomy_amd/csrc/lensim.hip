@@ -78,8 +78,12 @@ hipEvent_t timing_event(int slot) {
   return c.ev[slot];
 }
 
+// while a call is captured into a graph (lensim_graph_capture_loglike) no
+// timing events are recorded: they would become nodes of the graph
+thread_local bool g_capturing = false;
+
 void timing_mark(int slot, hipStream_t st) {
-  if (!g_timing) return;
+  if (!g_timing || g_capturing) return;
   hipEvent_t e = timing_event(slot);
   if (e) (void)hipEventRecord(e, st);
 }
@@ -89,6 +93,7 @@ void timing_mark(int slot, hipStream_t st) {
 // types[] holds the inline lens, source and lens-light types (no bulk).
 struct ModelArgs {
   int32_t n_lens, n_bulk, n_defl, n_source, n_light, n_prof, n_epl, has_halo, bulk_type;
+  int32_t has_nie_nfw;  // an SIE / NIE or NFW deflector (inline or bulk): the float form deflects them in double
   const int32_t *bulk_types;
   int32_t types[LENSIM_MAX_PROFILES];
   int32_t epl_index[LENSIM_MAX_PROFILES];
@@ -329,6 +334,7 @@ int model_args(const lensim_model *model, ModelArgs &m) {
       return fail(LENSIM_EUNSUPPORTED, "profile %d: type %d is not a supported light profile", k, t);
     if (t == LENSIM_EPL) m.epl_index[k] = m.n_epl++;
     if (t == LENSIM_NFW || t == LENSIM_TNFW) m.has_halo = 1;
+    if (lens && (t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW)) m.has_nie_nfw = 1;
   }
   if (m.n_bulk > 0) {
     m.bulk_type = model->bulk_type;
@@ -339,6 +345,9 @@ int model_args(const lensim_model *model, ModelArgs &m) {
     if (m.bulk_type == LENSIM_NULL && !m.bulk_types)
       return fail(LENSIM_EINVAL, "mixed bulk deflectors need d_bulk_types");
     if (m.bulk_type == LENSIM_NFW || m.bulk_type == LENSIM_TNFW || m.bulk_type == LENSIM_NULL) m.has_halo = 1;
+    if (m.bulk_type == LENSIM_SIE || m.bulk_type == LENSIM_NIE || m.bulk_type == LENSIM_NFW ||
+        m.bulk_type == LENSIM_NULL)
+      m.has_nie_nfw = 1;
   }
   m.alpha_scaling = model->alpha_scaling;
   m.flux_scaling = model->flux_scaling;
@@ -448,7 +457,7 @@ __device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double 
   return r;
 }
 
-template <typename T, bool HALO>
+template <typename T, bool HALO, bool DBL = false>
 __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
                                             double &ax, double &ay) {
   for (int k = 0; k < m.n_lens; ++k) {
@@ -456,7 +465,7 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
-    add_deflection<T, HALO>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+    add_deflection<T, HALO, DBL>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   }
   if (m.n_bulk == 0) return;
   if (m.bulk_type == LENSIM_TNFW) {
@@ -470,34 +479,34 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
   for (int k = m.n_lens; k < m.n_defl; ++k) {
     const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
-    add_deflection<T, HALO>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+    add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
 }
 
 // One deflector k (inline lens k < n_lens with its series coefficients, or bulk).
-template <typename T, bool HALO>
+template <typename T, bool HALO, bool DBL>
 __device__ __forceinline__ void deflect_one(const ModelArgs &m, const RecPtrs<T> &rp, int img, int k, double x,
                                             double y, double &ax, double &ay) {
   const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
   if (k < m.n_lens) {
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
-    add_deflection<T, HALO>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+    add_deflection<T, HALO, DBL>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   } else {
     const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
-    add_deflection<T, HALO>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+    add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
 }
 
 // Source-plane position of the ray at angle (x, y): single plane
 // beta = theta - alpha_scaling * alpha, or the multi-plane recursion of
 // MultiPlaneBulk.ray_shooting (LensModel/MultiPlane/multi_plane_bulk.py:152-212).
-template <typename T, bool HALO>
+template <typename T, bool HALO, bool DBL>
 __device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
                                           double &bx, double &by) {
   if (m.mp_n == 0) {
     double ax = 0.0, ay = 0.0;
-    deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
+    deflect_all<T, HALO, DBL>(m, rp, img, x, y, ax, ay);
     bx = x - ax * m.alpha_scaling;
     by = y - ay * m.alpha_scaling;
     return;
@@ -513,7 +522,7 @@ __device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &
       double ax = 0.0, ay = 0.0;
       int k = m.mp_order[p];
       k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
-      deflect_one<T, HALO>(m, rp, img, k, px / tz, py / tz, ax, ay);
+      deflect_one<T, HALO, DBL>(m, rp, img, k, px / tz, py / tz, ax, ay);
       const double f = m.mp_r2p[o + p];
       angx -= ax * f;
       angy -= ay * f;
@@ -549,7 +558,7 @@ __device__ __forceinline__ bool evaluated(const GridArgs &g, int i, int j) {
   return !g.mask || g.mask[size_t(j / g.mask_s) * g.mask_nx + i / g.mask_s] != 0;
 }
 
-template <typename T, bool HALO>
+template <typename T, bool HALO, bool DBL>
 __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
                                                      const float *__restrict__ recf,
                                                      const double *__restrict__ coefd,
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
         double bx, by;
-        trace_ray<T, HALO>(m, rp, img, x, y, bx, by);
+        trace_ray<T, HALO, DBL>(m, rp, img, x, y, bx, by);
         acc += light_sum<T>(m, rp, k_src0, k_src1, bx, by);
       }
       if (do_light) acc += light_sum<T>(m, rp, k_src1, k_l1, x, y);
@@ -700,8 +709,8 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
 int check_launch(const char *where);
 
 // Launches render_fast_kernel when the model is one of its compile-time lists
-// (EPL, SIE / NIE each with or without SHEAR after it, NFW + SIE / NIE in
-// either order; one Sersic source, none or one Sersic lens light), else
+// (EPL, SIE / NIE or NFW, each with or without SHEAR after it, NFW + SIE / NIE
+// in either order; one Sersic source, none or one Sersic lens light), else
 // render_kernel.
 template <typename T>
 int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, const float *recf, const double *coefd,
@@ -715,16 +724,29 @@ int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, con
   if (fast) {
     d0 = nie(m.types[0]);
     d1 = m.n_lens == 2 ? nie(m.types[1]) : 0;
-    const bool pair = ((d0 == LENSIM_EPL || d0 == LENSIM_NIE) && (d1 == 0 || d1 == LENSIM_SHEAR)) ||
+    const bool pair = ((d0 == LENSIM_EPL || d0 == LENSIM_NIE || d0 == LENSIM_NFW) && (d1 == 0 || d1 == LENSIM_SHEAR)) ||
                       (d0 == LENSIM_NFW && d1 == LENSIM_NIE) || (d0 == LENSIM_NIE && d1 == LENSIM_NFW);
     fast = pair && (d0 != LENSIM_EPL || m.epl_index[0] == 0);
     for (int k = m.n_lens; fast && k < m.n_lens + m.n_source + m.n_light; ++k) fast = sersic(m.types[k]);
   }
   if (!fast) {
-    if (m.has_halo)
-      render_kernel<T, true><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax);
-    else
-      render_kernel<T, false><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax);
+    // SIE / NIE and NFW deflect in double in the float form too; the kernel
+    // is instantiated both ways so other models keep the float forms' registers
+#define LENSIM_RENDER(H, D) \
+  render_kernel<T, H, D><<<rgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out, blkmax)
+    bool done = false;
+    if constexpr (sizeof(T) == sizeof(float)) {
+      if (m.has_nie_nfw) {
+        if (m.has_halo) LENSIM_RENDER(true, true);
+        else LENSIM_RENDER(false, true);
+        done = true;
+      }
+    }
+    if (!done) {
+      if (m.has_halo) LENSIM_RENDER(true, false);
+      else LENSIM_RENDER(false, false);
+    }
+#undef LENSIM_RENDER
     return check_launch("render_kernel");
   }
   const bool ll = m.n_light == 1;
@@ -740,12 +762,14 @@ int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, con
   else if (d0 == LENSIM_NIE && d1 == LENSIM_SHEAR) LENSIM_FAST(LENSIM_NIE, LENSIM_SHEAR);
   else if (d0 == LENSIM_NIE && d1 == LENSIM_NFW) LENSIM_FAST(LENSIM_NIE, LENSIM_NFW);
   else if (d0 == LENSIM_NIE) LENSIM_FAST(LENSIM_NIE, 0);
+  else if (d0 == LENSIM_NFW && d1 == LENSIM_SHEAR) LENSIM_FAST(LENSIM_NFW, LENSIM_SHEAR);
+  else if (d0 == LENSIM_NFW && d1 == 0) LENSIM_FAST(LENSIM_NFW, 0);
   else LENSIM_FAST(LENSIM_NFW, LENSIM_NIE);
 #undef LENSIM_FAST
   return check_launch("render_fast_kernel");
 }
 
-template <typename T, bool HALO>
+template <typename T, bool HALO, bool DBL>
 __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double *__restrict__ recd,
                                                        const float *__restrict__ recf,
                                                        const double *__restrict__ coefd,
@@ -757,7 +781,7 @@ __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double
   if (i >= n_ray) return;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   double tx, ty;
-  trace_ray<T, HALO>(m, rp, img, xs[i], ys[i], tx, ty);
+  trace_ray<T, HALO, DBL>(m, rp, img, xs[i], ys[i], tx, ty);
   bx[size_t(img) * n_ray + i] = tx;
   by[size_t(img) * n_ray + i] = ty;
 }
@@ -2565,21 +2589,19 @@ int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n
   const float *recf = reinterpret_cast<const float *>(w + L.recf);
   const double *coefd = reinterpret_cast<const double *>(w + L.coefd);
   const float *coeff = reinterpret_cast<const float *>(w + L.coeff);
+#define LENSIM_TRACE(T, H, D) \
+  raytrace_kernel<T, H, D><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x, d_beta_y)
   if (precision == LENSIM_FP64) {
-    if (m.has_halo)
-      raytrace_kernel<double, true><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
-                                                          d_beta_y);
-    else
-      raytrace_kernel<double, false><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
-                                                           d_beta_y);
+    if (m.has_halo) LENSIM_TRACE(double, true, false);
+    else LENSIM_TRACE(double, false, false);
+  } else if (m.has_nie_nfw) {  // SIE / NIE and NFW deflect in double (see nie_defl)
+    if (m.has_halo) LENSIM_TRACE(float, true, true);
+    else LENSIM_TRACE(float, false, true);
   } else {
-    if (m.has_halo)
-      raytrace_kernel<float, true><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
-                                                         d_beta_y);
-    else
-      raytrace_kernel<float, false><<<grid, 256, 0, st>>>(m, recd, recf, coefd, coeff, d_x, d_y, n_ray, d_beta_x,
-                                                          d_beta_y);
+    if (m.has_halo) LENSIM_TRACE(float, true, false);
+    else LENSIM_TRACE(float, false, false);
   }
+#undef LENSIM_TRACE
   return check_launch("raytrace_kernel");
 }
 
@@ -2836,6 +2858,69 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (precision == LENSIM_FP64)
     return run_grad<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
   return run_grad<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+}
+
+// ------------------------------------------------------ graph capture ---
+struct lensim_graph_s {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+int lensim_graph_capture_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                                 const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                                 double *d_logl, void *d_model_out, void *ws, size_t ws_bytes, void *stream,
+                                 lensim_graph *out) {
+  if (!out) return fail(LENSIM_EINVAL, "graph handle pointer is NULL");
+  *out = nullptr;
+  if (!stream) return fail(LENSIM_EINVAL, "graph capture needs a stream other than the NULL stream");
+  if (n_img <= 0) return fail(LENSIM_EINVAL, "graph capture needs n_img > 0");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+  g_capturing = true;
+  const int rc = lensim_loglike(model, grid, psf, data, d_params, n_img, precision, d_logl, d_model_out, ws,
+                                ws_bytes, stream);
+  g_capturing = false;
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(st, &graph);  // ends the capture whatever the call returned
+  if (rc != LENSIM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  lensim_graph g = static_cast<lensim_graph>(malloc(sizeof(lensim_graph_s)));
+  if (!g) {
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return fail(LENSIM_EINVAL, "out of host memory");
+  }
+  g->graph = graph;
+  g->exec = exec;
+  *out = g;
+  return LENSIM_OK;
+}
+
+int lensim_graph_launch(lensim_graph g, void *stream) {
+  if (!g) return fail(LENSIM_EINVAL, "graph is NULL");
+  const hipError_t e = hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  return LENSIM_OK;
+}
+
+int lensim_graph_destroy(lensim_graph g) {
+  if (!g) return LENSIM_OK;
+  const hipError_t e1 = hipGraphExecDestroy(g->exec);
+  const hipError_t e2 = hipGraphDestroy(g->graph);
+  free(g);
+  if (e1 != hipSuccess) return hip_fail(e1, "hipGraphExecDestroy");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipGraphDestroy");
+  return LENSIM_OK;
 }
 
 }  // extern "C"

@@ -549,14 +549,14 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
 }
 
 // NIE / SIE deflection, LensModel/Profiles/nie.py:86-107, :280-295, :333-342.
-// DBL (the fast render kernels, whose profile list is fixed at compile time):
-// the float form runs the FP64 form's arithmetic instead.  Its float
-// atan / atanh leave one-to-two-ulp random errors in alpha, which a compact
-// cuspy source (Sersic n ~ 4, R_sersic 0.17") behind an NFW + SIE lens turned
-// into a log-likelihood error of 1.8e-6 relative.  Kernels that switch over
-// the profile type at run time (bulk lists, multi-plane, gradients) keep the
-// float form: the double library atan / atanh would raise every such
-// kernel's registers (occupancy 4 -> 2 waves per SIMD) whatever its model.
+// DBL (every render and ray-tracing kernel of a model that holds an SIE / NIE
+// or NFW deflector): the float form runs the FP64 form's arithmetic instead.
+// Its float atan / atanh leave one-to-two-ulp random errors in alpha, which a
+// compact cuspy source (Sersic n ~ 4, R_sersic 0.17") behind an NFW + SIE lens
+// turned into a log-likelihood error of 1.8e-6 relative.  The kernels are
+// instantiated both ways, so the double library atan / atanh cost registers
+// only to the models that use them.  The gradient kernels keep the float form
+// (their bar is 1e-4).
 template <typename T, bool DBL = false>
 __device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
   const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
@@ -668,7 +668,7 @@ __device__ __forceinline__ float nfw_g(float X) {
 }
 
 // NFW deflection, LensModel/Profiles/nfw.py:86-105 with nfw_alpha :257-279.
-// DBL (fast render kernels): the float form runs in double too.  The float
+// DBL (kernels of models that hold an NFW): the float form runs in double too.  The float
 // g(X) = ln(X/2) + h(X) cancels around X ~ 1 (0.31 at X = 1 from terms of 0.7
 // and 1), where an NFW main deflector's Einstein ring sits; with a compact
 // cuspy source that moved the log-likelihood by 2.4e-6 relative.
@@ -805,8 +805,10 @@ __device__ __forceinline__ void shear_defl(const double *rd, double x, double y,
 // Adds one deflector's alpha at (x, y).  coef_d / coef_t: this profile's EPL
 // series coefficients (NULL for other profiles).
 // HALO: whether the NFW / TNFW branches are compiled in (kernels for models
-// without them keep fewer registers).
-template <typename T, bool HALO>
+// without them keep fewer registers).  DBL: SIE / NIE and NFW run the FP64
+// form's arithmetic in the float form too (kernels for models that hold
+// them; see nie_defl and nfw_defl).
+template <typename T, bool HALO, bool DBL = false>
 __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const double *__restrict__ coef_d,
                                                const T *__restrict__ coef_t, double x, double y, double &ax,
                                                double &ay) {
@@ -819,10 +821,10 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
       break;
     case LENSIM_SIE:
     case LENSIM_NIE:
-      nie_defl<T>(r, x, y, ax, ay);
+      nie_defl<T, DBL>(r, x, y, ax, ay);
       break;
     case LENSIM_NFW:
-      if constexpr (HALO) nfw_defl<T>(r, x, y, ax, ay);
+      if constexpr (HALO) nfw_defl<T, DBL>(r, x, y, ax, ay);
       break;
     case LENSIM_TNFW:
       if constexpr (HALO) tnfw_defl<T>(r, x, y, ax, ay);

@@ -30,11 +30,63 @@ LIGHT_PROFILES = {
 # keyword arguments with a default value in the reference's function signatures
 _DEFAULTS = {"center_x": 0.0, "center_y": 0.0, "ra_0": 0.0, "dec_0": 0.0}
 
+SIE_S_SCALE = 0.0000000001  # SIE's default core, LensModel/Profiles/sie.py:56
+
 
 def lens_type(name):
     if name not in LENS_PROFILES:
         raise ValueError("%s is not a supported lens model. Supported are: %s." % (name, sorted(LENS_PROFILES)))
     return LENS_PROFILES[name]
+
+
+def lens_layout(name, profile_kwargs=None):
+    """Device type and parameter-slot layout of lens profile ``name`` built with
+    ``profile_kwargs``, the constructor options of lens_class
+    (LensModel/profile_list_base.py:309-765).
+
+    Returns (type code, layout, param_names).  ``layout`` lists the slot
+    contents in order: a kwarg name, or (name, value) for a slot that the
+    option fixes to a constant.  Options:
+      * SIE(s_scale, gamma, NIE) (sie.py:56-124): NIE=False evaluates the EPL
+        with the fixed slope ``gamma`` (default 2); a non-default s_scale is
+        the NIE with that core; otherwise the SIE;
+      * NFW(interpol): interpol=True raises, as the reference does (nfw.py:60-63);
+        its other keywords are accepted and unused, as there;
+      * EPL / NIE (b, t|s, q, phi, static): static=True (the profile evaluated
+        at stored constants instead of its kwargs) is not supported;
+      * other profiles take no options (their constructors take none).
+    """
+    code, names = lens_type(name)
+    pk = dict(profile_kwargs or {})
+    layout = list(names)
+    if name == "SIE":
+        unknown = set(pk) - {"s_scale", "gamma", "NIE"}
+        if unknown:
+            raise TypeError("SIE got unexpected profile keyword arguments %s" % sorted(unknown))
+        if not pk.get("NIE", True):
+            layout = ["theta_E", ("gamma", float(pk.get("gamma", 2))), "e1", "e2", "center_x", "center_y"]
+            code = LENS_PROFILES["EPL"][0]
+        elif float(pk.get("s_scale", SIE_S_SCALE)) != SIE_S_SCALE:
+            layout = ["theta_E", "e1", "e2", ("s_scale", float(pk["s_scale"])), "center_x", "center_y"]
+            code = LENS_PROFILES["NIE"][0]
+    elif name in ("EPL", "NIE"):
+        allowed = {"b", "t", "q", "phi", "static"} if name == "EPL" else {"b", "s", "q", "phi", "static"}
+        unknown = set(pk) - allowed
+        if unknown:
+            raise TypeError("%s got unexpected profile keyword arguments %s" % (name, sorted(unknown)))
+        if pk.get("static", False):
+            raise NotImplementedError("%s(static=True) is not supported: pass the parameters as kwargs" % name)
+    elif name == "NFW":
+        if pk.get("interpol", False):
+            raise ValueError("This class no longer supports interpol functionality in JAXtronomy.")
+    elif pk:
+        raise TypeError("%s takes no profile keyword arguments (got %s)" % (name, sorted(pk)))
+    return code, layout, list(names)
+
+
+def kwarg_slots(layout):
+    """(slot index, kwarg name) of the slots of a layout that hold kwargs."""
+    return [(q, nm) for q, nm in enumerate(layout) if isinstance(nm, str)]
 
 
 def light_type(name):
@@ -67,19 +119,24 @@ def batch_size(kwargs_lists):
 def pack(names_lists, kwargs_lists, n_img):
     """Pack kwargs into a float64 [n_img][n_profiles][NPARAM] parameter block.
 
-    names_lists: per profile, its ordered parameter names.  Values may be
-    python scalars, numpy arrays or torch tensors (scalar or length n_img).
-    Missing parameters without a reference default raise TypeError, unknown
-    ones raise ValueError, as the reference's profile calls do.
+    names_lists: per profile, its slot layout: ordered parameter names, or
+    (name, value) for a slot fixed to a constant (see lens_layout).  Values
+    may be python scalars, numpy arrays or torch tensors (scalar or length
+    n_img).  Missing parameters without a reference default raise TypeError,
+    unknown ones raise ValueError, as the reference's profile calls do.
     """
     n_prof = len(names_lists)
     cols = []
     for names, kw in zip(names_lists, kwargs_lists):
+        accepted = [nm for nm in names if isinstance(nm, str)]
         for key in kw:
-            if key not in names:
-                raise ValueError("parameter %s is not part of profile parameters %s" % (key, names))
+            if key not in accepted:
+                raise ValueError("parameter %s is not part of profile parameters %s" % (key, accepted))
         slot = []
         for name in names:
+            if not isinstance(name, str):
+                slot.append(np.full(n_img, float(name[1])))
+                continue
             if name in kw:
                 v = kw[name]
             elif name in _DEFAULTS:

@@ -27,6 +27,11 @@ Pinning (how each function is checked independently of this file):
   convolve_same          scipy.signal.fftconvolve(..., 'same')
   gaussian_convolve      scipy.ndimage.gaussian_filter(mode='nearest')
   log_likelihood         hand-computed 3x3 case
+  gaussian_prior_logl    scipy.stats.norm.logpdf plus its normalisation
+  likelihood_module_logl hand-built cases of each branch (bounds, NaN, +-inf)
+  beta_double_source_plane  1 when the two sources coincide; the ratio of
+                         deflections of an SIS whose Einstein radius is set by
+                         the distances (scipy quadrature)
 """
 
 import numpy as np
@@ -42,6 +47,7 @@ __all__ = [
     "coordinate_grid", "make_subgrid", "subgrid_coordinates", "re_size",
     "convolve_same", "gaussian_kernel", "gaussian_convolve",
     "moffat_kernel", "image", "log_likelihood", "c_d_model",
+    "gaussian_prior_logl", "likelihood_module_logl", "beta_double_source_plane",
 ]
 
 # --------------------------------------------------------------------------
@@ -683,3 +689,40 @@ def multi_plane_ray_shooting(x, y, lens_model_list, kwargs_lens, T_ij_list, T_z_
             ax = ax - fx * reduced2physical_factor[p]
             ay = ay - fy * reduced2physical_factor[p]
     return px / T_z_list[-1], py / T_z_list[-1]
+
+
+# ---------------------------------------------------------------------------
+# Sampling/likelihood.py: the sampler-facing log-likelihood of one vector
+
+def gaussian_prior_logl(kwargs_list, prior_list):
+    """lenstronomy PriorLikelihood's Gaussian prior of one kwargs list (as constructed at
+    Sampling/likelihood.py:153-172 and evaluated at :353-356): for each
+    [index, param_name, mean, sigma], -(kwargs_list[index][param_name] - mean)^2 / sigma^2 / 2."""
+    logl = 0.0
+    for index, name, mean, sigma in prior_list or []:
+        logl -= (kwargs_list[index][name] - mean) ** 2 / sigma ** 2 / 2.0
+    return logl
+
+
+def likelihood_module_logl(args, lower, upper, image_logl, prior_logl=0.0, custom_logl=0.0, check_bounds=True):
+    """LikelihoodModule.logL / log_likelihood, Sampling/likelihood.py:290-371, for the
+    imaging likelihood of one parameter vector: a bound hit gives -1e18
+    (:312-318); otherwise the custom addition, the prior and the image
+    likelihood (NaN -> -1e15, image_likelihood.py:93) are summed in that order
+    and the sum's NaN becomes 1e-18, +-inf the largest finite value (:370)."""
+    args = np.asarray(args, float)
+    if check_bounds and bool(np.any((args < lower) | (args > upper))):
+        return -1e18
+    logl = 0.0
+    logl += custom_logl
+    logl += prior_logl
+    logl += float(np.nan_to_num(image_logl, nan=-1e15))
+    return float(np.nan_to_num(logl, nan=1e-18))
+
+
+def beta_double_source_plane(d_xy, z_lens, z_source_1, z_source_2):
+    """Deflection ratio for a source at z_source_1 when the kwargs are normalised
+    for z_source_2 (lenstronomy LensCosmo.beta_double_source_plane, called at
+    LensModel/lens_model.py:180-189): D_ds1 / D_s1 * D_s2 / D_ds2 with the
+    angular diameter distance function d_xy(z_observer, z_source)."""
+    return d_xy(z_lens, z_source_1) / d_xy(0.0, z_source_1) * d_xy(0.0, z_source_2) / d_xy(z_lens, z_source_2)

@@ -49,3 +49,139 @@ def test_param_linear_solver_drops_amplitudes():
     np.testing.assert_array_equal(f(np.ones((3, 2))), [2.0, 2.0, 2.0])
     with pytest.raises(ValueError):
         prepare_logL_func("tpu", None)
+
+
+def test_gaussian_prior_oracle_pinned_by_scipy():
+    """The oracle's Gaussian prior equals the normal log-density without its
+    normalisation, summed over entries (two profiles, three entries)."""
+    from scipy.stats import norm
+
+    from oracle import lensim_oracle as O
+
+    kw = [{"theta_E": 1.3, "gamma": 2.1}, {"gamma1": 0.02}]
+    priors = [[0, "theta_E", 1.2, 0.1], [0, "gamma", 2.0, 0.05], [1, "gamma1", 0.0, 0.03]]
+    want = sum(norm.logpdf(kw[i][nm], mu, s) + np.log(s * np.sqrt(2 * np.pi)) for i, nm, mu, s in priors)
+    assert abs(O.gaussian_prior_logl(kw, priors) - want) < 1e-12
+    assert O.gaussian_prior_logl(kw, None) == 0.0
+
+
+def test_likelihood_module_oracle_branches():
+    """Each branch of the reference's LikelihoodModule.logL on hand-built values."""
+    from oracle import lensim_oracle as O
+
+    lo, hi = np.array([0.0, -1.0]), np.array([2.0, 1.0])
+    assert O.likelihood_module_logl([3.0, 0.0], lo, hi, -10.0) == -1e18          # bound hit
+    assert O.likelihood_module_logl([1.0, 0.0], lo, hi, -10.0, -2.0, 5.0) == -7.0  # custom + prior + image
+    assert O.likelihood_module_logl([1.0, 0.0], lo, hi, np.nan, -2.0) == -1e15 - 2.0  # image NaN -> -1e15
+    assert O.likelihood_module_logl([1.0, 0.0], lo, hi, -10.0, 0.0, np.nan) == 1e-18  # total NaN -> 1e-18
+    assert O.likelihood_module_logl([1.0, 0.0], lo, hi, -np.inf) == -np.finfo(float).max
+    assert O.likelihood_module_logl([3.0, 0.0], lo, hi, -10.0, check_bounds=False) == -10.0
+    assert O.likelihood_module_logl([np.nan, 0.0], lo, hi, -10.0) == -10.0          # NaN is not a bound hit
+
+
+def test_likelihood_module_host_composition(monkeypatch):
+    """LikelihoodModule's host arithmetic (bounds, priors, custom addition, NaN
+    policies) against the oracle's composition, with the image likelihood
+    stubbed (no device)."""
+    import torch
+
+    from jaxtronomy_amd.Sampling import likelihood as L
+    from oracle import lensim_oracle as O
+
+    im = _im()
+    monkeypatch.setattr(L.backend, "to_device_f64", lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)))
+    image_ll = np.array([-5.0, np.nan, -7.0, -np.inf, -3.0, -4.0])
+    monkeypatch.setattr(im, "likelihood_batch", lambda params: torch.as_tensor(image_ll[:params.shape[0]]))
+    p = Param(im, kwargs_fixed_lens=[{"center_x": 0.0, "center_y": 0.0}, {"ra_0": 0.0, "dec_0": 0.0}],
+              kwargs_fixed_lens_light=[{"center_x": 0.0, "center_y": 0.0}],
+              kwargs_lower_lens=[{"theta_E": 0.1}, {}], kwargs_upper_lens=[{"theta_E": 3.0}, {}])
+    priors_lens = [[0, "theta_E", 1.2, 0.1], [1, "gamma1", 0.0, 0.03]]
+    priors_light = [[0, "R_sersic", 1.0, 0.2], [0, "center_x", 0.0, 0.01]]  # a prior on a fixed parameter too
+
+    def custom(kwargs_lens=None, kwargs_source=None, **kw):
+        v = kwargs_source[0]["n_sersic"]
+        return np.nan if v > 10 else -0.5 * v
+
+    lm = L.LikelihoodModule(im, p, custom_logL_addition=custom, prior_lens=priors_lens,
+                            prior_lens_light=priors_light)
+    kl, ks, kll = W.epl180_draw(0)
+    base = p.kwargs2args(kl, ks, kll, image_model=im)
+    args = np.repeat(base[None], 6, axis=0)
+    args[2, 0] = 5.0           # theta_E out of bounds
+    n_idx = p.names.index("n_sersic_source0")
+    args[4, n_idx] = 20.0      # custom addition NaN
+    got = lm.logL(args)
+    for j in range(6):
+        kw = p.params2kwargs(p.args2params(args[j:j + 1])[0], im)
+        prior = (O.gaussian_prior_logl(kw["kwargs_lens"], priors_lens) +
+                 O.gaussian_prior_logl(kw["kwargs_lens_light"], priors_light))
+        want = O.likelihood_module_logl(args[j], *p.param_limits(), image_ll[j], prior,
+                                        custom(**kw))
+        assert got[j] == want or abs(got[j] - want) <= 1e-12 * abs(want), (j, got[j], want)
+    assert got[2] == -1e18 and got[4] == 1e-18 and got[1] < -1e14
+    with pytest.raises(NotImplementedError):
+        L.LikelihoodModule(im, p, prior_ps=[[0, "ra_image", 0.0, 1.0]])
+    with pytest.raises(NotImplementedError):
+        L.LikelihoodModule(im, p, prior_lens_kde=[[0, "theta_E", np.ones(3)]])
+
+
+def test_profile_options_layout():
+    """Lens profile constructor options (profile_kwargs_list): SIE NIE=False is
+    the EPL at a fixed slope, a non-default s_scale the NIE; NFW interpol raises
+    as in the reference; unknown or unsupported options raise."""
+    from jaxtronomy_amd import profiles
+
+    code, layout, names = profiles.lens_layout("SIE", {"NIE": False, "gamma": 2.0})
+    assert code == profiles.LENS_PROFILES["EPL"][0] and layout[1] == ("gamma", 2.0)
+    assert names == profiles.LENS_PROFILES["SIE"][1]
+    code, layout, _ = profiles.lens_layout("SIE", {"s_scale": 0.05})
+    assert code == profiles.LENS_PROFILES["NIE"][0] and layout[3] == ("s_scale", 0.05)
+    assert profiles.lens_layout("SIE", {"s_scale": profiles.SIE_S_SCALE, "gamma": 2.3})[0] == \
+        profiles.LENS_PROFILES["SIE"][0]  # gamma only matters without the NIE
+    assert profiles.lens_layout("NFW", {"interpol": False, "num_interp_X": 100})[0] == 4
+    with pytest.raises(ValueError):
+        profiles.lens_layout("NFW", {"interpol": True})
+    with pytest.raises(NotImplementedError):
+        profiles.lens_layout("EPL", {"static": True})
+    with pytest.raises(TypeError):
+        profiles.lens_layout("SHEAR", {"foo": 1})
+    with pytest.raises(TypeError):
+        profiles.lens_layout("SIE", {"bar": 1})
+    # packing: the fixed slope sits in the EPL's gamma slot, SIE kwargs are accepted
+    p = profiles.pack([layout], [dict(theta_E=1.1, e1=0.1, e2=0.0)], 2)
+    np.testing.assert_array_equal(p[0, 0, :6], [1.1, 0.1, 0.0, 0.05, 0.0, 0.0])
+    lm = LensModel(["SIE", "SHEAR"], profile_kwargs_list=[{"NIE": False}, None])
+    assert lm._types[0] == 1 and lm.param_name_list[0] == profiles.LENS_PROFILES["SIE"][1]
+    cfg = dict(W.EPL180, numPix=20, supersampling=1)
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((20, 20)))), PSF("NONE"), lm,
+                    LightModel(["SERSIC_ELLIPSE"]))
+    prm = Param(im)
+    assert "gamma_lens0" not in prm.names and prm.num_param()[0] == 5 + 4 + 7
+    row = prm.args2params(np.arange(16, dtype=float))[0]
+    assert row[0, 1] == 2.0  # the slope slot carries the option's value
+
+
+def test_alpha_scaling_from_cosmology():
+    """z_source != z_source_convention scales the deflections by the ratio of
+    D_ds/D_s at the two source redshifts (distances by scipy quadrature)."""
+    from scipy import integrate
+
+    from jaxtronomy_amd.LensModel.lens_model import LensModel as LM
+    from jaxtronomy_amd.Util.cosmo import Background, FlatLambdaCDM
+    from oracle import lensim_oracle as O
+
+    cosmo = FlatLambdaCDM(70.0, 0.3)
+    f = lambda z: 1.0 / np.sqrt(0.3 * (1 + z) ** 3 + 0.7)
+    dc = lambda z: 299792.458 / 70.0 * integrate.quad(f, 0, z, epsabs=0, epsrel=1e-13)[0]
+    dxy = lambda z1, z2: (dc(z2) - dc(z1)) / (1 + z2)
+    want = O.beta_double_source_plane(dxy, 0.5, 2.0, 1.5)
+    assert abs(O.beta_double_source_plane(dxy, 0.5, 1.5, 1.5) - 1.0) < 1e-15
+    assert abs(O.beta_double_source_plane(dxy, 1e-9, 2.0, 1.5) - 1.0) < 1e-8   # lens at the observer
+    assert 1.0 < want < 1.2  # a farther source is deflected more
+    lm = LM(["SIE"], z_lens=0.5, z_source=2.0, z_source_convention=1.5, cosmo=cosmo)
+    assert abs(lm.alpha_scaling - want) < 1e-10
+    assert LM(["SIE"], z_lens=0.5, z_source=1.5, z_source_convention=1.5).alpha_scaling == 1.0
+    bkg = Background(cosmo)
+    assert abs(bkg.d_xy(0.5, 2.0) - dxy(0.5, 2.0)) < 1e-8 * dxy(0.5, 2.0)
+    with pytest.raises(ValueError):
+        LM(["SIE"], z_source=2.0, z_source_convention=1.5)
