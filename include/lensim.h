@@ -271,6 +271,21 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
                         const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
                         double *d_logl, double *d_grad, void *ws, size_t ws_bytes, void *stream);
 
+/* lensim_loglike without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * scratch memory is a file-scope device array of lensim_noalloc_scratch_bytes()
+ * (64 MiB) that every GPU the library runs on holds until the process ends;
+ * a call whose lensim_workspace_size exceeds it returns LENSIM_EWORKSPACE
+ * before any device work.  It calls no allocator.  Calls on one GPU share
+ * the array: calls that overlap in time on one GPU (different streams or
+ * threads) race and must be ordered by the caller.  Results equal
+ * lensim_loglike's. */
+size_t lensim_noalloc_scratch_bytes(void);
+int lensim_loglike_noalloc(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                           const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                           double *d_logl, void *d_model_out, void *stream);
+
 /* Graph capture of a log-likelihood call, for callers that replay one batch
  * shape many times (a sampler's likelihood, a request server's lanes) and
  * want one launch per call.  Replaces, for those callers, the jit-compiled

@@ -121,6 +121,8 @@ _SIGNATURES = {
     "lensim_grad_workspace_size": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
     "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                            ctypes.c_size_t, _P]),
+    "lensim_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "lensim_loglike_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
     "lensim_graph_capture_loglike": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
                                                     _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_void_p)]),
     "lensim_graph_launch": (ctypes.c_int, [_P, _P]),

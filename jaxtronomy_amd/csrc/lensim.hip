@@ -24,6 +24,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "grad.hpp"
 #include "profiles.hpp"
 
@@ -2503,6 +2505,40 @@ int run_linear(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int
   return LENSIM_OK;
 }
 
+// ------------------------------------------- allocation-free entry point --
+// lensim_loglike_noalloc keeps its scratch memory in this file-scope device
+// array (one copy per GPU, held until the process ends) instead of a
+// workspace argument.  In host code `scratch` names a host-side stand-in, not
+// the device array: its device address on the current GPU comes from
+// hipGetSymbolAddress (scratch_on_device) and only that address reaches
+// kernels.  No kernel of the log-likelihood pipeline reads scratch memory it
+// has not written earlier in the same call, so nothing is cleared between
+// calls.  Calls that overlap on one GPU share the array and race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current GPU, looked up once per GPU.
+int scratch_on_device(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(LENSIM_EINVAL, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(LENSIM_EDEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LENSIM_OK;
+}
+
 int check_ws(const Layout &L, void *ws, size_t ws_bytes) {
   if (L.total > ws_bytes)
     return fail(LENSIM_EWORKSPACE, "workspace of %zu B is smaller than the %zu B needed", ws_bytes, L.total);
@@ -2858,6 +2894,29 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (precision == LENSIM_FP64)
     return run_grad<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
   return run_grad<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+}
+
+size_t lensim_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int lensim_loglike_noalloc(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
+                           const lensim_data *data, const double *d_params, int32_t n_img, int32_t precision,
+                           double *d_logl, void *d_model_out, void *stream) {
+  if (n_img > 0) {
+    const size_t need = lensim_workspace_size(model, grid, psf, n_img, precision);
+    if (need == 0)  // a bad argument: lensim_loglike's own checks name it, before any device work
+      return lensim_loglike(model, grid, psf, data, d_params, n_img, precision, d_logl, d_model_out, nullptr, 0,
+                            stream);
+    if (need > kScratchBytes)
+      return fail(LENSIM_EWORKSPACE,
+                  "the call needs %zu B of scratch memory; lensim_loglike_noalloc holds %zu B (split the batch, "
+                  "or call lensim_loglike with a workspace)",
+                  need, kScratchBytes);
+  }
+  void *ws = nullptr;
+  const int rc = scratch_on_device(&ws);
+  if (rc) return rc;
+  return lensim_loglike(model, grid, psf, data, d_params, n_img, precision, d_logl, d_model_out, ws, kScratchBytes,
+                        stream);
 }
 
 // ------------------------------------------------------ graph capture ---
