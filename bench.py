@@ -132,6 +132,29 @@ def time_in_flight(im, params, steps, warmup, backend):
                 logl=[x.cpu().numpy() for x in logls])
 
 
+def clock_settle(im, params, seconds):
+    """Untimed sustained load before the timed phases: batches three in flight
+    (as the headline phase runs them) until ``seconds`` have passed, so the
+    timed steps start at settled clocks (an idle GPU takes ~15 ms of load to
+    reach them).  Returns the seconds spent and the batches run."""
+    import torch
+
+    lanes = [im] + list(im.twins)
+    streams = [torch.cuda.Stream() for _ in lanes]
+    logl = torch.empty(params.shape[0], dtype=torch.float64, device=params.device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(3 * len(lanes)):
+            with torch.cuda.stream(streams[n % len(lanes)]):
+                lanes[n % len(lanes)].likelihood_batch(params, logl=logl)
+            n += 1
+        streams[(n - 1) % len(lanes)].synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, n
+
+
 def measured_traffic(name):
     """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy's readings), or None."""
@@ -173,16 +196,30 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(cfg, kernel, params_np, data, n_sample, n_single=8):
-    """The C FP64 oracle timed on the host: ``n_sample`` log-likelihoods (the
-    benchmark batch repeated) on up to 16 threads, and ``n_single`` on one
-    thread.  Returns the baseline record and the log-likelihoods of the batch."""
+def cgroup_cpus():
+    """The CPU quota of this process's cgroup (cpu.max, in CPUs), or None."""
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        return None if quota == "max" else float(quota) / float(period)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(cfg, kernel, params_np, data, repeats=20, warmups=3):
+    """The C FP64 oracle timed on the host, as the measurement plan asks: 3
+    warm-up repeats, then the median of ``repeats`` timed repeats
+    (time.perf_counter, a steady clock), each repeat one log-likelihood per
+    thread (draws of the benchmark batch in turn), on the affinity set's
+    threads (capped at the cgroup's CPU quota, which otherwise throttles the
+    extra threads) and on one thread.  Returns the baseline record and the
+    log-likelihoods of the first draws."""
     from oracle import c_oracle
     from jaxtronomy_amd import workloads as W
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    affinity = len(os.sched_getaffinity(0))
+    quota = cgroup_cpus()
+    threads = affinity if quota is None else max(1, min(affinity, int(quota)))
     g = W.grid_dict(cfg)
-    sample = params_np[np.arange(n_sample) % params_np.shape[0]]
     c_oracle.lib()
 
     def run(p, n_threads):
@@ -192,14 +229,29 @@ def cpu_baseline(cfg, kernel, params_np, data, n_sample, n_single=8):
                               n_threads=n_threads)
         return ll, time.perf_counter() - t0
 
-    ll, dt = run(sample, threads)
-    _, dt1 = run(sample[:n_single], 1)
-    rec = dict(value=sample.shape[0] / dt, unit="images/s", cores=threads, kind="port",
-               sample="%d log-likelihood evaluations (the benchmark batch of %d, repeated; C FP64 restatement of the "
-                      "reference algorithm, OpenMP, %.1f s wall) on %s" % (sample.shape[0], params_np.shape[0], dt,
-                                                                           cpu_model()),
-               single_thread={"value": n_single / dt1, "sample": "%d evaluations, %.1f s" % (n_single, dt1)})
-    return rec, ll[:params_np.shape[0]]
+    def median_rate(n_threads):
+        times, first = [], None
+        for r in range(warmups + repeats):
+            idx = (np.arange(n_threads) + r * n_threads) % params_np.shape[0]
+            ll, dt = run(params_np[idx], n_threads)
+            if first is None:
+                first = ll
+            if r >= warmups:
+                times.append(dt)
+        med = float(np.median(times))
+        return n_threads / med, med, float(np.sum(times)), first
+
+    rate, med, tot, ll = median_rate(threads)
+    rate1, med1, tot1, _ = median_rate(1)
+    rec = dict(value=rate, unit="images/s", cores=threads, kind="port",
+               sample="median of %d repeats (after %d warm-ups) of %d log-likelihood evaluations, one per thread, of "
+                      "the benchmark's draws; C FP64 restatement of the reference algorithm, OpenMP; %.3f s per repeat, "
+                      "%.1f s timed; host %s, affinity set %d CPUs, cgroup quota %s CPUs"
+                      % (repeats, warmups, threads, med, tot, cpu_model(), affinity,
+                         "none" if quota is None else "%g" % quota),
+               single_thread={"value": rate1, "sample": "median of %d repeats (after %d warm-ups) of one evaluation, "
+                                                        "%.3f s each" % (repeats, warmups, med1)})
+    return rec, ll
 
 
 def build_substructure(batch, n_sub):
@@ -246,10 +298,11 @@ def substructure_work(params_np, n_ray, n_sub):
     return F + n_img * n_ray * 36.0 * n_sub, S + n_img * n_ray * 7.0 * n_sub
 
 
-def serve_measure(im, params_np, data, capacity, seconds, fractions):
+def serve_measure(im, params_np, data, capacity, seconds, fractions, label=""):
     """Served load on the request server: Poisson arrivals of log-likelihood
-    requests at fractions of the batched capacity; achieved rate and latency
-    percentiles per offered rate."""
+    requests at fractions of the batched capacity (a fraction above 1 offers
+    more than the server can take, so its achieved rate is the saturated
+    throughput); achieved rate and latency percentiles per offered rate."""
     from jaxtronomy_amd.serving import LensServer, latency_percentiles, poisson_load
 
     srv = LensServer(n_streams=4)
@@ -267,11 +320,11 @@ def serve_measure(im, params_np, data, capacity, seconds, fractions):
             futs, wall = poisson_load(srv, "epl180", params_np, rate, seconds, slots=slots, seed=i)
             pct = latency_percentiles(futs)
             nb = srv.batches - b0
-            points.append({"offered_per_s": round(rate, 1), "achieved_per_s": round(len(futs) / wall, 1),
+            points.append({"offered_per_s": round(rate, 1), "achieved_per_s": round(float(len(futs) / wall), 1),
                            "p50_ms": round(pct[50], 4), "p99_ms": round(pct[99], 4),
                            "mean_batch": round((srv.requests_done - n0) / max(nb, 1), 2)})
             del futs
-            log("bench: served %.0f/s offered -> %s" % (rate, points[-1]))
+            log("bench: served %s %.0f/s offered -> %s" % (label, rate, points[-1]))
     finally:
         srv.stop()
     return points
@@ -678,13 +731,14 @@ def main():
     ap.add_argument("--batch", type=int, default=None,
                     help="batch per step (default: 64 epl180, 16 substructure, 256 grad, as each workload is specified)")
     ap.add_argument("--fp64-steps", type=int, default=60)
-    ap.add_argument("--cpu-sample", type=int, default=128)
+    ap.add_argument("--settle", type=float, default=0.5, help="seconds of untimed sustained load before the timed "
+                    "phases (epl180)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60"), default="epl180")
     ap.add_argument("--in-flight", type=int, default=3, help="batches in flight for the headline value (epl180)")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
     ap.add_argument("--n-sub", type=int, default=1000)
-    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference for HBM counters)")
     args = ap.parse_args()
@@ -721,6 +775,8 @@ def main():
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch, args.in_flight)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2
+    settle_s, settle_n = clock_settle(im32, params, args.settle)
+    log("bench: clocks settled (%.2f s, %d batches)" % (settle_s, settle_n))
 
     log("bench: timing mixed precision, one batch at a time (%d steps)" % args.steps)
     mixed = time_steps(im32, params, args.steps, args.warmup, backend)
@@ -771,7 +827,13 @@ def main():
         },
         "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
         "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4),
-        "one_batch_at_a_time": {"value": round(alone, 1), "ms_per_step": round(mixed["total_ms"] / args.steps, 4)},
+        "latency": "p50_ms / p99_ms: one batch of 64 alone (HIP events); the served request latencies are under "
+                   "serving.mixed / serving.fp64 (p50_ms / p99_ms of the served line at 75% of capacity)",
+        "clock_settle": {"seconds": round(settle_s, 3), "batches": settle_n,
+                         "note": "untimed sustained load (three batches in flight) before the warm-up steps"},
+        "one_batch_at_a_time": {"value": round(alone, 1), "ms_per_step": round(mixed["total_ms"] / args.steps, 4),
+                                "p50_ms": round(float(np.percentile(mixed["per_step"], 50)), 4),
+                                "p99_ms": round(float(np.percentile(mixed["per_step"], 99)), 4)},
         "fp64": {"value": round(value64, 1), "ms_per_step": round(fp64_fl["total_ms"] / args.fp64_steps, 4),
                  "one_batch_at_a_time": {"value": round(alone64, 1),
                                          "ms_per_step": round(fp64["total_ms"] / args.fp64_steps, 4)},
@@ -791,20 +853,36 @@ def main():
                      "work": "per launch %.3g flops + 8 x %.3g special-function evaluations" % (F, S),
                      "algorithmic_bytes": 4 * n_ray // cfg["supersampling"] ** 2 * args.batch,
                      "conv_tflops": round(conv_flops / (mixed["conv_ms"] / 1e3) / 1e12, 3),
-                     "duration": "mean render launch of the one-batch-at-a-time phase, HIP events on its stream",
+                     "duration": "mean render launch over the %d timed one-batch-at-a-time steps: HIP events that "
+                                 "the library records on the launch stream around the render kernel (lensim_timing_*)"
+                                 "; the same command's rocprofv3 kernel trace is profiles/epl180_b64_kernel_trace.txt"
+                                 % args.steps,
                      "in_flight_achieved": round((F + 8 * S) / (mixed_fl["render_ms"] / 1e3) / 1e12, 3)},
     }
     if args.serve_seconds > 0:
         log("bench: served load")
+        served = {}
+        for form, im_, cap in (("mixed", im32, value), ("fp64", im64, value64)):
+            pts = serve_measure(im_, params_np, data, cap, args.serve_seconds, (0.25, 0.5, 0.75, 2.0), label=form)
+            load = pts[2]
+            served[form] = {"saturated_per_s": pts[-1]["achieved_per_s"],
+                            "p50_ms": load["p50_ms"], "p99_ms": load["p99_ms"],
+                            "stated_load": "75%% of the form's in-flight capacity (%.0f requests/s offered)"
+                                           % load["offered_per_s"],
+                            "points": pts}
         line["serving"] = {
             "workload": "Poisson arrivals of single log-likelihood requests (same model and data as above) on the "
                         "continuous-batching server: max batch 64, 3 graph lanes, 4 streams; arrivals and "
-                        "scheduler in one event loop",
-            "points": serve_measure(im32, params_np, data, value, args.serve_seconds, (0.25, 0.5, 0.75)),
+                        "scheduler in one event loop; offered at 25/50/75% of each form's in-flight capacity and "
+                        "at 200% (its achieved rate is the saturated throughput)",
+            "mixed": served["mixed"],
+            "fp64": served["fp64"],
+            "mixed_over_fp64_saturated": round(served["mixed"]["saturated_per_s"] /
+                                               served["fp64"]["saturated_per_s"], 2),
         }
     if not args.no_cpu:
-        log("bench: cpu baseline (%d images)" % args.cpu_sample)
-        cb, ll_cpu = cpu_baseline(cfg, kernel, params_np, data, args.cpu_sample)
+        log("bench: cpu baseline")
+        cb, ll_cpu = cpu_baseline(cfg, kernel, params_np, data)
         line["cpu_baseline"] = cb
         line["accuracy"]["fp64_logl_max_rel_vs_cpu_oracle"] = float(
             np.max(np.abs(fp64["logl"][:len(ll_cpu)] - ll_cpu) / np.abs(ll_cpu)))

@@ -254,3 +254,81 @@ def test_serving_mix_buckets_equal_alone():
             assert f.value == want
         else:
             assert torch.equal(f.value, im.image_batch(p).cpu()[0])
+
+
+def test_serving_at_stated_scale():
+    """The serving configuration at its stated scale: buckets of 240^2 at
+    supersampling 5 and 180^2 at supersampling 3 with 41x41 PSFs beside
+    100^2 / 64^2 buckets with 31x31 / 11x11 PSFs and the three deflector sets,
+    resident observations and noise maps, and Poisson arrivals fast enough that
+    more than 2000 requests are in flight at once.  A sample of every stream's
+    results equals the same requests run alone, bit for bit, and one
+    log-likelihood request per bucket equals the oracle's (1e-10 in the FP64
+    bucket, 1e-6 relative in the mixed ones)."""
+    from jaxtronomy_amd import profiles
+    from jaxtronomy_amd.serving import poisson_load_mix
+    from oracle import lensim_oracle as O
+
+    m = W.SERVE_MIX
+    combos = [("big", 240, 5, ("EPL", "SHEAR"), 41, "mixed"), ("c2", 180, 3, ("SIE", "SHEAR"), 41, "mixed"),
+              ("mid", 100, 3, ("NFW", "SIE"), 31, "mixed"), ("small", 64, 1, ("SIE", "SHEAR"), 11, "mixed"),
+              ("big64", 240, 5, ("EPL", "SHEAR"), 41, "fp64")]
+    srv = LensServer(n_streams=4)
+    streams, ref = [], {}
+    for bi, (key, n, s, defl, k, precision) in enumerate(combos):
+        cfg = dict(numPix=n, deltaPix=m["deltaPix"], supersampling=s, exposure_time=m["exposure_time"],
+                   background_rms=m["background_rms"])
+        kernel = W.moffat_kernel(k, m["psf_fwhm"] / m["deltaPix"], m["psf_beta"])
+        im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(list(defl)), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        kwargs_numerics={"supersampling_factor": s}, precision=precision)
+        srv.register(key, im, max_batch=32, n_slots=4, n_lanes=2, noise_maps=True)
+        rows = np.stack([im.pack(*W.mix_draw(defl, 20 * bi + d))[0][0] for d in range(8)])
+        clean = im.image_batch(torch.as_tensor(rows[:4], device="cuda")).double().cpu().numpy()
+        rng = np.random.default_rng(100 + bi)
+        obs, noise, slots = [], [], []
+        for o in range(4):
+            obs.append(W.noisy(clean[o], cfg["exposure_time"], cfg["background_rms"], rng))
+            noise.append(np.sqrt(cfg["background_rms"] ** 2 + np.maximum(clean[o], 0) / cfg["exposure_time"]))
+            slots.append(srv.put_data(key, obs[-1], noise[-1]))
+        srv.warmup(key, rows[0], kinds=("loglike", "image"))
+        w = 0.1 if precision == "fp64" else 1.0
+        streams.append(dict(key=key, params=rows, slots=np.array(slots)[np.arange(8) % 4], kind="loglike",
+                            weight=0.8 * w))
+        streams.append(dict(key=key, params=rows, kind="image", weight=0.2 * w))
+        ref[key] = (im, rows, obs, noise, cfg, kernel, defl, precision)
+    futs, wall, peak, (which, row) = poisson_load_mix(srv, streams, rate=2.0e6, duration=0.0025, seed=11)
+    srv.stop()
+    assert len(futs) == 5000 and all(f.done() for f in futs)
+    assert peak >= 2000, peak
+    assert len(srv.batch_sizes) > 1
+    # a sample of every stream: equal to the request alone
+    rng = np.random.default_rng(0)
+    for st in range(len(streams)):
+        idx = np.nonzero(which == st)[0]
+        assert idx.size > 0
+        for q in rng.choice(idx, size=min(6, idx.size), replace=False):
+            s = streams[st]
+            im, rows, obs, noise = ref[s["key"]][:4]
+            p = torch.as_tensor(rows[row[q]][None], device="cuda")
+            if s["kind"] == "loglike":
+                o = row[q] % 4
+                want = im.likelihood_batch(p, data=torch.as_tensor(obs[o][None], device="cuda"),
+                                           noise_map=torch.as_tensor(noise[o][None], device="cuda")).item()
+                assert futs[q].value == want, (s["key"], q)
+            else:
+                assert torch.equal(futs[q].value, im.image_batch(p).cpu()[0]), (s["key"], q)
+    # one log-likelihood request per bucket against the oracle
+    for st in range(0, len(streams), 2):
+        s = streams[st]
+        im, rows, obs, noise, cfg, kernel, defl, precision = ref[s["key"]]
+        q = int(np.nonzero(which == st)[0][0])
+        r = int(row[q])
+        kw = [{nm: float(rows[r][j][i]) for i, nm in profiles.kwarg_slots(im._names[j])}
+              for j in range(len(im._names))]
+        nd = len(defl)
+        want_img = O.image(W.grid_dict(cfg), list(defl), kw[:nd], ["SERSIC_ELLIPSE"], kw[nd:nd + 1],
+                           ["SERSIC_ELLIPSE"], kw[nd + 1:], kernel=kernel)
+        want = O.log_likelihood(want_img, obs[r % 4], noise_map=noise[r % 4])
+        tol = 1e-10 if precision == "fp64" else 1e-6
+        assert abs(futs[q].value - want) < tol * abs(want), (s["key"], futs[q].value, want)
