@@ -204,6 +204,9 @@ MfmaPlan mfma_plan(int nx, int ny, int n_img, int kx, int ky) {
   return best;
 }
 
+// 16 x 16 output tiles of an image (the matrix-core convolution's chi^2 partials)
+int n16_tiles(int nx, int ny) { return ((nx + 15) / 16) * ((ny + 15) / 16); }
+
 template <typename T>
 int n_tiles_of(int nx, int ny) {
   const int tx = (nx + ConvTile<T>::TX - 1) / ConvTile<T>::TX;
@@ -243,8 +246,13 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
   off = align256(off + size_t(n_img) * nx * ny * tsz);
   L.n_tiles = precision == LENSIM_MIXED ? n_tiles_of<float>(nx, ny) : n_tiles_of<double>(nx, ny);
   const size_t npix = size_t(nx) * ny, npix_r = ss.on ? npix * ss.s * ss.s : npix;  // rendered pixels
-  L.partial = off;  // per-tile (or per-block) chi^2 partials
-  off = align256(off + size_t(n_img) * (L.n_tiles > int((npix + 255) / 256) ? L.n_tiles : (npix + 255) / 256) * 8);
+  L.partial = off;  // per-tile (or per-block, or per 16 x 16 tile) chi^2 partials
+  {
+    size_t np_ = L.n_tiles;
+    np_ = np_ > (npix + 255) / 256 ? np_ : (npix + 255) / 256;
+    np_ = np_ > size_t(n16_tiles(nx, ny)) ? np_ : size_t(n16_tiles(nx, ny));
+    off = align256(off + size_t(n_img) * np_ * 8);
+  }
   L.blkmax = off;  // per render block max |value|, [n_img][ceil(rendered pixels / 256)]
   off = align256(off + size_t(n_img) * ((npix_r + 255) / 256) * 4);
   L.hr = L.chr = L.lowu = L.lowc = L.psf_low = off;
@@ -1242,10 +1250,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   if (a0 < ky_pad) body(bq[0], a0);  // an odd number of bodies: the last one
   // lane l holds D[4 (l / 16) + j][l % 16] of each tile
   const double unscale = ldexp(1.0, -(sa + sk));
-  double chi = 0.0;
+  // chi^2: one partial per 16 x 16 output tile, at [img][tile row][tile
+  // column] of the image's 16 x 16 tiling, so the order of the sum does not
+  // depend on the block width (which the plan picks by batch size): a request
+  // has the same log-likelihood in any batch
+  const int n16x = (c.nx + 15) >> 4, n16 = n16x * ((c.ny + 15) >> 4);
+  const int ty16 = (Y0 >> 4) + wave;
 #pragma unroll
   for (int t = 0; t < TL; ++t) {
     const int x = X0 + 16 * t + li;
+    double chi = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int y = Y0 + 16 * wave + 4 * lg + j;
@@ -1258,17 +1272,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
           static_cast<float *>(c.gout)[size_t(img) * c.nx * c.ny + p] = float(dlogl_dm(c, p, double(m), img));
       }
     }
-  }
-  if (partial) {
-    for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
-    __syncthreads();
-    double *red = reinterpret_cast<double *>(smem_raw);
-    if (lane == 0) red[wave] = chi;
-    __syncthreads();
-    if (tid == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kMfWaves; ++w) s += red[w];
-      partial[size_t(img) * gridDim.x + tile_id] = s;
+    if (partial) {
+      for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+      const int tx16 = (X0 >> 4) + t;
+      // tiles past the image's edge hold no pixel and have no slot
+      if (lane == 0 && 16 * tx16 < c.nx && 16 * ty16 < c.ny)
+        partial[size_t(img) * n16 + size_t(ty16) * n16x + tx16] = chi;
     }
   }
 }
@@ -1829,7 +1838,7 @@ int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int 
   int n_tiles;
   if (mfma) {
     const MfmaPlan pl = mfma_plan(g.nx, g.ny, n_img, kx, ky);
-    n_tiles = pl.n_tiles;
+    n_tiles = n16_tiles(g.nx, g.ny);  // chi^2 partials of the 16 x 16 output tiles
     rc = launch_conv_mfma(c, pl, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
                           (npix + 255) / 256, reinterpret_cast<float *>(out), partial, st);
   } else {
