@@ -96,6 +96,7 @@ void timing_mark(int slot, hipStream_t st) {
 struct ModelArgs {
   int32_t n_lens, n_bulk, n_defl, n_source, n_light, n_prof, n_epl, has_halo, bulk_type;
   int32_t has_nie_nfw;  // an SIE / NIE or NFW deflector (inline or bulk): the float form deflects them in double
+  SubSteps sub;         // sub-ray steps of the call's grid, for the records (with_grid); zero without a grid
   const int32_t *bulk_types;
   int32_t types[LENSIM_MAX_PROFILES];
   int32_t epl_index[LENSIM_MAX_PROFILES];
@@ -408,10 +409,15 @@ __device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, 
       dcf = dcoeff + co;
     }
   }
-  prepare_record(type, p, rec, coef, dcoef, cf, dcf);
+  prepare_record(type, p, rec, coef, m.sub, dcoef, cf, dcf);
   for (int i = 0; i < kRec; ++i) {
     recd[size_t(ri) * kRec + i] = rec[i];
     recf[size_t(ri) * kRec + i] = float(rec[i]);
+  }
+  if (type == LENSIM_EPL) {  // series lengths as int bits in the float copy
+    recf[size_t(ri) * kRec + EPL_NMAX] = __int_as_float(int(rec[EPL_NMAX]));
+    recf[size_t(ri) * kRec + EPL_NZ] = __int_as_float(int(rec[EPL_NZ]));
+    recf[size_t(ri) * kRec + EPL_NF] = __int_as_float(int(rec[EPL_NF]));
   }
 }
 
@@ -624,22 +630,103 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   }
 }
 
-// One deflector of a type fixed at compile time (EPL's series coefficients
-// at coef_t / coef_d; SIE records are NIE records).
+// The coordinates of a compile-time deflector that the sub-ray loop advances:
+// Z = (q xr, yr) for the EPL, (xr, yr) for SIE / NIE (major-axis frame), the
+// offset (dx, dy) from the centre for NFW.  They are affine in the ray
+// position, so after their value at the pixel's first sub-ray each sub-ray
+// adds a step (xs, ys) and each row a row step (xr_, yr_).
+template <int TYPE>
+struct DeflCoords {
+  double a, b, sa, sb, ra, rb;
+  __device__ __forceinline__ void init(const double *rd, const SubSteps &st, double x0, double y0) {
+    const double dx = x0 - rd[R_CX], dy = y0 - rd[R_CY];
+    if constexpr (TYPE == LENSIM_EPL) {
+      const double c = rd[EPL_C], s = rd[EPL_S];
+      a = rd[EPL_Q] * (dx * c + dy * s);
+      b = -dx * s + dy * c;
+      sa = rd[EPL_ZSX];
+      sb = rd[EPL_ZSXI];
+      ra = rd[EPL_ZRR];
+      rb = rd[EPL_ZRRI];
+    } else if constexpr (TYPE == LENSIM_NIE) {
+      const double c = rd[NIE_C], s = rd[NIE_S];
+      a = dx * c + dy * s;
+      b = -dx * s + dy * c;
+      sa = rd[NIE_XSX];
+      sb = rd[NIE_YSX];
+      ra = rd[NIE_XRR];
+      rb = rd[NIE_YRR];
+    } else {
+      a = dx;
+      b = dy;
+      sa = st.sx[0];
+      sb = st.sx[1];
+      ra = st.rr[0];
+      rb = st.rr[1];
+    }
+  }
+  __device__ __forceinline__ void step() {
+    a += sa;
+    b += sb;
+  }
+  __device__ __forceinline__ void row() {
+    a += ra;
+    b += rb;
+  }
+};
+
+// The exact position of sub-ray (sx, sy) of the pixel centred at (ra_c, dec_c).
+struct SubRay {
+  double ra_c, dec_c, o0, inv_s;
+  int sx, sy;
+  __device__ __forceinline__ void pos(const GridArgs &g, double &x, double &y) const {
+    // the counters pass through readfirstlane (they are uniform), which keeps
+    // the compiler from carrying them as float64 induction variables for
+    // this rarely taken branch
+    const double ox = o0 + __builtin_amdgcn_readfirstlane(sx) * inv_s;
+    const double oy = o0 + __builtin_amdgcn_readfirstlane(sy) * inv_s;
+    x = ra_c + g.m00 * ox + g.m01 * oy;
+    y = dec_c + g.m10 * ox + g.m11 * oy;
+  }
+};
+
+// beta -= alpha_scaling alpha of one compile-time deflector at its advanced
+// coordinates; the EPL and SIE / NIE rotate back with (alpha_scaling cos phi,
+// alpha_scaling sin phi) folded into the update.  An EPL sub-ray within 1e-12
+// of the centre recomputes Z from its exact position (so that a ray exactly
+// at the centre gets the reference's zero deflection, not the direction of
+// the advanced coordinates' rounding).
 template <typename T, int TYPE>
-__device__ __forceinline__ void defl_fixed(const Rec<T> &r, const T *coef_t, const double *coef_d, double x,
-                                           double y, double &ax, double &ay) {
+__device__ __forceinline__ void defl_fast(const Rec<T> &r, const T *coef_t, const double *coef_d,
+                                          DeflCoords<TYPE> &u, double asc, const GridArgs &g, const SubRay &sr,
+                                          double &bx, double &by) {
   if constexpr (TYPE == LENSIM_EPL) {
+    double R2 = fma(u.a, u.a, u.b * u.b);
+    if (R2 < 1e-24) {  // exact Z (the advance continues from it)
+      double x, y;
+      sr.pos(g, x, y);
+      const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+      const double c = r.d[EPL_C], s = r.d[EPL_S];
+      u.a = r.d[EPL_Q] * (dx * c + dy * s);
+      u.b = -dx * s + dy * c;
+      R2 = fma(u.a, u.a, u.b * u.b);
+    }
+    double fr, fi;
     if constexpr (sizeof(T) == sizeof(float))
-      epl_defl_f32(r, coef_t, coef_d, x, y, ax, ay);
+      epl_major_f32(r, coef_t, coef_d, u.a, u.b, R2, fr, fi);
     else
-      epl_defl_f64(r, coef_d, x, y, ax, ay);
+      epl_major_f64(r, coef_d, u.a, u.b, R2, fr, fi);
+    bx = fma(-r.d[EPL_AC], fr, fma(r.d[EPL_AS], fi, bx));
+    by = fma(-r.d[EPL_AS], fr, fma(-r.d[EPL_AC], fi, by));
   } else if constexpr (TYPE == LENSIM_NIE) {
-    nie_defl<T, true>(r, x, y, ax, ay);
+    double fx, fy;
+    nie_major_dbl(r.d, u.a, u.b, fx, fy);
+    bx = fma(-r.d[NIE_AC], fx, fma(r.d[NIE_AS], fy, bx));
+    by = fma(-r.d[NIE_AS], fx, fma(-r.d[NIE_AC], fy, by));
   } else if constexpr (TYPE == LENSIM_NFW) {
-    nfw_defl<T, true>(r, x, y, ax, ay);
-  } else if constexpr (TYPE == LENSIM_SHEAR) {
-    shear_defl(r.d, x, y, ax, ay);
+    const double f = asc * nfw_factor_dbl(r.d, u.a, u.b);
+    bx = fma(-f, u.a, bx);
+    by = fma(-f, u.b, by);
   }
 }
 
@@ -647,15 +734,22 @@ __device__ __forceinline__ void defl_fixed(const Rec<T> &r, const T *coef_t, con
 // time: deflector D0 (EPL, SIE / NIE or NFW) and optionally D1 (SHEAR, SIE /
 // NIE or NFW; 0: none), in that order, then NSRC Sersic sources and NLL Sersic
 // lens lights; one plane, no bulk list.  Per profile it is render_kernel's
-// arithmetic (the device functions are shared); what goes is the runtime walk
-// over the profile list, its type switches and the scalar state they keep
-// (render_kernel spills SGPRs to VGPR lanes for it).
+// arithmetic, rearranged so that everything affine in the ray position is
+// advanced by one addition per sub-ray instead of formed from the position:
+// the deflectors' coordinates (DeflCoords), the lens lights' scaled
+// elliptical coordinates, and w = theta - alpha_scaling alpha_SHEAR(theta)
+// (the external shear is linear, so beta = w - alpha_scaling (alpha of the
+// other deflectors)).  The records hold the steps (prepare_record's
+// SubSteps).  In the float form each Sersic profile's values of a pixel are
+// summed in float and multiplied by its amplitude in double at the end.
 template <typename T, int D0, int D1, int NSRC, int NLL>
 __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
                                                           const float *__restrict__ recf,
                                                           const double *__restrict__ coefd,
                                                           const float *__restrict__ coeff, unsigned flags,
                                                           T *__restrict__ out, float *__restrict__ blkmax) {
+  constexpr bool kF32 = sizeof(T) == sizeof(float);
+  static_assert(NSRC == 1, "the compile-time render takes one source");
   const int img = blockIdx.y;
   const int npix = g.nx * g.ny;
   const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -667,38 +761,122 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   const Rec<T> r0{rp.recd, rp.rect};
   const Rec<T> r1{rp.recd + kRec, rp.rect + kRec};
   const size_t co = size_t(img) * m.n_epl * LENSIM_EPL_NMAX;  // D0's coefficients when it is the EPL (epl_index 0)
+  // the pixel's first sub-ray (make_subgrid offsets, ImSim/Numerics/grid.py:60-62)
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
   const double inv_s = 1.0 / g.s;
   const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const double x0 = ra_c + g.m00 * o0 + g.m01 * o0;
+  const double y0 = dec_c + g.m10 * o0 + g.m11 * o0;
   const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && NSRC > 0;
   const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && NLL > 0;
   const double asc = m.alpha_scaling;
+  // w = theta - asc alpha_SHEAR(theta) and its steps
+  double wx, wy, wsx, wsy, wrx, wry;
+  if constexpr (D1 == LENSIM_SHEAR) {
+    const double dx = x0 - r1.d[R_CX], dy = y0 - r1.d[R_CY];
+    const double g1 = r1.d[SH_G1], g2 = r1.d[SH_G2];
+    wx = x0 - asc * (g1 * dx + g2 * dy);
+    wy = y0 - asc * (g2 * dx - g1 * dy);
+    wsx = r1.d[SH_WSX];
+    wsy = r1.d[SH_WSY];
+    wrx = r1.d[SH_WRX];
+    wry = r1.d[SH_WRY];
+  } else {
+    wx = x0;
+    wy = y0;
+    wsx = m.sub.sx[0];
+    wsy = m.sub.sx[1];
+    wrx = m.sub.rr[0];
+    wry = m.sub.rr[1];
+  }
+  // relative to the source's centre, so beta - c_source comes out directly
+  const double *rds = rp.recd + KS * kRec;
+  wx -= rds[R_CX];
+  wy -= rds[R_CY];
+  // the Sersic floors, canonicalised once (fmax then needs no per-ray canonicalisation)
+  const double umin_s = __builtin_canonicalize(rds[SE_UMIN]);
+  double umin_l[NLL > 0 ? NLL : 1];
+#pragma unroll
+  for (int k = 0; k < NLL; ++k) umin_l[k] = __builtin_canonicalize(rp.recd[(KS + NSRC + k) * kRec + SE_UMIN]);
+  DeflCoords<D0> u0;
+  u0.init(r0.d, m.sub, x0, y0);
+  constexpr int D1C = (D1 == LENSIM_SHEAR || D1 == 0) ? LENSIM_NFW : D1;  // unused when D1 is SHEAR or none
+  DeflCoords<D1C> u1;
+  if constexpr (D1 == LENSIM_NIE || D1 == LENSIM_NFW) u1.init(r1.d, m.sub, x0, y0);
+  // the lens lights' scaled elliptical coordinates
+  double lx[NLL > 0 ? NLL : 1], ly[NLL > 0 ? NLL : 1];
+#pragma unroll
+  for (int k = 0; k < NLL; ++k) {
+    const double *rd = rp.recd + (KS + NSRC + k) * kRec;
+    const double dx = x0 - rd[R_CX], dy = y0 - rd[R_CY];
+    lx[k] = fma(rd[SE_M00S], dx, rd[SE_M01S] * dy);
+    ly[k] = fma(rd[SE_M01S], dx, rd[SE_M11S] * dy);
+  }
+  // per-profile sums of the Sersic values (float form) or the flux (FP64 form)
+  float fs[NSRC > 0 ? NSRC : 1], fl[NLL > 0 ? NLL : 1];
+#pragma unroll
+  for (int k = 0; k < NSRC; ++k) fs[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < NLL; ++k) fl[k] = 0.f;
   double acc = 0.0;
   const bool eval = active && evaluated(g, i, j);
   for (int sy = 0; eval && sy < g.s; ++sy) {
-    const double oy = o0 + sy * inv_s;
     for (int sx = 0; sx < g.s; ++sx) {
-      const double ox = o0 + sx * inv_s;
-      // make_subgrid offsets (ImSim/Numerics/grid.py:60-62)
-      const double x = ra_c + g.m00 * ox + g.m01 * oy;
-      const double y = dec_c + g.m10 * ox + g.m11 * oy;
       if (do_src) {
-        // -0.0 + d is d for every d, so the first deflector's sum folds away
-        double ax = -0.0, ay = -0.0;
-        defl_fixed<T, D0>(r0, rp.coeft + co, rp.coefd + co, x, y, ax, ay);
-        if constexpr (D1 != 0) defl_fixed<T, D1>(r1, nullptr, nullptr, x, y, ax, ay);
-        const double bx = x - ax * asc, by = y - ay * asc;
-#pragma unroll
-        for (int k = 0; k < NSRC; ++k)
-          acc += sersic_light<T>(Rec<T>{rp.recd + (KS + k) * kRec, rp.rect + (KS + k) * kRec}, bx, by);
+        const SubRay sr{ra_c, dec_c, o0, inv_s, sx, sy};
+        double bx = wx, by = wy;
+        defl_fast<T, D0>(r0, rp.coeft + co, rp.coefd + co, u0, asc, g, sr, bx, by);
+        if constexpr (D1 == LENSIM_NIE || D1 == LENSIM_NFW)
+          defl_fast<T, D1C>(r1, nullptr, nullptr, u1, asc, g, sr, bx, by);
+        // (bx, by) = beta - c_source
+        const double xs = fma(rds[SE_M00S], bx, rds[SE_M01S] * by);
+        const double ys = fma(rds[SE_M01S], bx, rds[SE_M11S] * by);
+        if constexpr (kF32)
+          fs[0] += sersic_u_f32(rds, reinterpret_cast<const float *>(rp.rect) + KS * kRec, xs, ys, umin_s);
+        else
+          acc += rds[SE_AMP] * sersic_u_f64(rds, xs, ys, umin_s);
       }
       if (do_light) {
 #pragma unroll
-        for (int k = 0; k < NLL; ++k)
-          acc += sersic_light<T>(Rec<T>{rp.recd + (KS + NSRC + k) * kRec, rp.rect + (KS + NSRC + k) * kRec}, x, y);
+        for (int k = 0; k < NLL; ++k) {
+          const double *rd = rp.recd + (KS + NSRC + k) * kRec;
+          if constexpr (kF32)
+            fl[k] += sersic_u_f32(rd, reinterpret_cast<const float *>(rp.rect) + (KS + NSRC + k) * kRec, lx[k],
+                                  ly[k], umin_l[k]);
+          else
+            acc += rd[SE_AMP] * sersic_u_f64(rd, lx[k], ly[k], umin_l[k]);
+        }
+      }
+      // next sub-ray
+      wx += wsx;
+      wy += wsy;
+      u0.step();
+      if constexpr (D1 == LENSIM_NIE || D1 == LENSIM_NFW) u1.step();
+#pragma unroll
+      for (int k = 0; k < NLL; ++k) {
+        const double *rd = rp.recd + (KS + NSRC + k) * kRec;
+        lx[k] += rd[SE_XSX];
+        ly[k] += rd[SE_YSX];
       }
     }
+    // next row
+    wx += wrx;
+    wy += wry;
+    u0.row();
+    if constexpr (D1 == LENSIM_NIE || D1 == LENSIM_NFW) u1.row();
+#pragma unroll
+    for (int k = 0; k < NLL; ++k) {
+      const double *rd = rp.recd + (KS + NSRC + k) * kRec;
+      lx[k] += rd[SE_XRR];
+      ly[k] += rd[SE_YRR];
+    }
+  }
+  if constexpr (kF32) {
+#pragma unroll
+    for (int k = 0; k < NSRC; ++k) acc = fma(rp.recd[(KS + k) * kRec + SE_AMP], double(fs[k]), acc);
+#pragma unroll
+    for (int k = 0; k < NLL; ++k) acc = fma(rp.recd[(KS + NSRC + k) * kRec + SE_AMP], double(fl[k]), acc);
   }
   const T v = T(acc * (g.scale * m.flux_scaling));
   if (active) out[size_t(img) * npix + pix] = v;
@@ -1641,6 +1819,18 @@ int grid_args(const lensim_grid *grid, GridArgs &g) {
   return LENSIM_OK;
 }
 
+// The model with the sub-ray steps of grid g (SubSteps) for its records.
+ModelArgs with_grid(const ModelArgs &m, const GridArgs &g) {
+  ModelArgs mg = m;
+  const double inv_s = 1.0 / g.s;
+  mg.sub.sx[0] = g.m00 * inv_s;
+  mg.sub.sx[1] = g.m10 * inv_s;
+  mg.sub.rr[0] = g.m01 * inv_s - g.s * mg.sub.sx[0];
+  mg.sub.rr[1] = g.m11 * inv_s - g.s * mg.sub.sx[1];
+  mg.sub.asc = m.alpha_scaling;
+  return mg;
+}
+
 int psf_dims(const lensim_psf *psf, uint32_t flags, int &kx, int &ky, int &edge) {
   const bool none = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
   if (none) {
@@ -1773,9 +1963,10 @@ int launch_conv_mfma(ConvArgs c, const MfmaPlan &pl, int n_img, const float *in,
 
 // The image pipeline shared by lensim_image and lensim_loglike.
 template <typename T>
-int run_image(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+int run_image(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
               const lensim_data *data, const double *params, int n_img, unsigned flags, T *out, double *logl,
               unsigned char *ws, const Layout &L, hipStream_t st, T *gout = nullptr) {
+  const ModelArgs m = with_grid(m_in, g);
   const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
   const bool mfma = use_mfma<T>(identity, kx, ky);
   const int kyp = kyp_of(ky);
@@ -1976,13 +2167,14 @@ __global__ void unpool_kernel(const T *__restrict__ in, int nx, int ny, int s, i
 // low-resolution split kernel's convolution of the averaged unconvolved rays
 // is added.
 template <typename T>
-int run_image_ss(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
+int run_image_ss(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, int kx, int ky, int edge,
                  const SSPlan &ss, const lensim_data *data, const double *params, int n_img, unsigned flags, T *out,
                  double *logl, unsigned char *ws, const Layout &L, hipStream_t st, T *gout = nullptr) {
-  int rc = launch_prep(m, params, n_img, ws, L, st, gout != nullptr);
-  if (rc) return rc;
   const int s = ss.s;
   const GridArgs gh = ray_grid(g);
+  const ModelArgs m = with_grid(m_in, gh);
+  int rc = launch_prep(m, params, n_img, ws, L, st, gout != nullptr);
+  if (rc) return rc;
   const bool mfma = use_mfma<T>(false, kx, ky);
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);

@@ -24,7 +24,17 @@
 
 namespace lensim {
 
-constexpr int kRec = 16;  // derived constants per (image, profile)
+constexpr int kRec = 24;  // derived constants per (image, profile)
+
+// Sub-ray stepping of the supersampled grid, for the records' step constants:
+// the sky offset from one sub-ray to the next along a pixel's first axis (sx),
+// the offset from the end of a row of s sub-rays back to the start of the next
+// row (rr = one step along the second axis minus s first-axis steps), and the
+// deflection scaling alpha_scaling.  The render then updates every quantity
+// that is affine in the ray position by one addition per sub-ray.
+struct SubSteps {
+  double sx[2], rr[2], asc;
+};
 
 // Record slot names shared by all profiles: centre first.
 enum : int { R_CX = 0, R_CY = 1 };
@@ -77,16 +87,27 @@ enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX,
               EPL_F2,     // f^2 = |z|^2
               EPL_LNB,    // ln b
               EPL_TM1,    // t - 1
-              EPL_PB };   // 2 b / (1 + q)
-static_assert(EPL_PB < kRec, "EPL record exceeds kRec");
+              EPL_PB,     // 2 b / (1 + q)
+              EPL_ZSX,    // Z = (q xr, yr) steps: next sub-ray (real, imaginary) ...
+              EPL_ZSXI,
+              EPL_ZRR,    // ... and next row
+              EPL_ZRRI,
+              EPL_AC,     // alpha_scaling cos phi, alpha_scaling sin phi (the rotation back, scaled)
+              EPL_AS,
+              EPL_M2F };  // -2 f
+// In the float copy of an EPL record, EPL_NMAX, EPL_NZ and EPL_NF hold the
+// int's bits (read with __float_as_int), so no per-ray conversion is needed.
+static_assert(EPL_AS < kRec, "EPL record exceeds kRec");
 // The float form's z-series coefficients g_1..g_NZ sit at this offset of an
 // EPL's float coefficient array, past the w-series' F_n (nmax <= 50 whenever
 // the z-series is chosen).
-constexpr int kZMax = 24, kZOff = LENSIM_EPL_NMAX - kZMax;
+constexpr int kZMax = 24, kZOff = LENSIM_EPL_NMAX - kZMax - 4;
 static_assert(kZOff >= 50, "z-series coefficients overlap the w-series");
 // NIE/SIE (nie.py:182-200, :280-295): cos, sin, q (clamped), q^2, s, b/sqrt(1-q^2),
 //          sqrt(1-q^2), q^2 s
-enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S };
+//          then the steps of (xr, yr) and alpha_scaling (cos, sin) as for the EPL
+enum : int { NIE_C = 2, NIE_S, NIE_Q, NIE_Q2, NIE_SS, NIE_BSQ, NIE_SQ, NIE_Q2S, NIE_XSX, NIE_YSX, NIE_XRR,
+              NIE_YRR, NIE_AC, NIE_AS };
 // NFW     (nfw.py:86-105, :257-279, :384-395): Rs (floored), 4 rho0 Rs, 1/Rs
 enum : int { NFW_RS = 2, NFW_A, NFW_IRS };
 // TNFW    (tnfw.py:67-87, :226-248, :329-349): Rs, 4 rho0 Rs, tau, tau^2,
@@ -94,8 +115,9 @@ enum : int { NFW_RS = 2, NFW_A, NFW_IRS };
 enum : int { TN_RS = 2, TN_A, TN_TAU, TN_TAU2, TN_PRE, TN_C0, TN_C1, TN_RMIN, TN_T2P1, TN_IRS,
               TN_K,      // pi (tau^2 + 1)^2 / 2
               TN_T2M1 }; // tau^2 - 1
-// SHEAR   (shear.py:37-53): ra_0, dec_0 in the centre slots, gamma1, gamma2
-enum : int { SH_G1 = 2, SH_G2 };
+// SHEAR   (shear.py:37-53): ra_0, dec_0 in the centre slots, gamma1, gamma2; the
+//          steps of w = theta - alpha_scaling alpha_shear(theta) (affine in theta)
+enum : int { SH_G1 = 2, SH_G2, SH_WSX, SH_WSY, SH_WRX, SH_WRY };
 // SERSIC  (sersic_ellipse.py:62-93, sersic_utils.py:83-110, :239-267,
 //          param_util.py:103-124): amp, the product-average matrix
 //          ((1-e1), -e2; -e2, (1+e1))/norm, max(R_sersic, 1e-4), 1/n, b_n, 1/R_sersic_
@@ -104,8 +126,16 @@ enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS,
               SE_IRS2,    // 1 / R_sersic_^2
               SE_R2MAX,   // (1000 R_sersic_)^2  (the 1000 R_sersic window on R^2)
               SE_LNRS,    // ln R_sersic_
-              SE_BNLO };  // b_n - float(b_n): the float form carries b_n as a head + tail pair
-static_assert(SE_BNLO < kRec, "SERSIC record exceeds kRec");
+              SE_BNLO,    // b_n - float(b_n): the float form carries b_n as a head + tail pair
+              SE_M00S,    // the matrix / R_sersic_: (xs, ys) = M (x - c) / R_sersic_, u = xs^2 + ys^2
+              SE_M01S,
+              SE_M11S,
+              SE_UMIN,    // 1e-8 / R_sersic_^2 (the R^2 floor, on u)
+              SE_XSX,     // steps of (xs, ys): next sub-ray, next row
+              SE_YSX,
+              SE_XRR,
+              SE_YRR };
+static_assert(SE_YRR < kRec, "SERSIC record exceeds kRec");
 // GAUSSIAN light (gaussian.py:27-42): amp/(2 pi sigma^2), 1/sigma^2
 enum : int { GA_C = 2, GA_IS2 };
 
@@ -135,7 +165,8 @@ __host__ __device__ inline void ellipticity2phi_q(double e1, double e2, double &
 // For EPL also writes the series coefficients F_n = 2F1(-n, t/2; 2-t/2; 2),
 // n < nmax, from Gauss' contiguous recurrence (Util/hyp2f1_util.py:156-166),
 // their float copies (coef_f) and, for gradients, dF_n/dt (dcoef, dcoef_f).
-__device__ inline void prepare_record(int type, const double *p, double *rec, double *coef,
+// st: the grid's sub-ray steps (zero for calls without a grid).
+__device__ inline void prepare_record(int type, const double *p, double *rec, double *coef, const SubSteps &st,
                                       double *dcoef = nullptr, float *coef_f = nullptr, float *dcoef_f = nullptr) {
   for (int i = 0; i < kRec; ++i) rec[i] = 0.0;
   switch (type) {
@@ -163,6 +194,16 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[EPL_TM1] = t - 1.0;
       rec[EPL_PB] = 2.0 * b / (1.0 + q);
       rec[EPL_NF] = (double)nmax;
+      {
+        const double c = rec[EPL_C], sn = rec[EPL_S];
+        rec[EPL_ZSX] = q * (st.sx[0] * c + st.sx[1] * sn);
+        rec[EPL_ZSXI] = -st.sx[0] * sn + st.sx[1] * c;
+        rec[EPL_ZRR] = q * (st.rr[0] * c + st.rr[1] * sn);
+        rec[EPL_ZRRI] = -st.rr[0] * sn + st.rr[1] * c;
+        rec[EPL_AC] = st.asc * c;
+        rec[EPL_AS] = st.asc * sn;
+      }
+      rec[EPL_M2F] = -2.0 * f;
       if (coef) {
         // F_n by the recurrence, with float copies and (for gradients) dF_n/dt
         // written as they are formed: nothing is read back from memory
@@ -258,6 +299,15 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[NIE_BSQ] = b / sq;
       rec[NIE_SQ] = sq;
       rec[NIE_Q2S] = qc * qc * s;
+      {
+        const double c = rec[NIE_C], sn = rec[NIE_S];
+        rec[NIE_XSX] = st.sx[0] * c + st.sx[1] * sn;
+        rec[NIE_YSX] = -st.sx[0] * sn + st.sx[1] * c;
+        rec[NIE_XRR] = st.rr[0] * c + st.rr[1] * sn;
+        rec[NIE_YRR] = -st.rr[0] * sn + st.rr[1] * c;
+        rec[NIE_AC] = st.asc * c;
+        rec[NIE_AS] = st.asc * sn;
+      }
       break;
     }
     case LENSIM_NFW: {
@@ -296,6 +346,11 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[R_CY] = p[3];
       rec[SH_G1] = p[0];
       rec[SH_G2] = p[1];
+      // w = theta - asc (g1 dx + g2 dy, g2 dx - g1 dy): its steps
+      rec[SH_WSX] = st.sx[0] - st.asc * (p[0] * st.sx[0] + p[1] * st.sx[1]);
+      rec[SH_WSY] = st.sx[1] - st.asc * (p[1] * st.sx[0] - p[0] * st.sx[1]);
+      rec[SH_WRX] = st.rr[0] - st.asc * (p[0] * st.rr[0] + p[1] * st.rr[1]);
+      rec[SH_WRY] = st.rr[1] - st.asc * (p[1] * st.rr[0] - p[0] * st.rr[1]);
       break;
     }
     case LENSIM_SERSIC:
@@ -322,6 +377,14 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[SE_R2MAX] = (1000.0 * Rs_) * (1000.0 * Rs_);
       rec[SE_LNRS] = log(Rs_);
       rec[SE_BNLO] = rec[SE_BN] - double(float(rec[SE_BN]));
+      rec[SE_M00S] = rec[SE_M00] / Rs_;
+      rec[SE_M01S] = rec[SE_M01] / Rs_;
+      rec[SE_M11S] = rec[SE_M11] / Rs_;
+      rec[SE_UMIN] = 1e-8 * rec[SE_IRS2];
+      rec[SE_XSX] = rec[SE_M00S] * st.sx[0] + rec[SE_M01S] * st.sx[1];
+      rec[SE_YSX] = rec[SE_M01S] * st.sx[0] + rec[SE_M11S] * st.sx[1];
+      rec[SE_XRR] = rec[SE_M00S] * st.rr[0] + rec[SE_M01S] * st.rr[1];
+      rec[SE_YRR] = rec[SE_M01S] * st.rr[0] + rec[SE_M11S] * st.rr[1];
       break;
     }
     case LENSIM_GAUSSIAN: {
@@ -373,10 +436,22 @@ __device__ __forceinline__ void real_poly_at_complex(const ST *__restrict__ c, i
     b2 = a2;
     b1 = a3;
   }
-  for (; k >= 1; --k) {
-    const ST b0 = fma(sw, b1, fma(-tw, b2, c[k]));
+  // the last k < 4 steps, unrolled (no register rotation in a loop)
+  if (k == 3) {
+    const ST a0 = fma(sw, b1, fma(-tw, b2, c[3]));
+    const ST a1 = fma(sw, a0, fma(-tw, b1, c[2]));
+    const ST a2 = fma(sw, a1, fma(-tw, a0, c[1]));
+    b2 = a1;
+    b1 = a2;
+  } else if (k == 2) {
+    const ST a0 = fma(sw, b1, fma(-tw, b2, c[2]));
+    const ST a1 = fma(sw, a0, fma(-tw, b1, c[1]));
+    b2 = a0;
+    b1 = a1;
+  } else if (k == 1) {
+    const ST a0 = fma(sw, b1, fma(-tw, b2, c[1]));
     b2 = b1;
-    b1 = b0;
+    b1 = a0;
   }
   pr = fma(b1, wr, fma(-tw, b2, c[0]));
   pi = b1 * wi;
@@ -386,15 +461,11 @@ __device__ __forceinline__ void real_poly_at_complex(const ST *__restrict__ c, i
 // Metcalf 2015 eqs. 22-23), FP64 form.  The Lopez-Temme series
 // (Util/hyp2f1_util.py:136-176) in w = z/(z-2) with the reference's nmax terms
 // over the per-image coefficients F_n; the reference's nan_to_num policy.
-__device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double *__restrict__ coef, double x,
-                                             double y, double &ax, double &ay) {
-  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
-  const double c = r.d[EPL_C], s = r.d[EPL_S];
-  const double xr = dx * c + dy * s;
-  const double yr = -dx * s + dy * c;
-  const double q = r.d[EPL_Q], t = r.d[EPL_T], f = r.d[EPL_F];
-  const double Zr = q * xr, Zi = yr;
-  const double R2 = Zr * Zr + Zi * Zi;
+// The deflection in the major-axis frame (fr, fi) at Z = (Zr, Zi) = (q xr, yr),
+// R2 = |Z|^2.
+__device__ __forceinline__ void epl_major_f64(const Rec<double> &r, const double *__restrict__ coef, double Zr,
+                                              double Zi, double R2, double &fr, double &fi) {
+  const double t = r.d[EPL_T], f = r.d[EPL_F];
   double R = ::sqrt(R2);
   R = R > 0.000000001 ? R : 0.000000001;
   double Or, Oi;
@@ -424,8 +495,19 @@ __device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double 
   }
   // (b/R)^t = exp(t (ln b - ln R))
   const double scale = r.d[EPL_PREF] * ::exp(t * (r.d[EPL_LNB] - ::log(R)));
-  const double fr = nan_to_1e10(scale * Or);
-  const double fi = nan_to_1e10(scale * Oi);
+  fr = nan_to_1e10(scale * Or);
+  fi = nan_to_1e10(scale * Oi);
+}
+
+__device__ __forceinline__ void epl_defl_f64(const Rec<double> &r, const double *__restrict__ coef, double x,
+                                             double y, double &ax, double &ay) {
+  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+  const double c = r.d[EPL_C], s = r.d[EPL_S];
+  const double xr = dx * c + dy * s;
+  const double yr = -dx * s + dy * c;
+  const double Zr = r.d[EPL_Q] * xr, Zi = yr;
+  double fr, fi;
+  epl_major_f64(r, coef, Zr, Zi, Zr * Zr + Zi * Zi, fr, fi);
   ax += fr * c - fi * s;
   ay += fr * s + fi * c;
 }
@@ -466,17 +548,14 @@ __device__ __forceinline__ float expm1_f32(float x) {
 // Then alpha = 2b/(1+q) (b/R)^(t-1) (Z/R) (1 + delta), so its relative error
 // is about eps (|delta| + |(t-1) ln(b/R)|) instead of several float ulps.
 // Long series (nmax >= 100, f >= 0.73) form delta in double.
-__device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *__restrict__ coef_f,
-                                             const double *__restrict__ coef_d, double x, double y, double &ax,
-                                             double &ay) {
-  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
-  const double cd = r.d[EPL_C], sd = r.d[EPL_S];
-  const double Zrd = r.d[EPL_Q] * fma(dx, cd, dy * sd);
-  const double Zid = fma(-dx, sd, dy * cd);
-  const double R2d = fma(Zrd, Zrd, Zid * Zid);
+// The deflection in the major-axis frame (fr, fi) at Z = (Zrd, Zid) = (q xr, yr),
+// R2d = |Z|^2.
+__device__ __forceinline__ void epl_major_f32(const Rec<float> &r, const float *__restrict__ coef_f,
+                                              const double *__restrict__ coef_d, double Zrd, double Zid, double R2d,
+                                              double &fr, double &fi) {
   const float Zr = float(Zrd), Zi = float(Zid), R2 = float(R2d);
   float dr = 0.f, di = 0.f;  // delta (0 for f == 0, and at the centre where the reference's NaN becomes 0)
-  const int nf = (int)r[EPL_NF], nz = (int)r[EPL_NZ];
+  const int nf = __float_as_int(r[EPL_NF]), nz = __float_as_int(r[EPL_NZ]);
   // 1/|Z|, unclamped: the direction Z/|Z| enters 2F1 however small |Z| is
   // (float R^2 holds |Z| down to 1e-19; 0 at the centre, where delta = 0)
   const float iRs = R2 > 0.f ? __builtin_amdgcn_rsqf(R2) : 0.f;
@@ -486,12 +565,12 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
     const float f = r[EPL_F];
     const float ur = Zr * iRs, ui = Zi * iRs;
     const float zr = -f * ((ur - ui) * (ur + ui));
-    const float zi = -2.f * f * (ur * ui);
+    const float zi = r[EPL_M2F] * (ur * ui);
     float Pr, Pi;
     real_poly_at_complex<float>(coef_f + kZOff, nz, zr, zi, Pr, Pi);
     dr = fmaf(zr, Pr, -zi * Pi);
     di = fmaf(zr, Pi, zi * Pr);
-  } else if (r.d[EPL_NMAX] >= 100.0) {
+  } else if (__float_as_int(r[EPL_NMAX]) >= 100) {
     // f >= 0.73: |w| approaches f/(2-f) ~ 1 and the long series amplifies any
     // error of w, so delta is formed in double throughout (uniform per image)
     if (R2d > 0.0) {
@@ -535,15 +614,28 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
   const double Or = Zrd + double(fmaf(Zr, dr, -Zi * di));
   const double Oi = Zid + double(fmaf(Zr, di, Zi * dr));
   // 1/R with R = max(|Z|, 1e-9): one Newton step on the rsqrt, or 1e9 below the floor
-  const double R2c = R2d > 1e-18 ? R2d : 1e-18;
-  const float R2cf = float(R2c);
+  const bool above = R2d > 1e-18;
   double iR = double(iRs);
-  iR = R2d > 1e-18 ? iR * fma(-0.5 * R2c, iR * iR, 1.5) : 1e9;
+  iR = above ? iR * fma(-0.5 * R2d, iR * iR, 1.5) : 1e9;
   // (b/R)^(t-1)
+  const float R2cf = R2 > 1e-18f ? R2 : 1e-18f;
   const float lnbr = fmaf(-0.34657359f, __builtin_amdgcn_logf(R2cf), r[EPL_LNB]);  // ln b - ln(R^2)/2
   const float em = expm1_f32(r[EPL_TM1] * lnbr);
-  const double scale = r.d[EPL_PB] * iR * (1.0 + double(em));
-  const double fr = scale * Or, fi = scale * Oi;
+  const double pbr = r.d[EPL_PB] * iR;
+  const double scale = fma(pbr, double(em), pbr);  // 2b/(1+q) / R (b/R)^(t-1)
+  fr = scale * Or;
+  fi = scale * Oi;
+}
+
+__device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *__restrict__ coef_f,
+                                             const double *__restrict__ coef_d, double x, double y, double &ax,
+                                             double &ay) {
+  const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
+  const double cd = r.d[EPL_C], sd = r.d[EPL_S];
+  const double Zrd = r.d[EPL_Q] * fma(dx, cd, dy * sd);
+  const double Zid = fma(-dx, sd, dy * cd);
+  double fr, fi;
+  epl_major_f32(r, coef_f, coef_d, Zrd, Zid, fma(Zrd, Zrd, Zid * Zid), fr, fi);
   ax += fr * cd - fi * sd;
   ay += fr * sd + fi * cd;
 }
@@ -557,6 +649,15 @@ __device__ __forceinline__ void epl_defl_f32(const Rec<float> &r, const float *_
 // instantiated both ways, so the double library atan / atanh cost registers
 // only to the models that use them.  The gradient kernels keep the float form
 // (their bar is 1e-4).
+// The FP64 form's NIE in the major-axis frame at the rotated (xr, yr).
+__device__ __forceinline__ void nie_major_dbl(const double *rd, double xr, double yr, double &fx, double &fy) {
+  const double ss = rd[NIE_SS];
+  const double psi = ::sqrt(rd[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
+  const double sq = rd[NIE_SQ], bsq = rd[NIE_BSQ];
+  fx = bsq * ::atan(sq * xr / (psi + ss));
+  fy = bsq * ::atanh(sq * yr / (psi + rd[NIE_Q2S]));
+}
+
 template <typename T, bool DBL = false>
 __device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
   const double dx = x - r.d[R_CX], dy = y - r.d[R_CY];
@@ -564,11 +665,8 @@ __device__ __forceinline__ void nie_defl(const Rec<T> &r, double x, double y, do
   if constexpr (DBL || sizeof(T) == sizeof(double)) {
     const double xr = dx * cd + dy * sd;
     const double yr = -dx * sd + dy * cd;
-    const double ss = r.d[NIE_SS];
-    const double psi = ::sqrt(r.d[NIE_Q2] * (ss * ss + xr * xr) + yr * yr);
-    const double sq = r.d[NIE_SQ], bsq = r.d[NIE_BSQ];
-    const double fx = bsq * ::atan(sq * xr / (psi + ss));
-    const double fy = bsq * ::atanh(sq * yr / (psi + r.d[NIE_Q2S]));
+    double fx, fy;
+    nie_major_dbl(r.d, xr, yr, fx, fy);
     ax += fx * cd - fy * sd;
     ay += fx * sd + fy * cd;
   } else {
@@ -672,19 +770,23 @@ __device__ __forceinline__ float nfw_g(float X) {
 // g(X) = ln(X/2) + h(X) cancels around X ~ 1 (0.31 at X = 1 from terms of 0.7
 // and 1), where an NFW main deflector's Einstein ring sits; with a compact
 // cuspy source that moved the log-likelihood by 2.4e-6 relative.
+// The FP64 form's NFW deflection at (dx, dy) from its centre: alpha = a (dx, dy).
+__device__ __forceinline__ double nfw_factor_dbl(const double *rd, double dx, double dy) {
+  double R = ::sqrt(dx * dx + dy * dy);
+  R = R > 0.00000001 ? R : 0.00000001;
+  // X = R / Rs by division: g(X) has an exact X == 1 branch and cancels
+  // near it, so X must round as the reference's does.
+  const double X = R / rd[NFW_RS];
+  return rd[NFW_A] * nfw_g(X) / (X * X);
+}
+
 template <typename T, bool DBL = false>
 __device__ __forceinline__ void nfw_defl(const Rec<T> &r, double x, double y, double &ax, double &ay) {
   const double dxd = x - r.d[R_CX], dyd = y - r.d[R_CY];
   if constexpr (DBL || sizeof(T) == sizeof(double)) {
-    const double dx = dxd, dy = dyd;
-    double R = ::sqrt(dx * dx + dy * dy);
-    R = R > 0.00000001 ? R : 0.00000001;
-    // X = R / Rs by division: g(X) has an exact X == 1 branch and cancels
-    // near it, so X must round as the reference's does.
-    const double X = R / r.d[NFW_RS];
-    const double a = r.d[NFW_A] * nfw_g(X) / (X * X);
-    ax += a * dx;
-    ay += a * dy;
+    const double a = nfw_factor_dbl(r.d, dxd, dyd);
+    ax += a * dxd;
+    ay += a * dyd;
   } else {
     const T dx = T(dxd), dy = T(dyd);
     T R = Fn<T>::sqrt_(dx * dx + dy * dy);
@@ -888,6 +990,36 @@ __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Re
   const float e = r[SE_BN] * t;
   const float e_lo = fmaf(r[SE_BN], t, -e) + r[SE_BNLO] * t;
   return (R2 <= r.d[SE_R2MAX]) ? exp_accurate_f32(e, e_lo) : 0.f;
+}
+
+// The Sersic kernel (no amplitude) at the scaled elliptical coordinates
+// (xs, ys) = M (x - c) / R_sersic_, i.e. at u = R^2 / R_sersic_^2 (the
+// compile-time render's form: the matrix and 1/R_sersic_ are folded, and the
+// lens light's (xs, ys) advance by a constant step per sub-ray).  frac^(1/n)
+// = u^(1/(2n)); the floors and the window as sersic_kernel, on u.
+// umin: rd[SE_UMIN], which a caller may hoist (canonicalised) out of its loop.
+__device__ __forceinline__ double sersic_u_f64(const double *rd, double xs, double ys, double umin) {
+  const double u = fmax(fma(xs, xs, ys * ys), umin);
+  const double pw = ::exp(0.5 * ::log(u) * rd[SE_INVN]);
+  const double e = -rd[SE_BN] * (pw - 1.0);
+  double v = (u <= 1e6) ? ::exp(e) : 0.0;
+  if (v != v) v = 0.0;
+  if (v == INFINITY) v = 1.7976931348623157e308;
+  return v;
+}
+
+// Float form of sersic_u_f64 with sersic_value_f32's unbiased arithmetic.
+__device__ __forceinline__ float sersic_u_f32(const double *rd, const float *rf, double xs, double ys, double umin) {
+  const double u = fmax(fma(xs, xs, ys * ys), umin);
+  const float uf = float(u);
+  const float m = __builtin_amdgcn_frexp_mantf(uf);  // u = m 2^ex, m in [0.5, 1)
+  const int ex = __builtin_amdgcn_frexp_expf(uf);
+  const double L = double(ex) + double(__builtin_amdgcn_logf(m));
+  const float pw = __builtin_amdgcn_exp2f(float(rd[SE_C1] * L));
+  const float t = 1.f - pw;
+  const float e = rf[SE_BN] * t;
+  const float e_lo = fmaf(rf[SE_BN], t, -e) + rf[SE_BNLO] * t;
+  return (u <= 1e6) ? exp_accurate_f32(e, e_lo) : 0.f;
 }
 
 // One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
