@@ -1366,27 +1366,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     for (int j = 0; j < 4; ++j) dacc[t][j] = 0.0;
   const int rowbase = (16 * wave + li) * ls + 8 * lg;
   // One step per (kernel row a, K block ks).  The rows are padded with
-  // zero-B rows to a multiple of U; a body covers U rows (R = U * NKS steps)
-  // and ends with one float -> double fold.  Two register sets of B
-  // fragments (from L2) alternate between bodies: a set is refilled with the
-  // steps 2R ahead as soon as its body's MFMAs are issued, so its loads fly
-  // during the other set's body.  No load sits under a branch (the refills
-  // past the last step re-read the last fragments and are not used).
+  // zero-B rows to a multiple of U; a body covers U rows (R = U * NKS steps).
+  // Two register sets of B fragments (from L2) alternate between bodies: a
+  // set is refilled with the steps 2R ahead as soon as its body's MFMAs are
+  // issued, so its loads fly during the other set's body.  No load sits under
+  // a branch (the refills past the last step re-read the last fragments and
+  // are not used).  The float sums are folded into the float64 accumulators
+  // once per pair of bodies (2U kernel rows).
   constexpr int U = mfma_unroll(NKS), R = U * NKS;
   const int ky_pad = (c.ky + U - 1) / U * U, n_steps = ky_pad * NKS;
   half8 bq[2][R][2];
+  // The fragments are read as a buffer (resource in scalar registers): the
+  // step's offset is a scalar and the lane's a fixed vector offset, so a load
+  // needs no vector address arithmetic.  Reads past the n_steps fragments
+  // (none are issued) would return zeros.
+  const __amdgpu_buffer_rsrc_t frs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<half8 *>(frag), (short)0, n_steps * 2048, 0x00020000);
+  const int voff = lane * 16;
   auto refill = [&](half8 (&b)[R][2], int s0) {
 #pragma unroll
     for (int p = 0; p < R; ++p) {
-      const int q = s0 + p < n_steps ? s0 + p : n_steps - 1;
-      b[p][0] = frag[(q * 2 + 0) * 64 + lane];
-      b[p][1] = frag[(q * 2 + 1) * 64 + lane];
+      const int q = __builtin_amdgcn_readfirstlane(s0 + p < n_steps ? s0 + p : n_steps - 1);
+      b[p][0] = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(frs, voff, q * 2048, 0));
+      b[p][1] = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(frs, voff, q * 2048 + 1024, 0));
     }
   };
-  auto body = [&](const half8 (&b)[R][2], int a0) {
-    floatx4 acc[TL];
+  floatx4 acc[TL];
+  auto zero = [&]() {
 #pragma unroll
     for (int t = 0; t < TL; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  };
+  auto fold = [&]() {
+#pragma unroll
+    for (int t = 0; t < TL; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
+  };
+  auto body = [&](const half8 (&b)[R][2], int a0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int ar = a0 + u < c.ky ? a0 + u : c.ky - 1;  // padding rows re-read the last row (times zero B)
@@ -1411,21 +1427,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], b[p][0], acc[t], 0, 0, 0);
       }
     }
-#pragma unroll
-    for (int t = 0; t < TL; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) dacc[t][j] += double(acc[t][j]);
   };
   refill(bq[0], 0);
   refill(bq[1], R);
   int a0 = 0;
   for (; a0 + 2 * U <= ky_pad; a0 += 2 * U) {
+    zero();
     body(bq[0], a0);
     refill(bq[0], (a0 + 2 * U) * NKS);
     body(bq[1], a0 + U);
     refill(bq[1], (a0 + 3 * U) * NKS);
+    fold();
   }
-  if (a0 < ky_pad) body(bq[0], a0);  // an odd number of bodies: the last one
+  if (a0 < ky_pad) {  // an odd number of bodies: the last one
+    zero();
+    body(bq[0], a0);
+    fold();
+  }
   // lane l holds D[4 (l / 16) + j][l % 16] of each tile
   const double unscale = ldexp(1.0, -(sa + sk));
   // chi^2: one partial per 16 x 16 output tile, at [img][tile row][tile
