@@ -2,7 +2,10 @@
 
 hipcc cross-compiles without a GPU.  Packed-FP32 instructions are disabled on
 purpose (-target-feature -packed-fp32-ops); the host pass warns that it does
-not know that feature, which is harmless.
+not know that feature, which is harmless.  Matrix-core results stay in vector
+registers (-amdgpu-mfma-vgpr-form): with accumulation registers the compiler
+copied the float64 convolution's accumulators between the two register files
+on every loop iteration.
 """
 
 import os
@@ -21,6 +24,7 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-I" + os.path.join(ROOT, "include"),
 ]
 

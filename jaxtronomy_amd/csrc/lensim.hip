@@ -167,6 +167,25 @@ size_t mfma_smem(int kx, int ky, int tl) {
   return size_t(kMfTY + ky - 1) * mfma_ls(kx, tl) * 2 * sizeof(_Float16);
 }
 
+// ---- float64 matrix-core convolution (FP64 form) ----
+// v_mfma_f64_16x16x4_f64 (lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]
+// of a K-step): per kernel row a, D(16 x 16) += A(16 x 4 nk4) B(4 nk4 x 16)
+// with A = 16 image rows of the padded tile and B[k][n] = Kf[a][k - n]; nk4
+// K-steps of 4 cover 16 + kx - 1 columns.  The LDS row stride is 2 (mod 16)
+// doubles: the 32 lanes of each half of a ds_read_b64 (16 rows x 2 columns)
+// then fall on distinct banks.
+int mf64_nk4(int kx) { return (16 + kx - 1 + 3) / 4; }
+int mf64_tw(int kx, int tl) { return 16 * (tl - 1) + 4 * mf64_nk4(kx); }
+int mf64_ls(int kx, int tl) {
+  const int tw = mf64_tw(kx, tl);
+  return tw + ((2 - tw % 16) + 16) % 16;
+}
+size_t mf64_smem(int kx, int ky, int tl) { return size_t(kMfTY + ky - 1) * mf64_ls(kx, tl) * sizeof(double); }
+// K-steps (kernel rows x nk4) padded with zero fragments to a multiple of the
+// kernel's prefetch depth, so its step loop has no partial group
+constexpr int kMf64B = 8;
+int mf64_steps(int kx, int ky) { return (ky * mf64_nk4(kx) + kMf64B - 1) / kMf64B * kMf64B; }
+
 // Blocks of 64 rows x 16 tl columns.  All of a call's blocks are resident at
 // once on this workload's sizes, so the time is that of the busiest CU:
 // ceil(blocks / CUs) blocks of tl wave-tiles.  tl is the one in [2, 4] with
@@ -188,12 +207,12 @@ int cu_count() {
   }
   return n;
 }
-MfmaPlan mfma_plan(int nx, int ny, int n_img, int kx, int ky) {
+MfmaPlan mfma_plan(int nx, int ny, int n_img, int kx, int ky, bool f64 = false) {
   MfmaPlan best{0, 0, 0};
   long best_load = 0;
   const int cus = cu_count();
   for (int tl = 4; tl >= 2; --tl) {
-    if (mfma_smem(kx, ky, tl) > kMaxLds) continue;
+    if ((f64 ? mf64_smem(kx, ky, tl) : mfma_smem(kx, ky, tl)) > kMaxLds) continue;
     const int ntx = (nx + 16 * tl - 1) / (16 * tl), nt = ntx * ((ny + kMfTY - 1) / kMfTY);
     const long blocks = long(nt) * n_img;
     const long load = (blocks + cus - 1) / cus * tl;
@@ -271,8 +290,9 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
       off = align256(off + size_t(ss.kxl) * kyp_of(ss.kyl) * 8);
     }
   }
-  L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments
+  L.bfrag = off;  // int exponent (16 B slot) then the MFMA B fragments (float64 fragments in the FP64 form)
   if (precision == LENSIM_MIXED) off = align256(off + 16 + size_t(mfma_steps(kx, ky)) * 2 * 64 * 16);
+  else off = align256(off + size_t(mf64_steps(kx, ky)) * 64 * sizeof(double));
   L.total = off;
   return L;
 }
@@ -1241,7 +1261,21 @@ struct PsfJob {
 int psf_job_blocks(const PsfJob &pj) {
   if (pj.mode == 1) return (pj.kyp * pj.kx + 255) / 256;
   if (pj.mode == 2) return (pj.n_steps * 2 * 64 + 255) / 256;
+  if (pj.mode == 3) return (pj.n_steps * 64 + 255) / 256;
   return 0;
+}
+
+// B fragments of the float64 matrix-core convolution, [a][kk][lane] in the
+// lane layout of v_mfma_f64_16x16x4_f64 (lane l holds B[l / 16][l % 16] of
+// K-step kk): B[k][n] = Kf[a][k - n], Kf = the flipped kernel, zero outside;
+// the n_steps - ky nk4 padding steps are zero.
+__device__ void psf_mf64_prep_one(const double *__restrict__ k, int kx, int ky, int nk4, int n_steps,
+                                  double *__restrict__ frag, int idx) {
+  if (idx >= n_steps * 64) return;
+  const int lane = idx & 63, step = idx >> 6;
+  const int a = step / nk4, kk = step - a * nk4;
+  const int c = 4 * kk + (lane >> 4) - (lane & 15);  // tap of the flipped row a
+  frag[idx] = (a < ky && c >= 0 && c < kx) ? k[size_t(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
 }
 
 // Blocks [0, nb_prep): one thread per (image, profile) record; the rest: the PSF job.
@@ -1258,6 +1292,8 @@ __global__ __launch_bounds__(256) void prep_psf_kernel(ModelArgs m, const double
   const int b = blockIdx.x - nb_prep;
   if (pj.mode == 2) {
     psf_mfma_prep_block(pj.k, pj.kx, pj.ky, pj.nks, pj.n_steps, pj.hdr, pj.frag, b);
+  } else if (pj.mode == 3) {
+    psf_mf64_prep_one(pj.k, pj.kx, pj.ky, pj.nks, pj.n_steps, pj.kt, b * 256 + threadIdx.x);
   } else {
     const int idx = b * 256 + threadIdx.x;
     if (idx < pj.kyp * pj.kx) psf_prep_one(pj.k, pj.kx, pj.ky, pj.kyp, pj.kt, 1, idx);
@@ -1478,6 +1514,124 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   }
 }
 
+
+// Direct 'same' convolution in float64 on the matrix cores (FP64 form):
+// block = 64 rows x 16 TL columns (4 waves x TL tiles of 16 x 16), per
+// kernel row a and K-step kk one v_mfma_f64_16x16x4_f64 per tile; the padded
+// tile sits in LDS, the B fragments (psf_mf64_prep_one) come from L2 through
+// buffer loads.  Same outputs, chi^2 partials (per 16 x 16 tile) and
+// dlogL/dmodel as conv_mfma_kernel, all in float64.
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int TL>
+__global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int nk4, const double *__restrict__ in,
+                                                        const double *__restrict__ frag, double *__restrict__ out,
+                                                        double *__restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *tile = reinterpret_cast<double *>(smem_raw);
+  // XCD-aware order, as conv_mfma_kernel
+  int img = blockIdx.y, tile_id = blockIdx.x;
+  {
+    const int n_blk = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+    if ((n_blk & 7) == 0) {
+      const int logical = (lin & 7) * (n_blk >> 3) + (lin >> 3);
+      img = logical / gridDim.x;
+      tile_id = logical - img * gridDim.x;
+    }
+  }
+  const int tyi = tile_id / c.n_tiles_x, txi = tile_id - tyi * c.n_tiles_x;
+  const int Y0 = tyi * kMfTY, X0 = txi * 16 * TL;
+  const int cy = (c.ky - 1) / 2, cx = (c.kx - 1) / 2;
+  const int TH = kMfTY + c.ky - 1, TW = 16 * (TL - 1) + 4 * nk4;
+  const int tid = threadIdx.x;
+  const double *src = in + size_t(img) * c.nx * c.ny;
+  // padded tile: eight loads in flight per thread before the LDS stores
+  constexpr int kLd = 8;
+  for (int base = tid; base < TH * TW; base += 256 * kLd) {
+    double v[kLd];
+    int at[kLd];
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int idx = base + u * 256;
+      const int ly = idx / TW, lx = idx - ly * TW;
+      int gy = Y0 - cy + ly, gx = X0 - cx + lx;
+      v[u] = 0.0;
+      at[u] = idx < TH * TW ? ly * ls + lx : -1;
+      if (idx >= TH * TW) {
+      } else if (c.edge) {
+        gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
+        gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
+        v[u] = src[size_t(gy) * c.nx + gx];
+      } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
+        v[u] = src[size_t(gy) * c.nx + gx];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kLd; ++u)
+      if (at[u] >= 0) tile[at[u]] = v[u];
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, li = lane & 15, lg = lane >> 4;
+  // lane (li, lg) reads A[li][lg] of each K-step: tile row 16 wave + li + a,
+  // column 16 t + 4 kk + lg
+  const double *arow = tile + (16 * wave + li) * ls + lg;
+  // n_steps: a multiple of kMf64B; the steps past ky nk4 have zero B
+  // fragments and re-read A of the last kernel row (a stays at ky - 1)
+  const int n_steps = (c.ky * nk4 + kMf64B - 1) / kMf64B * kMf64B;
+  const __amdgpu_buffer_rsrc_t frs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(frag), (short)0, n_steps * 512, 0x00020000);
+  const int voff = lane * 8;
+  double4_t acc[TL];
+#pragma unroll
+  for (int t = 0; t < TL; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  // the B fragments of kMf64B steps are requested ahead of their MFMAs
+  double bq[kMf64B];
+#pragma unroll
+  for (int p = 0; p < kMf64B; ++p)
+    bq[p] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(frs, voff, p * 512, 0));
+  int a = 0, kk = 0;
+  for (int s0 = 0; s0 < n_steps; s0 += kMf64B) {
+#pragma unroll
+    for (int p = 0; p < kMf64B; ++p) {
+      const double *ap = arow + a * ls + 4 * kk;
+#pragma unroll
+      for (int t = 0; t < TL; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[16 * t], bq[p], acc[t], 0, 0, 0);
+      if (++kk == nk4) {
+        kk = 0;
+        if (a + 1 < c.ky) ++a;
+      }
+      // refill slot p with step s0 + p + kMf64B (the refills past the end re-read the last step, unused)
+      const int q = __builtin_amdgcn_readfirstlane(s0 + p + kMf64B < n_steps ? s0 + p + kMf64B : n_steps - 1);
+      bq[p] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(frs, voff, q * 512, 0));
+    }
+  }
+  // result register j of lane l holds D[l / 16 + 4 j][l % 16] of each tile
+  // (measured on gfx950); chi^2 partials per 16 x 16 tile
+  const int n16x = (c.nx + 15) >> 4, n16 = n16x * ((c.ny + 15) >> 4);
+  const int ty16 = (Y0 >> 4) + wave;
+#pragma unroll
+  for (int t = 0; t < TL; ++t) {
+    const int x = X0 + 16 * t + li;
+    double chi = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int y = Y0 + 16 * wave + lg + 4 * j;
+      if (x < c.nx && y < c.ny) {
+        const size_t p = size_t(y) * c.nx + x;
+        const double m = acc[t][j];
+        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (partial) chi += chi2_pixel<double>(c, p, m, img);
+        if (c.gout) static_cast<double *>(c.gout)[size_t(img) * c.nx * c.ny + p] = dlogl_dm(c, p, m, img);
+      }
+    }
+    if (partial) {
+      for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
+      const int tx16 = (X0 >> 4) + t;
+      if (lane == 0 && 16 * tx16 < c.nx && 16 * ty16 < c.ny)
+        partial[size_t(img) * n16 + size_t(ty16) * n16x + tx16] = chi;
+    }
+  }
+}
 
 // ------------------------------------------------------------- gradient ---
 // Number of kwargs of a profile type (its slots in a parameter block).
@@ -1963,6 +2117,40 @@ int launch_conv_mfma_tl(const ConvArgs &c, int tl, dim3 grid, const float *in, c
   }
 }
 
+// Whether the FP64 form convolves on the matrix cores (kernels of at least
+// 21 x 21 whose tile fits the LDS; smaller ones use the direct stencil).
+template <typename T>
+bool use_mf64(bool identity, int kx, int ky) {
+  if (sizeof(T) != sizeof(double) || identity) return false;
+  return kx >= 21 && ky >= 21 && mf64_smem(kx, ky, 2) <= kMaxLds;
+}
+
+template <int TL>
+int launch_conv_mf64_n(const ConvArgs &c, dim3 grid, const double *in, const double *frag, double *out,
+                       double *partial, hipStream_t st) {
+  const size_t smem = mf64_smem(c.kx, c.ky, TL);
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_mf64_kernel<TL>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(conv_mf64_kernel)");
+  }
+  conv_mf64_kernel<TL><<<grid, 256, smem, st>>>(c, mf64_ls(c.kx, TL), mf64_nk4(c.kx), in, frag, out, partial);
+  return check_launch("conv_mf64_kernel");
+}
+
+int launch_conv_mf64(ConvArgs c, const MfmaPlan &pl, int n_img, const double *in, const double *frag, double *out,
+                     double *partial, hipStream_t st) {
+  if (pl.tl == 0) return fail(LENSIM_EUNSUPPORTED, "psf %dx%d exceeds the float64 matrix-core tile's LDS", c.ky, c.kx);
+  c.n_tiles_x = pl.n_tiles_x;
+  const dim3 grid(pl.n_tiles, n_img);
+  switch (pl.tl) {
+    case 2: return launch_conv_mf64_n<2>(c, grid, in, frag, out, partial, st);
+    case 3: return launch_conv_mf64_n<3>(c, grid, in, frag, out, partial, st);
+    case 4: return launch_conv_mf64_n<4>(c, grid, in, frag, out, partial, st);
+    default: return fail(LENSIM_EINVAL, "matrix-core tile width %d", pl.tl);
+  }
+}
+
 // The MFMA convolution of n_img images by plan pl (c.kx, c.ky, c.edge set;
 // c.n_tiles_x is set here).
 int launch_conv_mfma(ConvArgs c, const MfmaPlan &pl, int n_img, const float *in, const int *hdr, const half8 *frag,
@@ -1987,23 +2175,29 @@ int run_image(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, i
   const ModelArgs m = with_grid(m_in, g);
   const bool identity = !psf || psf->type == LENSIM_PSF_NONE || (flags & LENSIM_IMAGE_UNCONVOLVED);
   const bool mfma = use_mfma<T>(identity, kx, ky);
+  const bool mf64 = use_mf64<T>(identity, kx, ky);
   const int kyp = kyp_of(ky);
   double *kf = reinterpret_cast<double *>(ws + L.psf);
   int *hdr = reinterpret_cast<int *>(ws + L.bfrag);
   half8 *frag = reinterpret_cast<half8 *>(ws + L.bfrag + 16);
+  double *frag64 = reinterpret_cast<double *>(ws + L.bfrag);
   // records and the PSF preparation in one launch
   PsfJob pj{};
-  pj.mode = mfma ? 2 : 1;
+  pj.mode = mfma ? 2 : (mf64 ? 3 : 1);
   pj.k = identity ? nullptr : psf->d_kernel;
   pj.kx = kx;
   pj.ky = ky;
   pj.kyp = kyp;
-  pj.kt = kf;
+  pj.kt = mf64 ? frag64 : kf;
   pj.hdr = hdr;
   pj.frag = frag;
   if (mfma) {
     pj.nks = mfma_nks(kx);
     pj.n_steps = mfma_steps(kx, ky);
+  }
+  if (mf64) {
+    pj.nks = mf64_nk4(kx);
+    pj.n_steps = mf64_steps(kx, ky);
   }
   const int nb_prep = (n_img * m.n_prof + 255) / 256;
   const bool dc = gout != nullptr;
@@ -2050,6 +2244,11 @@ int run_image(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, i
     n_tiles = n16_tiles(g.nx, g.ny);  // chi^2 partials of the 16 x 16 output tiles
     rc = launch_conv_mfma(c, pl, n_img, reinterpret_cast<const float *>(lowres), hdr, frag, blkmax,
                           (npix + 255) / 256, reinterpret_cast<float *>(out), partial, st);
+  } else if (mf64) {
+    const MfmaPlan pl = mfma_plan(g.nx, g.ny, n_img, kx, ky, true);
+    n_tiles = n16_tiles(g.nx, g.ny);
+    rc = launch_conv_mf64(c, pl, n_img, reinterpret_cast<const double *>(lowres), frag64,
+                          reinterpret_cast<double *>(out), partial, st);
   } else {
     rc = launch_conv_kernel<T>(c, n_img, lowres, kf, out, partial, &n_tiles, st);
   }
