@@ -1589,17 +1589,31 @@ __global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int 
 #pragma unroll
   for (int p = 0; p < kMf64B; ++p)
     bq[p] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(frs, voff, p * 512, 0));
+  // A of a step is read from LDS one step ahead, so its latency falls under
+  // the previous step's MFMAs (a block's LDS leaves room for one block per CU
+  // at the usual tile widths: no other wave hides it)
   int a = 0, kk = 0;
+  double av[TL];
+#pragma unroll
+  for (int t = 0; t < TL; ++t) av[t] = arow[16 * t];
   for (int s0 = 0; s0 < n_steps; s0 += kMf64B) {
 #pragma unroll
     for (int p = 0; p < kMf64B; ++p) {
-      const double *ap = arow + a * ls + 4 * kk;
-#pragma unroll
-      for (int t = 0; t < TL; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[16 * t], bq[p], acc[t], 0, 0, 0);
-      if (++kk == nk4) {
-        kk = 0;
-        if (a + 1 < c.ky) ++a;
+      int a1 = a, kk1 = kk + 1;
+      if (kk1 == nk4) {
+        kk1 = 0;
+        if (a1 + 1 < c.ky) ++a1;
       }
+      const double *apn = arow + a1 * ls + 4 * kk1;
+      double an[TL];
+#pragma unroll
+      for (int t = 0; t < TL; ++t) an[t] = apn[16 * t];
+#pragma unroll
+      for (int t = 0; t < TL; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bq[p], acc[t], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < TL; ++t) av[t] = an[t];
+      a = a1;
+      kk = kk1;
       // refill slot p with step s0 + p + kMf64B (the refills past the end re-read the last step, unused)
       const int q = __builtin_amdgcn_readfirstlane(s0 + p + kMf64B < n_steps ? s0 + p + kMf64B : n_steps - 1);
       bq[p] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(frs, voff, q * 512, 0));
