@@ -350,12 +350,15 @@ class ImageModel(object):
         backend.check(rc, "lensim_loglike")
         return logl
 
-    def likelihood_and_grad_batch(self, params, n_img=None):
+    def likelihood_and_grad_batch(self, params, n_img=None, model=None):
         """(logL [B], dlogL/dparams [B, n_profiles, 8]) of a packed device parameter
-        block; the gradient slot layout is the parameter-block layout."""
+        block; the gradient slot layout is the parameter-block layout.  A
+        LensModelBulk deflector passes the lensim_model of pack_bulk as model
+        (single-plane lists: multi-plane gradients are not available)."""
         lib = backend.lib()
-        if self._bulk:
-            raise NotImplementedError("gradients of bulk deflector lists are not supported")
+        if self._bulk and model is None:
+            raise ValueError("a LensModelBulk deflector needs the model of pack_bulk")
+        m = self._model if model is None else model
         if self._frame is not None:
             raise NotImplementedError("gradients with flux_evaluate_indexes sub-frames are not supported")
         if n_img is None:
@@ -364,12 +367,12 @@ class ImageModel(object):
         logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
         grad = torch.empty((n_img, params.shape[1], backend.NPARAM), dtype=torch.float64, device=params.device)
         d = self._data_struct()
-        nbytes = lib.lensim_grad_workspace_size(ctypes.byref(self._model), ctypes.byref(self._grid),
+        nbytes = lib.lensim_grad_workspace_size(ctypes.byref(m), ctypes.byref(self._grid),
                                                 ctypes.byref(self._psf), n_img, self._prec)
         if nbytes == 0:
             raise ValueError("gradient workspace query failed: %s" % lib.lensim_last_error().decode())
         ws = self._ws.get(nbytes)
-        rc = lib.lensim_loglike_grad(ctypes.byref(self._model), ctypes.byref(self._grid), ctypes.byref(self._psf),
+        rc = lib.lensim_loglike_grad(ctypes.byref(m), ctypes.byref(self._grid), ctypes.byref(self._psf),
                                      ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
                                      backend.ptr(grad), backend.ptr(ws), ws.numel(), backend.stream_handle())
         backend.check(rc, "lensim_loglike_grad")
@@ -383,10 +386,21 @@ class ImageModel(object):
     def likelihood_gradient(self, kwargs_lens=None, kwargs_source=None, kwargs_lens_light=None):
         """logL and its gradient in kwargs form: (logL, [grad dicts for lens],
         [source], [lens light]), like jax.grad of likelihood_data_given_model
-        (test/test_ImSim/test_image_model.py:274-335)."""
-        params, n_img = self.pack(kwargs_lens, kwargs_source, kwargs_lens_light)
-        logl, grad = self.likelihood_and_grad_batch(backend.to_device_f64(params), n_img)
-        g = grad.cpu().numpy()
+        (test/test_ImSim/test_image_model.py:274-335).  With a LensModelBulk
+        deflector, kwargs_lens is its ray_shooting_kwargs and the lens gradient is
+        a dict in the form of its "all_kwargs" (LensModelBulk.unpack_gradient)."""
+        if self._bulk:
+            params, n_img, m, keep = self.pack_bulk(kwargs_lens, kwargs_source, kwargs_lens_light)
+            logl, grad = self.likelihood_and_grad_batch(backend.to_device_f64(params), n_img, model=m)
+            del keep
+            g = grad.cpu().numpy()
+            n_defl = g.shape[1] - len(self._src_types) - len(self._light_types)
+            glens = self.LensModel.unpack_gradient(kwargs_lens, g[:, :n_defl])
+            g = g[:, n_defl:]
+        else:
+            params, n_img = self.pack(kwargs_lens, kwargs_source, kwargs_lens_light)
+            logl, grad = self.likelihood_and_grad_batch(backend.to_device_f64(params), n_img)
+            g = grad.cpu().numpy()
         out, k = [], 0
         for types in (self._lens_types, self._src_types, self._light_types):
             lst = []
@@ -395,6 +409,8 @@ class ImageModel(object):
                             for i, nm in profiles.kwarg_slots(self._names[k])})
                 k += 1
             out.append(lst)
+        if self._bulk:
+            out[0] = glens
         return (logl if n_img > 1 else logl[0]), out[0], out[1], out[2]
 
     # ------------------------------------------------------------ drop-in API --

@@ -234,7 +234,27 @@ class LensModelBulk(object):
         packed = {d: j for j, d in enumerate(inline_idx)}
         packed.update({d: len(inline_idx) + j for j, d in enumerate(bulk_idx)})
         self._plane_order = [packed[d] for d in range(n_def) if d in packed]  # packed index in dict order
+        self._packed = packed
         return n_img, it, ip, bt, bp
+
+    def unpack_gradient(self, ray_shooting_kwargs, grad):
+        """A gradient over the packed deflector rows of the last pack() (grad
+        [B, n_inline + n_bulk, 8]) in the form of ray_shooting_kwargs["all_kwargs"]:
+        {kwarg: [B, n_deflectors]} (or [n_deflectors] for B = 1), the entry of
+        deflector d the derivative in its own kwarg; kwargs a deflector's profile
+        does not have, and NULL padding, are zero -- the structure jax.grad gives
+        for the dict argument of ray_shooting."""
+        index_list = np.asarray(ray_shooting_kwargs["index_list"], dtype=int)
+        keys = ray_shooting_kwargs["all_kwargs"].keys()
+        g = np.asarray(grad)
+        out = {kw: np.zeros((g.shape[0], len(index_list))) for kw in keys}
+        for d, j in self._packed.items():
+            for i, pn in enumerate(self.param_name_list[index_list[d]]):
+                if pn in out:
+                    out[pn][:, d] += g[:, j, i]
+        if g.shape[0] == 1:
+            out = {kw: v[0] for kw, v in out.items()}
+        return out
 
     def _model(self, inline_types, bulk_types, source_types=(), light_types=(), flux_scaling=1.0,
                ray_shooting_kwargs=None):

@@ -185,13 +185,38 @@ __device__ void nie_g(const S &theta_E, const S &e1, const S &e2, const S &s_sca
   ay = ay + (fx * sn + fy * cs);
 }
 
-// NFW.g, nfw.py:345-365
+// F(X) of the NFW family: arctanh(sqrt(1 - X^2)) / sqrt(1 - X^2) = arccosh(1/X)
+// / sqrt(1 - X^2) below X = 1, arctan(sqrt(X^2 - 1)) / sqrt(X^2 - 1) = arccos(1/X)
+// / sqrt(X^2 - 1) above, 1 at X = 1 (nfw.py:345-365, tnfw.py:288-304).  Both
+// sides are the one series sum_n (-u)^n / (2n + 1) in u = X^2 - 1, used for
+// |u| < u0: there the closed forms' derivatives are differences of two terms
+// that agree to O(u), so their duals lose (epsilon / |u|) relative precision
+// (all of it in float within a few 1e-7 of X = 1); the series carries the
+// derivative with no cancellation.  u0 = 0.1 (float) leaves a truncation of
+// 0.1^9 / 19 after nine terms, u0 = 0.01 (double) 0.01^9 / 19.
+template <typename T, typename S>
+__device__ __forceinline__ S nfw_F_g(const S &X) {
+  const S u = X * X - T(1);
+  const T u0 = sizeof(T) == sizeof(float) ? T(0.1) : T(0.01);
+  if (val(u) < u0 && val(u) > -u0) {
+    S f = lift(T(1) / T(17), u);
+#pragma unroll
+    for (int n = 7; n >= 0; --n) f = T(1) / T(2 * n + 1) - u * f;
+    return f;
+  }
+  if (val(u) < T(0)) {
+    const S r = sqrt(-u);
+    return atanh(r) / r;
+  }
+  const S r = sqrt(u);
+  return atan(r) / r;
+}
+
+// NFW.g, nfw.py:345-365: log(X/2) + F(X)
 template <typename T, typename S>
 __device__ __forceinline__ S nfw_g_g(S X) {
   X = fmax(X, T(0.000001));
-  if (val(X) < T(1)) return log(X * T(0.5)) + T(1) / sqrt(T(1) - X * X) * acosh(T(1) / X);
-  if (val(X) == T(1)) return lift(T(1) + T(::log(0.5)), X);
-  return log(X * T(0.5)) + T(1) / sqrt(X * X - T(1)) * acos(T(1) / X);
+  return log(X * T(0.5)) + nfw_F_g<T>(X);
 }
 
 // NFW, nfw.py:86-105, :257-279, :384-395.  p: Rs, alpha_Rs, center_x, center_y.
@@ -218,16 +243,7 @@ __device__ void tnfw_g(const S *p, const S &x, const S &y, S &ax, S &ay) {
   const S X = fmax(R / Rs, T(0.001));
   const S tau = p[2] / Rs;
   const S t2 = tau * tau;
-  S F;
-  if (val(X) < T(1)) {
-    const S v = T(1) - X * X;
-    F = atanh(sqrt(v)) / sqrt(v);
-  } else if (val(X) > T(1)) {
-    const S v = X * X - T(1);
-    F = atan(sqrt(v)) / sqrt(v);
-  } else {
-    F = X;
-  }
+  const S F = nfw_F_g<T>(X);
   const S L = log(X / (tau + sqrt(t2 + X * X)));
   const S g = t2 / ((t2 + T(1)) * (t2 + T(1))) *
               ((t2 + T(1) + T(2) * (X * X - T(1))) * F + tau * T(M_PI) + (t2 - T(1)) * log(tau) +

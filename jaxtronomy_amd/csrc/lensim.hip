@@ -493,16 +493,10 @@ __device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double 
   return r;
 }
 
+// The deflection of the bulk list (records n_lens .. n_defl - 1) added to (ax, ay).
 template <typename T, bool HALO, bool DBL = false>
-__device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
-                                            double &ax, double &ay) {
-  for (int k = 0; k < m.n_lens; ++k) {
-    const int type = m.types[k];
-    const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
-    const int e = m.epl_index[k];
-    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
-    add_deflection<T, HALO, DBL>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
-  }
+__device__ __forceinline__ void deflect_bulk(const ModelArgs &m, const RecPtrs<T> &rp, double x, double y,
+                                             double &ax, double &ay) {
   if (m.n_bulk == 0) return;
   if (m.bulk_type == LENSIM_TNFW) {
     // common case of substructure: one profile type, no per-deflector switch
@@ -517,6 +511,19 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
     add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
+}
+
+template <typename T, bool HALO, bool DBL = false>
+__device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
+                                            double &ax, double &ay) {
+  for (int k = 0; k < m.n_lens; ++k) {
+    const int type = m.types[k];
+    const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
+    const int e = m.epl_index[k];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    add_deflection<T, HALO, DBL>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+  }
+  deflect_bulk<T, HALO, DBL>(m, rp, x, y, ax, ay);
 }
 
 // One deflector k (inline lens k < n_lens with its series coefficients, or bulk).
@@ -1649,7 +1656,7 @@ __global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int 
 
 // ------------------------------------------------------------- gradient ---
 // Number of kwargs of a profile type (its slots in a parameter block).
-__host__ __device__ inline int n_params_of(int type) {
+__host__ __device__ constexpr int n_params_of(int type) {
   switch (type) {
     case LENSIM_EPL: return 6;
     case LENSIM_SIE: return 5;
@@ -1740,6 +1747,7 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
       add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + co,
                               rp.coeft + co, x, y, axp, ayp);
     }
+    deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
     S ax = S(T(axp)), ay = S(T(ayp));
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
@@ -1772,6 +1780,35 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
   for (int i = 0; i < N; ++i) acc[i] += w * double(flux.d[i]);
 }
 
+// ray_grad for an inline NFW or TNFW deflector k: the other deflectors in the
+// forward form, this one's duals in double in both forms (the float closed
+// forms of the NFW family cancel, as for the bulk list's duals).
+template <typename T, int N, bool HALO>
+__device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+                                              int k, int type, double x, double y, double w, double *acc) {
+  using S = gm::Dual<double, N>;
+  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  double axp = 0.0, ayp = 0.0;
+  for (int j = 0; j < m.n_lens; ++j) {
+    if (j == k) continue;
+    const int e = m.epl_index[j];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + co,
+                            rp.coeft + co, x, y, axp, ayp);
+  }
+  deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
+  S ps[LENSIM_NPARAM];
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(pk[i], i) : S(pk[i]);
+  S ax = S(axp), ay = S(ayp);
+  gm::defl_g<double, S>(type, ps, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), 1,
+                        S(x), S(y), ax, ay);
+  const S bx = S(x) - ax * m.alpha_scaling, by = S(y) - ay * m.alpha_scaling;
+  const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, T(bx.v), T(by.v));
+#pragma unroll
+  for (int i = 0; i < N; ++i) acc[i] += w * (double(f2.d[0]) * bx.d[i] + double(f2.d[1]) * by.d[i]);
+}
+
 // Contribution of one ray to d flux / d(kwargs of the EPL deflector k).
 // Three dual seeds carry gamma, e1 and e2; theta_E and the centre are in
 // closed form: alpha scales as b^t with b = theta_E sqrt(q), so
@@ -1792,6 +1829,7 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
     add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
                             rp.coeft + coj, x, y, axp, ayp);
   }
+  deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
   S ps[LENSIM_NPARAM];
 #pragma unroll
   for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = (i >= 1 && i <= 3) ? S::seed(T(pk[i]), i - 1) : S(T(pk[i]));
@@ -1891,8 +1929,8 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc);
       else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_NFW) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_TNFW) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_NFW) halo_ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_TNFW) halo_ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SERSIC) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SERSIC_ELLIPSE) ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
@@ -1941,6 +1979,151 @@ int launch_grad_profile(const ModelArgs &m, const GridArgs &g, const GradArgs &g
   }
 #undef LENSIM_GRAD
   return check_launch("grad_kernel");
+}
+
+// d flux / d(kwargs) of the bulk deflectors (LensModelBulk lists, records
+// n_lens .. n_defl - 1).  With beta = theta - alpha_scaling sum_j alpha_j,
+// d(ray flux)/d p_k = -alpha_scaling grad S(beta) . d alpha_k / d p_k, so a
+// ray's total deflection and source gradient are needed once, and then each
+// deflector's own deflection in duals.  Block (b, img, c): the rays of pixels
+// [b kGradPix 256, (b + 1) kGradPix 256) of image img and deflectors
+// n_lens + c dpb + d, d < dpb (dpb = 64, 128 or 256 threads per deflector
+// group, 256 / dpb groups of whole waves).  Per tile of 256 rays each thread
+// traces one ray into LDS: (x, y) and g = w (-alpha_scaling) dS/dbeta; then
+// group grp's thread d runs its deflector's duals over the tile's rays grp,
+// grp + 256/dpb, ... (the same ray for every lane of a wave: the zero-weight
+// skip is uniform), and the groups' sums are added in a fixed order.
+// TYPE is the list's common type, or LENSIM_NULL for a list of several types
+// (a switch per deflector, outside the ray loop).
+// The deflectors' duals run in double in both forms: the NFW family's closed
+// forms cancel inside and outside the scale radius (the float forward
+// rewrites them; their duals would not follow), which costs a float dual all
+// its precision in the derivative by the centre.
+template <int TYPE, int N>
+__device__ __forceinline__ void bulk_tile_duals(const double *pk, const double *lx, const double *ly,
+                                                const double *lgx, const double *lgy, int r0, int step, int r1,
+                                                double *acc) {
+  using S = gm::Dual<double, N>;
+  S ps[LENSIM_NPARAM];
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(pk[i], i) : S(pk[i]);
+  for (int r = r0; r < r1; r += step) {
+    const double gx = lgx[r], gy = lgy[r];
+    if (gx == 0.0 && gy == 0.0) continue;
+    S ax = S(0.0), ay = S(0.0);
+    gm::defl_g<double, S>(TYPE, ps, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), 1,
+                          S(lx[r]), S(ly[r]), ax, ay);
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] += gx * ax.d[i] + gy * ay.d[i];
+  }
+}
+
+template <typename T, bool HALO, int TYPE>
+__global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
+                                                        const double *__restrict__ recd,
+                                                        const float *__restrict__ recf,
+                                                        const double *__restrict__ coefd,
+                                                        const float *__restrict__ coeff, int dpb) {
+  const int img = blockIdx.y;
+  const int npix = g.nx * g.ny, ss = g.s * g.s;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const int t = threadIdx.x, d = t % dpb, grp = t / dpb, ngrp = blockDim.x / dpb;
+  const int k = m.n_lens + blockIdx.z * dpb + d;
+  const bool mine = k < m.n_defl;
+  int type = LENSIM_NULL;
+  if (mine) type = TYPE != LENSIM_NULL ? TYPE : m.bulk_types[k - m.n_lens];
+  double pk[LENSIM_NPARAM], acc[LENSIM_NPARAM];
+#pragma unroll
+  for (int q = 0; q < LENSIM_NPARAM; ++q) {
+    pk[q] = mine ? ga.params[(size_t(img) * m.n_prof + k) * LENSIM_NPARAM + q] : 0.0;
+    acc[q] = 0.0;
+  }
+  __shared__ double lx[256], ly[256], lgx[256], lgy[256];
+  __shared__ double red[256][LENSIM_NPARAM];
+  const int p0 = blockIdx.x * kGradPix * 256, p1 = min(npix, p0 + kGradPix * 256);
+  const int r_end = (p1 - p0) * ss;  // rays of the block, pixel-major, s x s sub-rays per pixel
+  const double inv_s = 1.0 / g.s, o0 = -0.5 + 1.0 / (2.0 * g.s), as = m.alpha_scaling;
+  for (int base = 0; base < r_end; base += 256) {
+    // one ray per thread: position, weight and the source gradient at its beta
+    {
+      const int rr = base + t;
+      const bool active = rr < r_end;
+      const int pix = p0 + (active ? rr / ss : 0), sub = active ? rr % ss : 0;
+      const int i = pix % g.nx, j = pix / g.nx, sy = sub / g.s, sx = sub % g.s;
+      const double ox = o0 + sx * inv_s, oy = o0 + sy * inv_s;
+      const double x = i * g.m00 + j * g.m01 + g.ra0 + g.m00 * ox + g.m01 * oy;
+      const double y = i * g.m10 + j * g.m11 + g.dec0 + g.m10 * ox + g.m11 * oy;
+      const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
+                                    m.flux_scaling
+                              : 0.0;
+      double gx = 0.0, gy = 0.0;
+      if (w != 0.0 && m.n_source > 0 && evaluated(g, i, j)) {
+        double ax = 0.0, ay = 0.0;
+        deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
+        const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, T(x - ax * as), T(y - ay * as));
+        gx = -as * w * double(f2.d[0]);
+        gy = -as * w * double(f2.d[1]);
+      }
+      lx[t] = x;
+      ly[t] = y;
+      lgx[t] = gx;
+      lgy[t] = gy;
+    }
+    __syncthreads();
+    const int r1 = min(256, r_end - base);
+    if constexpr (TYPE != LENSIM_NULL) {
+      if (mine) bulk_tile_duals<TYPE, n_params_of(TYPE)>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+    } else {
+      switch (type) {
+        case LENSIM_SIE: bulk_tile_duals<LENSIM_SIE, 5>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_NIE: bulk_tile_duals<LENSIM_NIE, 6>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_NFW:
+          if constexpr (HALO) bulk_tile_duals<LENSIM_NFW, 4>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          break;
+        case LENSIM_TNFW:
+          if constexpr (HALO) bulk_tile_duals<LENSIM_TNFW, 5>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          break;
+        case LENSIM_SHEAR: bulk_tile_duals<LENSIM_SHEAR, 4>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        default: break;  // no kwargs: zero partials
+      }
+    }
+    __syncthreads();
+  }
+  // the groups' sums in group order
+#pragma unroll
+  for (int q = 0; q < LENSIM_NPARAM; ++q) red[t][q] = acc[q];
+  __syncthreads();
+  if (grp == 0 && mine) {
+#pragma unroll
+    for (int q = 0; q < LENSIM_NPARAM; ++q) {
+      double v = 0.0;
+      for (int gi = 0; gi < ngrp; ++gi) v += red[gi * dpb + d][q];
+      ga.gpart[((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + q] = v;
+    }
+  }
+}
+
+template <typename T, bool HALO>
+int launch_bulk_grad(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, const double *recd,
+                     const float *recf, int nblk, int n_img, hipStream_t st) {
+  const int dpb = m.n_bulk <= 64 ? 64 : (m.n_bulk <= 128 ? 128 : 256);
+  const dim3 grid(nblk, n_img, (m.n_bulk + dpb - 1) / dpb);
+#define LENSIM_BULK_GRAD(TY) \
+  bulk_grad_kernel<T, HALO, TY><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff, dpb)
+  switch (m.bulk_type) {
+    case LENSIM_SIE: LENSIM_BULK_GRAD(LENSIM_SIE); break;
+    case LENSIM_NIE: LENSIM_BULK_GRAD(LENSIM_NIE); break;
+    case LENSIM_NFW:
+      if constexpr (HALO) LENSIM_BULK_GRAD(LENSIM_NFW);
+      break;
+    case LENSIM_TNFW:
+      if constexpr (HALO) LENSIM_BULK_GRAD(LENSIM_TNFW);
+      break;
+    case LENSIM_SHEAR: LENSIM_BULK_GRAD(LENSIM_SHEAR); break;
+    default: LENSIM_BULK_GRAD(LENSIM_NULL); break;  // several types: d_bulk_types per deflector
+  }
+#undef LENSIM_BULK_GRAD
+  return check_launch("bulk_grad_kernel");
 }
 
 __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, int n, double *__restrict__ grad) {
@@ -2593,6 +2776,13 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
   for (int k = 0; k < m.n_prof; ++k) {
+    if (k == m.n_lens && m.n_bulk > 0) {  // the bulk list in one launch
+      rc = m.has_halo ? launch_bulk_grad<T, true>(m, gg, ga, recd, recf, nblk, n_img, st)
+                      : launch_bulk_grad<T, false>(m, gg, ga, recd, recf, nblk, n_img, st);
+      if (rc) return rc;
+      k = m.n_defl - 1;
+      continue;
+    }
     rc = m.has_halo ? launch_grad_profile<T, true>(m, gg, ga, recd, recf, nblk, n_img, k, st)
                     : launch_grad_profile<T, false>(m, gg, ga, recd, recf, nblk, n_img, k, st);
     if (rc) return rc;
@@ -3301,7 +3491,6 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   ModelArgs m;
   int rc = model_args(model, m);
   if (rc) return rc;
-  if (m.n_bulk) return fail(LENSIM_EUNSUPPORTED, "gradients of bulk deflector lists are not supported");
   if (m.mp_n) return fail(LENSIM_EUNSUPPORTED, "gradients of multi-plane ray tracing are not supported");
   GridArgs g;
   if ((rc = grid_args(grid, g))) return rc;

@@ -69,6 +69,29 @@ def test_grad_vs_central_fd(precision, tol):
     assert np.all(g[:, 1, 4:] == 0) and np.all(g[:, 0, 6:] == 0)
 
 
+def test_mixed_grad_c3_batch256_within_1e4_of_fp64():
+    """The gradient configuration at its stated size (120^2 at supersampling 2,
+    21x21 PSF, 256 points around a seeded truth, 18 free parameters): the mixed
+    form's gradient within 1e-4 of the FP64 form's, per parameter relative to
+    that parameter's gradient scale over the batch (the FP64 gradient itself is
+    held to central differences above and to the oracle's below)."""
+    im64, truth = _model("fp64")
+    im32, _ = _model("mixed")
+    params = torch.as_tensor(_perturbed(im64, truth, 256, 7), device="cuda")
+    l64, g64 = im64.likelihood_and_grad_batch(params)
+    l32, g32 = im32.likelihood_and_grad_batch(params)
+    g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
+    worst = 0.0
+    for (k, i), name in FREE.items():
+        scale = np.max(np.abs(g64[:, k, i]))
+        err = np.max(np.abs(g32[:, k, i] - g64[:, k, i])) / scale
+        worst = max(worst, err)
+        assert err < 1e-4, (name, k, i, err)
+    rel = torch.abs((l32 - l64) / l64).cpu().numpy()
+    assert np.max(rel) < 1e-6, np.max(rel)
+    print("mixed gradient vs fp64, worst over 18 parameters: %.3g; logL %.3g" % (worst, np.max(rel)))
+
+
 def test_grad_vs_oracle_fd_small():
     """Independent of the GPU forward: central differences of the CPU oracle."""
     cfg = dict(W.EPL180, numPix=24, supersampling=1, deltaPix=0.15)
@@ -199,3 +222,51 @@ def test_torch_autograd_backward():
     (2.0 * logl.sum()).backward()
     _, grad = im.likelihood_and_grad_batch(params.detach())
     assert torch.allclose(params.grad, 2.0 * grad)
+
+
+def test_inline_halo_grads_fp64_vs_fd_and_mixed_vs_fp64():
+    """EPL + NFW + TNFW + SHEAR inline (a halo on the Einstein ring, where X = 1
+    falls on the arc), Sersic source and lens light, 80x80 at 0.05" s=2: the
+    FP64 gradient of every parameter against central differences of the FP64
+    logL, and the mixed form's gradient within 1e-4 of the FP64 form's (per
+    parameter, relative to its scale over a batch of 16 points)."""
+    cfg = dict(W.EPL180, numPix=80, deltaPix=0.05, supersampling=2)
+    kernel = W.moffat_kernel(15, 2.0, 3.0)
+    kl, ks, kll = W.epl180_draw(4)
+    names = ["EPL", "NFW", "TNFW", "SHEAR"]
+    kl = [kl[0], dict(Rs=0.3, alpha_Rs=0.08, center_x=1.0, center_y=0.3),
+          dict(Rs=0.15, alpha_Rs=0.05, r_trunc=0.6, center_x=-0.7, center_y=-0.9), kl[1]]
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModel(names), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+
+    clean = model("fp64").image(kl, ks, kll).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(9))
+    im64, im32 = model("fp64", data), model("mixed", data)
+    base, _ = im64.pack(kl, ks, kll)
+    rng = np.random.default_rng(10)
+    nprm = [6, 4, 5, 2, 7, 7]
+    p = np.repeat(base, 16, axis=0)
+    for k, n in enumerate(nprm):
+        for i in range(n):
+            p[:, k, i] += rng.normal(0, 0.02, 16) * max(abs(base[0, k, i]), 0.1)
+    pt = torch.as_tensor(p, device="cuda")
+    l64, g64 = im64.likelihood_and_grad_batch(pt)
+    l32, g32 = im32.likelihood_and_grad_batch(pt)
+    g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
+    slots = [(k, i) for k, n in enumerate(nprm) for i in range(n)]
+    for (k, i) in slots:
+        scale = np.max(np.abs(g64[:, k, i]))
+        assert np.max(np.abs(g32[:, k, i] - g64[:, k, i])) / scale < 1e-4, (k, i)
+    # central differences of the FP64 logL at the first point, all in one batch
+    reps = np.repeat(p[:1], 2 * len(slots), axis=0)
+    hs = np.array([1e-6 * max(abs(p[0, k, i]), 0.05) for (k, i) in slots])
+    for q, (k, i) in enumerate(slots):
+        reps[2 * q, k, i] += hs[q]
+        reps[2 * q + 1, k, i] -= hs[q]
+    ll = im64.likelihood_batch(torch.as_tensor(reps, device="cuda")).cpu().numpy()
+    fd = (ll[0::2] - ll[1::2]) / (2 * hs)
+    got = np.array([g64[0, k, i] for (k, i) in slots])
+    assert np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))) < 1e-5
