@@ -30,6 +30,16 @@ so that a handful of graphs covers every batch size.
 Admission: every bucket's device memory is charged against an HBM budget at
 registration; ``put_data`` returns None while all of a bucket's slots are in
 use.  All device work is on streams of the default priority.
+
+Failures are per request: a batch whose launch fails (a library error, a
+failed graph launch) fails its own requests -- their Futures carry the
+exception, ``result()`` raises it and ``status`` is "error" -- and the lane and
+stream go back to the pool.  A watchdog fails the requests of a batch still
+running ``watchdog`` seconds after its launch (status "timeout", result()
+raises TimeoutError) and keeps its lane and stream out of use until the batch
+does finish, then returns them; ``health()`` reports the counts.  An error in
+the scheduler itself fails every outstanding request and stops the
+background thread.
 """
 
 import collections
@@ -66,10 +76,11 @@ _D2H = 2  # hipMemcpyDeviceToHost
 class Future(object):
     """Result handle of one request (set by the scheduler thread)."""
 
-    __slots__ = ("value", "t_submit", "t_done", "_cond")
+    __slots__ = ("value", "error", "t_submit", "t_done", "_cond")
 
     def __init__(self, cond, t_submit):
         self.value = None
+        self.error = None
         self.t_submit = t_submit
         self.t_done = None
         self._cond = cond
@@ -77,11 +88,24 @@ class Future(object):
     def done(self):
         return self.t_done is not None
 
+    @property
+    def status(self):
+        """"pending", "ok", "error" (the batch failed: see ``error``) or
+        "timeout" (the watchdog failed it)."""
+        if self.t_done is None:
+            return "pending"
+        if self.error is None:
+            return "ok"
+        return "timeout" if isinstance(self.error, TimeoutError) else "error"
+
     def result(self, timeout=None):
+        """The value; raises the batch's exception for a failed request."""
         if self.t_done is None:
             with self._cond:
                 if not self._cond.wait_for(self.done, timeout):
                     raise TimeoutError("request not completed")
+        if self.error is not None:
+            raise self.error
         return self.value
 
     @property
@@ -263,7 +287,7 @@ class LensServer(object):
     thread and ``start()`` / ``stop()`` from a background thread.
     """
 
-    def __init__(self, n_streams=4, hbm_budget=64 << 30, use_graphs=True):
+    def __init__(self, n_streams=4, hbm_budget=64 << 30, use_graphs=True, watchdog=30.0):
         backend.require_gpu()
         backend.timing_enable(False)  # event timing synchronises; it cannot sit in a graph
         self.streams = [torch.cuda.Stream() for _ in range(int(n_streams))]
@@ -279,6 +303,12 @@ class LensServer(object):
         self.batches = 0
         self.requests_done = 0
         self.batch_sizes = collections.Counter()
+        self.watchdog = float(watchdog)  # seconds a batch may run before its requests fail
+        self.overdue = []  # batches the watchdog failed, holding their lane and stream until they finish
+        self.failed_batches = 0
+        self.watchdog_trips = 0
+        self.requests_failed = 0
+        self.last_error = None
         self._thread = None
         self._stop = False
 
@@ -383,17 +413,46 @@ class LensServer(object):
         return futs
 
     # ---------------------------------------------------------- scheduling --
+    def _fail(self, futs, exc):
+        now = time.perf_counter()
+        for f in futs:
+            f.error = exc
+            f.t_done = now
+        self.requests_failed += len(futs)
+        self.last_error = exc
+        with self.cond:
+            self.cond.notify_all()
+
     def _retire(self):
+        if self.overdue:  # batches the watchdog gave up on: their lane and stream return once they finish
+            still = []
+            for item in self.overdue:
+                if item[0].query():
+                    item[3].busy = False
+                    self.free_streams.append(item[6])
+                else:
+                    still.append(item)
+            self.overdue = still
         if not self.inflight:
             return
-        still, done = [], []
+        still, done, late = [], [], []
+        now = time.perf_counter()
         for item in self.inflight:
-            (done if item[0].query() else still).append(item)
-        if not done:
+            if item[0].query():
+                done.append(item)
+            elif now - item[7] > self.watchdog:
+                late.append(item)
+            else:
+                still.append(item)
+        if not done and not late:
             return
         self.inflight = still
-        now = time.perf_counter()
-        for ev, images, b, lane, kind, futs, si in done:
+        for item in late:
+            self.watchdog_trips += 1
+            self.overdue.append(item)
+            self._fail(item[5], TimeoutError("batch of %d %s requests for bucket %r still running %.1f s after its "
+                                             "launch" % (len(item[5]), item[4], item[2].key, now - item[7])))
+        for ev, images, b, lane, kind, futs, si, t0 in done:
             n = len(futs)
             if kind == "loglike":
                 vals = lane.logl_host[:n].numpy().tolist()
@@ -447,8 +506,15 @@ class LensServer(object):
             np.concatenate(ss, axis=0, out=lane.slot_np[:n])
             si = self.free_streams.pop(0)
             lane.busy = True
-            ev, images = b.run(lane, kind, n, self.streams[si])
-            self.inflight.append((ev, images, b, lane, kind, futs, si))
+            try:
+                ev, images = b.run(lane, kind, n, self.streams[si])
+            except Exception as exc:  # the batch did not launch: its own requests fail
+                lane.busy = False
+                self.free_streams.append(si)
+                self.failed_batches += 1
+                self._fail(futs, exc)
+                continue
+            self.inflight.append((ev, images, b, lane, kind, futs, si, time.perf_counter()))
             self.batches += 1
             self.batch_sizes[n] += 1
             launched = True
@@ -463,6 +529,30 @@ class LensServer(object):
     def idle(self):
         with self.lock:
             return not self.inflight and not any(self.pending.values())
+
+    def health(self):
+        """Counters of the scheduler: batches launched and failed, watchdog
+        trips, requests done and failed, batches still holding a lane after a
+        watchdog trip, and the last error (None if there was none)."""
+        return {"batches": self.batches, "failed_batches": self.failed_batches,
+                "watchdog_trips": self.watchdog_trips, "overdue_batches": len(self.overdue),
+                "requests_done": self.requests_done, "requests_failed": self.requests_failed,
+                "free_streams": len(self.free_streams), "last_error": None if self.last_error is None
+                else "%s: %s" % (type(self.last_error).__name__, self.last_error)}
+
+    def _fail_all(self, exc):
+        """Fail every queued and in-flight request (an error in the scheduler)."""
+        with self.lock:
+            futs = []
+            for q in self.pending.values():
+                while q:
+                    blk = q.popleft()
+                    futs.extend(blk.futures[blk.pos:])
+            for item in self.inflight:
+                futs.extend(item[5])
+            self.overdue.extend(self.inflight)
+            self.inflight = []
+        self._fail(futs, exc)
 
     def drain(self, timeout=120.0):
         """Run the scheduler until every queued request has completed."""
@@ -483,8 +573,12 @@ class LensServer(object):
         def loop():
             torch.cuda.set_device(device)
             while not self._stop:
-                if not self.step():
-                    time.sleep(20e-6)
+                try:
+                    if not self.step():
+                        time.sleep(20e-6)
+                except Exception as exc:  # the scheduler itself failed: nothing outstanding is left waiting
+                    self._fail_all(exc)
+                    self._stop = True
 
         self._thread = threading.Thread(target=loop, daemon=True)
         self._thread.start()

@@ -2,6 +2,8 @@
 batched with others (continuous batching, padding, graph replay), and graph
 replays with eager work between them match eager launches."""
 
+import time
+
 import numpy as np
 import pytest
 import torch
@@ -128,6 +130,102 @@ def test_slot_pool_and_budget():
         srv.put_data("a", np.zeros((31, 32)))
     with pytest.raises(MemoryError):
         LensServer(n_streams=1, hbm_budget=b.nbytes - 1).register("a", im, max_batch=4, n_slots=2, n_lanes=1)
+
+
+def test_failed_batch_fails_only_its_requests():
+    """A lane whose workspace is too small: the library refuses its batches
+    before any device work (EWORKSPACE), their requests end with status "error"
+    and the library's message, the lane and stream return to the pool, and the
+    other lane's batches are served as usual."""
+    im, cfg = _image_model("mixed", n=48, s=2, k=9)
+    srv = LensServer(n_streams=2, use_graphs=False)
+    b = srv.register("a", im, max_batch=4, n_slots=2, n_lanes=2)
+    obs = _observations(im, cfg, 1, 5)[0]
+    slot = srv.put_data("a", obs)
+    bad = b.lanes[1]
+    bad.ws = torch.empty(256, dtype=torch.uint8, device="cuda")
+    params = _requests(im, 24, 60)
+    # the first batch takes lane 0; while it runs, the next ones take lane 1 and fail
+    futs = srv.submit_many("a", params, slots=[slot] * 24)
+    srv.drain(timeout=60)
+    st = [f.status for f in futs]
+    assert "ok" in st and "error" in st and set(st) == {"ok", "error"}
+    for p, f in zip(params, futs):
+        if f.status == "ok":
+            assert f.result() == _alone_logl(im, p, obs)
+        else:
+            with pytest.raises(RuntimeError, match="workspace"):
+                f.result()
+    h = srv.health()
+    assert h["failed_batches"] > 0 and h["requests_failed"] == st.count("error") and h["free_streams"] == 2
+    assert h["requests_done"] == st.count("ok") and "workspace" in h["last_error"]
+
+
+class _HeldEvent(object):
+    """A batch's completion event that reports 'not done' until released: the
+    host-side stand-in for a slow batch (the GPU work itself ends as usual)."""
+
+    def __init__(self, ev):
+        self.ev = ev
+        self.held = True
+
+    def query(self):
+        return (not self.held) and self.ev.query()
+
+
+def test_watchdog_fails_overdue_batch_and_recovers_its_lane():
+    im, cfg = _image_model("mixed", n=48, s=2, k=9)
+    srv = LensServer(n_streams=1, use_graphs=False, watchdog=0.05)
+    b = srv.register("a", im, max_batch=4, n_slots=1, n_lanes=1)
+    obs = _observations(im, cfg, 1, 6)[0]
+    slot = srv.put_data("a", obs)
+    run, held = b.run, []
+
+    def run_held(*a):
+        ev, images = run(*a)
+        held.append(_HeldEvent(ev))
+        return held[-1], images
+
+    b.run = run_held
+    params = _requests(im, 8, 80)
+    f1 = srv.submit_many("a", params[:4], slots=[slot] * 4)
+    srv.step()  # launches the first batch
+    f2 = srv.submit_many("a", params[4:], slots=[slot] * 4)
+    time.sleep(0.1)
+    srv.step()  # the watchdog fails the first batch; its lane and stream stay held
+    assert [f.status for f in f1] == ["timeout"] * 4 and all(f.status == "pending" for f in f2)
+    with pytest.raises(TimeoutError, match="still running"):
+        f1[0].result()
+    h = srv.health()
+    assert h["watchdog_trips"] == 1 and h["overdue_batches"] == 1 and h["free_streams"] == 0
+    held[0].ev.synchronize()
+    held[0].held = False  # the batch finishes: lane and stream come back, the queue moves on
+    b.run = run
+    srv.drain(timeout=60)
+    assert [f.status for f in f2] == ["ok"] * 4
+    for p, f in zip(params[4:], f2):
+        assert f.result() == _alone_logl(im, p, obs)
+    h = srv.health()
+    assert h["overdue_batches"] == 0 and h["free_streams"] == 1 and h["requests_done"] == 4
+
+
+def test_scheduler_error_fails_outstanding_requests():
+    im, cfg = _image_model("mixed", n=32, s=1, k=5)
+    srv = LensServer(n_streams=1, use_graphs=False)
+    srv.register("a", im, max_batch=4, n_slots=1, n_lanes=1)
+    slot = srv.put_data("a", np.zeros((32, 32)))
+
+    def broken():
+        raise RuntimeError("scheduler failure")
+
+    srv._dispatch = broken
+    futs = srv.submit_many("a", _requests(im, 6, 90), slots=[slot] * 6)
+    srv.start()
+    for f in futs:
+        with pytest.raises(RuntimeError, match="scheduler failure"):
+            f.result(timeout=30)
+    srv.stop()
+    assert srv.health()["requests_failed"] == 6 and srv.idle()
 
 
 def test_graph_replays_with_eager_work_between():
