@@ -722,6 +722,133 @@ def main_sie60(args):
     print(json.dumps(line), flush=True)
 
 
+LOS_PUBLISHED_S = 0.4684  # the reference's GPU time for this ray-shooting call (notebook cell 24)
+
+
+def main_los(args):
+    """Multi-plane ray shooting through a line-of-sight halo realisation
+    (workloads.los_halo_draw: 5130 TNFW + 126 CONVERGENCE sheets on 126
+    planes, 5500 deflector slots, z_source 2.5) at the 400 x 400 grid over
+    +-10", the call the reference times on its GPU.  value: seconds per
+    LensModelBulk.ray_shooting call from host arrays to device results
+    (the drop-in call as the reference's notebook times it), FP64 as the
+    reference runs it; the mixed form and the device-only kernel time beside."""
+    import torch
+
+    from jaxtronomy_amd import backend
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+
+    cfg = W.LOS
+    names, kw, z = W.los_halo_draw(0)
+    x, y = W.los_grid()
+    n_ray = x.size
+    n_tnfw, n_conv = names.count("TNFW"), names.count("CONVERGENCE")
+    lib = backend.lib()
+    res = {}
+    for prec in ("fp64", "mixed"):
+        lm = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True, precision=prec)
+        rs = lm.prepare_ray_shooting_kwargs(names, kw, z_source=cfg["z_source"], lens_redshift_list=z,
+                                            num_deflectors=cfg["num_deflectors"])
+        steps = args.steps if prec == "mixed" else args.fp64_steps
+        log("bench: los %s: drop-in calls (%d)" % (prec, steps))
+        for _ in range(max(1, args.warmup)):
+            lm.ray_shooting(x, y, rs)
+        torch.cuda.synchronize()
+        per = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            bx, by = lm.ray_shooting(x, y, rs)
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - t0)
+        # device-resident: the raytrace launch alone, inputs already in HBM, HIP events on its stream
+        n_img, it, ip, bt, bp = lm.pack(rs)
+        m, keep = lm._model(it, bt, ray_shooting_kwargs=rs)
+        params = backend.to_device_f64(np.concatenate([ip, bp], axis=1))
+        xt, yt = backend.to_device_f64(x), backend.to_device_f64(y)
+        ox = torch.empty((1, n_ray), dtype=torch.float64, device="cuda")
+        oy = torch.empty_like(ox)
+        pr = backend.PRECISIONS[prec]
+        ws = torch.empty(max(256, lib.lensim_workspace_size(ctypes.byref(m), None, None, 1, pr)), dtype=torch.uint8,
+                         device="cuda")
+
+        def launch():
+            backend.check(lib.lensim_raytrace(ctypes.byref(m), backend.ptr(params), 1, backend.ptr(xt),
+                                              backend.ptr(yt), n_ray, pr, backend.ptr(ox), backend.ptr(oy),
+                                              backend.ptr(ws), ws.numel(), backend.stream_handle()),
+                          "lensim_raytrace")
+
+        launch()
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
+            launch()
+            evs[i + 1].record()
+        torch.cuda.synchronize()
+        kern = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+        del keep
+        res[prec] = dict(call_s=float(np.median(per)), call_p99_s=float(np.percentile(per, 99)),
+                         kernel_ms=float(np.mean(kern)), bx=ox.cpu().numpy()[0], by=oy.cpu().numpy()[0],
+                         drop_in_equal=bool(np.array_equal(bx.cpu().numpy(), ox.cpu().numpy()[0])))
+    # algorithmic work per call: per (ray, TNFW) 36 flops + 7 special functions, per (ray, sheet) 4 flops,
+    # per (ray, plane) 10 flops of the plane recursion
+    F = n_ray * (36.0 * n_tnfw + 4.0 * n_conv + 10.0 * (n_tnfw + n_conv))
+    S = n_ray * 7.0 * n_tnfw
+    k64 = res["fp64"]["kernel_ms"] / 1e3
+    achieved = (F + 20 * S) / k64 / 1e12
+    scale = float(np.max(np.abs(np.concatenate([x, y]))))
+    mixed_err = max(float(np.max(np.abs(res["mixed"]["bx"] - res["fp64"]["bx"]))),
+                    float(np.max(np.abs(res["mixed"]["by"] - res["fp64"]["by"])))) / scale
+    line = {
+        "metric": "seconds per multi-plane ray-shooting call, 400^2 rays through 5130 TNFW + 126 CONVERGENCE "
+                  "planes (5500 deflector slots), FP64",
+        "value": round(res["fp64"]["call_s"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.fp64_steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["fp64"]["call_s"] * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(res["fp64"]["call_s"] / LOS_PUBLISHED_S, 5),
+        "dtype": "f64",
+        "data": "synthetic: a seeded realisation with the published counts (pyHalo is not available)",
+        "config": {"workload": "LensModelBulk(['TNFW', 'CONVERGENCE'], multi_plane=True).ray_shooting: z_lens 0.5, "
+                               "z_source 2.5, 126 planes, 400x400 grid over +-10\", host arrays in, device arrays "
+                               "out", "rays": n_ray, "deflectors": n_tnfw + n_conv,
+                   "published_reference_s": LOS_PUBLISHED_S},
+        "p99_ms": round(res["fp64"]["call_p99_s"] * 1e3, 4),
+        "kernel_ms": round(res["fp64"]["kernel_ms"], 4),
+        "mixed": {"value": round(res["mixed"]["call_s"], 6), "kernel_ms": round(res["mixed"]["kernel_ms"], 4),
+                  "max_err_rel_to_coordinate_scale_vs_fp64": mixed_err},
+        "accuracy": {"drop_in_equals_device_call": res["fp64"]["drop_in_equal"] and res["mixed"]["drop_in_equal"]},
+        "roofline": {"bound": "valu", "kernel": "raytrace_kernel<double> (multi-plane)", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                     "traffic": None, "duration": "mean raytrace launch over the timed steps, HIP events on its "
+                     "stream", "work": "per call %.3g flops + 20 x %.3g special-function evaluations" % (F, S)},
+    }
+    if not args.no_cpu:
+        from oracle import lensim_oracle as O
+
+        order = np.argsort(np.array(z))
+        rs = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True).prepare_ray_shooting_kwargs(
+            names, kw, z_source=cfg["z_source"], lens_redshift_list=z)
+        rng = np.random.default_rng(0)
+        sel = rng.choice(n_ray, 2000, replace=False)
+        t0 = time.perf_counter()
+        wx, wy = O.multi_plane_ray_shooting(x[sel], y[sel], [names[i] for i in order], [kw[i] for i in order],
+                                            rs["T_ij_list"], rs["T_z_list"], rs["reduced2physical_factor"])
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = dict(value=dt * n_ray / sel.size, unit="s", cores=1, kind="port",
+                                    sample="2000 of the 160000 rays through the numpy oracle's plane scan "
+                                           "(%.1f s), scaled to the full grid" % dt)
+        err = max(float(np.max(np.abs(res["fp64"]["bx"][sel] - wx))), float(np.max(np.abs(res["fp64"]["by"][sel] -
+                                                                                         wy)))) / scale
+        line["accuracy"]["fp64_max_err_rel_to_coordinate_scale_vs_oracle"] = err
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -734,7 +861,8 @@ def main():
     ap.add_argument("--settle", type=float, default=0.5, help="seconds of untimed sustained load before the timed "
                     "phases (epl180)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60"), default="epl180")
+    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60", "los"),
+                    default="epl180")
     ap.add_argument("--in-flight", type=int, default=3, help="batches in flight for the headline value (epl180)")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
     ap.add_argument("--n-sub", type=int, default=1000)
@@ -772,6 +900,8 @@ def main():
         return main_serve_mix(args)
     if args.workload == "sie60":
         return main_sie60(args)
+    if args.workload == "los":
+        return main_los(args)
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch, args.in_flight)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2

@@ -25,6 +25,7 @@
  *   NFW            Rs alpha_Rs center_x center_y             (LensModel/Profiles/nfw.py:48)
  *   TNFW           Rs alpha_Rs r_trunc center_x center_y     (LensModel/Profiles/tnfw.py:27)
  *   SHEAR          gamma1 gamma2 ra_0 dec_0                  (LensModel/Profiles/shear.py:16)
+ *   CONVERGENCE    kappa ra_0 dec_0                          (LensModel/Profiles/convergence.py:15)
  *   SERSIC_ELLIPSE amp R_sersic n_sersic e1 e2 center_x center_y
  *                                                 (LightModel/Profiles/sersic_ellipse.py:25)
  *   SERSIC         amp R_sersic n_sersic center_x center_y  (LightModel/Profiles/sersic.py)
@@ -40,7 +41,7 @@
 extern "C" {
 #endif
 
-#define LENSIM_ABI_VERSION 5
+#define LENSIM_ABI_VERSION 6
 #define LENSIM_NPARAM 8        /* float64 slots per profile in a parameter block */
 #define LENSIM_MAX_PROFILES 64 /* lens + source + lens-light profiles per model */
 #define LENSIM_EPL_NMAX 500    /* largest series length of the EPL hypergeometric sum */
@@ -67,6 +68,7 @@ typedef enum {
   LENSIM_NFW = 4,
   LENSIM_TNFW = 5,
   LENSIM_SHEAR = 6,
+  LENSIM_CONVERGENCE = 7, /* mass sheet */
   /* light profiles */
   LENSIM_SERSIC = 32,
   LENSIM_SERSIC_ELLIPSE = 33,
@@ -106,7 +108,7 @@ typedef struct {
  * (LensModel/lens_model.py:25-216, LightModel/light_model_base.py:63-228),
  * plus an optional bulk deflector list as packed by
  * SinglePlaneBulk.prepare_ray_shooting_kwargs (LensModel/single_plane_bulk.py:47-106):
- * bulk types may be SIE, NIE, NFW, TNFW, SHEAR or LENSIM_NULL (padding). */
+ * bulk types may be SIE, NIE, NFW, TNFW, SHEAR, CONVERGENCE or LENSIM_NULL (padding). */
 typedef struct {
   int32_t n_lens;
   int32_t n_source;

@@ -8,9 +8,16 @@ with point_source_supersampling_factor = s > 1 is sampled on the supersampled
 grid: it is kept (normalised) for supersampled convolution, and the regular
 kernel is its s x s block mean (degrade_kernel, odd s).
 
-lenstronomy itself is not available here; degrade_kernel and split_kernel are
-restated from its documented behaviour and pinned by their invariants (unit
-total flux, the hole under the supersampled cut, centring), not by its output.
+A PIXEL kernel given at the regular resolution and asked for on a grid
+supersampled by an odd factor s is interpolated there by subgrid_kernel
+(lenstronomy's PSF.kernel_point_source_supersampled: kernel_util.subgrid_kernel
+with odd=True and five iterations, then cut_psf to n s pixels), as
+ImSim/Numerics/numerics.py:103-116 asks for supersampled convolution.
+
+lenstronomy itself is not available here; degrade_kernel, split_kernel and
+subgrid_kernel are restated from its documented behaviour and pinned by their
+invariants (unit total flux, the hole under the supersampled cut, centring,
+block means that reproduce the regular kernel), not by its output.
 """
 
 import numpy as np
@@ -59,16 +66,30 @@ class PSF(object):
 
     def kernel_point_source_supersampled(self, supersampling_factor):
         """The PIXEL kernel sampled on a grid supersampled by supersampling_factor:
-        the kernel given at that factor (point_source_supersampling_factor), or
-        the regular kernel itself for a factor of 1."""
+        the kernel given at that factor (point_source_supersampling_factor), the
+        regular kernel itself for a factor of 1, else the regular kernel
+        interpolated onto the finer grid (subgrid_kernel, odd factors; the
+        result is cached per factor)."""
         if supersampling_factor == 1:
             return self._kernel
-        if self.psf_type == "PIXEL" and getattr(self, "_kernel_super", None) is not None and \
+        if self.psf_type != "PIXEL":
+            raise ValueError("a supersampled kernel is only defined for PIXEL PSFs (psf_type %s)" % self.psf_type)
+        if getattr(self, "_kernel_super", None) is not None and \
                 self._point_source_supersampling_factor == supersampling_factor:
             return self._kernel_super
-        raise NotImplementedError("a PIXEL kernel supersampled by %d must be given (kernel_point_source with "
-                                  "point_source_supersampling_factor=%d); interpolating a regular kernel onto a "
-                                  "finer grid is not supported" % (supersampling_factor, supersampling_factor))
+        if supersampling_factor % 2 == 0:
+            raise NotImplementedError("interpolating a PIXEL kernel onto a grid supersampled by an even factor (%d) "
+                                      "is not supported; give the supersampled kernel with "
+                                      "point_source_supersampling_factor" % supersampling_factor)
+        cache = self.__dict__.setdefault("_subgrid_cache", {})
+        if supersampling_factor not in cache:
+            k = subgrid_kernel(self._kernel, supersampling_factor, odd=True, num_iter=5)
+            n_new = len(self._kernel) * supersampling_factor
+            if n_new % 2 == 0:
+                n_new -= 1
+            k = cut_psf(k, n_new)
+            cache[supersampling_factor] = k / np.sum(k)
+        return cache[supersampling_factor]
 
     @property
     def fwhm(self):
@@ -95,6 +116,54 @@ def cut_psf(psf_data, psf_size):
     c = (n - 1) // 2
     r = (psf_size - 1) // 2
     return psf_data[c - r:c + r + 1, c - r:c + r + 1]
+
+
+def _resize_linear(values, n_out):
+    """A square array sampled at the cell centres of the unit square, linearly
+    interpolated (separably, in each axis) onto n_out x n_out cell centres;
+    beyond the outermost input centres the edge values hold (lenstronomy
+    image_util.re_size_array: linear interpolation of the cells' values,
+    constant past the input range)."""
+    n = values.shape[0]
+    d = 1.0 / n
+    c_in = np.linspace(d / 2, 1 - d / 2, n)
+    d_out = 1.0 / n_out
+    c_out = np.linspace(d_out / 2.0, 1 - d_out / 2.0, n_out)
+    rows = np.array([np.interp(c_out, c_in, values[j]) for j in range(n)])  # along x, per input row
+    return np.array([np.interp(c_out, c_in, rows[:, i]) for i in range(n_out)]).T  # along y
+
+
+def subgrid_kernel(kernel, subgrid_res, odd=True, num_iter=5):
+    """The kernel on a grid subgrid_res times finer (lenstronomy
+    kernel_util.subgrid_kernel): the regular kernel is interpolated onto the
+    finer grid and normalised; then, num_iter times, the finer kernel is
+    block-averaged back to the regular grid, the residual against the regular
+    kernel is added to the kernel being interpolated, and that is
+    interpolated again -- so the fine kernel's block means approach the
+    given kernel.  Odd subgrid_res only (the block mean of an odd factor is
+    centred); odd=True makes the fine size odd."""
+    subgrid_res = int(subgrid_res)
+    if subgrid_res == 1:
+        return kernel
+    if subgrid_res % 2 == 0:
+        raise NotImplementedError("even subgrid factors are not supported")
+    kernel = np.asarray(kernel, dtype=np.float64)
+    n = kernel.shape[0]
+    if kernel.ndim != 2 or kernel.shape[1] != n:
+        raise ValueError("subgrid_kernel takes a square kernel, not %s" % (kernel.shape,))
+    n_new = n * subgrid_res
+    if odd and n_new % 2 == 0:
+        n_new -= 1
+    source = kernel.copy()
+    fine = _resize_linear(source, n_new)
+    fine = fine / np.sum(fine)
+    for _ in range(max(num_iter, 1)):
+        coarse = fine.reshape(n, subgrid_res, n, subgrid_res).mean(axis=(1, 3))
+        coarse = coarse / np.sum(coarse)  # both at unit flux: the residual is in shape only
+        source = source + (kernel / np.sum(kernel) - coarse)
+        fine = _resize_linear(source, n_new)
+        fine = fine / np.sum(fine)
+    return fine
 
 
 def gaussian_pixel_kernel(fwhm, pixel_scale, truncation, supersampling_factor=1, supersampling_convolution=False):

@@ -30,7 +30,7 @@ import torch
 from .. import backend, profiles
 from ..Util.cosmo import Background
 
-_BULK_OK = ("SIE", "NIE", "NFW", "TNFW", "SHEAR")
+_BULK_OK = ("SIE", "NIE", "NFW", "TNFW", "SHEAR", "CONVERGENCE")
 
 
 class LensModelBulk(object):
@@ -219,14 +219,17 @@ class LensModelBulk(object):
                 bulk_idx = [d for d in rest if names[d] == major]
 
         def block(idx):
+            # one gather per (profile, parameter) over the deflectors of that profile
+            idx = np.asarray(idx, dtype=int)
             p = np.zeros((n_img, len(idx), backend.NPARAM))
             types = np.zeros(len(idx), dtype=np.int32)
-            for j, d in enumerate(idx):
-                u = index_list[d]
-                name = self.unique_lens_model_list[u]
-                types[j] = profiles.LENS_PROFILES[name][0]
+            us = index_list[idx] if len(idx) else idx
+            for u in np.unique(us):
+                sel = np.nonzero(us == u)[0]
+                types[sel] = profiles.LENS_PROFILES[self.unique_lens_model_list[u]][0]
                 for i, pn in enumerate(self.param_name_list[u]):
-                    p[:, j, i] = cols[pn][:, d] if pn in cols else 0.0
+                    if pn in cols:
+                        p[:, sel, i] = cols[pn][:, idx[sel]]
             return types, p
 
         it, ip = block(inline_idx)

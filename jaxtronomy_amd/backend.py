@@ -15,7 +15,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 NPARAM = 8
 MAX_PROFILES = 64
 EPL_NMAX = 500

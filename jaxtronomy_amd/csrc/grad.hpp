@@ -279,6 +279,10 @@ __device__ __forceinline__ void defl_g(int type, const S *p, const T *c, const T
       ay = ay + (p[1] * dx - p[0] * dy);
       break;
     }
+    case LENSIM_CONVERGENCE:  // convergence.py:34-46; p: kappa, ra_0, dec_0
+      ax = ax + p[0] * (x - p[1]);
+      ay = ay + p[0] * (y - p[2]);
+      break;
     default:
       break;
   }

@@ -334,7 +334,7 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
 
 bool bulk_type_ok(int t) {
   return t == LENSIM_NULL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW || t == LENSIM_TNFW ||
-         t == LENSIM_SHEAR;
+         t == LENSIM_SHEAR || t == LENSIM_CONVERGENCE;
 }
 
 int model_args(const lensim_model *model, ModelArgs &m) {
@@ -358,7 +358,7 @@ int model_args(const lensim_model *model, ModelArgs &m) {
     m.epl_index[k] = -1;
     const bool lens = k < m.n_lens;
     const bool lens_ok = t == LENSIM_EPL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW ||
-                         t == LENSIM_TNFW || t == LENSIM_SHEAR;
+                         t == LENSIM_TNFW || t == LENSIM_SHEAR || t == LENSIM_CONVERGENCE;
     const bool light_ok = t == LENSIM_SERSIC || t == LENSIM_SERSIC_ELLIPSE || t == LENSIM_GAUSSIAN;
     if (lens && !lens_ok) return fail(LENSIM_EUNSUPPORTED, "profile %d: type %d is not a supported deflector", k, t);
     if (!lens && !light_ok)
@@ -371,7 +371,7 @@ int model_args(const lensim_model *model, ModelArgs &m) {
     m.bulk_type = model->bulk_type;
     m.bulk_types = model->d_bulk_types;
     if (!bulk_type_ok(m.bulk_type))
-      return fail(LENSIM_EUNSUPPORTED, "bulk deflector type %d not supported (SIE, NIE, NFW, TNFW, SHEAR)",
+      return fail(LENSIM_EUNSUPPORTED, "bulk deflector type %d not supported (SIE, NIE, NFW, TNFW, SHEAR, CONVERGENCE)",
                   m.bulk_type);
     if (m.bulk_type == LENSIM_NULL && !m.bulk_types)
       return fail(LENSIM_EINVAL, "mixed bulk deflectors need d_bulk_types");
@@ -402,7 +402,7 @@ int model_args(const lensim_model *model, ModelArgs &m) {
 
 __device__ inline bool bulk_type_ok_dev(int t) {
   return t == LENSIM_NULL || t == LENSIM_SIE || t == LENSIM_NIE || t == LENSIM_NFW || t == LENSIM_TNFW ||
-         t == LENSIM_SHEAR;
+         t == LENSIM_SHEAR || t == LENSIM_CONVERGENCE;
 }
 
 // Record (and EPL coefficients) number idx < n_img * n_prof, profile-major
@@ -1664,6 +1664,7 @@ __host__ __device__ constexpr int n_params_of(int type) {
     case LENSIM_NFW: return 4;
     case LENSIM_TNFW: return 5;
     case LENSIM_SHEAR: return 4;
+    case LENSIM_CONVERGENCE: return 3;
     case LENSIM_SERSIC: return 5;
     case LENSIM_SERSIC_ELLIPSE: return 7;
     case LENSIM_GAUSSIAN: return 4;
@@ -1932,6 +1933,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       else if constexpr (type == LENSIM_NFW) halo_ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_TNFW) halo_ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      else if constexpr (type == LENSIM_CONVERGENCE) ray_grad<T, 3, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SERSIC) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_SERSIC_ELLIPSE) ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
       else if constexpr (type == LENSIM_GAUSSIAN) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
@@ -1972,6 +1974,7 @@ int launch_grad_profile(const ModelArgs &m, const GridArgs &g, const GradArgs &g
       if constexpr (HALO) LENSIM_GRAD(LENSIM_TNFW); else LENSIM_GRAD(LENSIM_NULL);
       break;
     case LENSIM_SHEAR: LENSIM_GRAD(LENSIM_SHEAR); break;
+    case LENSIM_CONVERGENCE: LENSIM_GRAD(LENSIM_CONVERGENCE); break;
     case LENSIM_SERSIC: LENSIM_GRAD(LENSIM_SERSIC); break;
     case LENSIM_SERSIC_ELLIPSE: LENSIM_GRAD(LENSIM_SERSIC_ELLIPSE); break;
     case LENSIM_GAUSSIAN: LENSIM_GRAD(LENSIM_GAUSSIAN); break;
@@ -2084,6 +2087,9 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
           if constexpr (HALO) bulk_tile_duals<LENSIM_TNFW, 5>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
           break;
         case LENSIM_SHEAR: bulk_tile_duals<LENSIM_SHEAR, 4>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_CONVERGENCE:
+          bulk_tile_duals<LENSIM_CONVERGENCE, 3>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          break;
         default: break;  // no kwargs: zero partials
       }
     }
@@ -2120,6 +2126,7 @@ int launch_bulk_grad(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, 
       if constexpr (HALO) LENSIM_BULK_GRAD(LENSIM_TNFW);
       break;
     case LENSIM_SHEAR: LENSIM_BULK_GRAD(LENSIM_SHEAR); break;
+    case LENSIM_CONVERGENCE: LENSIM_BULK_GRAD(LENSIM_CONVERGENCE); break;
     default: LENSIM_BULK_GRAD(LENSIM_NULL); break;  // several types: d_bulk_types per deflector
   }
 #undef LENSIM_BULK_GRAD

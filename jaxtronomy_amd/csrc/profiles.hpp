@@ -118,6 +118,8 @@ enum : int { TN_RS = 2, TN_A, TN_TAU, TN_TAU2, TN_PRE, TN_C0, TN_C1, TN_RMIN, TN
 // SHEAR   (shear.py:37-53): ra_0, dec_0 in the centre slots, gamma1, gamma2; the
 //          steps of w = theta - alpha_scaling alpha_shear(theta) (affine in theta)
 enum : int { SH_G1 = 2, SH_G2, SH_WSX, SH_WSY, SH_WRX, SH_WRY };
+// CONVERGENCE (convergence.py:34-46): ra_0, dec_0 in the centre slots, kappa
+enum : int { CV_KAPPA = 2 };
 // SERSIC  (sersic_ellipse.py:62-93, sersic_utils.py:83-110, :239-267,
 //          param_util.py:103-124): amp, the product-average matrix
 //          ((1-e1), -e2; -e2, (1+e1))/norm, max(R_sersic, 1e-4), 1/n, b_n, 1/R_sersic_
@@ -353,6 +355,11 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[SH_WRY] = st.rr[1] - st.asc * (p[1] * st.rr[0] - p[0] * st.rr[1]);
       break;
     }
+    case LENSIM_CONVERGENCE:
+      rec[R_CX] = p[1];
+      rec[R_CY] = p[2];
+      rec[CV_KAPPA] = p[0];
+      break;
     case LENSIM_SERSIC:
     case LENSIM_SERSIC_ELLIPSE: {
       const bool ell = (type == LENSIM_SERSIC_ELLIPSE);
@@ -904,6 +911,12 @@ __device__ __forceinline__ void shear_defl(const double *rd, double x, double y,
   ay += g2 * dx - g1 * dy;
 }
 
+// Mass sheet, LensModel/Profiles/convergence.py:34-46 (linear: kept in double).
+__device__ __forceinline__ void convergence_defl(const double *rd, double x, double y, double &ax, double &ay) {
+  ax += rd[CV_KAPPA] * (x - rd[R_CX]);
+  ay += rd[CV_KAPPA] * (y - rd[R_CY]);
+}
+
 // Adds one deflector's alpha at (x, y).  coef_d / coef_t: this profile's EPL
 // series coefficients (NULL for other profiles).
 // HALO: whether the NFW / TNFW branches are compiled in (kernels for models
@@ -933,6 +946,9 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
       break;
     case LENSIM_SHEAR:
       shear_defl(r.d, x, y, ax, ay);
+      break;
+    case LENSIM_CONVERGENCE:
+      convergence_defl(r.d, x, y, ax, ay);
       break;
     default:
       break;

@@ -3,7 +3,7 @@ ordered parameter names of each profile (the order of a parameter slot).
 
 Names and parameter orders follow the reference's ``param_names``:
 EPL LensModel/Profiles/epl.py:50, SIE sie.py:40, NIE nie.py, NFW nfw.py:48,
-TNFW tnfw.py:27, SHEAR shear.py:16, SERSIC_ELLIPSE
+TNFW tnfw.py:27, SHEAR shear.py:16, CONVERGENCE convergence.py:15, SERSIC_ELLIPSE
 LightModel/Profiles/sersic_ellipse.py:25, SERSIC sersic.py, GAUSSIAN gaussian.py.
 """
 
@@ -19,6 +19,7 @@ LENS_PROFILES = {
     "NFW": (4, ["Rs", "alpha_Rs", "center_x", "center_y"]),
     "TNFW": (5, ["Rs", "alpha_Rs", "r_trunc", "center_x", "center_y"]),
     "SHEAR": (6, ["gamma1", "gamma2", "ra_0", "dec_0"]),
+    "CONVERGENCE": (7, ["kappa", "ra_0", "dec_0"]),
 }
 
 LIGHT_PROFILES = {

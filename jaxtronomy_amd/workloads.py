@@ -223,3 +223,67 @@ def substructure_draw(draw, n_sub=None, cfg=None):
     subs = [dict(Rs=float(rs[i]), alpha_Rs=float(alpha_rs[i]), r_trunc=float(5.0 * rs[i]),
                  center_x=float(cx[i]), center_y=float(cy[i])) for i in range(n_sub)]
     return kwargs_lens, subs, kwargs_source
+
+
+# ---------------------------------------------------------------------------
+# Line-of-sight halo workload: the shape of the multi-plane ray-tracing
+# benchmark that the reference publishes for a pyHalo CDM realisation
+# (notebooks/ray_shooting_through_dark_matter_substructure_with_GPU.ipynb:
+# z_lens 0.5, z_source 2.5, cone opening 5.5", 5130 TNFW halos and 126
+# CONVERGENCE sheets, 5500 deflector slots, a 400 x 400 grid over +-10").
+# pyHalo is not available here: the halos are drawn to the same counts, with
+# M ~ M^-1.9 on [1e6, 1e10] M_sun, one redshift plane per sheet, field halos
+# per plane in proportion to the cone's comoving volume and a quarter of the
+# halos as subhalos on the main lens plane, positions uniform in the cone's
+# disc, r_trunc = 5 Rs, and each plane's sheet the negative of the mean
+# convergence its halos add over the disc (kappa = -sum alpha_Rs-mass / area).
+LOS = dict(z_lens=0.5, z_source=2.5, n_halo=5130, n_planes=126, cone_arcsec=5.5, num_deflectors=5500,
+           num_pix=400, half_width=10.0)
+
+
+def los_halo_draw(draw, cfg=None):
+    """(lens_model_list, kwargs_lens, lens_redshift_list) of one line-of-sight
+    realisation: n_halo TNFW and n_planes CONVERGENCE deflectors."""
+    cfg = LOS if cfg is None else cfg
+    rng = np.random.default_rng(seed(7, draw))
+    zl, zs, n_planes = cfg["z_lens"], cfg["z_source"], cfg["n_planes"]
+    z_planes = (np.arange(n_planes) + 0.5) * (zs / n_planes)
+    main = int(np.argmin(np.abs(z_planes - zl)))
+    z_planes[main] = zl
+    dc = np.array([_comoving(0.0, z, n=256) for z in z_planes])
+    vol = dc ** 2 * np.gradient(dc)  # comoving volume of a fixed-angle cone slice
+    n_sub = cfg["n_halo"] // 4
+    counts = np.floor((cfg["n_halo"] - n_sub) * vol / vol.sum()).astype(int)
+    counts[np.argsort(-vol)[:cfg["n_halo"] - n_sub - counts.sum()]] += 1
+    counts[main] += n_sub
+    u = rng.uniform(size=cfg["n_halo"])
+    a, lo, hi = -0.9, 1e6 ** -0.9, 1e10 ** -0.9  # M^-1.9: the CDF of M^(1 - 1.9)
+    masses = (lo + u * (hi - lo)) ** (1.0 / a)
+    radius = cfg["cone_arcsec"] / 2.0
+    names, kwargs, z_list = [], [], []
+    q = 0
+    for p, zp in enumerate(z_planes):
+        m = masses[q:q + counts[p]]
+        q += counts[p]
+        rs, alpha_rs = nfw_physical_to_angle(m, concentration(m), zp, zs)
+        r = radius * np.sqrt(rng.uniform(size=len(m)))
+        phi = rng.uniform(0, 2 * np.pi, len(m))
+        for i in range(len(m)):
+            names.append("TNFW")
+            kwargs.append(dict(Rs=float(rs[i]), alpha_Rs=float(alpha_rs[i]), r_trunc=float(5.0 * rs[i]),
+                               center_x=float(r[i] * np.cos(phi[i])), center_y=float(r[i] * np.sin(phi[i]))))
+            z_list.append(float(zp))
+        # the sheet: minus the halos' projected mass (~ alpha_Rs Rs each, in angle units) over the disc
+        kappa = -float(np.sum(alpha_rs * rs)) / (np.pi * radius ** 2) if len(m) else 0.0
+        names.append("CONVERGENCE")
+        kwargs.append(dict(kappa=kappa, ra_0=0.0, dec_0=0.0))
+        z_list.append(float(zp))
+    return names, kwargs, z_list
+
+
+def los_grid(cfg=None):
+    """The benchmark's ray coordinates: a num_pix x num_pix grid over
+    +-half_width arcsec (x fastest), as numpy float64 arrays."""
+    cfg = LOS if cfg is None else cfg
+    v = np.linspace(-cfg["half_width"], cfg["half_width"], cfg["num_pix"])
+    return np.tile(v, cfg["num_pix"]), np.repeat(v, cfg["num_pix"])

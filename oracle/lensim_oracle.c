@@ -21,7 +21,7 @@
 #include <omp.h>
 #endif
 
-enum { P_EPL = 1, P_SIE = 2, P_NIE = 3, P_NFW = 4, P_TNFW = 5, P_SHEAR = 6,
+enum { P_EPL = 1, P_SIE = 2, P_NIE = 3, P_NFW = 4, P_TNFW = 5, P_SHEAR = 6, P_CONVERGENCE = 7,
        P_SERSIC = 32, P_SERSIC_ELLIPSE = 33, P_GAUSSIAN = 34 };
 
 static void rotate(double x, double y, double a, double *xo, double *yo) {
@@ -180,6 +180,11 @@ static void deflect(int type, const double *p, double x, double y, double *ax, d
       *ay += p[1] * dx - p[0] * dy;
       break;
     }
+    case P_CONVERGENCE:
+      /* convergence.py:34-46 */
+      *ax += p[0] * (x - p[1]);
+      *ay += p[0] * (y - p[2]);
+      break;
     default: break;
   }
 }

@@ -21,6 +21,8 @@ Pinning (how each function is checked independently of this file):
   nfw/tnfw_derivatives   |alpha| = M_2d(R)/(pi R) with M_2d integrated from the
                          3-D density by scipy quadrature; alpha(Rs)=alpha_Rs
   shear_derivatives      exact linear form
+  convergence_derivatives  |alpha(R)| = M_2d(R)/(pi R) of a uniform sheet
+                         (quadrature), radial; div(alpha) = 2 kappa by FD
   sersic_*               closed-form flux integral vs fine-grid quadrature
   coordinate/sub grid    sub-pixel block means equal pixel centres, spacing
   re_size                exact block mean
@@ -40,7 +42,7 @@ __all__ = [
     "ellipticity2phi_q", "phi_q2_ellipticity", "rotate",
     "hyp2f1_lopez_temme_8", "epl_nmax", "epl_major_axis_derivatives",
     "epl_derivatives", "nie_derivatives", "sie_derivatives", "sis_derivatives",
-    "nfw_derivatives", "tnfw_derivatives", "shear_derivatives",
+    "nfw_derivatives", "tnfw_derivatives", "shear_derivatives", "convergence_derivatives",
     "LENS_FUNCS", "LIGHT_FUNCS",
     "sersic_elliptic", "sersic", "gaussian_light", "b_n",
     "alpha", "ray_shooting", "surface_brightness",
@@ -288,8 +290,14 @@ def shear_derivatives(x, y, gamma1, gamma2, ra_0=0.0, dec_0=0.0):
     return gamma1 * x_ + gamma2 * y_, gamma2 * x_ - gamma1 * y_
 
 
+def convergence_derivatives(x, y, kappa, ra_0=0.0, dec_0=0.0):
+    """Convergence.derivatives (a mass sheet), LensModel/Profiles/convergence.py:34-46."""
+    return kappa * (np.asarray(x, float) - ra_0), kappa * (np.asarray(y, float) - dec_0)
+
+
 # name -> (derivatives function, ordered param names as in each profile's
-# ``param_names``: epl.py:50, sie.py:40, nie.py, nfw.py:48, tnfw.py:27, shear.py:16)
+# ``param_names``: epl.py:50, sie.py:40, nie.py, nfw.py:48, tnfw.py:27, shear.py:16,
+# convergence.py:15)
 LENS_FUNCS = {
     "EPL": (epl_derivatives, ["theta_E", "gamma", "e1", "e2", "center_x", "center_y"]),
     "SIE": (sie_derivatives, ["theta_E", "e1", "e2", "center_x", "center_y"]),
@@ -298,6 +306,7 @@ LENS_FUNCS = {
     "NFW": (nfw_derivatives, ["Rs", "alpha_Rs", "center_x", "center_y"]),
     "TNFW": (tnfw_derivatives, ["Rs", "alpha_Rs", "r_trunc", "center_x", "center_y"]),
     "SHEAR": (shear_derivatives, ["gamma1", "gamma2", "ra_0", "dec_0"]),
+    "CONVERGENCE": (convergence_derivatives, ["kappa", "ra_0", "dec_0"]),
 }
 
 

@@ -8,8 +8,8 @@ from jaxtronomy_amd import workloads as W
 from oracle import c_oracle
 from oracle import lensim_oracle as O
 
-TYPES = {"EPL": 1, "SIE": 2, "NIE": 3, "NFW": 4, "TNFW": 5, "SHEAR": 6, "SERSIC": 32, "SERSIC_ELLIPSE": 33,
-         "GAUSSIAN": 34}
+TYPES = {"EPL": 1, "SIE": 2, "NIE": 3, "NFW": 4, "TNFW": 5, "SHEAR": 6, "CONVERGENCE": 7, "SERSIC": 32,
+         "SERSIC_ELLIPSE": 33, "GAUSSIAN": 34}
 
 
 def _pack(names, kwargs):
@@ -25,7 +25,8 @@ def _rel(a, b):
     return np.max(np.abs(a - b)) / np.max(np.abs(b))
 
 
-@pytest.mark.parametrize("lens", [["SIE", "SHEAR"], ["EPL", "SHEAR"], ["NFW", "SIE"], ["TNFW", "NIE"]])
+@pytest.mark.parametrize("lens", [["SIE", "SHEAR"], ["EPL", "SHEAR"], ["NFW", "SIE"], ["TNFW", "NIE"],
+                                  ["EPL", "CONVERGENCE"]])
 def test_c_oracle_image_matches_numpy(lens):
     rng = np.random.default_rng(3)
     kwl = []
@@ -38,6 +39,8 @@ def test_c_oracle_image_matches_numpy(lens):
             kw = dict(theta_E=1.2, gamma=2.1, e1=0.2, e2=0.05, center_x=0.0, center_y=0.0)
         elif name == "SHEAR":
             kw = dict(gamma1=0.03, gamma2=-0.01)
+        elif name == "CONVERGENCE":
+            kw = dict(kappa=-0.04, ra_0=0.1, dec_0=-0.05)
         elif name == "NFW":
             kw = dict(Rs=0.6, alpha_Rs=0.4, center_x=0.1, center_y=0.0)
         else:

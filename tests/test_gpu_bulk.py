@@ -198,6 +198,42 @@ def test_multi_plane_ray_shooting(precision, tol):
     assert np.max(np.abs(sx - wx)) > 100 * tol * scale
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-6)])
+def test_line_of_sight_realisation_vs_oracle(precision, tol):
+    """The line-of-sight workload (5130 TNFW + 126 CONVERGENCE sheets on 126
+    planes, z_source 2.5, 5500 deflector slots; the shape of the reference's
+    published multi-plane benchmark) on 1500 rays of its +-10" grid plus rays
+    on halo centres: the plane scan against the oracle's.  Both types go to
+    the bulk loop with a per-deflector type."""
+    names, kw, z = W.los_halo_draw(1)
+    lm = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True, precision=precision)
+    rs = lm.prepare_ray_shooting_kwargs(names, kw, z_source=W.LOS["z_source"], lens_redshift_list=z,
+                                        num_deflectors=W.LOS["num_deflectors"])
+    n_img, it, ip, bt, bp = lm.pack(rs)
+    assert len(it) == 0 and len(bt) == len(names) and set(bt) == {5, 7}
+    gx, gy = W.los_grid()
+    rng = np.random.default_rng(2)
+    sel = rng.choice(gx.size, 1500, replace=False)
+    x, y = gx[sel], gy[sel]
+    halos = [d for d in kw if "Rs" in d][:5]
+    x[:5] = [d["center_x"] for d in halos]
+    y[:5] = [d["center_y"] for d in halos]
+    bx, by = lm.ray_shooting(x, y, rs)
+    order = np.argsort(np.array(z))
+    wx, wy = O.multi_plane_ray_shooting(x, y, [names[i] for i in order], [kw[i] for i in order], rs["T_ij_list"],
+                                        rs["T_z_list"], rs["reduced2physical_factor"])
+    scale = max(np.max(np.abs(x - wx)), np.max(np.abs(y - wy)))
+    err = max(np.max(np.abs(_np(bx) - wx)), np.max(np.abs(_np(by) - wy))) / scale
+    assert err < tol, err
+    # the sheets matter: without them the rays land elsewhere
+    keep = [i for i in order if names[i] == "TNFW"]
+    zk = np.array(z)[keep]
+    rs2 = lm.prepare_ray_shooting_kwargs([names[i] for i in keep], [kw[i] for i in keep], z_source=2.5,
+                                         lens_redshift_list=list(zk))
+    nx, ny = lm.ray_shooting(x, y, rs2)
+    assert np.max(np.abs(_np(nx) - wx)) > 1e3 * tol * scale
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-5)])
 def test_multi_plane_image(precision, tol):
     names, kw, ks, z, cosmo = _multiplane_scene(n_sub=30)

@@ -270,3 +270,43 @@ def test_inline_halo_grads_fp64_vs_fd_and_mixed_vs_fp64():
     fd = (ll[0::2] - ll[1::2]) / (2 * hs)
     got = np.array([g64[0, k, i] for (k, i) in slots])
     assert np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))) < 1e-5
+
+
+def test_convergence_sheet_grads():
+    """EPL + CONVERGENCE (a mass sheet; its kappa and reference point are free):
+    the FP64 gradient against central differences of the FP64 logL, and the
+    mixed gradient within 1e-4 of the FP64 one."""
+    cfg = dict(W.EPL180, numPix=64, deltaPix=0.06, supersampling=2)
+    kernel = W.moffat_kernel(11, 2.0, 3.0)
+    kl, ks, kll = W.epl180_draw(6)
+    names = ["EPL", "CONVERGENCE"]
+    kl = [kl[0], dict(kappa=-0.04, ra_0=0.05, dec_0=-0.03)]
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModel(names), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+
+    clean = model("fp64").image(kl, ks, kll).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(12))
+    im64, im32 = model("fp64", data), model("mixed", data)
+    kl_p = [dict(kl[0], theta_E=kl[0]["theta_E"] * 1.02), dict(kappa=-0.035, ra_0=0.08, dec_0=-0.01)]
+    p, _ = im64.pack(kl_p, ks, kll)
+    pt = torch.as_tensor(p, device="cuda")
+    _, g64 = im64.likelihood_and_grad_batch(pt)
+    _, g32 = im32.likelihood_and_grad_batch(pt)
+    g64, g32 = g64.cpu().numpy()[0], g32.cpu().numpy()[0]
+    slots = [(0, i) for i in range(6)] + [(1, i) for i in range(3)] + [(2, i) for i in range(7)] + \
+            [(3, i) for i in range(7)]
+    reps = np.repeat(p, 2 * len(slots), axis=0)
+    hs = np.array([1e-6 * max(abs(p[0, k, i]), 0.05) for (k, i) in slots])
+    for q, (k, i) in enumerate(slots):
+        reps[2 * q, k, i] += hs[q]
+        reps[2 * q + 1, k, i] -= hs[q]
+    ll = im64.likelihood_batch(torch.as_tensor(reps, device="cuda")).cpu().numpy()
+    fd = (ll[0::2] - ll[1::2]) / (2 * hs)
+    got = np.array([g64[k, i] for (k, i) in slots])
+    assert np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))) < 1e-5
+    for (k, i) in slots:
+        assert abs(g32[k, i] - g64[k, i]) < 1e-4 * max(abs(g64[k, i]), 1e-3 * np.max(np.abs(got))), (k, i)
+    assert abs(g64[1, 0]) > 1e-3 * np.max(np.abs(got))  # the sheet's kappa matters

@@ -56,6 +56,7 @@ LENS_CASES = [
     ("NFW", dict(Rs=0.5, alpha_Rs=0.35, center_x=0.5, center_y=0.0)),  # X == 1 at (0, 0)
     ("TNFW", dict(Rs=0.4, alpha_Rs=0.2, r_trunc=2.0, center_x=0.1, center_y=0.1)),
     ("SHEAR", dict(gamma1=0.03, gamma2=-0.02, ra_0=0.1, dec_0=0.0)),
+    ("CONVERGENCE", dict(kappa=-0.03, ra_0=0.1, dec_0=-0.2)),
 ]
 
 
@@ -476,7 +477,7 @@ def test_mixed_gaussian_psf_edge_padding_and_signed_kernel():
     assert img_err(_np(im2.image(kl, ks)), want) < MIXED_IMG_TOL
 
 
-@pytest.mark.parametrize("variant", ["pixel_split", "pixel_full", "gaussian"])
+@pytest.mark.parametrize("variant", ["pixel_split", "pixel_full", "pixel_regular", "gaussian"])
 @pytest.mark.parametrize("precision,tol,ltol", [("fp64", FP64_TOL, FP64_TOL), ("mixed", MIXED_IMG_TOL,
                                                                                   MIXED_LOGL_TOL)])
 def test_supersampled_convolution(variant, precision, tol, ltol):
@@ -493,6 +494,13 @@ def test_supersampled_convolution(variant, precision, tol, ltol):
         kernel_hr = O.gaussian_kernel(0.2, cfg["deltaPix"], 3, s, True)
         kernel_low = None
         psf_type = "GAUSSIAN"
+    elif variant == "pixel_regular":
+        # a kernel at the regular resolution, interpolated onto the supersampled grid (subgrid_kernel)
+        psf = PSF("PIXEL", kernel_point_source=W.moffat_kernel(9, 2.0, 3.0))
+        psf_type = "PIXEL"
+        numerics["supersampling_kernel_size"] = None
+        kernel_low, kernel_hr = None, psf.kernel_point_source_supersampled(s)
+        assert kernel_hr.shape == (27, 27)
     else:
         k_super = W.moffat_kernel(7 * s, 2.0 * s, 3.0)
         psf = PSF("PIXEL", kernel_point_source=k_super, point_source_supersampling_factor=s)

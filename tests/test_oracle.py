@@ -179,6 +179,30 @@ def test_shear_linear():
     npt.assert_allclose(ay, v[1], rtol=1e-15)
 
 
+def test_convergence_mass_sheet():
+    """A sheet of constant convergence: |alpha(R)| = M_2d(R) / (pi R) with M_2d
+    integrated by quadrature (kappa pi R^2), radial from (ra_0, dec_0), and
+    div alpha = 2 kappa by central differences."""
+    from scipy.integrate import quad
+
+    kappa, ra0, dec0 = -0.013, 0.4, -0.7
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-3, 3, 50), rng.uniform(-3, 3, 50)
+    ax, ay = O.convergence_derivatives(x, y, kappa, ra0, dec0)
+    for xi, yi, axi, ayi in zip(x[:10], y[:10], ax[:10], ay[:10]):
+        R = np.hypot(xi - ra0, yi - dec0)
+        m2d = quad(lambda r: 2 * np.pi * r * kappa, 0, R)[0]
+        npt.assert_allclose(np.hypot(axi, ayi), abs(m2d) / (np.pi * R), rtol=1e-12)
+        npt.assert_allclose(axi * (yi - dec0) - ayi * (xi - ra0), 0.0, atol=1e-15)  # radial
+        assert np.sign(axi * (xi - ra0) + ayi * (yi - dec0)) == np.sign(kappa)
+    h = 1e-5
+    div = ((O.convergence_derivatives(x + h, y, kappa, ra0, dec0)[0] -
+            O.convergence_derivatives(x - h, y, kappa, ra0, dec0)[0]) +
+           (O.convergence_derivatives(x, y + h, kappa, ra0, dec0)[1] -
+            O.convergence_derivatives(x, y - h, kappa, ra0, dec0)[1])) / (2 * h)
+    npt.assert_allclose(div, 2 * kappa, rtol=1e-8)
+
+
 # ---------------------------------------------------------------- light ---
 
 def _sersic_total_flux(amp, R_sersic, n):
