@@ -2142,12 +2142,17 @@ __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, i
   grad[idx] = s;
 }
 
-__global__ void logl_kernel(const double *__restrict__ partial, int n_tiles, int n_img, double *__restrict__ logl) {
-  const int img = blockIdx.x * blockDim.x + threadIdx.x;
-  if (img >= n_img) return;
+// One wave per image: lane l sums partials l, l + 64, ... and a fixed
+// butterfly adds the 64 lane sums, so the order depends on n_tiles alone (not
+// on the batch); the partials of an image are read contiguously.
+__global__ __launch_bounds__(64) void logl_kernel(const double *__restrict__ partial, int n_tiles,
+                                                  double *__restrict__ logl) {
+  const int img = blockIdx.x;
+  const double *p = partial + size_t(img) * n_tiles;
   double s = 0.0;
-  for (int t = 0; t < n_tiles; ++t) s += partial[size_t(img) * n_tiles + t];
-  logl[img] = -s / 2.0;
+  for (int t = threadIdx.x; t < n_tiles; t += 64) s += p[t];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) logl[img] = -s / 2.0;
 }
 
 // ------------------------------------------------------------ launches ---
@@ -2459,7 +2464,7 @@ int run_image(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, i
   if (rc) return rc;
   timing_mark(2, st);
   if (data) {
-    logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, n_tiles, n_img, logl);
+    logl_kernel<<<n_img, 64, 0, st>>>(partial, n_tiles, logl);
     rc = check_launch("logl_kernel");
     if (rc) return rc;
   }
@@ -2669,7 +2674,7 @@ int run_image_ss(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf
   combine_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(cc, chr, lowc, s, out, partial);
   if ((rc = check_launch("combine_kernel"))) return rc;
   if (data) {
-    logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, nblk, n_img, logl);
+    logl_kernel<<<n_img, 64, 0, st>>>(partial, nblk, logl);
     return check_launch("logl_kernel");
   }
   return LENSIM_OK;
@@ -3123,7 +3128,7 @@ int run_linear(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int
   double *chi = reinterpret_cast<double *>(ws + LL.chi);
   lin_model_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(c, A, amp, n_lin, n_img, model_out, chi);
   if ((rc = check_launch("lin_model_kernel"))) return rc;
-  logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(chi, nblk, n_img, logl);
+  logl_kernel<<<n_img, 64, 0, st>>>(chi, nblk, logl);
   if ((rc = check_launch("logl_kernel"))) return rc;
   add_kernel<<<(n_img + 255) / 256, 256, 0, st>>>(logl, marg, n_img);
   if ((rc = check_launch("add_kernel"))) return rc;
@@ -3424,7 +3429,7 @@ int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const v
     data_chi2_kernel<float><<<dim3(nblk, n_img), 256, 0, st>>>(c, static_cast<const float *>(d_model), partial);
   int rc = check_launch("data_chi2_kernel");
   if (rc) return rc;
-  logl_kernel<<<(n_img + 63) / 64, 64, 0, st>>>(partial, nblk, n_img, d_logl);
+  logl_kernel<<<n_img, 64, 0, st>>>(partial, nblk, d_logl);
   return check_launch("logl_kernel");
 }
 
