@@ -592,7 +592,27 @@ struct GridArgs {
   double scale;  // pixel_width^2 / s^2
   const uint8_t *mask;  // evaluation mask of the image pixels (NULL: all), [mask_ny][mask_nx]
   int mask_nx, mask_s;  // pixel (i, j) of this grid is mask pixel (i / mask_s, j / mask_s)
+  // formed once on the host (grid_consts), so no thread divides: 1/s, the
+  // first sub-ray's offset -1/2 + 1/(2 s) (make_subgrid, ImSim/Numerics/grid.py:60-62),
+  // 1/nx and 1/s^2
+  double inv_s, o0, inv_nx, inv_ss;
 };
+
+void grid_consts(GridArgs &g) {
+  g.inv_s = 1.0 / g.s;
+  g.o0 = -0.5 + 1.0 / (2.0 * g.s);
+  g.inv_nx = 1.0 / g.nx;
+  g.inv_ss = 1.0 / (double(g.s) * double(g.s));
+}
+
+// q = n / d and n - q d for 0 <= n < 2^30 and 1 <= d, from inv_d = 1/d in
+// double: (n + 1/2) / d lies at least 1/(2d) from an integer, and the
+// product's rounding error is below n 2^-52 < 2^-22 / d, so truncation gives q.
+__device__ __forceinline__ int div_by(int n, int d, double inv_d, int &rem) {
+  const int q = int((double(n) + 0.5) * inv_d);
+  rem = n - q * d;
+  return q;
+}
 
 // Whether grid pixel (i, j) is evaluated (RegularGrid flux_evaluate_indexes,
 // ImSim/Numerics/grid.py:47-58: the rays of other pixels are not shot and
@@ -612,13 +632,14 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = pix0 < npix;
   const int pix = active ? pix0 : npix - 1;  // inactive lanes stay for the wave reduction
-  const int i = pix % g.nx, j = pix / g.nx;
+  int i;
+  const int j = div_by(pix, g.nx, g.inv_nx, i);
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   // pixel centre (lenstronomy grid_from_coordinate_transform order)
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
-  const double inv_s = 1.0 / g.s;
-  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const double inv_s = g.inv_s;
+  const double o0 = g.o0;
   const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && m.n_source > 0;
   const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
   const int k_src0 = m.n_defl, k_src1 = m.n_defl + m.n_source, k_l1 = m.n_prof;
@@ -782,7 +803,8 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   const int pix0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = pix0 < npix;
   const int pix = active ? pix0 : npix - 1;  // inactive lanes stay for the wave reduction
-  const int i = pix % g.nx, j = pix / g.nx;
+  int i;
+  const int j = div_by(pix, g.nx, g.inv_nx, i);
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   constexpr int KS = D1 ? 2 : 1;  // first source record
   const Rec<T> r0{rp.recd, rp.rect};
@@ -791,8 +813,8 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   // the pixel's first sub-ray (make_subgrid offsets, ImSim/Numerics/grid.py:60-62)
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
-  const double inv_s = 1.0 / g.s;
-  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const double inv_s = g.inv_s;
+  const double o0 = g.o0;
   const double x0 = ra_c + g.m00 * o0 + g.m01 * o0;
   const double y0 = dec_c + g.m10 * o0 + g.m11 * o0;
   const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && NSRC > 0;
@@ -1910,15 +1932,16 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
   const int pix0 = (blockIdx.x * kGradPix + pp) * blockDim.x + threadIdx.x;
   const bool active = pix0 < npix;
   const int pix = active ? pix0 : npix - 1;
-  const int i = pix % g.nx, j = pix / g.nx;
+  int i;
+  const int j = div_by(pix, g.nx, g.inv_nx, i);
   // the pixel's flux is scale * flux_scaling * (sum over its rays)
   const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
                                 m.flux_scaling
                           : 0.0;
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
   const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
-  const double inv_s = 1.0 / g.s;
-  const double o0 = -0.5 + 1.0 / (2.0 * g.s);
+  const double inv_s = g.inv_s;
+  const double o0 = g.o0;
   // a lens profile only matters if a source is there to be lensed
   const bool live = active && w != 0.0 && (k >= m.n_defl || m.n_source > 0) && evaluated(g, i, j);
   for (int sy = 0; live && sy < g.s; ++sy) {
@@ -2045,14 +2068,17 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
   __shared__ double red[256][LENSIM_NPARAM];
   const int p0 = blockIdx.x * kGradPix * 256, p1 = min(npix, p0 + kGradPix * 256);
   const int r_end = (p1 - p0) * ss;  // rays of the block, pixel-major, s x s sub-rays per pixel
-  const double inv_s = 1.0 / g.s, o0 = -0.5 + 1.0 / (2.0 * g.s), as = m.alpha_scaling;
+  const double inv_s = g.inv_s, o0 = g.o0, as = m.alpha_scaling;
   for (int base = 0; base < r_end; base += 256) {
     // one ray per thread: position, weight and the source gradient at its beta
     {
       const int rr = base + t;
       const bool active = rr < r_end;
-      const int pix = p0 + (active ? rr / ss : 0), sub = active ? rr % ss : 0;
-      const int i = pix % g.nx, j = pix / g.nx, sy = sub / g.s, sx = sub % g.s;
+      int sub0;
+      const int q = div_by(active ? rr : 0, ss, g.inv_ss, sub0);
+      const int pix = p0 + q, sub = sub0;
+      int i, sx;
+      const int j = div_by(pix, g.nx, g.inv_nx, i), sy = div_by(sub, g.s, g.inv_s, sx);
       const double ox = o0 + sx * inv_s, oy = o0 + sy * inv_s;
       const double x = i * g.m00 + j * g.m01 + g.ra0 + g.m00 * ox + g.m01 * oy;
       const double y = i * g.m10 + j * g.m11 + g.dec0 + g.m10 * ox + g.m11 * oy;
@@ -2197,6 +2223,7 @@ int grid_args(const lensim_grid *grid, GridArgs &g) {
   g.mask = grid->d_eval_mask;
   g.mask_nx = g.nx;
   g.mask_s = 1;
+  grid_consts(g);
   return LENSIM_OK;
 }
 
@@ -2570,6 +2597,7 @@ GridArgs ray_grid(const GridArgs &g) {
   gh.m11 = g.m11 / s;
   gh.scale = g.scale * double(s) * double(s);
   gh.mask_s = g.mask_s * s;
+  grid_consts(gh);
   return gh;
 }
 
