@@ -1039,9 +1039,21 @@ struct ConvArgs {
   int nx, ny, kx, ky, kyp, edge, n_tiles_x;  // kyp: kernel height padded to kWin
   const double *data, *noise, *expo, *mask;
   double exposure_time, bkg2;
+  double inv_t;  // 1 / exposure_time, formed on the host
   void *gout;  // optional dlogL/dmodel per pixel (compute type)
   int64_t dstride;  // data / noise elements between batch images (0: shared)
 };
+
+// 1/v to within two ulp: the hardware reciprocal refined by one Newton step
+// (a correctly rounded float64 division issues about three times the
+// instructions).  0, +-inf and NaN keep the hardware result (1/0 = inf,
+// 1/inf = 0), where the Newton step itself would give NaN.
+__device__ __forceinline__ double rcp_refined(double v) {
+  const double r0 = __builtin_amdgcn_rcp(v);
+  const double e = fma(-v, r0, 1.0);
+  const double r = fma(e, r0, r0);
+  return (e == e) ? r : r0;
+}
 
 // d logL / d model of one pixel: logL = -1/2 mask (m - d)^2 / V(m) with
 // V = max(m, 0)/t + bkg^2 (dV/dm = 1/t for m >= 0, the branch the reference's
@@ -1054,11 +1066,11 @@ __device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m
     var = c.noise[q] * c.noise[q];
     dvar = 0.0;
   } else {
-    const double t = c.expo ? c.expo[p] : c.exposure_time;
-    var = (m >= 0.0 ? m : 0.0) / t + c.bkg2;
-    dvar = m >= 0.0 ? 1.0 / t : 0.0;
+    const double it = c.expo ? rcp_refined(c.expo[p]) : c.inv_t;  // 1 / exposure
+    var = fma(m >= 0.0 ? m : 0.0, it, c.bkg2);
+    dvar = m >= 0.0 ? it : 0.0;
   }
-  const double r = (m - d) / var;
+  const double r = (m - d) * rcp_refined(var);
   double gp = -(r - 0.5 * r * r * dvar);
   if (c.mask) gp *= c.mask[p];
   return gp;
@@ -1074,11 +1086,11 @@ __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T mode
     const double nm = c.noise[q];
     var = nm * nm;
   } else {
-    const double t = c.expo ? c.expo[p] : c.exposure_time;
+    const double it = c.expo ? rcp_refined(c.expo[p]) : c.inv_t;  // 1 / exposure
     const double mpos = (m >= 0.0) ? m : 0.0;
-    var = mpos / t + c.bkg2;
+    var = fma(mpos, it, c.bkg2);
   }
-  double chi = (m - d) * (m - d) / var;
+  double chi = (m - d) * (m - d) * rcp_refined(var);
   if (c.mask) chi *= c.mask[p];
   return chi;
 }
@@ -2470,6 +2482,7 @@ int run_image(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf, i
     c.mask = data->d_mask;
     c.exposure_time = data->exposure_time;
     c.bkg2 = data->background_rms * data->background_rms;
+    c.inv_t = 1.0 / data->exposure_time;
     c.gout = gout;
     c.dstride = data->image_stride;
     partial = reinterpret_cast<double *>(ws + L.partial);
@@ -2694,6 +2707,7 @@ int run_image_ss(const ModelArgs &m_in, const GridArgs &g, const lensim_psf *psf
     cc.mask = data->d_mask;
     cc.exposure_time = data->exposure_time;
     cc.bkg2 = data->background_rms * data->background_rms;
+    cc.inv_t = 1.0 / data->exposure_time;
     cc.dstride = data->image_stride;
     cc.gout = gout;
     partial = reinterpret_cast<double *>(ws + L.partial);
@@ -3145,6 +3159,7 @@ int run_linear(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int
   c.mask = data->d_mask;
   c.exposure_time = data->exposure_time;
   c.bkg2 = data->background_rms * data->background_rms;
+  c.inv_t = 1.0 / data->exposure_time;
   c.dstride = data->image_stride;
   double *part = reinterpret_cast<double *>(ws + LL.part);
   normal_kernel<T><<<dim3(nblk, n_img), 256, 0, st>>>(c, A, n_lin, n_img, part);
@@ -3448,6 +3463,7 @@ int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const v
   c.mask = data->d_mask;
   c.exposure_time = data->exposure_time;
   c.bkg2 = data->background_rms * data->background_rms;
+  c.inv_t = 1.0 / data->exposure_time;
   c.dstride = data->image_stride;
   double *partial = static_cast<double *>(ws);
   const int nblk = (nx * ny + 255) / 256;
