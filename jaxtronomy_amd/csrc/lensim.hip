@@ -117,6 +117,7 @@ __host__ __device__ inline int type_of(const ModelArgs &m, int k) {
 struct Layout {
   size_t recd, recf, coefd, coeff, dcoefd, dcoeff, psf, lowres, partial, blkmax, bfrag;
   size_t gmodel, glow, gpart;  // gradient: dlogL/dmodel, dlogL/dlowres, per-block partial gradients
+  size_t gbeta;                // gradient: dS/dbeta of the sources at every ray (0: none)
   size_t gpad, gext;           // edge-padded PSF adjoint: padded dlogL/dmodel and its correlation
   size_t gchr, glu;            // supersampled convolution adjoint buffers
   size_t hr, chr, lowu, lowc, psf_low;  // supersampled convolution: ray fluxes, their convolution,
@@ -300,7 +301,7 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
 // Layout of lensim_loglike_grad: the image layout plus the gradient buffers
 // (on the ray grid when the convolution is supersampled).
 Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, int edge,
-                   const SSPlan &ss = SSPlan()) {
+                   const SSPlan &ss, int s) {
   Layout L = layout(m, nx, ny, kx, ky, n_img, precision, ss);
   const size_t tsz = precision == LENSIM_MIXED ? 4 : 8;
   const size_t npix = size_t(nx) * ny;
@@ -314,6 +315,12 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
   L.gpart = off;
   off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 256 * kGradPix - 1) / (256 * kGradPix)) *
                            LENSIM_NPARAM * 8);
+  L.gbeta = 0;
+  if (m.n_lens > 0 && m.n_source > 0) {  // dS/dbeta per ray (two values, compute type)
+    L.gbeta = off;
+    const size_t rays = npix_r * (ss.on ? 1 : size_t(s) * s);
+    off = align256(off + size_t(n_img) * rays * 2 * tsz);
+  }
   L.gpad = L.gext = L.gchr = L.glu = off;
   if (edge) {  // extended domain (ny cs + ky - 1) x (nx cs + kx - 1)
     const size_t next = size_t(nx * cs + kx - 1) * (ny * cs + ky - 1);
@@ -1714,6 +1721,7 @@ struct GradArgs {
   const float *dcoeff;
   const void *glow;       // [n_img][ny][nx] dlogL / d(unconvolved image), compute type
   double *gpart;          // [n_img][n_prof][nblk][8]
+  void *gbeta;            // [n_img][ny][nx][s][s][2] dS/dbeta of the sources' sum, compute type, or NULL
 };
 
 // A light profile's flux and its derivatives in its own kwargs (the N seeds
@@ -1763,15 +1771,37 @@ __device__ __forceinline__ gm::Dual<T, 2> sources_at(const ModelArgs &m, const G
 
 // Contribution of one ray to d flux / d(kwargs of profile k), with the
 // profile's own kwargs as N dual seeds; w = dlogL / d(ray flux).
+// gb: this ray's dS/dbeta of the sources' sum (two values), when the model
+// has lenses and sources: the first source's launch writes it, the lens
+// launches (which run after it) read it instead of deflecting by the other
+// deflectors and evaluating the sources again.
 template <typename T, int N, bool HALO>
 __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
-                                         int k, int type, double x, double y, double w, double *acc) {
+                                         int k, int type, double x, double y, double w, double *acc, T *gb) {
   using S = gm::Dual<T, N>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
   S ps[LENSIM_NPARAM];
 #pragma unroll
   for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(T(pk[i]), i) : S(T(pk[i]));
   S flux;
+  if (k < m.n_lens && gb) {
+    // d flux = dS/dbeta . d beta, d beta = -alpha_scaling d alpha_k
+    S ax = S(T(0)), ay = S(T(0));
+    const int e = m.epl_index[k];
+    const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    int nser = 1;
+    if (type == LENSIM_EPL)
+      nser = int(sizeof(T) == sizeof(float) ? rp.recd[k * kRec + EPL_NF] : rp.recd[k * kRec + EPL_NMAX]);
+    const T *c = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.coeff + co)
+                                            : reinterpret_cast<const T *>(ga.coefd + co);
+    const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
+                                             : reinterpret_cast<const T *>(ga.dcoefd + co);
+    gm::defl_g<T, S>(type, ps, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
+    const T gx = gb[0], gy = gb[1], as = T(m.alpha_scaling);
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] += w * double(-as * (gx * ax.d[i] + gy * ay.d[i]));
+    return;
+  }
   if (k < m.n_lens) {
     // d/d(lens kwargs): the other deflectors in the forward form, this one in duals
     double axp = 0.0, ayp = 0.0;
@@ -1806,7 +1836,27 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
     // d/d(source kwargs) at the deflected position
     double ax = 0.0, ay = 0.0;
     deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
-    flux = light_grad<T, N>(type, ps, pk, T(x - ax * m.alpha_scaling), T(y - ay * m.alpha_scaling));
+    const T bxs = T(x - ax * m.alpha_scaling), bys = T(y - ay * m.alpha_scaling);
+    bool done = false;
+    if (gb) {  // the first source's launch: also dS/dbeta of the sources' sum for the lens launches
+      if constexpr (N == 7) {
+        if (m.n_source == 1 && type == LENSIM_SERSIC_ELLIPSE) {  // one evaluation gives both
+          T gp[7], gx, gy;
+          flux.v = gm::sersic_ellipse_grad<T>(pk, bxs, bys, gp, gx, gy);
+#pragma unroll
+          for (int i = 0; i < 7; ++i) flux.d[i] = gp[i];
+          gb[0] = gx;
+          gb[1] = gy;
+          done = true;
+        }
+      }
+      if (!done) {
+        const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxs, bys);
+        gb[0] = f2.d[0];
+        gb[1] = f2.d[1];
+      }
+    }
+    if (!done) flux = light_grad<T, N>(type, ps, pk, bxs, bys);
   } else {
     // d/d(lens light kwargs) at the ray
     flux = light_grad<T, N>(type, ps, pk, T(x), T(y));
@@ -1820,9 +1870,22 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
 // forms of the NFW family cancel, as for the bulk list's duals).
 template <typename T, int N, bool HALO>
 __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
-                                              int k, int type, double x, double y, double w, double *acc) {
+                                              int k, int type, double x, double y, double w, double *acc,
+                                              const T *gb) {
   using S = gm::Dual<double, N>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  if (gb) {  // dS/dbeta from the first source's launch
+    S ps[LENSIM_NPARAM];
+#pragma unroll
+    for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(pk[i], i) : S(pk[i]);
+    S ax = S(0.0), ay = S(0.0);
+    gm::defl_g<double, S>(type, ps, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), 1,
+                          S(x), S(y), ax, ay);
+    const double gx = double(gb[0]), gy = double(gb[1]), as = m.alpha_scaling;
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] += w * (-as * (gx * ax.d[i] + gy * ay.d[i]));
+    return;
+  }
   double axp = 0.0, ayp = 0.0;
   for (int j = 0; j < m.n_lens; ++j) {
     if (j == k) continue;
@@ -1853,18 +1916,20 @@ __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs
 // kappa = (2 - t)/2 (b/R)^t and the shear of its eq. (17) from alpha itself.
 template <typename T, bool HALO>
 __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
-                                             int k, double x, double y, double w, double *acc) {
+                                             int k, double x, double y, double w, double *acc, const T *gb) {
   using S = gm::Dual<T, 3>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
-  double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form
-  for (int j = 0; j < m.n_lens; ++j) {
-    if (j == k) continue;
-    const int ej = m.epl_index[j];
-    const size_t coj = (ej >= 0) ? (size_t(img) * m.n_epl + ej) * LENSIM_EPL_NMAX : 0;
-    add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
-                            rp.coeft + coj, x, y, axp, ayp);
+  double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form (not needed with gb)
+  if (!gb) {
+    for (int j = 0; j < m.n_lens; ++j) {
+      if (j == k) continue;
+      const int ej = m.epl_index[j];
+      const size_t coj = (ej >= 0) ? (size_t(img) * m.n_epl + ej) * LENSIM_EPL_NMAX : 0;
+      add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
+                              rp.coeft + coj, x, y, axp, ayp);
+    }
+    deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
   }
-  deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
   S ps[LENSIM_NPARAM];
 #pragma unroll
   for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = (i >= 1 && i <= 3) ? S::seed(T(pk[i]), i - 1) : S(T(pk[i]));
@@ -1917,10 +1982,18 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   dax[5] = -fxy;
   day[5] = -fyy;
   const T as = T(m.alpha_scaling);
-  const T bxv = T(x) - (T(axp) + a_x) * as, byv = T(y) - (T(ayp) + a_y) * as;
-  const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxv, byv);
+  T gx, gy;
+  if (gb) {
+    gx = gb[0];
+    gy = gb[1];
+  } else {
+    const T bxv = T(x) - (T(axp) + a_x) * as, byv = T(y) - (T(ayp) + a_y) * as;
+    const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxv, byv);
+    gx = f2.d[0];
+    gy = f2.d[1];
+  }
 #pragma unroll
-  for (int i = 0; i < 6; ++i) acc[i] += w * double(-as * (f2.d[0] * dax[i] + f2.d[1] * day[i]));
+  for (int i = 0; i < 6; ++i) acc[i] += w * double(-as * (gx * dax[i] + gy * day[i]));
 }
 
 // One block = kGradPix x 256 pixels x one image of profile k, whose type
@@ -1937,6 +2010,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
   const int npix = g.nx * g.ny;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   constexpr int type = TYPE;
+  T *gbeta = static_cast<T *>(ga.gbeta);
   double acc[LENSIM_NPARAM];
 #pragma unroll
   for (int q = 0; q < LENSIM_NPARAM; ++q) acc[q] = 0.0;
@@ -1962,16 +2036,22 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       const double ox = o0 + sx * inv_s;
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
-      if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc);
-      else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_NFW) halo_ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_TNFW) halo_ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_CONVERGENCE) ray_grad<T, 3, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_SERSIC) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_SERSIC_ELLIPSE) ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
-      else if constexpr (type == LENSIM_GAUSSIAN) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc);
+      // this ray's dS/dbeta slot: written by the first source's launch, read by the lens launches
+      T *gb = (gbeta && (k < m.n_lens || k == m.n_defl))
+                  ? gbeta + ((size_t(img) * npix + pix) * size_t(g.s * g.s) + size_t(sy * g.s + sx)) * 2
+                  : nullptr;
+      if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_NFW) halo_ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_TNFW) halo_ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_CONVERGENCE)
+        ray_grad<T, 3, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_SERSIC) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_SERSIC_ELLIPSE)
+        ray_grad<T, 7, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_GAUSSIAN) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
     }
   }
   }
@@ -2826,15 +2906,19 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   ga.dcoeff = reinterpret_cast<const float *>(ws + L.dcoeff);
   ga.glow = glow;
   ga.gpart = reinterpret_cast<double *>(ws + L.gpart);
+  ga.gbeta = L.gbeta ? static_cast<void *>(ws + L.gbeta) : nullptr;
   const int npix = gg.nx * gg.ny, nblk = (npix + 256 * kGradPix - 1) / (256 * kGradPix);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
-  for (int k = 0; k < m.n_prof; ++k) {
+  // the sources first: the first source's launch stores dS/dbeta at every ray
+  // (ga.gbeta), which the inline lens launches after it read
+  for (int q = 0; q < m.n_prof; ++q) {
+    const int k = q < m.n_source ? m.n_defl + q : (q < m.n_source + m.n_defl ? q - m.n_source : q);
     if (k == m.n_lens && m.n_bulk > 0) {  // the bulk list in one launch
       rc = m.has_halo ? launch_bulk_grad<T, true>(m, gg, ga, recd, recf, nblk, n_img, st)
                       : launch_bulk_grad<T, false>(m, gg, ga, recd, recf, nblk, n_img, st);
       if (rc) return rc;
-      k = m.n_defl - 1;
+      q += m.n_bulk - 1;
       continue;
     }
     rc = m.has_halo ? launch_grad_profile<T, true>(m, gg, ga, recd, recf, nblk, n_img, k, st)
@@ -3538,7 +3622,7 @@ size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *
   GridArgs g;
   SSPlan ss;
   if (grid_args(grid, g) || psf_ss(psf, g, 0, ss)) return 0;
-  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision, edge, ss).total;
+  return grad_layout(m, grid->nx, grid->ny, kx, ky, n_img, precision, edge, ss, g.s).total;
 }
 
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -3564,7 +3648,7 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");
   if (!d_logl || !d_grad || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
   if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
-  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge, ss);
+  const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge, ss, g.s);
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
