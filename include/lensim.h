@@ -318,6 +318,25 @@ int lensim_graph_destroy(lensim_graph graph);
 int lensim_timing_enable(int on);
 int lensim_timing_read(double *render_ms, double *conv_ms, int64_t *calls);
 
+/* Bounds-checked debug build (no counterpart in the reference: JAX checks
+ * nothing of the kind).  liblensim_checked.so is this library compiled with
+ * -DLENSIM_BOUNDS_CHECK.  There every compute entry point registers the buffers
+ * of its call (parameters, model arrays, mask, PSF, data, outputs, and the
+ * workspace up to the size it needs), and every kernel access to global
+ * memory is tested against them.  An access outside them is not made (a
+ * store goes to a private sink, a load reads it); the entry point waits for
+ * its stream and returns LENSIM_EDEVICE with the source line of the first such
+ * access.  Calls under graph capture are not checked.  The registered ranges
+ * are global: calls of the checked build must not overlap in time.
+ * lensim_bounds_checked returns 1 in the checked build, 0 otherwise.
+ * lensim_bounds_selftest registers [d_buf, d_buf + bytes) and launches a
+ * kernel that stores bytes 0 .. bytes - 1 (byte i = i mod 256) and then
+ * byte `bytes`, one past the range.  The checked build diverts that last
+ * store and returns LENSIM_EDEVICE; the normal build launches nothing and
+ * returns LENSIM_EUNSUPPORTED.  bytes is at most 2^20. */
+int lensim_bounds_checked(void);
+int lensim_bounds_selftest(void *d_buf, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,7 @@
-"""Builds liblensim.so for gfx950 in-tree (jaxtronomy_amd/_lib/).
+"""Builds liblensim.so for gfx950 in-tree (jaxtronomy_amd/_lib/), and its
+bounds-checked debug form liblensim_checked.so (-DLENSIM_BOUNDS_CHECK: every
+kernel access to a caller or workspace buffer is tested against the call's
+buffers; selected by LENSIM_CHECKED=1 or backend.use_checked()).
 
 hipcc cross-compiles without a GPU.  Packed-FP32 instructions are disabled on
 purpose (-target-feature -packed-fp32-ops); the host pass warns that it does
@@ -16,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "lensim.hip")
 OUT = os.path.join(HERE, "_lib", "liblensim.so")
+OUT_CHECKED = os.path.join(HERE, "_lib", "liblensim_checked.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -36,10 +40,11 @@ def hipcc():
     return "hipcc"
 
 
-def build_library(verbose=False):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp%d" % os.getpid()
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, SRC]
+def build_library(verbose=False, checked=False):
+    out_path = OUT_CHECKED if checked else OUT
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    tmp = out_path + ".tmp%d" % os.getpid()
+    cmd = [hipcc()] + HIPCC_FLAGS + (["-DLENSIM_BOUNDS_CHECK"] if checked else []) + ["-o", tmp, SRC]
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
     if res.returncode != 0:
@@ -48,10 +53,11 @@ def build_library(verbose=False):
         raise RuntimeError("hipcc failed (%d):\n%s" % (res.returncode, out))
     if verbose and out.strip():
         print(out)
-    os.replace(tmp, OUT)  # atomic: a process that already loaded the old .so keeps it
-    return OUT
+    os.replace(tmp, out_path)  # atomic: a process that already loaded the old .so keeps it
+    return out_path
 
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_library(verbose=True, checked=True))
     sys.exit(0)

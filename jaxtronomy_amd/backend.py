@@ -14,6 +14,11 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "liblensim.so")
+# the bounds-checked debug build: every kernel access to a caller or workspace
+# buffer is tested against the call's buffers, and a call with an access
+# outside them fails with LENSIM_EDEVICE naming the source line (each call
+# also waits for its stream); LENSIM_CHECKED=1 loads it in place of LIB_PATH
+LIB_CHECKED_PATH = os.path.join(_HERE, "_lib", "liblensim_checked.so")
 
 ABI_VERSION = 6
 NPARAM = 8
@@ -122,6 +127,8 @@ _SIGNATURES = {
     "lensim_loglike_grad": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P,
                                            ctypes.c_size_t, _P]),
     "lensim_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "lensim_bounds_checked": (ctypes.c_int, []),
+    "lensim_bounds_selftest": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "lensim_loglike_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
     "lensim_graph_capture_loglike": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
                                                     _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_void_p)]),
@@ -139,22 +146,41 @@ class LensimError(RuntimeError):
     pass
 
 
+def _load(path):
+    if not os.path.exists(path):
+        raise LensimError(
+            "%s is not built (%s); run __graft_entry__.build() first" % (os.path.basename(path), path))
+    handle = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    if handle.lensim_abi_version() != ABI_VERSION:
+        raise LensimError("liblensim ABI version mismatch")
+    return handle
+
+
 def lib():
-    """Load liblensim.so (built in-tree by ``__graft_entry__.build()``)."""
+    """Load liblensim.so (built in-tree by ``__graft_entry__.build()``), or
+    liblensim_checked.so when LENSIM_CHECKED=1."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise LensimError(
-                "liblensim.so is not built (%s); run __graft_entry__.build() first" % LIB_PATH)
-        handle = ctypes.CDLL(LIB_PATH)
-        for name, (res, args) in _SIGNATURES.items():
-            fn = getattr(handle, name)
-            fn.restype = res
-            fn.argtypes = args
-        if handle.lensim_abi_version() != ABI_VERSION:
-            raise LensimError("liblensim ABI version mismatch")
-        _lib = handle
+        _lib = _load(LIB_CHECKED_PATH if os.environ.get("LENSIM_CHECKED") == "1" else LIB_PATH)
     return _lib
+
+
+def use_checked(on=True):
+    """Route every later call through the bounds-checked build (on=True) or
+    the normal one; returns the handle in use before, for restore_lib."""
+    global _lib
+    before = _lib
+    _lib = _load(LIB_CHECKED_PATH if on else LIB_PATH)
+    return before
+
+
+def restore_lib(handle):
+    global _lib
+    _lib = handle
 
 
 def require_gpu():

@@ -49,6 +49,109 @@ int hip_fail(hipError_t e, const char *where) {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// ---- bounds-checked build (-DLENSIM_BOUNDS_CHECK, liblensim_checked.so) ----
+// Each entry point registers the buffers of its call (workspace, parameters,
+// data, PSF, outputs) before its launches.  A checked access AT(p, i) tests
+// that element i of p lies inside one of them; one that does not is diverted
+// to a private sink and the first such access's source line is recorded, so
+// the kernel touches no memory outside the call's buffers and still ends
+// normally.  After the call the entry point waits for its stream and returns
+// LENSIM_EDEVICE naming that line, and clears the ranges again: a call that
+// registers none (graph capture, which cannot wait for its stream) runs
+// unchecked.  In the normal build AT(p, i) is p[i].
+constexpr int kMaxRanges = 24;
+struct CheckRanges {
+  int n;
+  unsigned long long lo[kMaxRanges], hi[kMaxRanges];
+};
+#ifdef LENSIM_BOUNDS_CHECK
+__constant__ CheckRanges g_ranges;
+__device__ unsigned int g_violation;
+__device__ __align__(16) unsigned char g_sink[64];
+
+__device__ __forceinline__ bool in_ranges(const void *p, size_t n) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  if (g_ranges.n == 0) return true;  // no ranges registered: unchecked
+  for (int r = 0; r < g_ranges.n; ++r)
+    if (a >= g_ranges.lo[r] && a + n <= g_ranges.hi[r]) return true;
+  return false;
+}
+
+__device__ __noinline__ void report_violation(int line) { atomicCAS(&g_violation, 0u, unsigned(line)); }
+
+template <typename T>
+__device__ __forceinline__ T &checked_at(T *p, size_t i, int line) {
+  if (!in_ranges(p + i, sizeof(T))) {
+    report_violation(line);
+    return *reinterpret_cast<T *>(g_sink);
+  }
+  return p[i];
+}
+#define AT(p, i) checked_at((p), size_t(i), __LINE__)
+// a base pointer and extent the kernel reads from with fixed offsets (records)
+#define CHECK_SPAN(p, n) \
+  do {                   \
+    if (!in_ranges((p), (n))) report_violation(__LINE__); \
+  } while (0)
+#else
+#define AT(p, i) (p)[i]
+#define CHECK_SPAN(p, n) \
+  do {                   \
+  } while (0)
+#endif
+
+// while a call is captured into a graph (lensim_graph_capture_loglike) no
+// timing events are recorded (they would become nodes of the graph) and the
+// checked build registers no ranges (it would have to wait for the stream)
+thread_local bool g_capturing = false;
+
+// The buffers of one call, for the checked build (no-ops otherwise).
+class CallRanges {
+ public:
+  void add(const void *p, size_t bytes) {
+    if (!p || !bytes) return;
+    if (r_.n >= kMaxRanges) {
+      overflow_ = true;
+      return;
+    }
+    r_.lo[r_.n] = reinterpret_cast<unsigned long long>(p);
+    r_.hi[r_.n] = r_.lo[r_.n] + bytes;
+    ++r_.n;
+  }
+#ifdef LENSIM_BOUNDS_CHECK
+  int begin(hipStream_t st) {
+    if (g_capturing) return LENSIM_OK;
+    if (overflow_) return fail(LENSIM_EINVAL, "bounds check: more than %d buffers in one call", kMaxRanges);
+    const unsigned int zero = 0;
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_ranges), &r_, sizeof(r_), 0, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_violation), &zero, sizeof(zero), 0, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // the host copies above read r_ and zero
+    return e == hipSuccess ? LENSIM_OK : hip_fail(e, "bounds check set-up");
+  }
+  int end(hipStream_t st, int rc) {
+    if (g_capturing) return rc;
+    unsigned int line = 0;
+    const int none = 0;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipMemcpyFromSymbol(&line, HIP_SYMBOL(g_violation), sizeof(line));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_ranges), &none, sizeof(none));
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "bounds check");
+    if (line)
+      return fail(LENSIM_EDEVICE, "bounds check: an access at lensim.hip:%u fell outside the call's buffers", line);
+    return LENSIM_OK;
+  }
+#else
+  int begin(hipStream_t) { return LENSIM_OK; }
+  int end(hipStream_t, int rc) { return rc; }
+#endif
+
+ private:
+  CheckRanges r_{};
+  bool overflow_ = false;
+};
+
 constexpr int kWin = 8;  // register window (outputs per thread) of the convolution
 constexpr int kGradPix = 4;  // pixels per thread of the gradient kernel
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
@@ -80,9 +183,6 @@ hipEvent_t timing_event(int slot) {
   return c.ev[slot];
 }
 
-// while a call is captured into a graph (lensim_graph_capture_loglike) no
-// timing events are recorded: they would become nodes of the graph
-thread_local bool g_capturing = false;
 
 void timing_mark(int slot, hipStream_t st) {
   if (!g_timing || g_capturing) return;
@@ -420,9 +520,11 @@ __device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, 
                          float *__restrict__ coeff, double *__restrict__ dcoefd, float *__restrict__ dcoeff) {
   const int k = idx / n_img, img = idx - k * n_img;
   const int ri = img * m.n_prof + k;
+  if (k >= m.n_lens && k < m.n_defl && !m.bulk_type) CHECK_SPAN(m.bulk_types + (k - m.n_lens), sizeof(int32_t));
   int type = type_of(m, k);
   if (k >= m.n_lens && k < m.n_defl && !bulk_type_ok_dev(type)) type = LENSIM_NULL;
   const double *p = params + size_t(ri) * LENSIM_NPARAM;
+  CHECK_SPAN(p, LENSIM_NPARAM * sizeof(double));
   double rec[kRec];
   double *coef = nullptr, *dcoef = nullptr;
   float *cf = nullptr, *dcf = nullptr;
@@ -431,20 +533,24 @@ __device__ void prep_one(const ModelArgs &m, const double *__restrict__ params, 
     const size_t co = (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
     coef = coefd + co;
     cf = coeff + co;
+    CHECK_SPAN(coef, LENSIM_EPL_NMAX * sizeof(double));
+    CHECK_SPAN(cf, LENSIM_EPL_NMAX * sizeof(float));
     if (dcoefd) {
       dcoef = dcoefd + co;
       dcf = dcoeff + co;
+      CHECK_SPAN(dcoef, LENSIM_EPL_NMAX * sizeof(double));
+      CHECK_SPAN(dcf, LENSIM_EPL_NMAX * sizeof(float));
     }
   }
   prepare_record(type, p, rec, coef, m.sub, dcoef, cf, dcf);
   for (int i = 0; i < kRec; ++i) {
-    recd[size_t(ri) * kRec + i] = rec[i];
-    recf[size_t(ri) * kRec + i] = float(rec[i]);
+    AT(recd, size_t(ri) * kRec + i) = rec[i];
+    AT(recf, size_t(ri) * kRec + i) = float(rec[i]);
   }
   if (type == LENSIM_EPL) {  // series lengths as int bits in the float copy
-    recf[size_t(ri) * kRec + EPL_NMAX] = __int_as_float(int(rec[EPL_NMAX]));
-    recf[size_t(ri) * kRec + EPL_NZ] = __int_as_float(int(rec[EPL_NZ]));
-    recf[size_t(ri) * kRec + EPL_NF] = __int_as_float(int(rec[EPL_NF]));
+    AT(recf, size_t(ri) * kRec + EPL_NMAX) = __int_as_float(int(rec[EPL_NMAX]));
+    AT(recf, size_t(ri) * kRec + EPL_NZ) = __int_as_float(int(rec[EPL_NZ]));
+    AT(recf, size_t(ri) * kRec + EPL_NF) = __int_as_float(int(rec[EPL_NF]));
   }
 }
 
@@ -462,12 +568,12 @@ __device__ void psf_prep_one(const double *__restrict__ k, int kx, int ky, int k
                              int flip, int idx) {
   const int c = idx / kyp, a = idx - c * kyp;
   if (!k) {  // identity kernel (no PSF / unconvolved)
-    out[idx] = (a == 0 && c == 0) ? 1.0 : 0.0;
+    AT(out, idx) = (a == 0 && c == 0) ? 1.0 : 0.0;
     return;
   }
   // flip = 0 stores the kernel itself: the convolution then correlates with K,
   // which is the adjoint of convolving with K (zero padding)
-  out[idx] = (a < ky) ? (flip ? k[(ky - 1 - a) * kx + (kx - 1 - c)] : k[a * kx + c]) : 0.0;
+  AT(out, idx) = (a < ky) ? (flip ? AT(k, (ky - 1 - a) * kx + (kx - 1 - c)) : AT(k, a * kx + c)) : 0.0;
 }
 
 __global__ void psf_prep_kernel(const double *__restrict__ k, int kx, int ky, int kyp, double *__restrict__ out,
@@ -489,8 +595,10 @@ __device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double 
                                                const double *coefd, const float *coeff, int img) {
   RecPtrs<T> r;
   r.recd = recd + size_t(img) * m.n_prof * kRec;
+  CHECK_SPAN(r.recd, size_t(m.n_prof) * kRec * sizeof(double));
   if constexpr (sizeof(T) == sizeof(float)) {
     r.rect = recf + size_t(img) * m.n_prof * kRec;
+    CHECK_SPAN(r.rect, size_t(m.n_prof) * kRec * sizeof(float));
     r.coeft = coeff;
   } else {
     r.rect = r.recd;
@@ -514,7 +622,7 @@ __device__ __forceinline__ void deflect_bulk(const ModelArgs &m, const RecPtrs<T
     return;
   }
   for (int k = m.n_lens; k < m.n_defl; ++k) {
-    const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
+    const int type = m.bulk_type ? m.bulk_type : AT(m.bulk_types, k - m.n_lens);
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
     add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
@@ -528,6 +636,7 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    if (e >= 0) CHECK_SPAN(rp.coeft + co, LENSIM_EPL_NMAX * sizeof(T));
     add_deflection<T, HALO, DBL>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   }
   deflect_bulk<T, HALO, DBL>(m, rp, x, y, ax, ay);
@@ -541,9 +650,10 @@ __device__ __forceinline__ void deflect_one(const ModelArgs &m, const RecPtrs<T>
   if (k < m.n_lens) {
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
+    if (e >= 0) CHECK_SPAN(rp.coeft + co, LENSIM_EPL_NMAX * sizeof(T));
     add_deflection<T, HALO, DBL>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   } else {
-    const int type = m.bulk_type ? m.bulk_type : m.bulk_types[k - m.n_lens];
+    const int type = m.bulk_type ? m.bulk_type : AT(m.bulk_types, k - m.n_lens);
     add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
 }
@@ -564,16 +674,16 @@ __device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &
   const size_t o = size_t(img) * size_t(m.mp_stride);
   double angx = x, angy = y, px = 0.0, py = 0.0, tz = 1.0;
   for (int p = 0; p < m.mp_n; ++p) {
-    const double dt = m.mp_tij[o + p];
-    tz = m.mp_tz[o + p];
+    const double dt = AT(m.mp_tij, o + p);
+    tz = AT(m.mp_tz, o + p);
     px += angx * dt;
     py += angy * dt;
     if (p + 1 < m.mp_n) {
       double ax = 0.0, ay = 0.0;
-      int k = m.mp_order[p];
+      int k = AT(m.mp_order, p);
       k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
       deflect_one<T, HALO, DBL>(m, rp, img, k, px / tz, py / tz, ax, ay);
-      const double f = m.mp_r2p[o + p];
+      const double f = AT(m.mp_r2p, o + p);
       angx -= ax * f;
       angy -= ay * f;
     }
@@ -625,7 +735,7 @@ __device__ __forceinline__ int div_by(int n, int d, double inv_d, int &rem) {
 // ImSim/Numerics/grid.py:47-58: the rays of other pixels are not shot and
 // their flux is zero).
 __device__ __forceinline__ bool evaluated(const GridArgs &g, int i, int j) {
-  return !g.mask || g.mask[size_t(j / g.mask_s) * g.mask_nx + i / g.mask_s] != 0;
+  return !g.mask || AT(g.mask, size_t(j / g.mask_s) * g.mask_nx + i / g.mask_s) != 0;
 }
 
 template <typename T, bool HALO, bool DBL>
@@ -668,7 +778,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
     }
   }
   const T v = T(acc * (g.scale * m.flux_scaling));
-  if (active) out[size_t(img) * npix + pix] = v;
+  if (active) AT(out, size_t(img) * npix + pix) = v;
   if (blkmax) {
     // max |v| of this block (the MFMA convolution scales its fp16 operands
     // by the image max); one slot per block, no atomics
@@ -680,7 +790,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
     if (threadIdx.x == 0) {
       float b = wmax[0];
       for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
-      blkmax[size_t(img) * gridDim.x + blockIdx.x] = b;
+      AT(blkmax, size_t(img) * gridDim.x + blockIdx.x) = b;
     }
   }
 }
@@ -935,7 +1045,7 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
     for (int k = 0; k < NLL; ++k) acc = fma(rp.recd[(KS + NSRC + k) * kRec + SE_AMP], double(fl[k]), acc);
   }
   const T v = T(acc * (g.scale * m.flux_scaling));
-  if (active) out[size_t(img) * npix + pix] = v;
+  if (active) AT(out, size_t(img) * npix + pix) = v;
   if (blkmax) {
     __shared__ float wmax[256 / 64];
     float mv = fabsf(float(v));
@@ -945,7 +1055,7 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
     if (threadIdx.x == 0) {
       float b = wmax[0];
       for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
-      blkmax[size_t(img) * gridDim.x + blockIdx.x] = b;
+      AT(blkmax, size_t(img) * gridDim.x + blockIdx.x) = b;
     }
   }
 }
@@ -1025,9 +1135,9 @@ __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double
   if (i >= n_ray) return;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   double tx, ty;
-  trace_ray<T, HALO, DBL>(m, rp, img, xs[i], ys[i], tx, ty);
-  bx[size_t(img) * n_ray + i] = tx;
-  by[size_t(img) * n_ray + i] = ty;
+  trace_ray<T, HALO, DBL>(m, rp, img, AT(xs, i), AT(ys, i), tx, ty);
+  AT(bx, size_t(img) * n_ray + i) = tx;
+  AT(by, size_t(img) * n_ray + i) = ty;
 }
 
 template <typename T>
@@ -1039,7 +1149,7 @@ __global__ __launch_bounds__(256) void light_kernel(ModelArgs m, const double *_
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n_ray) return;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, nullptr, nullptr, img);
-  flux[size_t(img) * n_ray + i] = light_sum<T>(m, rp, k0, k1, xs[i], ys[i]);
+  AT(flux, size_t(img) * n_ray + i) = light_sum<T>(m, rp, k0, k1, AT(xs, i), AT(ys, i));
 }
 
 struct ConvArgs {
@@ -1067,19 +1177,19 @@ __device__ __forceinline__ double rcp_refined(double v) {
 // jnp.where(data >= 0, ...) takes) or the noise map's square (dV/dm = 0).
 __device__ __forceinline__ double dlogl_dm(const ConvArgs &c, size_t p, double m, int img) {
   const size_t q = p + size_t(img) * size_t(c.dstride);  // per-image observation
-  const double d = c.data[q];
+  const double d = AT(c.data, q);
   double var, dvar;
   if (c.noise) {
-    var = c.noise[q] * c.noise[q];
+    var = AT(c.noise, q) * AT(c.noise, q);
     dvar = 0.0;
   } else {
-    const double it = c.expo ? rcp_refined(c.expo[p]) : c.inv_t;  // 1 / exposure
+    const double it = c.expo ? rcp_refined(AT(c.expo, p)) : c.inv_t;  // 1 / exposure
     var = fma(m >= 0.0 ? m : 0.0, it, c.bkg2);
     dvar = m >= 0.0 ? it : 0.0;
   }
   const double r = (m - d) * rcp_refined(var);
   double gp = -(r - 0.5 * r * r * dvar);
-  if (c.mask) gp *= c.mask[p];
+  if (c.mask) gp *= AT(c.mask, p);
   return gp;
 }
 
@@ -1087,18 +1197,18 @@ template <typename T>
 __device__ __forceinline__ double chi2_pixel(const ConvArgs &c, size_t p, T model_t, int img) {
   const size_t q = p + size_t(img) * size_t(c.dstride);  // per-image observation
   const double m = double(model_t);
-  const double d = c.data[q];
+  const double d = AT(c.data, q);
   double var;
   if (c.noise) {
-    const double nm = c.noise[q];
+    const double nm = AT(c.noise, q);
     var = nm * nm;
   } else {
-    const double it = c.expo ? rcp_refined(c.expo[p]) : c.inv_t;  // 1 / exposure
+    const double it = c.expo ? rcp_refined(AT(c.expo, p)) : c.inv_t;  // 1 / exposure
     const double mpos = (m >= 0.0) ? m : 0.0;
     var = fma(mpos, it, c.bkg2);
   }
   double chi = (m - d) * (m - d) * rcp_refined(var);
-  if (c.mask) chi *= c.mask[p];
+  if (c.mask) chi *= AT(c.mask, p);
   return chi;
 }
 
@@ -1148,9 +1258,9 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
       } else if (c.edge) {
         gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
         gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
-        v[u] = src[gy * c.nx + gx];
+        v[u] = AT(src, gy * c.nx + gx);
       } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
-        v[u] = src[gy * c.nx + gx];
+        v[u] = AT(src, gy * c.nx + gx);
       }
     }
 #pragma unroll
@@ -1160,7 +1270,9 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
   const int nch = kyp / kWin, nsteps = c.kx * nch;
   double *wl = reinterpret_cast<double *>(smem_raw + ((size_t(THp) * TW * sizeof(T) + 7) & ~size_t(7)));
   if constexpr (LW) {
-    for (int idx = tid; idx < nsteps * kWin; idx += NT) wl[idx] = kt[idx];
+    for (int idx = tid; idx < nsteps * kWin; idx += NT) wl[idx] = AT(kt, idx);
+  } else {
+    CHECK_SPAN(kt, size_t(nsteps + 1) * kWin * sizeof(double));
   }
   __syncthreads();
   const int lx = tid % TX, tg = tid / TX;
@@ -1221,9 +1333,9 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
       if (y < c.ny) {
         const size_t p = size_t(y) * c.nx + x;
         const T m = T(acc[r]);
-        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (out) AT(out, size_t(img) * c.nx * c.ny + p) = m;
         if (partial) chi += chi2_pixel<T>(c, p, m, img);
-        if (c.gout) static_cast<T *>(c.gout)[size_t(img) * c.nx * c.ny + p] = T(dlogl_dm(c, p, double(m), img));
+        if (c.gout) AT(static_cast<T *>(c.gout), size_t(img) * c.nx * c.ny + p) = T(dlogl_dm(c, p, double(m), img));
       }
     }
   }
@@ -1238,7 +1350,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs c, const T *__restri
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += red[w];
-      partial[size_t(img) * gridDim.x + tile_id] = s;
+      AT(partial, size_t(img) * gridDim.x + tile_id) = s;
     }
   }
 }
@@ -1259,19 +1371,19 @@ __device__ void psf_mfma_prep_block(const double *__restrict__ k, int kx, int ky
                                     int *__restrict__ hdr, half8 *__restrict__ frag, int b) {
   __shared__ double red[4];
   double m = 0.0;
-  for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(k[i]));
+  for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) m = fmax(m, fabs(AT(k, i)));
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   const int sk = (mx > 0.0 && isfinite(mx)) ? 14 - ilogb(mx) : 0;
-  if (b == 0 && threadIdx.x == 0) hdr[0] = sk;
+  if (b == 0 && threadIdx.x == 0) AT(hdr, 0) = sk;
   const double scale = ldexp(1.0, sk);  // exact power of two
   const int idx = b * 256 + threadIdx.x;
   if (idx >= n_steps * 2 * 64) return;
   const int lane = idx & 63, s = (idx >> 6) & 1, ak = idx >> 7;
   if (ak >= ky * nks) {
-    frag[idx] = half8{};
+    AT(frag, idx) = half8{};
     return;
   }
   const int a = ak / nks, ks = ak - a * nks;
@@ -1281,11 +1393,11 @@ __device__ void psf_mfma_prep_block(const double *__restrict__ k, int kx, int ky
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int c = 32 * ks + 8 * g + j - n;  // tap of the flipped kernel row a
-    const double w = (c >= 0 && c < kx) ? krow[kx - 1 - c] * scale : 0.0;
+    const double w = (c >= 0 && c < kx) ? AT(krow, kx - 1 - c) * scale : 0.0;
     const _Float16 h = (_Float16)w;
     v[j] = s == 0 ? h : (_Float16)(w - (double)h);
   }
-  frag[idx] = v;
+  AT(frag, idx) = v;
 }
 
 __global__ __launch_bounds__(256) void psf_mfma_prep_kernel(const double *__restrict__ k, int kx, int ky, int nks,
@@ -1323,7 +1435,7 @@ __device__ void psf_mf64_prep_one(const double *__restrict__ k, int kx, int ky, 
   const int lane = idx & 63, step = idx >> 6;
   const int a = step / nk4, kk = step - a * nk4;
   const int c = 4 * kk + (lane >> 4) - (lane & 15);  // tap of the flipped row a
-  frag[idx] = (a < ky && c >= 0 && c < kx) ? k[size_t(ky - 1 - a) * kx + (kx - 1 - c)] : 0.0;
+  AT(frag, idx) = (a < ky && c >= 0 && c < kx) ? AT(k, size_t(ky - 1 - a) * kx + (kx - 1 - c)) : 0.0;
 }
 
 // Blocks [0, nb_prep): one thread per (image, profile) record; the rest: the PSF job.
@@ -1402,7 +1514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
       for (int q = 0; q < kLdCols; ++q) {
         const int gx0 = X0 - cx + 64 * q + lane;
         const int gx = gx0 < 0 ? 0 : (gx0 >= c.nx ? c.nx - 1 : gx0);
-        const float x = src[size_t(gy) * c.nx + gx];
+        const float x = AT(src, size_t(gy) * c.nx + gx);
         v[r][q] = (row_ok && (c.edge || gx0 == gx)) ? x : 0.f;
       }
     }
@@ -1412,14 +1524,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   __shared__ float smax[4];
   {
     float mv = 0.f;
-    for (int b = tid; b < nblk; b += 256) mv = fmaxf(mv, blkmax[size_t(img) * nblk + b]);
+    for (int b = tid; b < nblk; b += 256) mv = fmaxf(mv, AT(blkmax, size_t(img) * nblk + b));
     for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
     if ((tid & 63) == 0) smax[tid >> 6] = mv;
     __syncthreads();
   }
   const float mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
   const int sa = (mx > 0.f && isfinite(mx)) ? 14 - ilogbf(mx) : 0;
-  const int sk = hdr[0];
+  const int sk = AT(hdr, 0);
   const float scale_a = ldexpf(1.0f, sa);  // exact power of two
   auto store_batch = [&](int ly0) {
 #pragma unroll
@@ -1467,6 +1579,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
   const __amdgpu_buffer_rsrc_t frs =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<half8 *>(frag), (short)0, n_steps * 2048, 0x00020000);
   const int voff = lane * 16;
+  CHECK_SPAN(frag, size_t(n_steps) * 2048);
   auto refill = [&](half8 (&b)[R][2], int s0) {
 #pragma unroll
     for (int p = 0; p < R; ++p) {
@@ -1546,10 +1659,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
       if (x < c.nx && y < c.ny) {
         const size_t p = size_t(y) * c.nx + x;
         const float m = float(dacc[t][j] * unscale);
-        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (out) AT(out, size_t(img) * c.nx * c.ny + p) = m;
         if (partial) chi += chi2_pixel<float>(c, p, m, img);
         if (c.gout)
-          static_cast<float *>(c.gout)[size_t(img) * c.nx * c.ny + p] = float(dlogl_dm(c, p, double(m), img));
+          AT(static_cast<float *>(c.gout), size_t(img) * c.nx * c.ny + p) = float(dlogl_dm(c, p, double(m), img));
       }
     }
     if (partial) {
@@ -1557,7 +1670,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
       const int tx16 = (X0 >> 4) + t;
       // tiles past the image's edge hold no pixel and have no slot
       if (lane == 0 && 16 * tx16 < c.nx && 16 * ty16 < c.ny)
-        partial[size_t(img) * n16 + size_t(ty16) * n16x + tx16] = chi;
+        AT(partial, size_t(img) * n16 + size_t(ty16) * n16x + tx16) = chi;
     }
   }
 }
@@ -1609,9 +1722,9 @@ __global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int 
       } else if (c.edge) {
         gy = gy < 0 ? 0 : (gy >= c.ny ? c.ny - 1 : gy);
         gx = gx < 0 ? 0 : (gx >= c.nx ? c.nx - 1 : gx);
-        v[u] = src[size_t(gy) * c.nx + gx];
+        v[u] = AT(src, size_t(gy) * c.nx + gx);
       } else if (gy >= 0 && gy < c.ny && gx >= 0 && gx < c.nx) {
-        v[u] = src[size_t(gy) * c.nx + gx];
+        v[u] = AT(src, size_t(gy) * c.nx + gx);
       }
     }
 #pragma unroll
@@ -1629,6 +1742,7 @@ __global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int 
   const __amdgpu_buffer_rsrc_t frs =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(frag), (short)0, n_steps * 512, 0x00020000);
   const int voff = lane * 8;
+  CHECK_SPAN(frag, size_t(n_steps) * 512);
   double4_t acc[TL];
 #pragma unroll
   for (int t = 0; t < TL; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
@@ -1681,16 +1795,16 @@ __global__ __launch_bounds__(256) void conv_mf64_kernel(ConvArgs c, int ls, int 
       if (x < c.nx && y < c.ny) {
         const size_t p = size_t(y) * c.nx + x;
         const double m = acc[t][j];
-        if (out) out[size_t(img) * c.nx * c.ny + p] = m;
+        if (out) AT(out, size_t(img) * c.nx * c.ny + p) = m;
         if (partial) chi += chi2_pixel<double>(c, p, m, img);
-        if (c.gout) static_cast<double *>(c.gout)[size_t(img) * c.nx * c.ny + p] = dlogl_dm(c, p, m, img);
+        if (c.gout) AT(static_cast<double *>(c.gout), size_t(img) * c.nx * c.ny + p) = dlogl_dm(c, p, m, img);
       }
     }
     if (partial) {
       for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
       const int tx16 = (X0 >> 4) + t;
       if (lane == 0 && 16 * tx16 < c.nx && 16 * ty16 < c.ny)
-        partial[size_t(img) * n16 + size_t(ty16) * n16x + tx16] = chi;
+        AT(partial, size_t(img) * n16 + size_t(ty16) * n16x + tx16) = chi;
     }
   }
 }
@@ -1753,6 +1867,7 @@ __device__ __forceinline__ gm::Dual<T, 2> sources_at(const ModelArgs &m, const G
   S2 f2 = S2(T(0));
   for (int j = m.n_defl; j < m.n_defl + m.n_source; ++j) {
     const double *pj = ga.params + (size_t(img) * m.n_prof + j) * LENSIM_NPARAM;
+    CHECK_SPAN(pj, LENSIM_NPARAM * sizeof(double));
     const int tj = m.types[j - m.n_bulk];
     if (tj == LENSIM_SERSIC_ELLIPSE) {
       T gp[7], gx, gy;
@@ -1780,6 +1895,7 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
                                          int k, int type, double x, double y, double w, double *acc, T *gb) {
   using S = gm::Dual<T, N>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
   S ps[LENSIM_NPARAM];
 #pragma unroll
   for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(T(pk[i]), i) : S(T(pk[i]));
@@ -1796,8 +1912,12 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
                                             : reinterpret_cast<const T *>(ga.coefd + co);
     const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
                                              : reinterpret_cast<const T *>(ga.dcoefd + co);
+    if (e >= 0) {
+      CHECK_SPAN(c, LENSIM_EPL_NMAX * sizeof(T));
+      CHECK_SPAN(dc, LENSIM_EPL_NMAX * sizeof(T));
+    }
     gm::defl_g<T, S>(type, ps, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
-    const T gx = gb[0], gy = gb[1], as = T(m.alpha_scaling);
+    const T gx = AT(gb, 0), gy = AT(gb, 1), as = T(m.alpha_scaling);
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] += w * double(-as * (gx * ax.d[i] + gy * ay.d[i]));
     return;
@@ -1823,6 +1943,10 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
                                             : reinterpret_cast<const T *>(ga.coefd + co);
     const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
                                              : reinterpret_cast<const T *>(ga.dcoefd + co);
+    if (e >= 0) {
+      CHECK_SPAN(c, LENSIM_EPL_NMAX * sizeof(T));
+      CHECK_SPAN(dc, LENSIM_EPL_NMAX * sizeof(T));
+    }
     const S xs = S(T(x)), ys = S(T(y));
     gm::defl_g<T, S>(type, ps, c, dc, nser, xs, ys, ax, ay);
     const S bx = xs - ax * T(m.alpha_scaling), by = ys - ay * T(m.alpha_scaling);
@@ -1845,15 +1969,15 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
           flux.v = gm::sersic_ellipse_grad<T>(pk, bxs, bys, gp, gx, gy);
 #pragma unroll
           for (int i = 0; i < 7; ++i) flux.d[i] = gp[i];
-          gb[0] = gx;
-          gb[1] = gy;
+          AT(gb, 0) = gx;
+          AT(gb, 1) = gy;
           done = true;
         }
       }
       if (!done) {
         const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxs, bys);
-        gb[0] = f2.d[0];
-        gb[1] = f2.d[1];
+        AT(gb, 0) = f2.d[0];
+        AT(gb, 1) = f2.d[1];
       }
     }
     if (!done) flux = light_grad<T, N>(type, ps, pk, bxs, bys);
@@ -1874,6 +1998,7 @@ __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs
                                               const T *gb) {
   using S = gm::Dual<double, N>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
   if (gb) {  // dS/dbeta from the first source's launch
     S ps[LENSIM_NPARAM];
 #pragma unroll
@@ -1881,7 +2006,7 @@ __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs
     S ax = S(0.0), ay = S(0.0);
     gm::defl_g<double, S>(type, ps, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), 1,
                           S(x), S(y), ax, ay);
-    const double gx = double(gb[0]), gy = double(gb[1]), as = m.alpha_scaling;
+    const double gx = double(AT(gb, 0)), gy = double(AT(gb, 1)), as = m.alpha_scaling;
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] += w * (-as * (gx * ax.d[i] + gy * ay.d[i]));
     return;
@@ -1919,6 +2044,7 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
                                              int k, double x, double y, double w, double *acc, const T *gb) {
   using S = gm::Dual<T, 3>;
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
   double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form (not needed with gb)
   if (!gb) {
     for (int j = 0; j < m.n_lens; ++j) {
@@ -1940,6 +2066,8 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
                                           : reinterpret_cast<const T *>(ga.coefd + co);
   const T *dc = sizeof(T) == sizeof(float) ? reinterpret_cast<const T *>(ga.dcoeff + co)
                                            : reinterpret_cast<const T *>(ga.dcoefd + co);
+  CHECK_SPAN(c, LENSIM_EPL_NMAX * sizeof(T));
+  CHECK_SPAN(dc, LENSIM_EPL_NMAX * sizeof(T));
   S ax = S(T(0)), ay = S(T(0));
   gm::defl_g<T, S>(LENSIM_EPL, ps, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
   const T a_x = ax.v, a_y = ay.v;
@@ -1984,8 +2112,8 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   const T as = T(m.alpha_scaling);
   T gx, gy;
   if (gb) {
-    gx = gb[0];
-    gy = gb[1];
+    gx = AT(gb, 0);
+    gy = AT(gb, 1);
   } else {
     const T bxv = T(x) - (T(axp) + a_x) * as, byv = T(y) - (T(ayp) + a_y) * as;
     const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bxv, byv);
@@ -2021,7 +2149,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
   int i;
   const int j = div_by(pix, g.nx, g.inv_nx, i);
   // the pixel's flux is scale * flux_scaling * (sum over its rays)
-  const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
+  const double w = active ? double(AT(static_cast<const T *>(ga.glow), size_t(img) * npix + pix)) * g.scale *
                                 m.flux_scaling
                           : 0.0;
   const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
@@ -2040,6 +2168,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
       T *gb = (gbeta && (k < m.n_lens || k == m.n_defl))
                   ? gbeta + ((size_t(img) * npix + pix) * size_t(g.s * g.s) + size_t(sy * g.s + sx)) * 2
                   : nullptr;
+      if (gb) CHECK_SPAN(gb, 2 * sizeof(T));
       if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc, gb);
       else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
       else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
@@ -2067,7 +2196,7 @@ __global__ __launch_bounds__(256) void grad_kernel(ModelArgs m, GridArgs g, Grad
   if (threadIdx.x < LENSIM_NPARAM) {
     double v = 0.0;
     for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[wv][threadIdx.x];
-    ga.gpart[((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + threadIdx.x] = v;
+    AT(ga.gpart, ((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + threadIdx.x) = v;
   }
 }
 
@@ -2149,11 +2278,11 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
   const int k = m.n_lens + blockIdx.z * dpb + d;
   const bool mine = k < m.n_defl;
   int type = LENSIM_NULL;
-  if (mine) type = TYPE != LENSIM_NULL ? TYPE : m.bulk_types[k - m.n_lens];
+  if (mine) type = TYPE != LENSIM_NULL ? TYPE : AT(m.bulk_types, k - m.n_lens);
   double pk[LENSIM_NPARAM], acc[LENSIM_NPARAM];
 #pragma unroll
   for (int q = 0; q < LENSIM_NPARAM; ++q) {
-    pk[q] = mine ? ga.params[(size_t(img) * m.n_prof + k) * LENSIM_NPARAM + q] : 0.0;
+    pk[q] = mine ? AT(ga.params, (size_t(img) * m.n_prof + k) * LENSIM_NPARAM + q) : 0.0;
     acc[q] = 0.0;
   }
   __shared__ double lx[256], ly[256], lgx[256], lgy[256];
@@ -2174,7 +2303,7 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
       const double ox = o0 + sx * inv_s, oy = o0 + sy * inv_s;
       const double x = i * g.m00 + j * g.m01 + g.ra0 + g.m00 * ox + g.m01 * oy;
       const double y = i * g.m10 + j * g.m11 + g.dec0 + g.m10 * ox + g.m11 * oy;
-      const double w = active ? double(static_cast<const T *>(ga.glow)[size_t(img) * npix + pix]) * g.scale *
+      const double w = active ? double(AT(static_cast<const T *>(ga.glow), size_t(img) * npix + pix)) * g.scale *
                                     m.flux_scaling
                               : 0.0;
       double gx = 0.0, gy = 0.0;
@@ -2222,7 +2351,7 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
     for (int q = 0; q < LENSIM_NPARAM; ++q) {
       double v = 0.0;
       for (int gi = 0; gi < ngrp; ++gi) v += red[gi * dpb + d][q];
-      ga.gpart[((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + q] = v;
+      AT(ga.gpart, ((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM + q) = v;
     }
   }
 }
@@ -2256,8 +2385,8 @@ __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, i
   if (idx >= n) return;
   const int q = idx % LENSIM_NPARAM, ik = idx / LENSIM_NPARAM;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += gpart[(size_t(ik) * nblk + b) * LENSIM_NPARAM + q];
-  grad[idx] = s;
+  for (int b = 0; b < nblk; ++b) s += AT(gpart, (size_t(ik) * nblk + b) * LENSIM_NPARAM + q);
+  AT(grad, idx) = s;
 }
 
 // One wave per image: lane l sums partials l, l + 64, ... and a fixed
@@ -2268,9 +2397,9 @@ __global__ __launch_bounds__(64) void logl_kernel(const double *__restrict__ par
   const int img = blockIdx.x;
   const double *p = partial + size_t(img) * n_tiles;
   double s = 0.0;
-  for (int t = threadIdx.x; t < n_tiles; t += 64) s += p[t];
+  for (int t = threadIdx.x; t < n_tiles; t += 64) s += AT(p, t);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (threadIdx.x == 0) logl[img] = -s / 2.0;
+  if (threadIdx.x == 0) AT(logl, img) = -s / 2.0;
 }
 
 // ------------------------------------------------------------ launches ---
@@ -2603,8 +2732,8 @@ __global__ void pool_kernel(const T *__restrict__ in, int nx, int ny, int s, T *
   const T *src = in + size_t(img) * nx * ny * s * s + size_t(y) * s * nxh + size_t(x) * s;
   double acc = 0.0;
   for (int sy = 0; sy < s; ++sy)
-    for (int sx = 0; sx < s; ++sx) acc += double(src[size_t(sy) * nxh + sx]);
-  out[size_t(img) * nx * ny + pix] = T(acc / (double(s) * double(s)));
+    for (int sx = 0; sx < s; ++sx) acc += double(AT(src, size_t(sy) * nxh + sx));
+  AT(out, size_t(img) * nx * ny + pix) = T(acc / (double(s) * double(s)));
 }
 
 // Final image of a supersampled convolution: the s x s mean of the convolved
@@ -2623,20 +2752,20 @@ __global__ __launch_bounds__(256) void combine_kernel(ConvArgs c, const T *__res
     const T *src = chr + size_t(img) * npix * s * s + size_t(y) * s * nxh + size_t(x) * s;
     double acc = 0.0;
     for (int sy = 0; sy < s; ++sy)
-      for (int sx = 0; sx < s; ++sx) acc += double(src[size_t(sy) * nxh + sx]);
+      for (int sx = 0; sx < s; ++sx) acc += double(AT(src, size_t(sy) * nxh + sx));
     double v = acc / (double(s) * double(s));
-    if (lowc) v += double(lowc[size_t(img) * npix + pix]);
+    if (lowc) v += double(AT(lowc, size_t(img) * npix + pix));
     const T m = T(v);
-    if (out) out[size_t(img) * npix + pix] = m;
+    if (out) AT(out, size_t(img) * npix + pix) = m;
     if (partial) chi = chi2_pixel<T>(c, pix, m, img);
-    if (c.gout) static_cast<T *>(c.gout)[size_t(img) * npix + pix] = T(dlogl_dm(c, pix, double(m), img));
+    if (c.gout) AT(static_cast<T *>(c.gout), size_t(img) * npix + pix) = T(dlogl_dm(c, pix, double(m), img));
   }
   if (partial) {
     __shared__ double red[4];
     for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
     __syncthreads();
-    if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) AT(partial, size_t(img) * gridDim.x + blockIdx.x) = red[0] + red[1] + red[2] + red[3];
   }
 }
 
@@ -2648,12 +2777,12 @@ __global__ __launch_bounds__(256) void data_chi2_kernel(ConvArgs c, const T *__r
   const int npix = c.nx * c.ny;
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   double chi = 0.0;
-  if (pix < npix) chi = chi2_pixel<T>(c, pix, model[size_t(img) * npix + pix], img);
+  if (pix < npix) chi = chi2_pixel<T>(c, pix, AT(model, size_t(img) * npix + pix), img);
   __shared__ double red[4];
   for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
   __syncthreads();
-  if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) AT(partial, size_t(img) * gridDim.x + blockIdx.x) = red[0] + red[1] + red[2] + red[3];
 }
 
 // Launches conv_kernel<T> ('same', zero or edge padding) of [n_img][ny][nx]
@@ -2702,9 +2831,9 @@ __global__ void unpool_kernel(const T *__restrict__ in, int nx, int ny, int s, i
   if (idx >= nxh * nyh) return;
   const int img = blockIdx.y;
   const int J = idx / nxh, I = idx - J * nxh;
-  const double v = double(in[size_t(img) * nx * ny + size_t(J / s) * nx + I / s]) / (double(s) * double(s));
-  T *o = out + size_t(img) * nxh * nyh + idx;
-  *o = accumulate ? T(double(*o) + v) : T(v);
+  const double v = double(AT(in, size_t(img) * nx * ny + size_t(J / s) * nx + I / s)) / (double(s) * double(s));
+  T &o = AT(out, size_t(img) * nxh * nyh + idx);
+  o = accumulate ? T(double(o) + v) : T(v);
 }
 
 // Supersampled convolution (supersampling_convolution=True,
@@ -2811,8 +2940,8 @@ __global__ void pad_kernel(const T *__restrict__ in, int nx, int ny, int cx, int
   const int img = blockIdx.y;
   const int Y = idx / nxe, X = idx - Y * nxe;
   const int y = Y - cy, x = X - cx;
-  const T v = (y >= 0 && y < ny && x >= 0 && x < nx) ? in[size_t(img) * nx * ny + size_t(y) * nx + x] : T(0);
-  out[size_t(img) * nxe * nye + idx] = v;
+  const T v = (y >= 0 && y < ny && x >= 0 && x < nx) ? AT(in, size_t(img) * nx * ny + size_t(y) * nx + x) : T(0);
+  AT(out, size_t(img) * nxe * nye + idx) = v;
 }
 
 // Adjoint of edge clamping: pixel (y, x) receives every extended pixel
@@ -2829,8 +2958,8 @@ __global__ void fold_kernel(const T *__restrict__ ext, int nx, int ny, int cx, i
   const T *e = ext + size_t(img) * nxe * nye;
   double s = 0.0;
   for (int Y = y0; Y <= y1; ++Y)
-    for (int X = x0; X <= x1; ++X) s += double(e[size_t(Y) * nxe + X]);
-  out[size_t(img) * nx * ny + idx] = T(s);
+    for (int X = x0; X <= x1; ++X) s += double(AT(e, size_t(Y) * nxe + X));
+  AT(out, size_t(img) * nx * ny + idx) = T(s);
 }
 
 // Adjoint of the 'same' convolution of [n_img][ny][nx] by a kernel prepared
@@ -2945,7 +3074,7 @@ __global__ void amp_one_kernel(const double *__restrict__ params, int n_img, int
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_img * n_prof * LENSIM_NPARAM) return;
   const int q = idx % LENSIM_NPARAM, k = (idx / LENSIM_NPARAM) % n_prof;
-  out[idx] = (k >= k_light0 && q == 0) ? 1.0 : params[idx];
+  AT(out, idx) = (k >= k_light0 && q == 0) ? 1.0 : AT(params, idx);
 }
 
 // Weights mask / C_D(data) (ImageModel.error_response, image_model.py:577-596:
@@ -2962,18 +3091,18 @@ __global__ __launch_bounds__(256) void normal_kernel(ConvArgs c, const T *__rest
   for (int q = 0; q < kLinAcc; ++q) acc[q] = 0.0;
   if (p < npix) {
     const size_t pd = p + size_t(img) * size_t(c.dstride);
-    const double d = c.data[pd];
+    const double d = AT(c.data, pd);
     double var;
     if (c.noise) {
-      var = c.noise[pd] * c.noise[pd];
+      var = AT(c.noise, pd) * AT(c.noise, pd);
     } else {
-      const double t = c.expo ? c.expo[p] : c.exposure_time;
+      const double t = c.expo ? AT(c.expo, p) : c.exposure_time;
       var = (d >= 0.0 ? d : 0.0) / t + c.bkg2;
     }
-    const double w = (c.mask ? c.mask[p] : 1.0) / var;
+    const double w = (c.mask ? AT(c.mask, p) : 1.0) / var;
     double a[kMaxLin];
 #pragma unroll
-    for (int l = 0; l < kMaxLin; ++l) a[l] = l < n_lin ? double(A[(size_t(l) * n_img + img) * npix + p]) : 0.0;
+    for (int l = 0; l < kMaxLin; ++l) a[l] = l < n_lin ? double(AT(A, (size_t(l) * n_img + img) * npix + p)) : 0.0;
     if (w != 0.0) {
       int q = 0;
 #pragma unroll
@@ -2995,7 +3124,7 @@ __global__ __launch_bounds__(256) void normal_kernel(ConvArgs c, const T *__rest
   }
   __syncthreads();
   for (int q = threadIdx.x; q < kLinAcc; q += blockDim.x)
-    part[(size_t(img) * gridDim.x + blockIdx.x) * kLinAcc + q] = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+    AT(part, (size_t(img) * gridDim.x + blockIdx.x) * kLinAcc + q) = red[0][q] + red[1][q] + red[2][q] + red[3][q];
 }
 
 // Eigenvalues of a symmetric n x n matrix (cyclic Jacobi rotations).
@@ -3078,7 +3207,7 @@ __global__ void lin_solve_kernel(const double *__restrict__ part, int nblk, int 
   double acc[kLinAcc];
   for (int q = 0; q < kLinAcc; ++q) acc[q] = 0.0;
   for (int b = 0; b < nblk; ++b)
-    for (int q = 0; q < kLinAcc; ++q) acc[q] += part[(size_t(img) * nblk + b) * kLinAcc + q];
+    for (int q = 0; q < kLinAcc; ++q) acc[q] += AT(part, (size_t(img) * nblk + b) * kLinAcc + q);
   double M[kMaxLin][kMaxLin], W[kMaxLin][kMaxLin], R[kMaxLin];
   int q = 0;
   for (int i = 0; i < kMaxLin; ++i)
@@ -3110,7 +3239,7 @@ __global__ void lin_solve_kernel(const double *__restrict__ part, int nblk, int 
     lu_solve(W, piv, n, B);
     for (int i = 0; i < n; ++i) finite = finite && isfinite(B[i]);
   }
-  for (int i = 0; i < n; ++i) amp[size_t(img) * n + i] = finite ? B[i] : 0.0;
+  for (int i = 0; i < n; ++i) AT(amp, size_t(img) * n + i) = finite ? B[i] : 0.0;
   if (cov) {
     for (int j = 0; j < n; ++j) {
       double e[kMaxLin];
@@ -3118,7 +3247,7 @@ __global__ void lin_solve_kernel(const double *__restrict__ part, int nblk, int 
       if (stable && parity != 0) lu_solve(W, piv, n, e);
       for (int i = 0; i < n; ++i) {
         const double v = (stable && parity != 0 && isfinite(e[i])) ? e[i] : 0.0;
-        cov[(size_t(img) * n + i) * n + j] = v;
+        AT(cov, (size_t(img) * n + i) * n + j) = v;
       }
     }
   }
@@ -3147,7 +3276,7 @@ __global__ void lin_solve_kernel(const double *__restrict__ part, int nblk, int 
       mc = isnan(log_det) ? -1e15 : log_det / 2.0 + n / 2.0 * log(M_PI / 2.0) - n * log(d_prior);
     }
   }
-  marg[img] = mc;
+  AT(marg, img) = mc;
 }
 
 // Solved model (zero outside the likelihood mask, array_masked2image) and its chi^2.
@@ -3161,22 +3290,23 @@ __global__ __launch_bounds__(256) void lin_model_kernel(ConvArgs c, const T *__r
   double chi = 0.0;
   if (p < npix) {
     double m = 0.0;
-    if (!c.mask || c.mask[p] != 0.0)
-      for (int l = 0; l < n_lin; ++l) m += amp[size_t(img) * n_lin + l] * double(A[(size_t(l) * n_img + img) * npix + p]);
+    if (!c.mask || AT(c.mask, p) != 0.0)
+      for (int l = 0; l < n_lin; ++l)
+        m += AT(amp, size_t(img) * n_lin + l) * double(AT(A, (size_t(l) * n_img + img) * npix + p));
     const T mt = T(m);
-    if (out) out[size_t(img) * npix + p] = mt;
+    if (out) AT(out, size_t(img) * npix + p) = mt;
     chi = chi2_pixel<T>(c, p, mt, img);
   }
   __shared__ double red[4];
   for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chi;
   __syncthreads();
-  if (threadIdx.x == 0) partial[size_t(img) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) AT(partial, size_t(img) * gridDim.x + blockIdx.x) = red[0] + red[1] + red[2] + red[3];
 }
 
 __global__ void add_kernel(double *__restrict__ x, const double *__restrict__ y, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += y[i];
+  if (i < n) AT(x, i) += AT(y, i);
 }
 
 // Workspace of the linear solve: the image layout of one column, then the
@@ -3308,6 +3438,43 @@ int check_ws(const Layout &L, void *ws, size_t ws_bytes) {
   return LENSIM_OK;
 }
 
+// The caller's buffers that a call reads: parameters, the bulk type list, the
+// multi-plane arrays, the evaluation mask, the PSF kernels and the data, each
+// with the extent the header gives it.
+void add_inputs(CallRanges &cr, const lensim_model *model, const ModelArgs &m, const double *params, int n_img,
+                const lensim_grid *grid, const lensim_psf *psf, const lensim_data *data, int nx, int ny) {
+  cr.add(params, size_t(n_img) * m.n_prof * LENSIM_NPARAM * sizeof(double));
+  if (m.bulk_types) cr.add(m.bulk_types, size_t(m.n_bulk) * sizeof(int32_t));
+  if (const lensim_multiplane *mp = model->multiplane) {
+    const size_t n = mp->stride ? size_t(n_img - 1) * size_t(mp->stride) + mp->n_planes : size_t(mp->n_planes);
+    cr.add(mp->d_order, size_t(mp->n_planes - 1) * sizeof(int32_t));
+    cr.add(mp->d_T_ij, n * sizeof(double));
+    cr.add(mp->d_T_z, n * sizeof(double));
+    cr.add(mp->d_r2p, n * sizeof(double));
+  }
+  if (grid) cr.add(grid->d_eval_mask, size_t(grid->nx) * grid->ny);
+  if (psf) {
+    cr.add(psf->d_kernel, size_t(psf->kx) * psf->ky * sizeof(double));
+    cr.add(psf->d_kernel_low, size_t(psf->kx_low) * psf->ky_low * sizeof(double));
+  }
+  if (data) {
+    const size_t npix = size_t(nx) * ny;
+    const size_t nd = data->image_stride ? size_t(n_img - 1) * size_t(data->image_stride) + npix : npix;
+    cr.add(data->d_data, nd * sizeof(double));
+    cr.add(data->d_noise_map, nd * sizeof(double));
+    cr.add(data->d_exposure_map, npix * sizeof(double));
+    cr.add(data->d_mask, npix * sizeof(double));
+  }
+}
+
+#ifdef LENSIM_BOUNDS_CHECK
+// Stores bytes 0 .. n - 1 of buf and then byte n (one past the registered range).
+__global__ void bounds_selftest_kernel(unsigned char *buf, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i <= n) AT(buf, i) = static_cast<unsigned char>(i & 255);
+}
+#endif
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI -----
@@ -3346,6 +3513,34 @@ int lensim_timing_read(double *render_ms, double *conv_ms, int64_t *calls) {
 
 const char *lensim_last_error(void) { return g_err; }
 
+int lensim_bounds_checked(void) {
+#ifdef LENSIM_BOUNDS_CHECK
+  return 1;
+#else
+  return 0;
+#endif
+}
+
+int lensim_bounds_selftest(void *d_buf, size_t bytes, void *stream) {
+#ifdef LENSIM_BOUNDS_CHECK
+  if (!d_buf) return fail(LENSIM_EINVAL, "NULL buffer");
+  if (bytes == 0 || bytes > (size_t(1) << 20)) return fail(LENSIM_EINVAL, "bytes must be in [1, 2^20]");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  CallRanges cr;
+  cr.add(d_buf, bytes);
+  int rc = cr.begin(st);
+  if (rc) return rc;
+  bounds_selftest_kernel<<<unsigned((bytes + 1 + 255) / 256), 256, 0, st>>>(static_cast<unsigned char *>(d_buf),
+                                                                             bytes);
+  return cr.end(st, check_launch("bounds_selftest_kernel"));
+#else
+  (void)d_buf;
+  (void)bytes;
+  (void)stream;
+  return fail(LENSIM_EUNSUPPORTED, "lensim_bounds_selftest needs the bounds-checked build (liblensim_checked.so)");
+#endif
+}
+
 size_t lensim_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                              int32_t n_img, int32_t precision) {
   ModelArgs m;
@@ -3380,7 +3575,15 @@ int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
-  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, nullptr, nullptr, nullptr, 0, 0);
+  cr.add(d_x, size_t(n_ray) * sizeof(double));
+  cr.add(d_y, size_t(n_ray) * sizeof(double));
+  cr.add(d_beta_x, size_t(n_img) * size_t(n_ray) * sizeof(double));
+  cr.add(d_beta_y, size_t(n_img) * size_t(n_ray) * sizeof(double));
+  cr.add(ws, L.total);
+  if ((rc = cr.begin(st))) return rc;
+  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return cr.end(st, rc);
   dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
   const double *recd = reinterpret_cast<const double *>(w + L.recd);
   const float *recf = reinterpret_cast<const float *>(w + L.recf);
@@ -3399,7 +3602,7 @@ int lensim_raytrace(const lensim_model *model, const double *d_params, int32_t n
     else LENSIM_TRACE(float, false, false);
   }
 #undef LENSIM_TRACE
-  return check_launch("raytrace_kernel");
+  return cr.end(st, check_launch("raytrace_kernel"));
 }
 
 int lensim_surface_brightness(const lensim_model *model, const double *d_params, int32_t n_img, int32_t which,
@@ -3419,7 +3622,14 @@ int lensim_surface_brightness(const lensim_model *model, const double *d_params,
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
-  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return rc;
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, nullptr, nullptr, nullptr, 0, 0);
+  cr.add(d_x, size_t(n_ray) * sizeof(double));
+  cr.add(d_y, size_t(n_ray) * sizeof(double));
+  cr.add(d_flux, size_t(n_img) * size_t(n_ray) * sizeof(double));
+  cr.add(ws, L.total);
+  if ((rc = cr.begin(st))) return rc;
+  if ((rc = launch_prep(m, d_params, n_img, w, L, st))) return cr.end(st, rc);
   const int k0 = which == 0 ? m.n_defl : m.n_defl + m.n_source;
   const int k1 = which == 0 ? m.n_defl + m.n_source : m.n_prof;
   dim3 grid((unsigned)((n_ray + 255) / 256), n_img);
@@ -3431,7 +3641,7 @@ int lensim_surface_brightness(const lensim_model *model, const double *d_params,
     light_kernel<float><<<grid, 256, 0, st>>>(m, reinterpret_cast<double *>(w + L.recd),
                                                reinterpret_cast<float *>(w + L.recf), k0, k1, d_x, d_y, n_ray,
                                                d_flux);
-  return check_launch("light_kernel");
+  return cr.end(st, check_launch("light_kernel"));
 }
 
 int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf, const double *d_params,
@@ -3459,18 +3669,27 @@ int lensim_image(const lensim_model *model, const lensim_grid *grid, const lensi
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
+  const size_t tsz = precision == LENSIM_FP64 ? sizeof(double) : sizeof(float);
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, grid, psf, nullptr, g.nx, g.ny);
+  cr.add(d_out, size_t(n_img) * g.nx * g.ny * tsz);
+  cr.add(ws, L.total);
+  if ((rc = cr.begin(st))) return rc;
   if (ss.on) {
     if (precision == LENSIM_FP64)
-      return run_image_ss<double>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
-                                  static_cast<double *>(d_out), nullptr, w, L, st);
-    return run_image_ss<float>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
+      rc = run_image_ss<double>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
+                                static_cast<double *>(d_out), nullptr, w, L, st);
+    else
+      rc = run_image_ss<float>(m, g, psf, kx, ky, edge, ss, nullptr, d_params, n_img, flags,
                                static_cast<float *>(d_out), nullptr, w, L, st);
-  }
-  if (precision == LENSIM_FP64)
-    return run_image<double>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags,
-                             static_cast<double *>(d_out), nullptr, w, L, st);
-  return run_image<float>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags, static_cast<float *>(d_out),
+  } else if (precision == LENSIM_FP64) {
+    rc = run_image<double>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags, static_cast<double *>(d_out),
+                           nullptr, w, L, st);
+  } else {
+    rc = run_image<float>(m, g, psf, kx, ky, edge, nullptr, d_params, n_img, flags, static_cast<float *>(d_out),
                           nullptr, w, L, st);
+  }
+  return cr.end(st, rc);
 }
 
 int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -3500,18 +3719,28 @@ int lensim_loglike(const lensim_model *model, const lensim_grid *grid, const len
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
   const unsigned flags = LENSIM_IMAGE_SOURCE | LENSIM_IMAGE_LENS_LIGHT;
+  const size_t tsz = precision == LENSIM_FP64 ? sizeof(double) : sizeof(float);
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, grid, psf, data, g.nx, g.ny);
+  cr.add(d_logl, size_t(n_img) * sizeof(double));
+  cr.add(d_model_out, size_t(n_img) * g.nx * g.ny * tsz);
+  cr.add(ws, L.total);
+  if ((rc = cr.begin(st))) return rc;
   if (ss.on) {
     if (precision == LENSIM_FP64)
-      return run_image_ss<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
-                                  static_cast<double *>(d_model_out), d_logl, w, L, st);
-    return run_image_ss<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
+      rc = run_image_ss<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
+                                static_cast<double *>(d_model_out), d_logl, w, L, st);
+    else
+      rc = run_image_ss<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, flags,
                                static_cast<float *>(d_model_out), d_logl, w, L, st);
-  }
-  if (precision == LENSIM_FP64)
-    return run_image<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags,
-                             static_cast<double *>(d_model_out), d_logl, w, L, st);
-  return run_image<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<float *>(d_model_out),
+  } else if (precision == LENSIM_FP64) {
+    rc = run_image<double>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<double *>(d_model_out),
+                           d_logl, w, L, st);
+  } else {
+    rc = run_image<float>(m, g, psf, kx, ky, edge, data, d_params, n_img, flags, static_cast<float *>(d_model_out),
                           d_logl, w, L, st);
+  }
+  return cr.end(st, rc);
 }
 
 size_t lensim_data_loglike_workspace_size(int32_t nx, int32_t ny, int32_t n_img) {
@@ -3551,14 +3780,25 @@ int lensim_data_loglike(const lensim_data *data, int32_t nx, int32_t ny, const v
   c.dstride = data->image_stride;
   double *partial = static_cast<double *>(ws);
   const int nblk = (nx * ny + 255) / 256;
+  CallRanges cr;
+  const size_t npix = size_t(nx) * ny;
+  const size_t nd = data->image_stride ? size_t(n_img - 1) * size_t(data->image_stride) + npix : npix;
+  cr.add(data->d_data, nd * sizeof(double));
+  cr.add(data->d_noise_map, nd * sizeof(double));
+  cr.add(data->d_exposure_map, npix * sizeof(double));
+  cr.add(data->d_mask, npix * sizeof(double));
+  cr.add(d_model, size_t(n_img) * npix * (precision == LENSIM_FP64 ? sizeof(double) : sizeof(float)));
+  cr.add(d_logl, size_t(n_img) * sizeof(double));
+  cr.add(ws, need);
+  int rc = cr.begin(st);
+  if (rc) return rc;
   if (precision == LENSIM_FP64)
     data_chi2_kernel<double><<<dim3(nblk, n_img), 256, 0, st>>>(c, static_cast<const double *>(d_model), partial);
   else
     data_chi2_kernel<float><<<dim3(nblk, n_img), 256, 0, st>>>(c, static_cast<const float *>(d_model), partial);
-  int rc = check_launch("data_chi2_kernel");
-  if (rc) return rc;
+  if ((rc = check_launch("data_chi2_kernel"))) return cr.end(st, rc);
   logl_kernel<<<n_img, 64, 0, st>>>(partial, nblk, d_logl);
-  return check_launch("logl_kernel");
+  return cr.end(st, check_launch("logl_kernel"));
 }
 
 size_t lensim_linear_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -3604,13 +3844,26 @@ int lensim_linear_solve(const lensim_model *model, const lensim_grid *grid, cons
   if (!ws || (reinterpret_cast<uintptr_t>(ws) & 255)) return fail(LENSIM_EINVAL, "workspace must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
+  const size_t tsz = precision == LENSIM_FP64 ? sizeof(double) : sizeof(float);
+  const size_t npix = size_t(g.nx) * g.ny;
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, grid, psf, data, g.nx, g.ny);
+  cr.add(d_amp, size_t(n_img) * n_lin * sizeof(double));
+  cr.add(d_logl, size_t(n_img) * sizeof(double));
+  cr.add(d_model_out, size_t(n_img) * npix * tsz);
+  cr.add(d_cov, size_t(n_img) * n_lin * n_lin * sizeof(double));
+  cr.add(d_response_out, size_t(n_lin) * n_img * npix * tsz);
+  cr.add(ws, LL.total);
+  if ((rc = cr.begin(st))) return rc;
   if (precision == LENSIM_FP64)
-    return run_linear<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
-                              d_logl, static_cast<double *>(d_model_out), d_cov,
-                              static_cast<double *>(d_response_out), w, LL, st);
-  return run_linear<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
+    rc = run_linear<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
+                            d_logl, static_cast<double *>(d_model_out), d_cov,
+                            static_cast<double *>(d_response_out), w, LL, st);
+  else
+    rc = run_linear<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, source_marg, linear_prior, d_amp,
                            d_logl, static_cast<float *>(d_model_out), d_cov, static_cast<float *>(d_response_out), w,
                            LL, st);
+  return cr.end(st, rc);
 }
 
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
@@ -3652,9 +3905,17 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if ((rc = check_ws(L, ws, ws_bytes))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char *w = static_cast<unsigned char *>(ws);
+  CallRanges cr;
+  add_inputs(cr, model, m, d_params, n_img, grid, psf, data, g.nx, g.ny);
+  cr.add(d_logl, size_t(n_img) * sizeof(double));
+  cr.add(d_grad, size_t(n_img) * m.n_prof * LENSIM_NPARAM * sizeof(double));
+  cr.add(ws, L.total);
+  if ((rc = cr.begin(st))) return rc;
   if (precision == LENSIM_FP64)
-    return run_grad<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
-  return run_grad<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+    rc = run_grad<double>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+  else
+    rc = run_grad<float>(m, g, psf, kx, ky, edge, ss, data, d_params, n_img, d_logl, d_grad, w, L, st);
+  return cr.end(st, rc);
 }
 
 size_t lensim_noalloc_scratch_bytes(void) { return kScratchBytes; }

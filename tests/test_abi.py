@@ -30,6 +30,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.lensim_abi_version() == backend.ABI_VERSION
 
 
+def test_checked_library_exports_every_declared_symbol():
+    """liblensim_checked.so (the bounds-checked debug build) has the same C ABI
+    and says it is the checked build; the normal one says it is not."""
+    checked = backend._load(backend.LIB_CHECKED_PATH)
+    for name in _declared_functions():
+        assert hasattr(checked, name), name
+    assert checked.lensim_abi_version() == backend.ABI_VERSION
+    assert checked.lensim_bounds_checked() == 1
+    assert backend.lib().lensim_bounds_checked() == 0
+    # the normal build refuses the self-test before any device work
+    assert backend.lib().lensim_bounds_selftest(None, 16, None) == -2
+
+
 def test_struct_layouts():
     assert ctypes.sizeof(backend.LensimModel) == 12 + 4 * 64 + 4 + 16 + 8 + 8 + 8  # padded to 8 before the doubles
     assert ctypes.sizeof(backend.LensimMultiplane) == 8 + 4 * 8 + 8

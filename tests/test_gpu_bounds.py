@@ -1,0 +1,226 @@
+"""The bounds-checked debug build (liblensim_checked.so, -DLENSIM_BOUNDS_CHECK)
+on the GPU: its self-test diverts a store one past the registered buffer and
+reports it, and every entry point on the representative workloads (the
+compile-time render with the matrix-core convolutions of both forms, the
+generic render, supersampled convolution, sub-frames with per-image data, the
+gradients with inline and bulk deflectors, multi-plane ray shooting, the
+light profiles and the linear solve) runs with no access outside its call's
+buffers and returns what the normal build returns."""
+
+import numpy as np
+import pytest
+import torch
+
+from jaxtronomy_amd import backend
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.imaging_data import ImageData
+from jaxtronomy_amd.Data.psf import PSF
+from jaxtronomy_amd.ImSim.image_linear_solve import ImageLinearFit
+from jaxtronomy_amd.ImSim.image_model import ImageModel
+from jaxtronomy_amd.LensModel.lens_model import LensModel
+from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+from jaxtronomy_amd.LightModel.light_model import LightModel
+
+pytestmark = pytest.mark.gpu
+
+EUNSUPPORTED, EDEVICE = -2, -3
+
+
+def _np(t):
+    return t.detach().cpu().numpy() if torch.is_tensor(t) else np.asarray(t)
+
+
+def _both(fn):
+    """fn() through the normal library, then through the checked one (every
+    call of which waits for its stream and raises on an access outside its
+    buffers); the two results as numpy trees."""
+    want = fn()
+    torch.cuda.synchronize()
+    before = backend.use_checked(True)
+    try:
+        got = fn()
+        torch.cuda.synchronize()
+    finally:
+        backend.restore_lib(before)
+    conv = lambda r: tuple(_np(x) for x in r) if isinstance(r, tuple) else (_np(r),)  # noqa: E731
+    return conv(want), conv(got)
+
+
+def _same(want, got, rtol):
+    for w, g in zip(want, got):
+        w = np.asarray(w, dtype=np.float64)
+        g = np.asarray(g, dtype=np.float64)
+        assert w.shape == g.shape
+        assert np.all(np.isfinite(g))
+        scale = max(np.max(np.abs(w)), 1e-300)
+        assert np.max(np.abs(g - w)) <= rtol * scale, (np.max(np.abs(g - w)), scale)
+
+
+def test_selftest_diverts_the_store_past_the_buffer():
+    checked = backend._load(backend.LIB_CHECKED_PATH)
+    n = 1000
+    buf = torch.zeros(n + 256, dtype=torch.uint8, device="cuda")  # 256 guard bytes after the registered range
+    torch.cuda.synchronize()
+    rc = checked.lensim_bounds_selftest(backend.ptr(buf), n, backend.stream_handle())
+    assert rc == EDEVICE
+    assert b"bounds check" in checked.lensim_last_error()
+    b = _np(buf)
+    np.testing.assert_array_equal(b[:n], np.arange(n) % 256)
+    assert not b[n:].any()  # the store one past the range was not made
+    # the normal build refuses without launching anything
+    assert backend.lib().lensim_bounds_selftest(backend.ptr(buf), n, backend.stream_handle()) == EUNSUPPORTED
+
+
+def _epl_model(precision, n_pix=60, s=2, k=21, numerics=None, psf=None):
+    cfg = dict(W.EPL180, numPix=n_pix, supersampling=s, deltaPix=0.08)
+    kernel = W.moffat_kernel(k, 2.0, 3.0)
+    probe = ImageModel(ImageData(**W.data_kwargs(cfg)), PSF("PIXEL", kernel_point_source=kernel),
+                       LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                       kwargs_numerics={"supersampling_factor": s})
+    kl, ks, kll = W.epl180_draw(3)
+    data = W.noisy(probe.image(kl, ks, kll).cpu().numpy(), cfg["exposure_time"], cfg["background_rms"],
+                   np.random.default_rng(4))
+    nm = {"supersampling_factor": s}
+    nm.update(numerics or {})
+    return ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf or PSF("PIXEL", kernel_point_source=kernel),
+                      LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                      kwargs_numerics=nm, precision=precision)
+
+
+def _params(im, n, seed=10):
+    draws = [W.epl180_draw(seed + i) for i in range(n)]
+    p, _ = im.pack(W.stack_kwargs([d[0] for d in draws]), W.stack_kwargs([d[1] for d in draws]),
+                   W.stack_kwargs([d[2] for d in draws]))
+    return torch.as_tensor(p, device="cuda")
+
+
+RTOL = {"fp64": 1e-12, "mixed": 1e-6}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_loglike_image_and_gradient(precision):
+    """The compile-time render, the matrix-core convolution with chi^2
+    partials, and the gradient (first-source dS/dbeta buffer, adjoint
+    convolution) on a batch of 5."""
+    im = _epl_model(precision)
+    p = _params(im, 5)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+
+    def run():
+        model = torch.empty((5,) + im._full_shape, dtype=dtype, device="cuda")
+        logl = im.likelihood_batch(p, model_out=model)
+        ll2, grad = im.likelihood_and_grad_batch(p)
+        return logl, model, ll2, grad
+
+    want, got = _both(run)
+    _same(want, got, RTOL[precision])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_supersampled_convolution_and_edge_padding(precision):
+    s = 3
+    k_super = W.moffat_kernel(5 * s, 1.5 * s, 3.0)
+    psf = PSF("PIXEL", kernel_point_source=k_super, point_source_supersampling_factor=s)
+    im = _epl_model(precision, n_pix=24, s=s, psf=psf,
+                    numerics={"supersampling_convolution": True, "supersampling_kernel_size": 3})
+    p = _params(im, 3)
+
+    def run():
+        logl = im.likelihood_batch(p)
+        ll2, grad = im.likelihood_and_grad_batch(p)
+        return logl, ll2, grad
+
+    want, got = _both(run)
+    _same(want, got, RTOL[precision])
+
+
+def test_checked_sub_frame_with_per_image_data():
+    """flux_evaluate_indexes (evaluation mask on the grid) and per-image
+    observations (data stride) through lensim_image and lensim_data_loglike."""
+    mask = np.zeros((40, 40), bool)
+    mask[6:30, 9:33] = True
+    im = _epl_model("fp64", n_pix=40, numerics={"flux_evaluate_indexes": mask})
+    p = _params(im, 3)
+    obs = torch.as_tensor(np.random.default_rng(1).normal(0, 0.01, (3, 40, 40)), device="cuda")
+    want, got = _both(lambda: im.likelihood_batch(p, data=obs))
+    _same(want, got, 1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_generic_render_and_bulk_gradient(precision):
+    """A LensModelBulk list (EPL + SHEAR inline, 40 TNFW and SIE in the bulk
+    loop with a per-deflector type list): the generic render and the bulk
+    gradient kernel."""
+    rng = np.random.default_rng(3)
+    names, kws = ["EPL", "SHEAR"], []
+    kl, ks, _ = W.epl180_draw(2)
+    kws += kl
+    for q in range(40):
+        cx, cy = rng.uniform(-1.5, 1.5, 2)
+        if q % 2:
+            names.append("SIE")
+            kws.append(dict(theta_E=rng.uniform(0.02, 0.06), e1=0.05, e2=-0.02, center_x=cx, center_y=cy))
+        else:
+            rs = rng.uniform(0.05, 0.2)
+            names.append("TNFW")
+            kws.append(dict(Rs=rs, alpha_Rs=rng.uniform(0.01, 0.05), r_trunc=5 * rs, center_x=cx, center_y=cy))
+    cfg = dict(W.EPL180, numPix=40, supersampling=2, deltaPix=0.1)
+    kernel = W.moffat_kernel(11, 2.0, 3.0)
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((40, 40)))), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModelBulk(["EPL", "SHEAR", "TNFW", "SIE"], precision=precision),
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2},
+                    precision=precision)
+    lm = im.LensModel
+    rs = lm.prepare_ray_shooting_kwargs(names, kws, num_deflectors=48)
+
+    def run():
+        img = im.image(rs, ks)
+        logl, glens, gsrc, _ = im.likelihood_gradient(rs, ks)
+        return img, np.array([float(logl)]), np.concatenate([np.ravel(v) for v in glens.values()])
+
+    want, got = _both(run)
+    _same(want, got, RTOL[precision])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_multiplane_raytrace_and_light(precision):
+    names, kw, z = W.los_halo_draw(1)
+    lm = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True, precision=precision)
+    rs = lm.prepare_ray_shooting_kwargs(names, kw, z_source=W.LOS["z_source"], lens_redshift_list=z,
+                                        num_deflectors=W.LOS["num_deflectors"])
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-10, 10, 700)
+    y = rng.uniform(-10, 10, 700)
+    light = LightModel(["SERSIC_ELLIPSE", "SERSIC"], precision=precision)
+    kl = [dict(amp=3.0, R_sersic=0.4, n_sersic=2.5, e1=0.1, e2=-0.05, center_x=0.1, center_y=0.0),
+          dict(amp=1.0, R_sersic=1.0, n_sersic=1.0, center_x=0.0, center_y=0.2)]
+
+    def run():
+        bx, by = lm.ray_shooting(x, y, rs)
+        return bx, by, light.surface_brightness(x, y, kl)
+
+    want, got = _both(run)
+    _same(want, got, 1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_linear_solve(precision):
+    cfg = dict(W.EPL180, numPix=40, supersampling=2, deltaPix=0.08)
+    kernel = W.moffat_kernel(11, 2.0, 3.0)
+    kl, ks, kll = W.epl180_draw(9)
+    data = np.random.default_rng(12).normal(0, 0.02, (40, 40))
+    mask = np.ones((40, 40))
+    mask[:4] = 0
+    im = ImageLinearFit(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        kwargs_numerics={"supersampling_factor": 2}, likelihood_mask=mask, precision=precision)
+    ks_in = [dict(ks[0], amp=1.0)]
+    kll_in = [dict(kll[0], amp=1.0)]
+
+    def run():
+        model, err, cov, amp = im.image_linear_solve(kl, [dict(ks_in[0])], [dict(kll_in[0])], inv_bool=True)
+        logl, _ = im.likelihood_data_given_model(kl, ks_in, kll_in, source_marg=True)
+        return model, np.asarray(cov), np.asarray(amp), np.array([float(logl)])
+
+    want, got = _both(run)
+    _same(want, got, 1e-10 if precision == "fp64" else 1e-5)
