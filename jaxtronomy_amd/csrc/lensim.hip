@@ -960,11 +960,17 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   const double *rds = rp.recd + KS * kRec;
   wx -= rds[R_CX];
   wy -= rds[R_CY];
-  // the Sersic floors, canonicalised once (fmax then needs no per-ray canonicalisation)
+  // the Sersic floors: canonicalised once for the FP64 form's fmax, as float
+  // bits for the float form's integer max
   const double umin_s = __builtin_canonicalize(rds[SE_UMIN]);
+  const int uminb_s = __float_as_int(float(rds[SE_UMIN]));
   double umin_l[NLL > 0 ? NLL : 1];
+  int uminb_l[NLL > 0 ? NLL : 1];
 #pragma unroll
-  for (int k = 0; k < NLL; ++k) umin_l[k] = __builtin_canonicalize(rp.recd[(KS + NSRC + k) * kRec + SE_UMIN]);
+  for (int k = 0; k < NLL; ++k) {
+    umin_l[k] = __builtin_canonicalize(rp.recd[(KS + NSRC + k) * kRec + SE_UMIN]);
+    uminb_l[k] = __float_as_int(float(rp.recd[(KS + NSRC + k) * kRec + SE_UMIN]));
+  }
   DeflCoords<D0> u0;
   u0.init(r0.d, m.sub, x0, y0);
   constexpr int D1C = (D1 == LENSIM_SHEAR || D1 == 0) ? LENSIM_NFW : D1;  // unused when D1 is SHEAR or none
@@ -999,7 +1005,7 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
         const double xs = fma(rds[SE_M00S], bx, rds[SE_M01S] * by);
         const double ys = fma(rds[SE_M01S], bx, rds[SE_M11S] * by);
         if constexpr (kF32)
-          fs[0] += sersic_u_f32(rds, reinterpret_cast<const float *>(rp.rect) + KS * kRec, xs, ys, umin_s);
+          fs[0] += sersic_u_f32(rds, reinterpret_cast<const float *>(rp.rect) + KS * kRec, xs, ys, uminb_s);
         else
           acc += rds[SE_AMP] * sersic_u_f64(rds, xs, ys, umin_s);
       }
@@ -1009,7 +1015,7 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
           const double *rd = rp.recd + (KS + NSRC + k) * kRec;
           if constexpr (kF32)
             fl[k] += sersic_u_f32(rd, reinterpret_cast<const float *>(rp.rect) + (KS + NSRC + k) * kRec, lx[k],
-                                  ly[k], umin_l[k]);
+                                  ly[k], uminb_l[k]);
           else
             acc += rd[SE_AMP] * sersic_u_f64(rd, lx[k], ly[k], umin_l[k]);
         }

@@ -128,7 +128,7 @@ enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS,
               SE_IRS2,    // 1 / R_sersic_^2
               SE_R2MAX,   // (1000 R_sersic_)^2  (the 1000 R_sersic window on R^2)
               SE_LNRS,    // ln R_sersic_
-              SE_BNLO,    // b_n - float(b_n): the float form carries b_n as a head + tail pair
+              SE_K2LO,    // K2 - float(K2): the float form carries K2 as a head + tail pair
               SE_M00S,    // the matrix / R_sersic_: (xs, ys) = M (x - c) / R_sersic_, u = xs^2 + ys^2
               SE_M01S,
               SE_M11S,
@@ -136,8 +136,9 @@ enum : int { SE_AMP = 2, SE_M00, SE_M01, SE_M11, SE_RS, SE_INVN, SE_BN, SE_IRS,
               SE_XSX,     // steps of (xs, ys): next sub-ray, next row
               SE_YSX,
               SE_XRR,
-              SE_YRR };
-static_assert(SE_YRR < kRec, "SERSIC record exceeds kRec");
+              SE_YRR,
+              SE_K2 };    // K2 = b_n log2(e): the profile is 2^(K2 (1 - frac^(1/n)))
+static_assert(SE_K2 < kRec, "SERSIC record exceeds kRec");
 // GAUSSIAN light (gaussian.py:27-42): amp/(2 pi sigma^2), 1/sigma^2
 enum : int { GA_C = 2, GA_IS2 };
 
@@ -383,7 +384,8 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
       rec[SE_IRS2] = 1.0 / (Rs_ * Rs_);
       rec[SE_R2MAX] = (1000.0 * Rs_) * (1000.0 * Rs_);
       rec[SE_LNRS] = log(Rs_);
-      rec[SE_BNLO] = rec[SE_BN] - double(float(rec[SE_BN]));
+      rec[SE_K2] = rec[SE_BN] * 1.4426950408889634;
+      rec[SE_K2LO] = rec[SE_K2] - double(float(rec[SE_K2]));
       rec[SE_M00S] = rec[SE_M00] / Rs_;
       rec[SE_M01S] = rec[SE_M01] / Rs_;
       rec[SE_M11S] = rec[SE_M11] / Rs_;
@@ -416,6 +418,13 @@ struct Rec {
   const T *t;       // kRec values of T (== d for T = double)
   __device__ __forceinline__ T operator[](int i) const { return t[i]; }
 };
+
+// max(u, floor) of a non-negative float by its bits (the order of non-negative
+// floats is that of their bit patterns): one integer max, where fmax would
+// first quiet its operands.
+__device__ __forceinline__ float floor_nonneg_f32(float u, int floor_bits) {
+  return __int_as_float(max(__float_as_int(u), floor_bits));
+}
 
 // Sum_{n < N} c_n w^n for real c_n at complex w, by dividing by the real
 // quadratic (v - w)(v - conj w) = v^2 - s v + t (s = 2 Re w, t = |w|^2):
@@ -625,7 +634,7 @@ __device__ __forceinline__ void epl_major_f32(const Rec<float> &r, const float *
   double iR = double(iRs);
   iR = above ? iR * fma(-0.5 * R2d, iR * iR, 1.5) : 1e9;
   // (b/R)^(t-1)
-  const float R2cf = R2 > 1e-18f ? R2 : 1e-18f;
+  const float R2cf = floor_nonneg_f32(R2, 0x219392ef);  // max(R^2, 1e-18f); R2 >= 0
   const float lnbr = fmaf(-0.34657359f, __builtin_amdgcn_logf(R2cf), r[EPL_LNB]);  // ln b - ln(R^2)/2
   const float em = expm1_f32(r[EPL_TM1] * lnbr);
   const double pbr = r.d[EPL_PB] * iR;
@@ -970,14 +979,13 @@ __device__ __forceinline__ double sersic_kernel(double R2, const Rec<double> &r)
   return v;
 }
 
-// exp(e + e_lo) in float to ~2 ulp (|e_lo| << ulp(e)): exp2 of e log2(e) split
-// into a float head and the rounding error of the product, plus e_lo log2(e),
-// as 2^hi (1 + lo ln 2).
-__device__ __forceinline__ float exp_accurate_f32(float e, float e_lo = 0.f) {
-  const float L = 1.4426950408889634f;
-  const float hi = e * L;
-  float lo = fmaf(e, L, -hi) + e * 1.925963033500011e-8f;  // log2(e) - L
-  lo = fmaf(e_lo, L, lo);
+// The Sersic profile exp(b_n t) = 2^(K2 t), t = 1 - frac^(1/n), in float to
+// ~2 ulp: K2 = b_n log2(e) is a float head + tail pair (rf[SE_K2],
+// rf[SE_K2LO]), the head product's rounding error is kept, and the result is
+// 2^hi (1 + lo ln 2), so the exponent carries no one-signed rounding error.
+__device__ __forceinline__ float sersic_exp2_f32(const float *rf, float t) {
+  const float hi = rf[SE_K2] * t;
+  const float lo = fmaf(rf[SE_K2], t, -hi) + rf[SE_K2LO] * t;
   return __builtin_amdgcn_exp2f(hi) * fmaf(lo, 0.6931471805599453f, 1.f);
 }
 
@@ -989,8 +997,8 @@ __device__ __forceinline__ float exp_accurate_f32(float e, float e_lo = 0.f) {
 //  * v_log_f32 of a large |log2 u|: gfx950's v_log_f32 reads about 0.45 ulp
 //    low on average, so the logarithm is split as exponent + log2 of the
 //    mantissa in [0.5, 1), and only the small second term is approximated;
-//  * a float b_n: the exponent b_n (1 - frac^(1/n)) takes b_n as a head + tail
-//    pair with the head product's rounding error.
+//  * a float b_n: the exponent b_n log2(e) (1 - frac^(1/n)) takes b_n log2(e)
+//    as a head + tail pair with the head product's rounding error.
 // frac^(1/n) = exp2(C1 log2 u) and exp of the exponent are then float
 // (v_exp_f32 is unbiased to 0.03 ulp).  Window: R^2 <= (1000 R_sersic)^2.
 __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Rec<float> &r) {
@@ -1002,10 +1010,7 @@ __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Re
   const int ex = __builtin_amdgcn_frexp_expf(u);
   const double L = double(ex) + double(__builtin_amdgcn_logf(m));
   const float pw = __builtin_amdgcn_exp2f(float(r.d[SE_C1] * L));
-  const float t = 1.f - pw;
-  const float e = r[SE_BN] * t;
-  const float e_lo = fmaf(r[SE_BN], t, -e) + r[SE_BNLO] * t;
-  return (R2 <= r.d[SE_R2MAX]) ? exp_accurate_f32(e, e_lo) : 0.f;
+  return (R2 <= r.d[SE_R2MAX]) ? sersic_exp2_f32(r.t, 1.f - pw) : 0.f;
 }
 
 // The Sersic kernel (no amplitude) at the scaled elliptical coordinates
@@ -1024,18 +1029,18 @@ __device__ __forceinline__ double sersic_u_f64(const double *rd, double xs, doub
   return v;
 }
 
-// Float form of sersic_u_f64 with sersic_value_f32's unbiased arithmetic.
-__device__ __forceinline__ float sersic_u_f32(const double *rd, const float *rf, double xs, double ys, double umin) {
-  const double u = fmax(fma(xs, xs, ys * ys), umin);
-  const float uf = float(u);
+// Float form of sersic_u_f64 with sersic_value_f32's unbiased arithmetic.  u
+// is formed in double and floored in float (umin_bits: the bits of
+// float(rd[SE_UMIN]); the floor is monotone, so u below the double floor gets
+// the float floor); the window u <= 1e6 is tested in float (1e6 is a float).
+__device__ __forceinline__ float sersic_u_f32(const double *rd, const float *rf, double xs, double ys,
+                                              int umin_bits) {
+  const float uf = floor_nonneg_f32(float(fma(xs, xs, ys * ys)), umin_bits);
   const float m = __builtin_amdgcn_frexp_mantf(uf);  // u = m 2^ex, m in [0.5, 1)
   const int ex = __builtin_amdgcn_frexp_expf(uf);
   const double L = double(ex) + double(__builtin_amdgcn_logf(m));
   const float pw = __builtin_amdgcn_exp2f(float(rd[SE_C1] * L));
-  const float t = 1.f - pw;
-  const float e = rf[SE_BN] * t;
-  const float e_lo = fmaf(rf[SE_BN], t, -e) + rf[SE_BNLO] * t;
-  return (u <= 1e6) ? exp_accurate_f32(e, e_lo) : 0.f;
+  return (uf <= 1e6f) ? sersic_exp2_f32(rf, 1.f - pw) : 0.f;
 }
 
 // One light profile at (x, y): SERSIC_ELLIPSE / SERSIC (sersic_ellipse.py:62-93,
