@@ -738,6 +738,37 @@ __device__ __forceinline__ bool evaluated(const GridArgs &g, int i, int j) {
   return !g.mask || AT(g.mask, size_t(j / g.mask_s) * g.mask_nx + i / g.mask_s) != 0;
 }
 
+// The maximum over the 64 lanes of a wave (every lane active) of a
+// non-negative float, by the integer max of the bit patterns (ordered as the
+// values are): DPP quad permutations and row rotations give each row of 16
+// lanes its maximum in every lane, readlane the four rows'.  Eight VALU
+// instructions where a shuffle butterfly takes about fifty.
+__device__ __forceinline__ float wave_max_nonneg(float x) {
+  int v = __float_as_int(x);
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1, 0, 3, 2]
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2, 3, 0, 1]
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xF, false));  // row_ror:4
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false));  // row_ror:8
+  const int r = max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+                    max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+  return __int_as_float(r);
+}
+
+// The block's maximum of the threads' non-negative x (every thread of the
+// block reaches this call) into *slot: one slot per block, no atomics.  The
+// MFMA convolution scales its fp16 operands by the image maximum.
+__device__ __forceinline__ void block_max_store(float x, float *slot) {
+  __shared__ float wmax[256 / 64];
+  const float mv = wave_max_nonneg(x);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = wmax[0];
+    for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
+    AT(slot, 0) = b;
+  }
+}
+
 template <typename T, bool HALO, bool DBL>
 __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
                                                      const float *__restrict__ recf,
@@ -779,20 +810,7 @@ __global__ __launch_bounds__(256) void render_kernel(ModelArgs m, GridArgs g, co
   }
   const T v = T(acc * (g.scale * m.flux_scaling));
   if (active) AT(out, size_t(img) * npix + pix) = v;
-  if (blkmax) {
-    // max |v| of this block (the MFMA convolution scales its fp16 operands
-    // by the image max); one slot per block, no atomics
-    __shared__ float wmax[256 / 64];
-    float mv = fabsf(float(v));
-    for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mv;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float b = wmax[0];
-      for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
-      AT(blkmax, size_t(img) * gridDim.x + blockIdx.x) = b;
-    }
-  }
+  if (blkmax) block_max_store(fabsf(float(v)), blkmax + size_t(img) * gridDim.x + blockIdx.x);
 }
 
 // The coordinates of a compile-time deflector that the sub-ray loop advances:
@@ -1052,18 +1070,7 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
   }
   const T v = T(acc * (g.scale * m.flux_scaling));
   if (active) AT(out, size_t(img) * npix + pix) = v;
-  if (blkmax) {
-    __shared__ float wmax[256 / 64];
-    float mv = fabsf(float(v));
-    for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mv;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float b = wmax[0];
-      for (int w = 1; w < int(blockDim.x >> 6); ++w) b = fmaxf(b, wmax[w]);
-      AT(blkmax, size_t(img) * gridDim.x + blockIdx.x) = b;
-    }
-  }
+  if (blkmax) block_max_store(fabsf(float(v)), blkmax + size_t(img) * gridDim.x + blockIdx.x);
 }
 
 int check_launch(const char *where);
