@@ -493,35 +493,36 @@ __device__ __forceinline__ void real_poly_at_complex(const ST *__restrict__ c, i
 __device__ __forceinline__ void epl_major_f64(const Rec<double> &r, const double *__restrict__ coef, double Zr,
                                               double Zi, double R2, double &fr, double &fi) {
   const double t = r.d[EPL_T], f = r.d[EPL_F];
-  double R = ::sqrt(R2);
-  R = R > 0.000000001 ? R : 0.000000001;
+  // R = max(|Z|, 1e-9) enters only through ln R = ln(max(R^2, 1e-18)) / 2
+  const double R2c = R2 > 1e-18 ? R2 : 1e-18;
   double Or, Oi;
   const int nmax = (int)r.d[EPL_NMAX];
   if (nmax == 1) {  // f == 0: the reference returns 2F1 = 1 (epl.py:294, :317)
     Or = Zr;
     Oi = Zi;
   } else {
-    // z = -f Z / conj(Z) = -f Z^2 / |Z|^2 (NaN at Z = 0, as the reference)
-    const double inv = 1.0 / R2;
+    // z = -f Z / conj(Z) = -f Z^2 / |Z|^2 (NaN at Z = 0, as the reference);
+    // the reciprocals refined to two ulp, not divided
+    const double inv = rcp_refined(R2);
     const double zr = -f * (Zr * Zr - Zi * Zi) * inv;
     const double zi = -f * (2.0 * Zr * Zi) * inv;
     // w = z / (z - 2)
     const double dr = zr - 2.0;
-    const double dd = 1.0 / (dr * dr + zi * zi);
+    const double dd = rcp_refined(dr * dr + zi * zi);
     const double wr = (zr * dr + zi * zi) * dd;
     const double wi = (zi * dr - zr * zi) * dd;
     double Sr, Si;
     real_poly_at_complex<double>(coef, nmax, wr, wi, Sr, Si);
     // 2F1 = S / (1 - z/2)
     const double er = 1.0 - zr * 0.5, ei = -zi * 0.5;
-    const double ed = 1.0 / (er * er + ei * ei);
+    const double ed = rcp_refined(er * er + ei * ei);
     const double hr = (Sr * er + Si * ei) * ed;
     const double hi = (Si * er - Sr * ei) * ed;
     Or = Zr * hr - Zi * hi;
     Oi = Zr * hi + Zi * hr;
   }
   // (b/R)^t = exp(t (ln b - ln R))
-  const double scale = r.d[EPL_PREF] * ::exp(t * (r.d[EPL_LNB] - ::log(R)));
+  const double scale = r.d[EPL_PREF] * ::exp(t * (r.d[EPL_LNB] - 0.5 * ::log(R2c)));
   fr = nan_to_1e10(scale * Or);
   fi = nan_to_1e10(scale * Oi);
 }
