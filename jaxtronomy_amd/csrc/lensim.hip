@@ -2158,8 +2158,11 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
                                            : reinterpret_cast<const T *>(ga.dcoefd + co);
   CHECK_SPAN(c, LENSIM_EPL_NMAX * sizeof(T));
   CHECK_SPAN(dc, LENSIM_EPL_NMAX * sizeof(T));
+  // q, cos phi, sin phi and their derivatives from the record, not per ray
+  const double *rdk = rp.recd + k * kRec;
+  const gm::EplGeo<T> geo = gm::epl_geo<T>(rdk, T(pk[2]), T(pk[3]));
   S ax = S(T(0)), ay = S(T(0));
-  gm::defl_g<T, S>(LENSIM_EPL, ps, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
+  gm::epl_g_geo<T>(ps, geo.q, geo.cs, geo.sn, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
   const T a_x = ax.v, a_y = ay.v;
   const T thetaE = T(pk[0]), t = T(pk[1]) - T(1);
   T dax[6], day[6];
@@ -2172,10 +2175,7 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
     day[i + 1] = ay.d[i];
   }
   // Hessian in the major-axis frame, rotated back
-  T phi, q;
-  gm::phi_q<T>(T(pk[2]), T(pk[3]), phi, q);
-  const T b = thetaE * gm::sqrt(q);
-  const T cs = gm::cos(phi), sn = gm::sin(phi);
+  const T q = geo.q.v, cs = geo.cs.v, sn = geo.sn.v;
   const T dx = T(x) - T(pk[4]), dy = T(y) - T(pk[5]);
   const T xr = dx * cs + dy * sn, yr = dy * cs - dx * sn;
   const T r = gm::sqrt(xr * xr + yr * yr);
@@ -2183,7 +2183,7 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   if (r > T(0)) {
     T R = gm::sqrt(q * q * xr * xr + yr * yr);
     R = R >= T(0.00000001) ? R : T(0.00000001);
-    const T kappa = (T(2) - t) / T(2) * gm::exp(t * gm::log(b / R));
+    const T kappa = (T(2) - t) / T(2) * gm::exp(t * (T(rdk[EPL_LNB]) - gm::log(R)));  // (b/R)^t
     const T ct = xr / r, st = yr / r;
     const T c2 = ct * ct * T(2) - T(1), s2 = st * ct * T(2);
     const T fr = a_x * cs + a_y * sn, fi = a_y * cs - a_x * sn;  // alpha in the major-axis frame
