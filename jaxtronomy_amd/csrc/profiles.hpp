@@ -1034,11 +1034,13 @@ __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Re
 // lens light's (xs, ys) advance by a constant step per sub-ray).  frac^(1/n)
 // = u^(1/(2n)); the floors and the window as sersic_kernel, on u.
 // umin: rd[SE_UMIN], which a caller may hoist (canonicalised) out of its loop.
+// In base 2: frac^(1/n) = 2^(C1 log2 u) and exp(b_n (1 - frac^(1/n))) =
+// 2^(K2 (1 - frac^(1/n))), K2 = b_n log2(e) (the exp2 / log2 pair issues fewer
+// instructions than exp / log).
 __device__ __forceinline__ double sersic_u_f64(const double *rd, double xs, double ys, double umin) {
   const double u = fmax(fma(xs, xs, ys * ys), umin);
-  const double pw = ::exp(0.5 * ::log(u) * rd[SE_INVN]);
-  const double e = -rd[SE_BN] * (pw - 1.0);
-  double v = (u <= 1e6) ? ::exp(e) : 0.0;
+  const double pw = ::exp2(::log2(u) * rd[SE_C1]);
+  double v = (u <= 1e6) ? ::exp2(rd[SE_K2] * (1.0 - pw)) : 0.0;
   if (v != v) v = 0.0;
   if (v == INFINITY) v = 1.7976931348623157e308;
   return v;
