@@ -105,10 +105,11 @@ enum : int { EPL_C = 2, EPL_S, EPL_Q, EPL_B, EPL_T, EPL_F, EPL_PREF, EPL_NMAX,
               EPL_ZRRI,
               EPL_AC,     // alpha_scaling cos phi, alpha_scaling sin phi (the rotation back, scaled)
               EPL_AS,
-              EPL_M2F };  // -2 f
+              EPL_M2F,    // -2 f
+              EPL_LB2 };  // log2 b
 // In the float copy of an EPL record, EPL_NMAX, EPL_NZ and EPL_NF hold the
 // int's bits (read with __float_as_int), so no per-ray conversion is needed.
-static_assert(EPL_AS < kRec, "EPL record exceeds kRec");
+static_assert(EPL_LB2 < kRec, "EPL record exceeds kRec");
 // The float form's z-series coefficients g_1..g_NZ sit at this offset of an
 // EPL's float coefficient array, past the w-series' F_n (nmax <= 50 whenever
 // the z-series is chosen).
@@ -218,6 +219,7 @@ __device__ inline void prepare_record(int type, const double *p, double *rec, do
         rec[EPL_AS] = st.asc * sn;
       }
       rec[EPL_M2F] = -2.0 * f;
+      rec[EPL_LB2] = log2(b);
       if (coef) {
         // F_n by the recurrence, with float copies and (for gradients) dF_n/dt
         // written as they are formed: nothing is read back from memory
@@ -521,8 +523,8 @@ __device__ __forceinline__ void epl_major_f64(const Rec<double> &r, const double
     Or = Zr * hr - Zi * hi;
     Oi = Zr * hi + Zi * hr;
   }
-  // (b/R)^t = exp(t (ln b - ln R))
-  const double scale = r.d[EPL_PREF] * ::exp(t * (r.d[EPL_LNB] - 0.5 * ::log(R2c)));
+  // (b/R)^t = 2^(t (log2 b - log2(R^2) / 2))
+  const double scale = r.d[EPL_PREF] * ::exp2(t * (r.d[EPL_LB2] - 0.5 * ::log2(R2c)));
   fr = nan_to_1e10(scale * Or);
   fi = nan_to_1e10(scale * Oi);
 }
