@@ -81,6 +81,62 @@ __device__ __forceinline__ double rcp_refined(double v) {
   return (e == e) ? r : r0;
 }
 
+// log2 x for finite x > 0 (the callers floor their arguments), to a few ulp
+// in about a third of the library log2's instructions: x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <=
+// 0.1716, and atanh(s) = s (1 + z / 3 + z^2 / 5 + ... + z^9 / 19), z = s^2 (the
+// next term, s^21 / 21, is below 2.4e-17 s).  m + 1 lies in [1.7, 2.5], so the
+// reciprocal's Newton step needs no guard.
+__device__ __forceinline__ double log2_pos(double x) {
+  int e;
+  double m = ::frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+  if (m < 0.7071067811865476) {
+    m += m;
+    --e;
+  }
+  const double d = m + 1.0;
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  const double sv = (m - 1.0) * r;
+  const double z = sv * sv;
+  double q = 0.05263157894736842;  // 1/19
+  q = fma(q, z, 0.058823529411764705);
+  q = fma(q, z, 0.06666666666666667);
+  q = fma(q, z, 0.07692307692307693);
+  q = fma(q, z, 0.09090909090909091);
+  q = fma(q, z, 0.1111111111111111);
+  q = fma(q, z, 0.14285714285714285);
+  q = fma(q, z, 0.2);
+  q = fma(q, z, 0.3333333333333333);
+  const double at = fma(sv * z, q, sv);  // atanh(s)
+  return fma(at, 2.8853900817779268, double(e));  // 2 atanh(s) / ln 2 + e
+}
+
+// 2^x for finite x, to a few ulp in half the library exp2's instructions:
+// x = n + r, n = rint(x), |r| <= 1/2, 2^r = sum_k (r ln 2)^k / k! to k = 13
+// (the next term is below 4.2e-18), times 2^n by ldexp; x is clamped to
+// [-1100, 1100] first, so the result is 0 below 2^-1075 and inf above 2^1024.
+__device__ __forceinline__ double exp2_fin(double x) {
+  x = fmin(fmax(x, -1100.0), 1100.0);
+  const double n = ::rint(x);
+  const double r = x - n;
+  double p = 1.3691488853904128e-12;
+  p = fma(p, r, 2.5678435993488206e-11);
+  p = fma(p, r, 4.4455382718708116e-10);
+  p = fma(p, r, 7.054911620801123e-09);
+  p = fma(p, r, 1.01780860092397e-07);
+  p = fma(p, r, 1.321548679014431e-06);
+  p = fma(p, r, 1.5252733804059841e-05);
+  p = fma(p, r, 0.0001540353039338161);
+  p = fma(p, r, 0.0013333558146428443);
+  p = fma(p, r, 0.009618129107628477);
+  p = fma(p, r, 0.05550410866482158);
+  p = fma(p, r, 0.24022650695910072);
+  p = fma(p, r, 0.6931471805599453);
+  p = fma(p, r, 1.0);
+  return ::ldexp(p, int(n));
+}
+
 // jnp.nan_to_num(v, posinf=1e10, neginf=-1e10) as used at LensModel/Profiles/epl.py:391-392
 template <typename T>
 __device__ __forceinline__ T nan_to_1e10(T v) {
@@ -524,7 +580,7 @@ __device__ __forceinline__ void epl_major_f64(const Rec<double> &r, const double
     Oi = Zr * hi + Zi * hr;
   }
   // (b/R)^t = 2^(t (log2 b - log2(R^2) / 2))
-  const double scale = r.d[EPL_PREF] * ::exp2(t * (r.d[EPL_LB2] - 0.5 * ::log2(R2c)));
+  const double scale = r.d[EPL_PREF] * exp2_fin(t * (r.d[EPL_LB2] - 0.5 * log2_pos(R2c)));
   fr = nan_to_1e10(scale * Or);
   fi = nan_to_1e10(scale * Oi);
 }
@@ -986,11 +1042,12 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
 // Sersic kernel, LensModel/Profiles/sersic_utils.py:239-267 (FP64 form), on
 // R^2: R_ = max(1e-4, R), frac = R_ / R_sersic_, frac^(1/n) = exp((ln R_ -
 // ln R_sersic_) / n), zero beyond frac = 1000, NaN -> 0.
+// In base 2 (log2_pos, exp2_fin): frac^(1/n) = 2^((log2(R^2) / 2 - log2 R_s) / n)
+// and exp(b_n (1 - frac^(1/n))) = 2^(K2 (1 - frac^(1/n))).
 __device__ __forceinline__ double sersic_kernel(double R2, const Rec<double> &r) {
   const double R2c = fmax(1e-8, R2);
-  const double pw = ::exp((0.5 * ::log(R2c) - r.d[SE_LNRS]) * r.d[SE_INVN]);
-  const double e = -r.d[SE_BN] * (pw - 1.0);
-  double v = (R2c <= r.d[SE_R2MAX]) ? ::exp(e) : 0.0;
+  const double pw = exp2_fin((0.5 * log2_pos(R2c) - r.d[SE_LNRS] * 1.4426950408889634) * r.d[SE_INVN]);
+  double v = (R2c <= r.d[SE_R2MAX]) ? exp2_fin(r.d[SE_K2] * (1.0 - pw)) : 0.0;
   if (v != v) v = 0.0;
   if (v == INFINITY) v = 1.7976931348623157e308;
   return v;
@@ -1037,12 +1094,11 @@ __device__ __forceinline__ float sersic_value_f32(double dx, double dy, const Re
 // = u^(1/(2n)); the floors and the window as sersic_kernel, on u.
 // umin: rd[SE_UMIN], which a caller may hoist (canonicalised) out of its loop.
 // In base 2: frac^(1/n) = 2^(C1 log2 u) and exp(b_n (1 - frac^(1/n))) =
-// 2^(K2 (1 - frac^(1/n))), K2 = b_n log2(e) (the exp2 / log2 pair issues fewer
-// instructions than exp / log).
+// 2^(K2 (1 - frac^(1/n))), K2 = b_n log2(e), by log2_pos and exp2_fin.
 __device__ __forceinline__ double sersic_u_f64(const double *rd, double xs, double ys, double umin) {
   const double u = fmax(fma(xs, xs, ys * ys), umin);
-  const double pw = ::exp2(::log2(u) * rd[SE_C1]);
-  double v = (u <= 1e6) ? ::exp2(rd[SE_K2] * (1.0 - pw)) : 0.0;
+  const double pw = exp2_fin(log2_pos(u) * rd[SE_C1]);
+  double v = (u <= 1e6) ? exp2_fin(rd[SE_K2] * (1.0 - pw)) : 0.0;
   if (v != v) v = 0.0;
   if (v == INFINITY) v = 1.7976931348623157e308;
   return v;
