@@ -977,7 +977,9 @@ __device__ __forceinline__ void tnfw_defl(const Rec<T> &r, double x, double y, d
   const T sqtx = Fn<T>::sqrt_(tau2 + X2);
   // the FP64 form's divisions as refined reciprocals (within two ulp)
   constexpr bool kDbl = sizeof(T) == sizeof(double);
-  const T L = Fn<T>::log_(kDbl ? X * T(rcp_refined(double(tau + sqtx))) : X / (tau + sqtx));
+  // (the FP64 logarithm of the positive ratio by log2_pos)
+  const T L = kDbl ? T(log2_pos(double(X) * rcp_refined(double(tau + sqtx))) * 0.6931471805599453)
+                   : Fn<T>::log_(X / (tau + sqtx));
   const T g = r[TN_PRE] * ((r[TN_T2P1] + T(2) * (X2 - T(1))) * tnfw_F(X) + r[TN_C0] +
                            sqtx * (T(-M_PI) + L * r[TN_C1]));
   const T a = kDbl ? r[TN_A] * g * T(rcp_refined(double(X2))) : r[TN_A] * g / X2;
