@@ -28,8 +28,6 @@ class ImageLinearFit(ImageModel):
                                              kwargs_pixelbased, precision)
         if getattr(self.Data, "likelihood_method", lambda: "diagonal")() != "diagonal":
             raise ValueError("only the diagonal likelihood method is supported")
-        if self._frame is not None:
-            raise NotImplementedError("the linear solve with flux_evaluate_indexes sub-frames is not supported")
         self._n_lin = len(self._src_types) + len(self._light_types)
 
     # ------------------------------------------------------------ batch API --
@@ -46,7 +44,11 @@ class ImageLinearFit(ImageModel):
         params = params.contiguous()
         dev = params.device
         dtype = torch.float64 if self._prec == backend.FP64 else torch.float32
-        ny, nx = self._full_shape
+        if data is not None and data.shape[0] < n_img:
+            raise ValueError("per-image data holds %d images for a batch of %d" % (data.shape[0], n_img))
+        # with a sub-frame the solve runs on its rectangle (the responses are zero
+        # outside it), the images are completed and logL gains the outside term
+        ny, nx = self._full_shape if self._frame is None else (self._grid.ny, self._grid.nx)
         out = {"logl": torch.empty(n_img, dtype=torch.float64, device=dev),
                "amp": torch.empty((n_img, self._n_lin), dtype=torch.float64, device=dev)}
         if want_model:
@@ -55,7 +57,11 @@ class ImageLinearFit(ImageModel):
             out["cov"] = torch.empty((n_img, self._n_lin, self._n_lin), dtype=torch.float64, device=dev)
         if want_response:
             out["response"] = torch.empty((self._n_lin, n_img, ny, nx), dtype=dtype, device=dev)
-        d = self._data_struct(data)
+        outside = None
+        if self._frame is None:
+            d = self._data_struct(data)
+        else:
+            d, outside, keep = self._sub_frame_data(data, None, n_img)
         nbytes = lib.lensim_linear_workspace_size(ctypes.byref(m), ctypes.byref(self._grid),
                                                   ctypes.byref(self._psf), n_img, self._prec)
         if nbytes == 0:
@@ -69,6 +75,12 @@ class ImageLinearFit(ImageModel):
                                      backend.ptr(out.get("response")), backend.ptr(ws), ws.numel(),
                                      backend.stream_handle())
         backend.check(rc, "lensim_linear_solve")
+        if outside is not None:
+            out["logl"] += outside
+            fy, fx = self._full_shape
+            for key in ("model", "response"):
+                if key in out:
+                    out[key] = self._complete(out[key], out[key].new_empty(out[key].shape[:-2] + (fy, fx)))
         return out
 
     def _packed(self, kwargs_lens, kwargs_source, kwargs_lens_light):

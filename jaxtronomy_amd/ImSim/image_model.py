@@ -103,6 +103,8 @@ class ImageModel(object):
                     self._psf.ky_low, self._psf.kx_low = self._kernel_low.shape
                     self._psf.d_kernel_low = self._kernel_low_dev.data_ptr()
         self._mask_dev = None
+        self._sub_masks = None  # sub-frame: likelihood mask inside / outside the rectangle
+        self._sub_cache = None  # sub-frame: (device buffers of the ImageData, lensim_data, outside logL, keep)
         self._bulk = isinstance(self.LensModel, LensModelBulk)
         self._lens_types = [] if self._bulk else list(self.LensModel._types)
         self._src_types = list(self.SourceModel._types)
@@ -280,10 +282,8 @@ class ImageModel(object):
                               backend.ptr(params), n_img, self._prec, flags, backend.ptr(sub), backend.ptr(ws),
                               ws.numel(), backend.stream_handle())
         backend.check(rc, "lensim_image")
-        if self._frame is not None:  # NumericsSubFrame._complete_frame (numerics_subframe.py:135-148)
-            x_min, y_min, n = self._frame
-            out.zero_()
-            out[:, x_min:x_min + n, y_min:y_min + n] = sub
+        if self._frame is not None:
+            self._complete(sub, out)
         return out
 
     def _data_struct(self, data=None, noise_map=None):
@@ -317,6 +317,63 @@ class ImageModel(object):
         elif noise_map is not None:
             raise ValueError("per-image noise maps need per-image data")
         return d
+
+    def _sub_frame_data(self, data=None, noise_map=None, n_img=1):
+        """(lensim_data cut to the sub-frame rectangle, float64 logL [1, or n_img
+        with per-image data] of the zero model on the pixels outside it, tensors
+        to keep alive).  The model is zero outside the rectangle
+        (NumericsSubFrame._complete_frame, numerics_subframe.py:135-148), so those
+        pixels add a term to ImageData.log_likelihood that no parameter changes,
+        and nothing to the gradient or to the weighted normal equations of the
+        linear solve (their response rows are zero): both run on the rectangle
+        with the data, noise map, exposure map and likelihood mask cut to it."""
+        lib = backend.lib()
+        buf = self.Data.device_buffers()
+        if data is None and self._sub_cache is not None and self._sub_cache[0] is buf:
+            return self._sub_cache[1:]
+        x0, y0, n = self._frame
+        ny, nx = self._full_shape
+        if self._sub_masks is None:
+            inside = self.likelihood_mask[x0:x0 + n, y0:y0 + n]
+            outside = self.likelihood_mask.copy()
+            outside[x0:x0 + n, y0:y0 + n] = False
+            self._sub_masks = (backend.to_device_f64(np.ascontiguousarray(inside, dtype=np.float64)),
+                               backend.to_device_f64(outside.astype(np.float64)))
+        # the outside term: ImageData.log_likelihood of a zero model under the outside mask
+        full = self._data_struct(data, noise_map)
+        full.d_mask = self._sub_masks[1].data_ptr()
+        n_out = 1 if data is None else int(n_img)
+        dtype = torch.float64 if self._prec == backend.FP64 else torch.float32
+        zero = torch.zeros((n_out, ny, nx), dtype=dtype, device=backend.device())
+        out_logl = torch.empty(n_out, dtype=torch.float64, device=backend.device())
+        ws = self._ws.get(lib.lensim_data_loglike_workspace_size(nx, ny, n_out))
+        rc = lib.lensim_data_loglike(ctypes.byref(full), nx, ny, backend.ptr(zero), n_out, self._prec,
+                                     backend.ptr(out_logl), backend.ptr(ws), ws.numel(), backend.stream_handle())
+        backend.check(rc, "lensim_data_loglike")
+
+        def cut(t):
+            return None if t is None else t[..., x0:x0 + n, y0:y0 + n].contiguous()
+        keep = [cut(buf["data"] if data is None else data), cut(noise_map if data is not None else buf["noise"]),
+                cut(buf["expo_map"]), self._sub_masks[0], zero]
+        d = backend.LensimData()
+        d.d_data = keep[0].data_ptr()
+        d.d_noise_map = None if keep[1] is None else keep[1].data_ptr()
+        d.d_exposure_map = None if keep[2] is None else keep[2].data_ptr()
+        d.d_mask = keep[3].data_ptr()
+        d.exposure_time = full.exposure_time
+        d.background_rms = full.background_rms
+        d.image_stride = n * n if data is not None else 0
+        if data is None:
+            self._sub_cache = (buf, d, out_logl, keep)
+        return d, out_logl, keep
+
+    def _complete(self, sub, full):
+        """NumericsSubFrame._complete_frame (numerics_subframe.py:135-148): the
+        sub-frame images [..., n, n] placed in zeroed full frames [..., ny, nx]."""
+        x0, y0, n = self._frame
+        full.zero_()
+        full[..., x0:x0 + n, y0:y0 + n] = sub
+        return full
 
     def likelihood_batch(self, params, n_img=None, model_out=None, logl=None, model=None, data=None,
                          noise_map=None):
@@ -359,14 +416,16 @@ class ImageModel(object):
         if self._bulk and model is None:
             raise ValueError("a LensModelBulk deflector needs the model of pack_bulk")
         m = self._model if model is None else model
-        if self._frame is not None:
-            raise NotImplementedError("gradients with flux_evaluate_indexes sub-frames are not supported")
         if n_img is None:
             n_img = params.shape[0]
         params = params.contiguous()
         logl = torch.empty(n_img, dtype=torch.float64, device=params.device)
         grad = torch.empty((n_img, params.shape[1], backend.NPARAM), dtype=torch.float64, device=params.device)
-        d = self._data_struct()
+        outside = None
+        if self._frame is None:
+            d = self._data_struct()
+        else:  # the sub-frame rectangle, plus the parameter-independent outside term
+            d, outside, _ = self._sub_frame_data()
         nbytes = lib.lensim_grad_workspace_size(ctypes.byref(m), ctypes.byref(self._grid),
                                                 ctypes.byref(self._psf), n_img, self._prec)
         if nbytes == 0:
@@ -376,6 +435,8 @@ class ImageModel(object):
                                      ctypes.byref(d), backend.ptr(params), n_img, self._prec, backend.ptr(logl),
                                      backend.ptr(grad), backend.ptr(ws), ws.numel(), backend.stream_handle())
         backend.check(rc, "lensim_loglike_grad")
+        if outside is not None:
+            logl += outside
         return logl, grad
 
     def log_likelihood_torch(self, params):

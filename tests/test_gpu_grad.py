@@ -166,6 +166,66 @@ def test_grad_gaussian_psf_vs_oracle_fd(precision, tol):
                 assert abs(grads[j][name] - fd) <= tol * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-5), ("mixed", 1e-4)])
+def test_sub_frame_grad_vs_oracle_fd(precision, tol):
+    """flux_evaluate_indexes sub-frame: the gradient runs on the square around
+    the evaluated pixels, logL covers the whole frame (the pixels outside the
+    square, where the model is zero, add a constant); logL against the oracle's
+    completed-frame likelihood and the gradient against its central
+    differences, with a likelihood mask that cuts through the square."""
+    n = 24
+    cfg = dict(W.EPL180, numPix=n, supersampling=2, deltaPix=0.15)
+    g = W.grid_dict(cfg)
+    kernel = W.moffat_kernel(5, 1.5, 3.0)
+    kl, ks, kll = W.epl180_draw(2)
+    ev = np.zeros((n, n), bool)
+    yy, xx = np.mgrid[0:n, 0:n]
+    ev[(yy - 11) ** 2 + (xx - 13) ** 2 < 8 ** 2] = True
+    lmask = np.ones((n, n))
+    lmask[:, :9] = 0
+    m = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel)
+    data = W.noisy(m, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(7))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": ev},
+                    likelihood_mask=lmask, precision=precision)
+    assert im._frame == (4, 6, 15)  # rows 4..18, columns 6..20
+    kl = [dict(kl[0], theta_E=kl[0]["theta_E"] * 1.03, gamma=kl[0]["gamma"] - 0.02), kl[1]]
+    logl, glens, gsrc, glight = im.likelihood_gradient(kl, ks, kll)
+
+    def ll(klx, ksx, kllx):
+        mm = O.image(g, ["EPL", "SHEAR"], klx, ["SERSIC_ELLIPSE"], ksx, ["SERSIC_ELLIPSE"], kllx, kernel=kernel,
+                     flux_evaluate_indexes=ev)
+        return O.log_likelihood(mm, data, mask=lmask, background_rms=cfg["background_rms"],
+                                exposure_map=cfg["exposure_time"])
+
+    ll0 = ll(kl, ks, kll)
+    ltol = 1e-9 if precision == "fp64" else 1e-6
+    assert abs(float(logl) - ll0) < ltol * abs(ll0)
+    assert abs(float(im.likelihood_data_given_model(kl, ks, kll)) - ll0) < ltol * abs(ll0)
+    for which, grads, kwl in ((0, glens, kl), (1, gsrc, ks), (2, glight, kll)):
+        for j, kw in enumerate(kwl):
+            for name in grads[j]:
+                h = 1e-6 * max(abs(kw[name]), 0.1)
+                lists = [[dict(d) for d in kl], [dict(d) for d in ks], [dict(d) for d in kll]]
+                lists[which][j][name] = kw[name] + h
+                up = ll(*lists)
+                lists[which][j][name] = kw[name] - h
+                dn = ll(*lists)
+                fd = (up - dn) / (2 * h)
+                assert abs(grads[j][name] - fd) <= tol * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
+    # a second call reuses the cut data; new data replaces it
+    logl2, _, _, _ = im.likelihood_gradient(kl, ks, kll)
+    assert float(logl2) == float(logl)
+    im.Data.update_data(data * 1.01)
+    logl3, _, _, _ = im.likelihood_gradient(kl, ks, kll)
+    mm = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel,
+                 flux_evaluate_indexes=ev)
+    ll3 = O.log_likelihood(mm, data * 1.01, mask=lmask, background_rms=cfg["background_rms"],
+                           exposure_map=cfg["exposure_time"])
+    assert abs(float(logl3) - ll3) < ltol * abs(ll3)
+
+
 @pytest.mark.parametrize("variant", ["pixel_split", "gaussian"])
 def test_grad_supersampled_convolution_vs_oracle_fd(variant):
     """supersampling_convolution=True: the adjoint runs through the s x s mean,

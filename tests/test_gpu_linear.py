@@ -129,3 +129,62 @@ def test_likelihood_module_batch():
         kl_i = [dict(draws[i][0][0], center_x=0.0, center_y=0.0), draws[i][0][1]]
         want, _ = im.likelihood_data_given_model(kl_i, [dict(draws[i][1][0])], [dict(draws[i][2][0])])
         assert abs(logl_l[i] - float(want)) <= 1e-12 * abs(float(want))
+
+
+@pytest.mark.parametrize("precision,tol,ltol", [("fp64", 1e-10, 1e-10), ("mixed", 1e-5, 1e-6)])
+def test_linear_solve_sub_frame_vs_oracle(precision, tol, ltol):
+    """flux_evaluate_indexes sub-frame: the responses vanish outside the square
+    around the evaluated pixels, so the solve runs there and logL covers the
+    whole frame; amplitudes, logL (with the marginalisation term), the
+    completed model, M^-1, the response matrix and a batch with one
+    observation per image against the oracle."""
+    n = 40
+    cfg = dict(W.EPL180, numPix=n, supersampling=2, deltaPix=0.08)
+    g = W.grid_dict(cfg)
+    kernel = W.moffat_kernel(7, 2.0, 3.0)
+    kl, ks, kll = W.epl180_draw(9)
+    clean = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel)
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(13))
+    ev = np.zeros((n, n), bool)
+    yy, xx = np.mgrid[0:n, 0:n]
+    ev[(yy - 19) ** 2 + (xx - 21) ** 2 < 13 ** 2] = True
+    mask = np.ones((n, n))
+    mask[:, :14] = 0
+    im = ImageLinearFit(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                        LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                        kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": ev},
+                        likelihood_mask=mask, precision=precision)
+    assert im._frame == (7, 9, 25)
+    ks_in = [dict(ks[0], amp=1.0)]
+    kll_in = [dict(kll[0], amp=1.0)]
+    oargs = dict(mask=mask, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"],
+                 kernel=kernel, flux_evaluate_indexes=ev)
+    logl, amp = im.likelihood_data_given_model(kl, ks_in, kll_in, source_marg=True)
+    want, B, model, M_inv = O.linear_solve(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll,
+                                           data, source_marg=True, **oargs)
+    npt.assert_allclose(amp, B, rtol=tol)
+    assert abs(float(logl) - want) < ltol * abs(want)
+    got_model, _, cov, _ = im.image_linear_solve(kl, [dict(ks_in[0])], [dict(kll_in[0])], inv_bool=True)
+    g_m = got_model.cpu().numpy()
+    assert g_m.shape == (n, n)
+    assert np.max(np.abs(g_m - model)) < tol * np.max(np.abs(model))
+    assert np.all(g_m[:7] == 0) and np.all(g_m[:, :14] == 0)
+    npt.assert_allclose(cov, M_inv, rtol=1e-6 if precision == "mixed" else 1e-9)
+    A = im.linear_response_matrix(kl, ks_in, kll_in).cpu().numpy()
+    sel = mask.ravel() > 0
+    col0 = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], [dict(ks[0], amp=1.0)], [], [], kernel=kernel,
+                   flux_evaluate_indexes=ev)
+    assert A.shape == (2, sel.sum())
+    assert np.max(np.abs(A[0] - col0.ravel()[sel])) < tol * np.max(np.abs(col0))
+    # one observation per batch image
+    data2 = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(14))
+    params, n_img, _, _ = im._packed(kl, ks_in, kll_in)
+    r = im.linear_solve_batch(torch.cat([params, params]), 2, model=None, want_model=True,
+                              data=torch.as_tensor(np.stack([data, data2]), device=params.device))
+    for b, dd in enumerate((data, data2)):
+        w_b, B_b, model_b, _ = O.linear_solve(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"],
+                                              kll, dd, **oargs)
+        npt.assert_allclose(r["amp"][b].cpu().numpy(), B_b, rtol=tol)
+        assert abs(float(r["logl"][b]) - w_b) < ltol * abs(w_b)
+        got_b = r["model"][b].to(torch.float64).cpu().numpy()
+        assert np.max(np.abs(got_b - model_b)) < tol * np.max(np.abs(model_b))

@@ -612,8 +612,6 @@ def test_flux_evaluate_indexes_sub_frame(psf_type, precision, tol, ltol):
     ll = float(im.likelihood_data_given_model(kl, ks, kll))
     ll_want = O.log_likelihood(want, data, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"])
     assert abs(ll - ll_want) < ltol * abs(ll_want)
-    with pytest.raises(NotImplementedError):
-        im.likelihood_gradient(kl, ks, kll)
 
 
 _PAIR_KWARGS = {
