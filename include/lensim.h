@@ -265,8 +265,11 @@ int lensim_linear_solve(const lensim_model *model, const lensim_grid *grid, cons
  * (ImSim/image_model.py:162-221, as exercised by
  * test/test_ImSim/test_image_model.py:274-335; EPL's custom JVP epl.py:321-340).
  * Adjoint path: dlogL/dmodel (model-based variance included) -> correlation
- * with the PSF -> per-ray forward-mode derivatives of each profile.
- * Not supported: bulk deflector lists. */
+ * with the PSF -> per-ray forward-mode derivatives of each profile.  Bulk
+ * deflector lists are differentiated per deflector; a multi-plane model
+ * (model->multiplane) by a reverse sweep over its planes
+ * (LensModel/MultiPlane/multi_plane_bulk.py:152-212); an evaluation mask
+ * (grid->d_eval_mask) restricts the rays as in lensim_image. */
 size_t lensim_grad_workspace_size(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,
                                   int32_t n_img, int32_t precision);
 int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, const lensim_psf *psf,

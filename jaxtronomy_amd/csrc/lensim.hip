@@ -195,7 +195,8 @@ void timing_mark(int slot, hipStream_t st) {
 // types[] holds the inline lens, source and lens-light types (no bulk).
 struct ModelArgs {
   int32_t n_lens, n_bulk, n_defl, n_source, n_light, n_prof, n_epl, has_halo, bulk_type;
-  int32_t has_nie_nfw;  // an SIE / NIE or NFW deflector (inline or bulk): the float form deflects them in double
+  int32_t has_nie_nfw;  // an SIE / NIE or NFW deflector (inline or bulk), or halos on several planes: the float
+                        // form deflects SIE / NIE and the NFW family in double
   SubSteps sub;         // sub-ray steps of the call's grid, for the records (with_grid); zero without a grid
   const int32_t *bulk_types;
   int32_t types[LENSIM_MAX_PROFILES];
@@ -416,7 +417,7 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
   off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 256 * kGradPix - 1) / (256 * kGradPix)) *
                            LENSIM_NPARAM * 8);
   L.gbeta = 0;
-  if (m.n_lens > 0 && m.n_source > 0) {  // dS/dbeta per ray (two values, compute type)
+  if ((m.n_lens > 0 || m.mp_n > 0) && m.n_source > 0) {  // dS/dbeta or beta per ray (two values, compute type)
     L.gbeta = off;
     const size_t rays = npix_r * (ss.on ? 1 : size_t(s) * s);
     off = align256(off + size_t(n_img) * rays * 2 * tsz);
@@ -501,6 +502,11 @@ int model_args(const lensim_model *model, ModelArgs &m) {
     m.mp_tz = mp->d_T_z;
     m.mp_r2p = mp->d_r2p;
     m.mp_stride = mp->stride;
+    // with several planes a deflector's error moves the rays at every later
+    // plane, where a strong lens magnifies it: the float form's render and
+    // gradient kernels deflect the NFW family in double there too (the float
+    // TNFW left logL 2e-6 off)
+    if (m.has_halo) m.has_nie_nfw = 1;
   }
   return LENSIM_OK;
 }
@@ -609,26 +615,28 @@ __device__ __forceinline__ RecPtrs<T> rec_ptrs(const ModelArgs &m, const double 
 }
 
 // The deflection of the bulk list (records n_lens .. n_defl - 1) added to (ax, ay).
-template <typename T, bool HALO, bool DBL = false>
+template <typename T, bool HALO, bool DBL = false, bool DBL_TNFW = DBL>
 __device__ __forceinline__ void deflect_bulk(const ModelArgs &m, const RecPtrs<T> &rp, double x, double y,
                                              double &ax, double &ay) {
   if (m.n_bulk == 0) return;
   if (m.bulk_type == LENSIM_TNFW) {
     // common case of substructure: one profile type, no per-deflector switch
     if constexpr (HALO) {
-      for (int k = m.n_lens; k < m.n_defl; ++k)
-        tnfw_defl<T>(Rec<T>{rp.recd + k * kRec, rp.rect + k * kRec}, x, y, ax, ay);
+      for (int k = m.n_lens; k < m.n_defl; ++k) {
+        if constexpr (DBL_TNFW) tnfw_defl<double>(Rec<double>{rp.recd + k * kRec, rp.recd + k * kRec}, x, y, ax, ay);
+        else tnfw_defl<T>(Rec<T>{rp.recd + k * kRec, rp.rect + k * kRec}, x, y, ax, ay);
+      }
     }
     return;
   }
   for (int k = m.n_lens; k < m.n_defl; ++k) {
     const int type = m.bulk_type ? m.bulk_type : AT(m.bulk_types, k - m.n_lens);
     const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
-    add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+    add_deflection<T, HALO, DBL, DBL_TNFW>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
 }
 
-template <typename T, bool HALO, bool DBL = false>
+template <typename T, bool HALO, bool DBL = false, bool DBL_TNFW = DBL>
 __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
                                             double &ax, double &ay) {
   for (int k = 0; k < m.n_lens; ++k) {
@@ -637,13 +645,13 @@ __device__ __forceinline__ void deflect_all(const ModelArgs &m, const RecPtrs<T>
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
     if (e >= 0) CHECK_SPAN(rp.coeft + co, LENSIM_EPL_NMAX * sizeof(T));
-    add_deflection<T, HALO, DBL>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+    add_deflection<T, HALO, DBL, DBL_TNFW>(type, r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   }
-  deflect_bulk<T, HALO, DBL>(m, rp, x, y, ax, ay);
+  deflect_bulk<T, HALO, DBL, DBL_TNFW>(m, rp, x, y, ax, ay);
 }
 
 // One deflector k (inline lens k < n_lens with its series coefficients, or bulk).
-template <typename T, bool HALO, bool DBL>
+template <typename T, bool HALO, bool DBL, bool DBL_TNFW = DBL>
 __device__ __forceinline__ void deflect_one(const ModelArgs &m, const RecPtrs<T> &rp, int img, int k, double x,
                                             double y, double &ax, double &ay) {
   const Rec<T> r{rp.recd + k * kRec, rp.rect + k * kRec};
@@ -651,22 +659,22 @@ __device__ __forceinline__ void deflect_one(const ModelArgs &m, const RecPtrs<T>
     const int e = m.epl_index[k];
     const size_t co = (e >= 0) ? (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX : 0;
     if (e >= 0) CHECK_SPAN(rp.coeft + co, LENSIM_EPL_NMAX * sizeof(T));
-    add_deflection<T, HALO, DBL>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
+    add_deflection<T, HALO, DBL, DBL_TNFW>(m.types[k], r, rp.coefd + co, rp.coeft + co, x, y, ax, ay);
   } else {
     const int type = m.bulk_type ? m.bulk_type : AT(m.bulk_types, k - m.n_lens);
-    add_deflection<T, HALO, DBL>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
+    add_deflection<T, HALO, DBL, DBL_TNFW>(type == LENSIM_EPL ? LENSIM_NULL : type, r, nullptr, nullptr, x, y, ax, ay);
   }
 }
 
 // Source-plane position of the ray at angle (x, y): single plane
 // beta = theta - alpha_scaling * alpha, or the multi-plane recursion of
 // MultiPlaneBulk.ray_shooting (LensModel/MultiPlane/multi_plane_bulk.py:152-212).
-template <typename T, bool HALO, bool DBL>
+template <typename T, bool HALO, bool DBL, bool DBL_TNFW = DBL>
 __device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &rp, int img, double x, double y,
                                           double &bx, double &by) {
   if (m.mp_n == 0) {
     double ax = 0.0, ay = 0.0;
-    deflect_all<T, HALO, DBL>(m, rp, img, x, y, ax, ay);
+    deflect_all<T, HALO, DBL, DBL_TNFW>(m, rp, img, x, y, ax, ay);
     bx = x - ax * m.alpha_scaling;
     by = y - ay * m.alpha_scaling;
     return;
@@ -682,7 +690,7 @@ __device__ __forceinline__ void trace_ray(const ModelArgs &m, const RecPtrs<T> &
       double ax = 0.0, ay = 0.0;
       int k = AT(m.mp_order, p);
       k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
-      deflect_one<T, HALO, DBL>(m, rp, img, k, px / tz, py / tz, ax, ay);
+      deflect_one<T, HALO, DBL, DBL_TNFW>(m, rp, img, k, px / tz, py / tz, ax, ay);
       const double f = AT(m.mp_r2p, o + p);
       angx -= ax * f;
       angy -= ay * f;
@@ -1148,7 +1156,7 @@ __global__ __launch_bounds__(256) void raytrace_kernel(ModelArgs m, const double
   if (i >= n_ray) return;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
   double tx, ty;
-  trace_ray<T, HALO, DBL>(m, rp, img, AT(xs, i), AT(ys, i), tx, ty);
+  trace_ray<T, HALO, DBL, false>(m, rp, img, AT(xs, i), AT(ys, i), tx, ty);
   AT(bx, size_t(img) * n_ray + i) = tx;
   AT(by, size_t(img) * n_ray + i) = ty;
 }
@@ -1221,7 +1229,7 @@ __global__ __launch_bounds__(256) void raytrace_mp_kernel(ModelArgs m, const dou
       k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
       const double tz = AT(tzs, q);
       double dx = 0.0, dy = 0.0;
-      deflect_one<T, HALO, DBL>(m, rp, img, k, px / tz, py / tz, dx, dy);
+      deflect_one<T, HALO, DBL, false>(m, rp, img, k, px / tz, py / tz, dx, dy);
       const double f = AT(r2p, q);
       ax = dx * f;
       ay = dy * f;
@@ -2048,11 +2056,18 @@ __device__ __forceinline__ void ray_grad(const ModelArgs &m, const GradArgs &ga,
     for (int i = 0; i < N; ++i) flux.d[i] = f2.d[0] * bx.d[i] + f2.d[1] * by.d[i];
   } else if (k < m.n_defl + m.n_source) {
     // d/d(source kwargs) at the deflected position
-    double ax = 0.0, ay = 0.0;
-    deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
-    const T bxs = T(x - ax * m.alpha_scaling), bys = T(y - ay * m.alpha_scaling);
+    T bxs, bys;
+    if (m.mp_n) {  // multi-plane: mp_grad_kernel stored beta at every ray
+      bxs = AT(gb, 0);
+      bys = AT(gb, 1);
+    } else {
+      double ax = 0.0, ay = 0.0;
+      deflect_all<T, HALO>(m, rp, img, x, y, ax, ay);
+      bxs = T(x - ax * m.alpha_scaling);
+      bys = T(y - ay * m.alpha_scaling);
+    }
     bool done = false;
-    if (gb) {  // the first source's launch: also dS/dbeta of the sources' sum for the lens launches
+    if (gb && !m.mp_n) {  // the first source's launch: also dS/dbeta of the sources' sum for the lens launches
       if constexpr (N == 7) {
         if (m.n_source == 1 && type == LENSIM_SERSIC_ELLIPSE) {  // one evaluation gives both
           T gp[7], gx, gy;
@@ -2122,30 +2137,19 @@ __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs
   for (int i = 0; i < N; ++i) acc[i] += w * (double(f2.d[0]) * bx.d[i] + double(f2.d[1]) * by.d[i]);
 }
 
-// Contribution of one ray to d flux / d(kwargs of the EPL deflector k).
-// Three dual seeds carry gamma, e1 and e2; theta_E and the centre are in
-// closed form: alpha scales as b^t with b = theta_E sqrt(q), so
-// d alpha / d theta_E = t alpha / theta_E, and d alpha / d center = -(the
-// Hessian), written as the reference's EPLMajorAxis.hessian
-// (LensModel/Profiles/epl.py:398-440, and the rotation of :203-232):
-// kappa = (2 - t)/2 (b/R)^t and the shear of its eq. (17) from alpha itself.
-template <typename T, bool HALO>
-__device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
-                                             int k, double x, double y, double w, double *acc, const T *gb) {
+// The deflection of the EPL deflector k at (x, y) and its derivatives by the
+// six kwargs (theta_E, gamma, e1, e2, center_x, center_y).  Three dual seeds
+// carry gamma, e1 and e2; theta_E and the centre are in closed form: alpha
+// scales as b^t with b = theta_E sqrt(q), so d alpha / d theta_E =
+// t alpha / theta_E, and d alpha / d center = -(the Hessian), written as the
+// reference's EPLMajorAxis.hessian (LensModel/Profiles/epl.py:398-440, and
+// the rotation of :203-232): kappa = (2 - t)/2 (b/R)^t and the shear of its
+// eq. (17) from alpha itself.
+template <typename T>
+__device__ __forceinline__ void epl_dalpha(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+                                           int k, const double *pk, double x, double y, T &a_x, T &a_y, T *dax,
+                                           T *day) {
   using S = gm::Dual<T, 3>;
-  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
-  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
-  double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form (not needed with gb)
-  if (!gb) {
-    for (int j = 0; j < m.n_lens; ++j) {
-      if (j == k) continue;
-      const int ej = m.epl_index[j];
-      const size_t coj = (ej >= 0) ? (size_t(img) * m.n_epl + ej) * LENSIM_EPL_NMAX : 0;
-      add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
-                              rp.coeft + coj, x, y, axp, ayp);
-    }
-    deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
-  }
   S ps[LENSIM_NPARAM];
 #pragma unroll
   for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = (i >= 1 && i <= 3) ? S::seed(T(pk[i]), i - 1) : S(T(pk[i]));
@@ -2163,9 +2167,9 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   const gm::EplGeo<T> geo = gm::epl_geo<T>(rdk, T(pk[2]), T(pk[3]));
   S ax = S(T(0)), ay = S(T(0));
   gm::epl_g_geo<T>(ps, geo.q, geo.cs, geo.sn, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
-  const T a_x = ax.v, a_y = ay.v;
+  a_x = ax.v;
+  a_y = ay.v;
   const T thetaE = T(pk[0]), t = T(pk[1]) - T(1);
-  T dax[6], day[6];
   const T dth = thetaE != T(0) ? t / thetaE : T(0);
   dax[0] = a_x * dth;
   day[0] = a_y * dth;
@@ -2199,6 +2203,27 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
   day[4] = -fxy;
   dax[5] = -fxy;
   day[5] = -fyy;
+}
+
+// Contribution of one ray to d flux / d(kwargs of the EPL deflector k).
+template <typename T, bool HALO>
+__device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+                                             int k, double x, double y, double w, double *acc, const T *gb) {
+  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
+  double axp = 0.0, ayp = 0.0;  // the other deflectors in the forward form (not needed with gb)
+  if (!gb) {
+    for (int j = 0; j < m.n_lens; ++j) {
+      if (j == k) continue;
+      const int ej = m.epl_index[j];
+      const size_t coj = (ej >= 0) ? (size_t(img) * m.n_epl + ej) * LENSIM_EPL_NMAX : 0;
+      add_deflection<T, HALO>(m.types[j], Rec<T>{rp.recd + j * kRec, rp.rect + j * kRec}, rp.coefd + coj,
+                              rp.coeft + coj, x, y, axp, ayp);
+    }
+    deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
+  }
+  T a_x, a_y, dax[6], day[6];
+  epl_dalpha<T>(m, ga, rp, img, k, pk, x, y, a_x, a_y, dax, day);
   const T as = T(m.alpha_scaling);
   T gx, gy;
   if (gb) {
@@ -2263,8 +2288,9 @@ grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
       const double ox = o0 + sx * inv_s;
       const double x = ra_c + g.m00 * ox + g.m01 * oy;
       const double y = dec_c + g.m10 * ox + g.m11 * oy;
-      // this ray's dS/dbeta slot: written by the first source's launch, read by the lens launches
-      T *gb = (gbeta && (k < m.n_lens || k == m.n_defl))
+      // this ray's slot: dS/dbeta, written by the first source's launch and read by the inline lens
+      // launches; with several planes, beta, written by mp_grad_kernel and read by the source launches
+      T *gb = (gbeta && (k < m.n_lens || k == m.n_defl || (m.mp_n && k < m.n_defl + m.n_source)))
                   ? gbeta + ((size_t(img) * npix + pix) * size_t(g.s * g.s) + size_t(sy * g.s + sx)) * 2
                   : nullptr;
       if (gb) CHECK_SPAN(gb, 2 * sizeof(T));
@@ -2477,6 +2503,225 @@ int launch_bulk_grad(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, 
   }
 #undef LENSIM_BULK_GRAD
   return check_launch("bulk_grad_kernel");
+}
+
+// Deflector k's deflection at (x, y) and its derivatives by its kwargs,
+// slots 0 .. N-1 (N returned; 0 for a type without kwargs): the EPL's closed
+// forms in the compute type, the other types' duals in double (as in the bulk
+// list's gradient).  The last two slots of every deflector type are its
+// centre, whose derivative is minus the column of d alpha / d(x, y).
+template <int TYPE, int N>
+__device__ __forceinline__ void dalpha_duals(const double *pk, double x, double y, double &ax, double &ay,
+                                             double *dax, double *day, double *dcen) {
+  using S = gm::Dual<double, N>;
+  S ps[LENSIM_NPARAM];
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i < N ? S::seed(pk[i], i) : S(pk[i]);
+  S sx = S(0.0), sy = S(0.0);
+  gm::defl_g<double, S>(TYPE, ps, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), 1,
+                        S(x), S(y), sx, sy);
+  ax = sx.v;
+  ay = sy.v;
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) {
+    dax[i] = i < N ? sx.d[i] : 0.0;
+    day[i] = i < N ? sy.d[i] : 0.0;
+  }
+  // the centre's derivatives (slots N-2, N-1): minus d alpha / dx and d alpha / dy
+  dcen[0] = sx.d[N - 2];
+  dcen[1] = sy.d[N - 2];
+  dcen[2] = sx.d[N - 1];
+  dcen[3] = sy.d[N - 1];
+}
+
+template <typename T, bool HALO>
+__device__ __forceinline__ void deflector_dalpha(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp,
+                                                 int img, int k, int type, double x, double y, double &ax,
+                                                 double &ay, double *dax, double *day, double *dcen) {
+  const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
+  CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
+  switch (type) {
+    case LENSIM_EPL: {
+      T a_x, a_y, dx[6], dy[6];
+      epl_dalpha<T>(m, ga, rp, img, k, pk, x, y, a_x, a_y, dx, dy);
+      ax = double(a_x);
+      ay = double(a_y);
+#pragma unroll
+      for (int i = 0; i < LENSIM_NPARAM; ++i) {
+        dax[i] = i < 6 ? double(dx[i]) : 0.0;
+        day[i] = i < 6 ? double(dy[i]) : 0.0;
+      }
+      dcen[0] = dax[4];
+      dcen[1] = day[4];
+      dcen[2] = dax[5];
+      dcen[3] = day[5];
+      break;
+    }
+    case LENSIM_SIE: dalpha_duals<LENSIM_SIE, 5>(pk, x, y, ax, ay, dax, day, dcen); break;
+    case LENSIM_NIE: dalpha_duals<LENSIM_NIE, 6>(pk, x, y, ax, ay, dax, day, dcen); break;
+    case LENSIM_NFW:
+      if constexpr (HALO) dalpha_duals<LENSIM_NFW, 4>(pk, x, y, ax, ay, dax, day, dcen);
+      break;
+    case LENSIM_TNFW:
+      if constexpr (HALO) dalpha_duals<LENSIM_TNFW, 5>(pk, x, y, ax, ay, dax, day, dcen);
+      break;
+    case LENSIM_SHEAR: dalpha_duals<LENSIM_SHEAR, 4>(pk, x, y, ax, ay, dax, day, dcen); break;
+    case LENSIM_CONVERGENCE: dalpha_duals<LENSIM_CONVERGENCE, 3>(pk, x, y, ax, ay, dax, day, dcen); break;
+    default: break;
+  }
+}
+
+// d flux / d(kwargs) of every deflector of a multi-plane model, the
+// recursion of trace_ray (MultiPlaneBulk.ray_shooting,
+// LensModel/MultiPlane/multi_plane_bulk.py:152-212)
+//   P_p = P_{p-1} + T_p A_{p-1},  A_p = A_{p-1} - f_p alpha_p(P_p / Tz_p),
+//   beta = P_n / Tz_n  (P_{-1} = 0, A_{-1} = the ray's angle)
+// differentiated in reverse.  From the adjoints (gP, gA) of (P_p, A_p):
+//   lambda = -f_p gA  (the adjoint of alpha_p),
+//   d flux / d kwargs_p = lambda . d alpha_p / d kwargs_p,
+//   gP += (d alpha_p / d x)^T lambda / Tz_p,  then gA += T_p gP,
+// starting from gP = w dS/dbeta / Tz_n, gA = 0 at the source plane.  The
+// plane states are not stored: each backward step inverts its forward one,
+// A_{p-1} = A_p + f_p alpha_p and P_{p-1} = P_p - T_p A_{p-1}, with alpha_p
+// the value of the duals the step evaluates anyway.  (d alpha / d x)^T lambda
+// is minus lambda's product with the centre's derivative.
+// Block (b, img) takes the rays of pixels [b kGradPix 256, (b + 1) kGradPix
+// 256), one per thread at a time; all threads sweep the planes together, so a
+// plane's contributions are summed over the block (waves, then the waves in
+// order) and added to its deflector's partial, which the block owns.  Each
+// ray's beta is also stored (ga.gbeta) for the source profiles' launches.
+template <typename T, bool HALO>
+__global__ __launch_bounds__(256) void mp_grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
+                                                      const double *__restrict__ recd,
+                                                      const float *__restrict__ recf,
+                                                      const double *__restrict__ coefd,
+                                                      const float *__restrict__ coeff) {
+  const int img = blockIdx.y;
+  const int npix = g.nx * g.ny, ss = g.s * g.s;
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const int t = threadIdx.x;
+  const size_t o = size_t(img) * size_t(m.mp_stride);
+  const int nd = m.mp_n - 1;  // deflector planes; plane nd is the source's
+  auto slot = [&](int k) {
+    return ((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM;
+  };
+  // this block's partials of every deflector start at zero
+  for (int q = t; q < m.n_defl * LENSIM_NPARAM; q += blockDim.x)
+    AT(ga.gpart, slot(q / LENSIM_NPARAM) + q % LENSIM_NPARAM) = 0.0;
+  __syncthreads();
+  __shared__ double red[2][256 / 64][LENSIM_NPARAM];
+  const int p0 = blockIdx.x * kGradPix * 256, p1 = min(npix, p0 + kGradPix * 256);
+  const int r_end = (p1 - p0) * ss;  // rays of the block, pixel-major, s x s sub-rays per pixel
+  int step = 0;  // plane steps so far: consecutive ones alternate between the two halves of red
+  for (int base = 0; base < r_end; base += 256) {
+    const int rr = base + t;
+    const bool active = rr < r_end;
+    int sub;
+    const int q0 = div_by(active ? rr : 0, ss, g.inv_ss, sub);
+    const int pix = p0 + q0;
+    int i, sx;
+    const int j = div_by(pix, g.nx, g.inv_nx, i), sy = div_by(sub, g.s, g.inv_s, sx);
+    const double ox = g.o0 + sx * g.inv_s, oy = g.o0 + sy * g.inv_s;
+    const double x = i * g.m00 + j * g.m01 + g.ra0 + g.m00 * ox + g.m01 * oy;
+    const double y = i * g.m10 + j * g.m11 + g.dec0 + g.m10 * ox + g.m11 * oy;
+    const double w = active ? double(AT(static_cast<const T *>(ga.glow), size_t(img) * npix + pix)) * g.scale *
+                                  m.flux_scaling
+                            : 0.0;
+    const bool live = w != 0.0 && m.n_source > 0 && evaluated(g, i, j);
+    // forwards to the source plane
+    double Ax = x, Ay = y, Px = 0.0, Py = 0.0;
+    double gPx = 0.0, gPy = 0.0, gAx = 0.0, gAy = 0.0;
+    if (live) {
+      for (int p = 0; p <= nd; ++p) {
+        const double dt = AT(m.mp_tij, o + p);
+        Px += Ax * dt;
+        Py += Ay * dt;
+        if (p < nd) {
+          const double tz = AT(m.mp_tz, o + p);
+          double ax = 0.0, ay = 0.0;
+          int k = AT(m.mp_order, p);
+          k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);
+          deflect_one<T, HALO, true>(m, rp, img, k, Px / tz, Py / tz, ax, ay);
+          const double f = AT(m.mp_r2p, o + p);
+          Ax -= ax * f;
+          Ay -= ay * f;
+        }
+      }
+      const double tzn = AT(m.mp_tz, o + nd);
+      const T bx = T(Px / tzn), by = T(Py / tzn);
+      if (ga.gbeta) {  // beta for the source launches
+        T *gb = static_cast<T *>(ga.gbeta) + ((size_t(img) * npix + pix) * size_t(ss) + size_t(sub)) * 2;
+        AT(gb, 0) = bx;
+        AT(gb, 1) = by;
+      }
+      const gm::Dual<T, 2> f2 = sources_at<T>(m, ga, img, bx, by);
+      gPx = w * double(f2.d[0]) / tzn;
+      gPy = w * double(f2.d[1]) / tzn;
+      // the source plane's step (no deflector): A_{nd-1} = A_nd
+      const double dt = AT(m.mp_tij, o + nd);
+      gAx += dt * gPx;
+      gAy += dt * gPy;
+      Px -= dt * Ax;
+      Py -= dt * Ay;
+    }
+    // backwards over the deflector planes
+    for (int p = nd - 1; p >= 0; --p) {
+      int k = AT(m.mp_order, p);
+      k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);
+      const int type = k < m.n_lens ? m.types[k] : (m.bulk_type ? m.bulk_type : AT(m.bulk_types, k - m.n_lens));
+      const int n = (type == LENSIM_EPL && k >= m.n_lens) ? 0 : n_params_of(type);  // bulk EPL: NULL
+      double c[LENSIM_NPARAM];
+#pragma unroll
+      for (int q = 0; q < LENSIM_NPARAM; ++q) c[q] = 0.0;
+      if (live && n > 0) {
+        const double tz = AT(m.mp_tz, o + p), f = AT(m.mp_r2p, o + p), dt = AT(m.mp_tij, o + p);
+        double ax = 0.0, ay = 0.0, dax[LENSIM_NPARAM], day[LENSIM_NPARAM], dcen[4];
+        deflector_dalpha<T, HALO>(m, ga, rp, img, k, type, Px / tz, Py / tz, ax, ay, dax, day, dcen);
+        const double lx = -f * gAx, ly = -f * gAy;
+#pragma unroll
+        for (int q = 0; q < LENSIM_NPARAM; ++q) c[q] = lx * dax[q] + ly * day[q];
+        gPx -= (lx * dcen[0] + ly * dcen[1]) / tz;
+        gPy -= (lx * dcen[2] + ly * dcen[3]) / tz;
+        Ax += f * ax;
+        Ay += f * ay;
+        gAx += dt * gPx;
+        gAy += dt * gPy;
+        Px -= dt * Ax;
+        Py -= dt * Ay;
+      } else if (live) {  // a deflector without kwargs deflects by zero
+        const double dt = AT(m.mp_tij, o + p);
+        gAx += dt * gPx;
+        gAy += dt * gPy;
+        Px -= dt * Ax;
+        Py -= dt * Ay;
+      }
+      // the plane's sum over the block, added to deflector k's partial
+      const int buf = step++ & 1;
+#pragma unroll
+      for (int q = 0; q < LENSIM_NPARAM; ++q) {
+        double v = c[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((t & 63) == 0) red[buf][t >> 6][q] = v;
+      }
+      __syncthreads();
+      if (t < n) {
+        double v = 0.0;
+        for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[buf][wv][t];
+        AT(ga.gpart, slot(k) + t) += v;
+      }
+    }
+  }
+}
+
+template <typename T>
+int launch_mp_grad(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, const double *recd, const float *recf,
+                   int nblk, int n_img, hipStream_t st) {
+  const dim3 grid(nblk, n_img);
+  if (m.has_halo)
+    mp_grad_kernel<T, true><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+  else
+    mp_grad_kernel<T, false><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+  return check_launch("mp_grad_kernel");
 }
 
 __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, int n, double *__restrict__ grad) {
@@ -3138,10 +3383,14 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   const int npix = gg.nx * gg.ny, nblk = (npix + 256 * kGradPix - 1) / (256 * kGradPix);
   const double *recd = reinterpret_cast<const double *>(ws + L.recd);
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
+  // multi-plane: every deflector in one launch, which also stores beta at every
+  // ray (ga.gbeta) for the source launches
+  if (m.mp_n && (rc = launch_mp_grad<T>(m, gg, ga, recd, recf, nblk, n_img, st))) return rc;
   // the sources first: the first source's launch stores dS/dbeta at every ray
   // (ga.gbeta), which the inline lens launches after it read
   for (int q = 0; q < m.n_prof; ++q) {
     const int k = q < m.n_source ? m.n_defl + q : (q < m.n_source + m.n_defl ? q - m.n_source : q);
+    if (m.mp_n && k < m.n_defl) continue;  // multi-plane: mp_grad_kernel above
     if (k == m.n_lens && m.n_bulk > 0) {  // the bulk list in one launch
       rc = m.has_halo ? launch_bulk_grad<T, true>(m, gg, ga, recd, recf, nblk, n_img, st)
                       : launch_bulk_grad<T, false>(m, gg, ga, recd, recf, nblk, n_img, st);
@@ -3991,7 +4240,6 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   ModelArgs m;
   int rc = model_args(model, m);
   if (rc) return rc;
-  if (m.mp_n) return fail(LENSIM_EUNSUPPORTED, "gradients of multi-plane ray tracing are not supported");
   GridArgs g;
   if ((rc = grid_args(grid, g))) return rc;
   int kx, ky, edge;

@@ -1060,8 +1060,10 @@ __device__ __forceinline__ void convergence_defl(const double *rd, double x, dou
 // HALO: whether the NFW / TNFW branches are compiled in (kernels for models
 // without them keep fewer registers).  DBL: SIE / NIE and NFW run the FP64
 // form's arithmetic in the float form too (kernels for models that hold
-// them; see nie_defl and nfw_defl).
-template <typename T, bool HALO, bool DBL = false>
+// SIE / NIE or NFW, or halos on several planes; see nie_defl and nfw_defl);
+// DBL_TNFW: TNFW too (the render and gradient kernels; the ray-shooting
+// kernels keep the float TNFW, whose rays stay within 2e-6 of the FP64 form's).
+template <typename T, bool HALO, bool DBL = false, bool DBL_TNFW = DBL>
 __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const double *__restrict__ coef_d,
                                                const T *__restrict__ coef_t, double x, double y, double &ax,
                                                double &ay) {
@@ -1080,7 +1082,10 @@ __device__ __forceinline__ void add_deflection(int type, const Rec<T> &r, const 
       if constexpr (HALO) nfw_defl<T, DBL>(r, x, y, ax, ay);
       break;
     case LENSIM_TNFW:
-      if constexpr (HALO) tnfw_defl<T>(r, x, y, ax, ay);
+      if constexpr (HALO) {
+        if constexpr (DBL_TNFW) tnfw_defl<double>(Rec<double>{r.d, r.d}, x, y, ax, ay);
+        else tnfw_defl<T>(r, x, y, ax, ay);
+      }
       break;
     case LENSIM_SHEAR:
       shear_defl(r.d, x, y, ax, ay);

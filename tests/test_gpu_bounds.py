@@ -3,8 +3,8 @@ on the GPU: its self-test diverts a store one past the registered buffer and
 reports it, and every entry point on the representative workloads (the
 compile-time render with the matrix-core convolutions of both forms, the
 generic render, supersampled convolution, sub-frames with per-image data, the
-gradients with inline and bulk deflectors, multi-plane ray shooting, the
-light profiles and the linear solve) runs with no access outside its call's
+gradients with inline, bulk and multi-plane deflectors, multi-plane ray
+shooting, the light profiles and the linear solve) runs with no access outside its call's
 buffers and returns what the normal build returns."""
 
 import numpy as np
@@ -201,6 +201,35 @@ def test_checked_multiplane_raytrace_and_light(precision):
 
     want, got = _both(run)
     _same(want, got, 1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checked_multiplane_gradient(precision):
+    """The multi-plane gradient (reverse plane sweep, beta stored for the
+    source launches): EPL + SHEAR inline and 12 TNFW on their own planes."""
+    from jaxtronomy_amd.Util.cosmo import FlatLambdaCDM
+
+    kl, subs, ks = W.substructure_draw(4, n_sub=12)
+    rng = np.random.default_rng(21)
+    names = ["EPL", "SHEAR"] + ["TNFW"] * 12
+    z = [0.5, 0.5] + list(rng.uniform(0.2, 1.3, 12))
+    cfg = dict(W.SUBSTRUCTURE, numPix=32, deltaPix=0.12, supersampling=2)
+    lm = LensModelBulk(["EPL", "SHEAR", "TNFW"], multi_plane=True, cosmo=FlatLambdaCDM(70.0, 0.3),
+                       precision=precision)
+    data = np.random.default_rng(3).normal(0, 0.02, (32, 32))
+    psf = PSF("PIXEL", kernel_point_source=W.moffat_kernel(7, 1.5, 3.0))
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf, lm, LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+    rs = lm.prepare_ray_shooting_kwargs(names, kl + subs, 1.5, z)
+    params, n, m, keep = im.pack_bulk(rs, ks)
+    pt = backend.to_device_f64(np.repeat(params, 3, axis=0))
+
+    def run():
+        return im.likelihood_and_grad_batch(pt, 3, model=m)
+
+    want, got = _both(run)
+    _same(want, got, 1e-12 if precision == "fp64" else 1e-6)
+    del keep
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

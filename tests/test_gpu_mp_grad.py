@@ -1,0 +1,230 @@
+"""Gradients of the log-likelihood through multi-plane ray tracing
+(MultiPlaneBulk, LensModel/MultiPlane/multi_plane_bulk.py:152-212): every
+deflector's kwargs and the source's against central differences of the
+oracle's plane scan, a line-of-sight list of two types in the bulk loop
+against central differences of the library's own FP64 logL, and the mixed
+form against the FP64 form."""
+
+import numpy as np
+import pytest
+
+from jaxtronomy_amd import backend, profiles
+from jaxtronomy_amd import workloads as W
+from jaxtronomy_amd.Data.imaging_data import ImageData
+from jaxtronomy_amd.Data.psf import PSF
+from jaxtronomy_amd.ImSim.image_model import ImageModel
+from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+from jaxtronomy_amd.LightModel.light_model import LightModel
+from oracle import lensim_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(seed=3):
+    """EPL + SHEAR at z = 0.5, six TNFW near the Einstein ring between z = 0.3
+    and 1.2 (three of them sharing the main plane), a CONVERGENCE sheet at
+    z = 0.8; source at z = 1.5."""
+    from jaxtronomy_amd.Util.cosmo import FlatLambdaCDM
+
+    kl, ks, _ = W.epl180_draw(seed)
+    kl = [dict(kl[0], center_x=0.0, center_y=0.0), dict(kl[1], ra_0=0.05, dec_0=-0.03)]
+    rng = np.random.default_rng(seed + 50)
+    subs = []
+    for _ in range(6):
+        r, phi = rng.uniform(0.7, 1.5), rng.uniform(0, 2 * np.pi)
+        rs = rng.uniform(0.08, 0.2)
+        subs.append(dict(Rs=rs, alpha_Rs=rng.uniform(0.05, 0.15), r_trunc=5 * rs, center_x=r * np.cos(phi),
+                         center_y=r * np.sin(phi)))
+    names = ["EPL", "SHEAR"] + ["TNFW"] * 6 + ["CONVERGENCE"]
+    kws = kl + subs + [dict(kappa=0.03, ra_0=0.1, dec_0=-0.2)]
+    z = [0.5, 0.5, 0.3, 0.45, 0.5, 0.5, 0.9, 1.2, 0.8]
+    return names, kws, ks, z, FlatLambdaCDM(70.0, 0.3)
+
+
+def _perturb(kwargs, seed, frac=0.03):
+    rng = np.random.default_rng(seed)
+    out = []
+    for kw in kwargs:
+        d = dict(kw)
+        for k in d:
+            d[k] = d[k] + rng.normal(0, frac) * max(abs(d[k]), 0.05)
+        out.append(d)
+    return out
+
+
+def _oracle_logl(cfg, kernel, data, names_s, kw_s, ks, rs):
+    """logL of the oracle's multi-plane image: plane scan on the s x s sub-rays,
+    source flux, s x s mean, PSF, x pixel area."""
+    g = W.grid_dict(cfg)
+    n, s = cfg["numPix"], cfg["supersampling"]
+    x, y = O.subgrid_coordinates(n, n, g["transform_pix2angle"], g["ra_at_xy_0"], g["dec_at_xy_0"], s)
+    bx, by = O.multi_plane_ray_shooting(np.ravel(x), np.ravel(y), names_s, kw_s, rs["T_ij_list"], rs["T_z_list"],
+                                        rs["reduced2physical_factor"])
+    flux = O.surface_brightness(bx, by, ["SERSIC_ELLIPSE"], ks)
+    img = O.convolve_same(O.re_size(flux.reshape(n * s, n * s), s), kernel) * cfg["deltaPix"] ** 2
+    return O.log_likelihood(img, data, background_rms=cfg["background_rms"], exposure_map=cfg["exposure_time"])
+
+
+_CFG = dict(W.EPL180, numPix=30, deltaPix=0.12, supersampling=2)
+_UNIQUE = ["EPL", "SHEAR", "TNFW", "CONVERGENCE"]
+
+
+def _scene_model(precision):
+    """(ImageModel on noisy data of the scene's truth, names, truth, source kwargs,
+    redshifts), 30x30 at 0.12" s=2, 7x7 Moffat."""
+    names, truth, ks, z, cosmo = _scene()
+    kernel = W.moffat_kernel(7, 1.5, 3.0)
+    lm = LensModelBulk(_UNIQUE, multi_plane=True, cosmo=cosmo, precision="fp64")
+    probe = ImageModel(ImageData(**W.data_kwargs(_CFG)), PSF("PIXEL", kernel_point_source=kernel), lm,
+                       LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2})
+    clean = probe.image(lm.prepare_ray_shooting_kwargs(names, truth, 1.5, z), ks).cpu().numpy()
+    data = W.noisy(clean, _CFG["exposure_time"], _CFG["background_rms"], np.random.default_rng(8))
+    lm = LensModelBulk(_UNIQUE, multi_plane=True, cosmo=cosmo, precision=precision)
+    im = ImageModel(ImageData(**W.data_kwargs(_CFG, data)), PSF("PIXEL", kernel_point_source=kernel), lm,
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+    return im, names, truth, ks, z, kernel, data
+
+
+def test_multi_plane_grad_vs_oracle_fd():
+    """Inline EPL, SHEAR and CONVERGENCE and six TNFW in the bulk loop on seven
+    planes: the FP64 logL against the oracle's plane scan and every kwarg's
+    derivative against its central differences, per parameter kind relative to
+    the largest of the kind."""
+    im, names, truth, ks, z, kernel, data = _scene_model("fp64")
+    lm = im.LensModel
+    order = np.argsort(np.array(z))
+    names_s = [names[i] for i in order]
+    point = _perturb(truth, 9)
+    ks_p = _perturb(ks, 10, 0.02)
+    rs = lm.prepare_ray_shooting_kwargs(names, point, 1.5, z)
+    logl, glens, gsrc, _ = im.likelihood_gradient(rs, ks_p)
+    kw_s = [point[i] for i in order]
+
+    def ll(kw_lens, kw_src):
+        return _oracle_logl(_CFG, kernel, data, names_s, kw_lens, kw_src, rs)
+
+    base = ll(kw_s, ks_p)
+    assert abs(float(logl) - base) < 1e-10 * abs(base)
+    got, fd, kind = [], [], []
+    for d, nm in enumerate(names_s):
+        for pn in profiles.LENS_PROFILES[nm][1]:
+            h = 1e-6 * max(abs(kw_s[d][pn]), 0.05)
+            up = [dict(k) for k in kw_s]
+            dn = [dict(k) for k in kw_s]
+            up[d][pn] += h
+            dn[d][pn] -= h
+            fd.append((ll(up, ks_p) - ll(dn, ks_p)) / (2 * h))
+            got.append(glens[pn][d])
+            kind.append(nm + ":" + pn)
+    for pn in profiles.LIGHT_PROFILES["SERSIC_ELLIPSE"][1]:
+        h = 1e-6 * max(abs(ks_p[0][pn]), 0.05)
+        up, dn = [dict(ks_p[0])], [dict(ks_p[0])]
+        up[0][pn] += h
+        dn[0][pn] -= h
+        fd.append((ll(kw_s, up) - ll(kw_s, dn)) / (2 * h))
+        got.append(gsrc[0][pn])
+        kind.append("source:" + pn)
+    got, fd = np.array(got), np.array(fd)
+    for kd in set(kind):
+        sel = np.array([k == kd for k in kind])
+        scale = max(np.max(np.abs(fd[sel])), 1e-3)
+        err = np.max(np.abs(got[sel] - fd[sel])) / scale
+        assert err < 1e-5, (kd, err, got[sel], fd[sel])
+    # kwargs that do not belong to a deflector's profile are zero there
+    tnfw = [d for d, nm in enumerate(names_s) if nm == "TNFW"]
+    assert np.all(glens["theta_E"][tnfw] == 0) and np.all(glens["kappa"][tnfw] == 0)
+    # the subhalos' gradients are a sizeable part of the whole
+    sub_kinds = np.array([k.startswith("TNFW") for k in kind])
+    assert np.max(np.abs(fd[sub_kinds])) > 1e-2 * np.max(np.abs(fd))
+
+
+def test_multi_plane_mixed_grad_vs_fp64_batch():
+    """The mixed form's gradient against the FP64 form's at 32 perturbed points
+    of the scene, each parameter relative to its gradient scale over the batch
+    (its largest magnitude there), and the mixed logL within 1e-6."""
+    im64, names, truth, ks, z, _, data = _scene_model("fp64")
+    im32 = _scene_model("mixed")[0]
+    rs = im64.LensModel.prepare_ray_shooting_kwargs(names, truth, 1.5, z)
+    params, n, m, keep = im64.pack_bulk(rs, ks)
+    rng = np.random.default_rng(17)
+    B = 32
+    batch = np.repeat(params, B, axis=0)
+    nprof = params.shape[1]
+    for j in range(nprof):
+        for i in range(backend.NPARAM):
+            batch[:, j, i] += rng.normal(0, 0.03, B) * max(abs(params[0, j, i]), 0.05) * (params[0, j, i] != 0)
+    pt = backend.to_device_f64(batch)
+    l64, g64 = im64.likelihood_and_grad_batch(pt, B, model=m)
+    l32, g32 = im32.likelihood_and_grad_batch(pt, B, model=m)
+    l64, l32 = l64.cpu().numpy(), l32.cpu().numpy()
+    assert np.all(np.abs(l32 - l64) < 1e-6 * np.abs(l64))
+    g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
+    scale = np.max(np.abs(g64), axis=0)
+    live = scale > 0
+    err = np.abs(g32 - g64).max(axis=0)[live] / scale[live]
+    assert err.max() < 1e-4, err.max()
+    del keep
+
+
+def _los_model(precision, cfg, data=None):
+    lm = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True, precision=precision)
+    psf = PSF("PIXEL", kernel_point_source=W.moffat_kernel(5, 1.2, 3.0))
+    return ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf, lm, LightModel(["SERSIC_ELLIPSE"]),
+                      kwargs_numerics={"supersampling_factor": 1}, precision=precision)
+
+
+def test_line_of_sight_grad_fd_and_mixed_vs_fp64():
+    """A line-of-sight list of 600 TNFW and 80 CONVERGENCE sheets on 80 planes
+    (more sheets than the inline slots, so both types share the bulk loop with a
+    per-deflector type), 40x40 at 0.25": the FP64 gradient against central
+    differences of the library's own FP64 logL for the heaviest halos and a
+    spread of sheets, and the mixed form against the FP64 form at 1e-4 of each
+    parameter kind's scale."""
+    los = dict(W.LOS, n_halo=600, n_planes=80, cone_arcsec=9.0)
+    names, kw, z = W.los_halo_draw(2, los)
+    cfg = dict(W.EPL180, numPix=40, deltaPix=0.25, supersampling=1)
+    ks = [dict(amp=40.0, R_sersic=0.8, n_sersic=1.5, e1=0.1, e2=-0.05, center_x=0.3, center_y=-0.2)]
+    probe = _los_model("fp64", cfg)
+    rs = probe.LensModel.prepare_ray_shooting_kwargs(names, kw, z_source=los["z_source"], lens_redshift_list=z)
+    clean = probe.image(rs, ks).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(4))
+    ks_p = [dict(ks[0], R_sersic=0.75, center_x=0.32)]
+    im64, im32 = _los_model("fp64", cfg, data), _los_model("mixed", cfg, data)
+    params, n, m, keep = im64.pack_bulk(rs, ks_p)
+    assert m.n_lens == 0 and m.n_bulk == len(names) and m.bulk_type == 0  # two types: per-deflector codes
+    pt = backend.to_device_f64(params)
+    l64, g64 = im64.likelihood_and_grad_batch(pt, n, model=m)
+    l32, g32 = im32.likelihood_and_grad_batch(pt, n, model=m)
+    g64, g32 = g64.cpu().numpy()[0], g32.cpu().numpy()[0]
+    assert abs(float(l32[0]) - float(l64[0])) < 1e-6 * abs(float(l64[0]))
+    # central differences of the FP64 logL, all perturbations in one batch
+    types = im64.LensModel.pack(rs)[3]
+    heavy = np.argsort(-params[0, :m.n_bulk, 1] * (types == 5))[:4]  # the four largest alpha_Rs
+    sheets = np.nonzero(types == 7)[0][::16]
+    # (a sheet's centre moves only kappa x the ray's offset: too little for differences of logL)
+    slots = [(int(j), i) for j in heavy for i in range(5)] + [(int(j), 0) for j in sheets] + \
+            [(m.n_bulk, i) for i in range(7)]
+    reps = np.repeat(params, 2 * len(slots), axis=0)
+    hs = []
+    for q, (j, i) in enumerate(slots):
+        h = 1e-6 * max(abs(params[0, j, i]), 0.05)
+        reps[2 * q, j, i] += h
+        reps[2 * q + 1, j, i] -= h
+        hs.append(h)
+    ll = im64.likelihood_batch(backend.to_device_f64(reps), reps.shape[0], model=m).cpu().numpy()
+    fd = (ll[0::2] - ll[1::2]) / (2 * np.array(hs))
+    got = np.array([g64[j, i] for (j, i) in slots])
+    kind = [("src%d" % i if j == m.n_bulk else ("tnfw%d" % i if types[j] == 5 else "sheet%d" % i)) for (j, i) in slots]
+    for kd in set(kind):
+        sel = np.array([k == kd for k in kind])
+        scale = max(np.max(np.abs(fd[sel])), 1.0)
+        assert np.max(np.abs(got[sel] - fd[sel])) / scale < 1e-5, (kd, got[sel], fd[sel])
+    # mixed against FP64, per (type, slot) relative to the largest of the kind
+    for t, npar in ((5, 5), (7, 3)):
+        rows = np.nonzero(types == t)[0]
+        for i in range(npar):
+            scale = np.max(np.abs(g64[rows, i]))
+            assert np.max(np.abs(g32[rows, i] - g64[rows, i])) <= 1e-4 * scale, (t, i)
+    src = g64[m.n_bulk, :7]
+    assert np.max(np.abs(g32[m.n_bulk, :7] - src) / np.maximum(np.abs(src), 1e-3)) < 1e-4
+    del keep
