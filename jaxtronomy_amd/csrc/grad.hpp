@@ -118,7 +118,7 @@ __device__ __forceinline__ void series_g(const T *__restrict__ c, const T *__res
 
 // EPL, LensModel/Profiles/epl.py:109-124, :178-200, :364-394.  p: theta_E,
 // gamma, e1, e2, center_x, center_y.  nser: series length (from the value of f).
-// q, cs, sn: the axis ratio and cos / sin of the orientation (epl_geo or phi_q).
+// q, cs, sn: the axis ratio and cos / sin of the orientation (from the record, or phi_q).
 template <typename T, typename S>
 __device__ __forceinline__ void epl_g_geo(const S *p, const S &q, const S &cs, const S &sn,
                                           const T *__restrict__ c, const T *__restrict__ dc, int nser, const S &x,
@@ -167,41 +167,6 @@ __device__ void epl_g(const S *p, const T *__restrict__ c, const T *__restrict__
   S phi, q;
   phi_q<T>(p[2], p[3], phi, q);
   epl_g_geo<T>(p, q, cos(phi), sin(phi), c, dc, nser, x, y, ax, ay);
-}
-
-// The EPL's q, cos phi and sin phi as duals in the seeds (gamma, e1, e2) of
-// the EPL gradient: the values from the record (prepare_record's float64
-// ellipticity2phi_q), the derivatives from the closed forms of
-// Util/param_util.py:88-100 (phi = atan2(e2, e1) / 2, c = min(|e|, 0.9999),
-// q = (1 - c) / (1 + c); dq = 0 where c is clamped).  Per ray this replaces the
-// dual atan2, square root, cosine and sine of per-image values.
-template <typename T>
-struct EplGeo {
-  Dual<T, 3> q, cs, sn;
-};
-
-template <typename T>
-__device__ __forceinline__ EplGeo<T> epl_geo(const double *rd, T e1, T e2) {
-  const T c2 = e1 * e1 + e2 * e2;
-  const T c = sqrt(c2);
-  const T cs = T(rd[EPL_C]), sn = T(rd[EPL_S]);
-  const T h = T(0.5) / c2;  // d phi / d(e1, e2) = (-e2, e1) / (2 c^2)
-  const T dp1 = -e2 * h, dp2 = e1 * h;
-  const T dqdc = c < T(0.9999) ? T(-2) / ((T(1) + c) * (T(1) + c)) : T(0);
-  EplGeo<T> g;
-  g.q.v = T(rd[EPL_Q]);
-  g.q.d[0] = T(0);
-  g.q.d[1] = dqdc * e1 / c;
-  g.q.d[2] = dqdc * e2 / c;
-  g.cs.v = cs;
-  g.cs.d[0] = T(0);
-  g.cs.d[1] = -sn * dp1;
-  g.cs.d[2] = -sn * dp2;
-  g.sn.v = sn;
-  g.sn.d[0] = T(0);
-  g.sn.d[1] = cs * dp1;
-  g.sn.d[2] = cs * dp2;
-  return g;
 }
 
 // NIE / SIE, LensModel/Profiles/nie.py:86-107, :182-200, :280-295.

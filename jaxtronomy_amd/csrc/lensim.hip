@@ -2138,21 +2138,27 @@ __device__ __forceinline__ void halo_ray_grad(const ModelArgs &m, const GradArgs
 }
 
 // The deflection of the EPL deflector k at (x, y) and its derivatives by the
-// six kwargs (theta_E, gamma, e1, e2, center_x, center_y).  Three dual seeds
-// carry gamma, e1 and e2; theta_E and the centre are in closed form: alpha
-// scales as b^t with b = theta_E sqrt(q), so d alpha / d theta_E =
-// t alpha / theta_E, and d alpha / d center = -(the Hessian), written as the
-// reference's EPLMajorAxis.hessian (LensModel/Profiles/epl.py:398-440, and
-// the rotation of :203-232): kappa = (2 - t)/2 (b/R)^t and the shear of its
-// eq. (17) from alpha itself.
+// six kwargs (theta_E, gamma, e1, e2, center_x, center_y).  Two dual seeds
+// carry gamma and the axis ratio q; the rest is in closed form:
+//  * alpha scales as b^t with b = theta_E sqrt(q), so d alpha / d theta_E =
+//    t alpha / theta_E;
+//  * d alpha / d center = -H, H = d alpha / dx the Hessian, written as the
+//    reference's EPLMajorAxis.hessian (LensModel/Profiles/epl.py:398-440, and
+//    the rotation of :203-232): kappa = (2 - t)/2 (b/R)^t and the shear of
+//    its eq. (17) from alpha itself;
+//  * alpha(x) = R(phi) alpha_M(R(phi)^T (x - c)), so d alpha / d phi =
+//    J alpha - H J (x - c) with J the quarter turn;
+//  * (e1, e2) -> (phi, q) by the closed forms of Util/param_util.py:88-100
+//    (phi = atan2(e2, e1) / 2, c = min(|e|, 0.9999), q = (1 - c) / (1 + c);
+//    dq = 0 where c is clamped).
 template <typename T>
 __device__ __forceinline__ void epl_dalpha(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
                                            int k, const double *pk, double x, double y, T &a_x, T &a_y, T *dax,
                                            T *day) {
-  using S = gm::Dual<T, 3>;
+  using S = gm::Dual<T, 2>;
   S ps[LENSIM_NPARAM];
 #pragma unroll
-  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = (i >= 1 && i <= 3) ? S::seed(T(pk[i]), i - 1) : S(T(pk[i]));
+  for (int i = 0; i < LENSIM_NPARAM; ++i) ps[i] = i == 1 ? S::seed(T(pk[1]), 0) : S(T(pk[i]));
   const int e = m.epl_index[k];
   const size_t co = (size_t(img) * m.n_epl + e) * LENSIM_EPL_NMAX;
   const int nser = int(sizeof(T) == sizeof(float) ? rp.recd[k * kRec + EPL_NF] : rp.recd[k * kRec + EPL_NMAX]);
@@ -2162,24 +2168,20 @@ __device__ __forceinline__ void epl_dalpha(const ModelArgs &m, const GradArgs &g
                                            : reinterpret_cast<const T *>(ga.dcoefd + co);
   CHECK_SPAN(c, LENSIM_EPL_NMAX * sizeof(T));
   CHECK_SPAN(dc, LENSIM_EPL_NMAX * sizeof(T));
-  // q, cos phi, sin phi and their derivatives from the record, not per ray
+  // q, cos phi and sin phi from the record, not per ray
   const double *rdk = rp.recd + k * kRec;
-  const gm::EplGeo<T> geo = gm::epl_geo<T>(rdk, T(pk[2]), T(pk[3]));
+  const T q = T(rdk[EPL_Q]), cs = T(rdk[EPL_C]), sn = T(rdk[EPL_S]);
   S ax = S(T(0)), ay = S(T(0));
-  gm::epl_g_geo<T>(ps, geo.q, geo.cs, geo.sn, c, dc, nser, S(T(x)), S(T(y)), ax, ay);
+  gm::epl_g_geo<T>(ps, S::seed(q, 1), S(cs), S(sn), c, dc, nser, S(T(x)), S(T(y)), ax, ay);
   a_x = ax.v;
   a_y = ay.v;
   const T thetaE = T(pk[0]), t = T(pk[1]) - T(1);
   const T dth = thetaE != T(0) ? t / thetaE : T(0);
   dax[0] = a_x * dth;
   day[0] = a_y * dth;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    dax[i + 1] = ax.d[i];
-    day[i + 1] = ay.d[i];
-  }
+  dax[1] = ax.d[0];
+  day[1] = ay.d[0];
   // Hessian in the major-axis frame, rotated back
-  const T q = geo.q.v, cs = geo.cs.v, sn = geo.sn.v;
   const T dx = T(x) - T(pk[4]), dy = T(y) - T(pk[5]);
   const T xr = dx * cs + dy * sn, yr = dy * cs - dx * sn;
   const T r = gm::sqrt(xr * xr + yr * yr);
@@ -2199,6 +2201,18 @@ __device__ __forceinline__ void epl_dalpha(const ModelArgs &m, const GradArgs &g
     fyy = kappa - G1;
     fxy = G2;
   }
+  // orientation: d alpha / d phi = J alpha - H J (x - c)
+  const T dphx = fxx * dy - fxy * dx - a_y, dphy = fxy * dy - fyy * dx + a_x;
+  const T e1 = T(pk[2]), e2 = T(pk[3]);
+  const T c2 = e1 * e1 + e2 * e2, ce = gm::sqrt(c2);
+  const T h = T(0.5) / c2;  // d phi / d(e1, e2) = (-e2, e1) / (2 c^2)
+  const T dp1 = -e2 * h, dp2 = e1 * h;
+  const T dqdc = ce < T(0.9999) ? T(-2) / ((T(1) + ce) * (T(1) + ce)) : T(0);
+  const T dq1 = dqdc * e1 / ce, dq2 = dqdc * e2 / ce;
+  dax[2] = ax.d[1] * dq1 + dphx * dp1;
+  day[2] = ay.d[1] * dq1 + dphy * dp1;
+  dax[3] = ax.d[1] * dq2 + dphx * dp2;
+  day[3] = ay.d[1] * dq2 + dphy * dp2;
   dax[4] = -fxx;
   day[4] = -fxy;
   dax[5] = -fxy;
