@@ -219,6 +219,7 @@ struct Layout {
   size_t recd, recf, coefd, coeff, dcoefd, dcoeff, psf, lowres, partial, blkmax, bfrag;
   size_t gmodel, glow, gpart;  // gradient: dlogL/dmodel, dlogL/dlowres, per-block partial gradients
   size_t gbeta;                // gradient: dS/dbeta of the sources at every ray (0: none)
+  size_t gpart_mp;             // gradient, several planes: per-block partials of the deflectors (0: none)
   size_t gpad, gext;           // edge-padded PSF adjoint: padded dlogL/dmodel and its correlation
   size_t gchr, glu;            // supersampled convolution adjoint buffers
   size_t hr, chr, lowu, lowc, psf_low;  // supersampled convolution: ray fluxes, their convolution,
@@ -401,6 +402,15 @@ Layout layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int
 
 // Layout of lensim_loglike_grad: the image layout plus the gradient buffers
 // (on the ray grid when the convolution is supersampled).
+// Blocks per image of the multi-plane gradient: one 256-ray sweep each where
+// that keeps the whole launch near 2048 blocks, else several sweeps per block
+// (so that the per-block partials of every deflector stay bounded).
+int mp_grad_blocks(size_t rays, int n_img) {
+  const long sweeps = long((rays + 255) / 256);
+  const long want = (2048 + n_img - 1) / n_img;
+  return int(sweeps < want ? sweeps : (want > 0 ? want : 1));
+}
+
 Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img, int precision, int edge,
                    const SSPlan &ss, int s) {
   Layout L = layout(m, nx, ny, kx, ky, n_img, precision, ss);
@@ -417,10 +427,15 @@ Layout grad_layout(const ModelArgs &m, int nx, int ny, int kx, int ky, int n_img
   off = align256(off + size_t(n_img) * m.n_prof * ((npix_r + 256 * kGradPix - 1) / (256 * kGradPix)) *
                            LENSIM_NPARAM * 8);
   L.gbeta = 0;
+  const size_t rays = npix_r * (ss.on ? 1 : size_t(s) * s);
   if ((m.n_lens > 0 || m.mp_n > 0) && m.n_source > 0) {  // dS/dbeta or beta per ray (two values, compute type)
     L.gbeta = off;
-    const size_t rays = npix_r * (ss.on ? 1 : size_t(s) * s);
     off = align256(off + size_t(n_img) * rays * 2 * tsz);
+  }
+  L.gpart_mp = 0;
+  if (m.mp_n > 0 && m.n_defl > 0) {
+    L.gpart_mp = off;
+    off = align256(off + size_t(n_img) * m.n_defl * mp_grad_blocks(rays, n_img) * LENSIM_NPARAM * 8);
   }
   L.gpad = L.gext = L.gchr = L.glu = off;
   if (edge) {  // extended domain (ny cs + ky - 1) x (nx cs + kx - 1)
@@ -2599,17 +2614,20 @@ __device__ __forceinline__ void deflector_dalpha(const ModelArgs &m, const GradA
 // A_{p-1} = A_p + f_p alpha_p and P_{p-1} = P_p - T_p A_{p-1}, with alpha_p
 // the value of the duals the step evaluates anyway.  (d alpha / d x)^T lambda
 // is minus lambda's product with the centre's derivative.
-// Block (b, img) takes the rays of pixels [b kGradPix 256, (b + 1) kGradPix
-// 256), one per thread at a time; all threads sweep the planes together, so a
-// plane's contributions are summed over the block (waves, then the waves in
-// order) and added to its deflector's partial, which the block owns.  Each
-// ray's beta is also stored (ga.gbeta) for the source profiles' launches.
+// Block (b, img) takes sweeps [b spb, (b + 1) spb) of 256 rays of image img
+// (the rays pixel-major, s x s sub-rays per pixel), one ray per thread per
+// sweep; all threads sweep the planes together, so a plane's contributions
+// are summed over the block (waves, then the waves in order) and added to its
+// deflector's partial in gpart_mp, which the block owns; mp_partial_reduce
+// sums the blocks in order.  Each ray's beta is also stored (ga.gbeta) for
+// the source profiles' launches.
 template <typename T, bool HALO>
 __global__ __launch_bounds__(256) void mp_grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
                                                       const double *__restrict__ recd,
                                                       const float *__restrict__ recf,
                                                       const double *__restrict__ coefd,
-                                                      const float *__restrict__ coeff) {
+                                                      const float *__restrict__ coeff, double *__restrict__ gpart_mp,
+                                                      int spb) {
   const int img = blockIdx.y;
   const int npix = g.nx * g.ny, ss = g.s * g.s;
   const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
@@ -2617,22 +2635,21 @@ __global__ __launch_bounds__(256) void mp_grad_kernel(ModelArgs m, GridArgs g, G
   const size_t o = size_t(img) * size_t(m.mp_stride);
   const int nd = m.mp_n - 1;  // deflector planes; plane nd is the source's
   auto slot = [&](int k) {
-    return ((size_t(img) * m.n_prof + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM;
+    return ((size_t(img) * m.n_defl + k) * gridDim.x + blockIdx.x) * LENSIM_NPARAM;
   };
   // this block's partials of every deflector start at zero
   for (int q = t; q < m.n_defl * LENSIM_NPARAM; q += blockDim.x)
-    AT(ga.gpart, slot(q / LENSIM_NPARAM) + q % LENSIM_NPARAM) = 0.0;
+    AT(gpart_mp, slot(q / LENSIM_NPARAM) + q % LENSIM_NPARAM) = 0.0;
   __syncthreads();
   __shared__ double red[2][256 / 64][LENSIM_NPARAM];
-  const int p0 = blockIdx.x * kGradPix * 256, p1 = min(npix, p0 + kGradPix * 256);
-  const int r_end = (p1 - p0) * ss;  // rays of the block, pixel-major, s x s sub-rays per pixel
+  const int r_all = npix * ss;
+  const int r0 = min(r_all, int(blockIdx.x) * spb * 256), r_end = min(r_all, r0 + spb * 256);
   int step = 0;  // plane steps so far: consecutive ones alternate between the two halves of red
-  for (int base = 0; base < r_end; base += 256) {
+  for (int base = r0; base < r_end; base += 256) {
     const int rr = base + t;
     const bool active = rr < r_end;
     int sub;
-    const int q0 = div_by(active ? rr : 0, ss, g.inv_ss, sub);
-    const int pix = p0 + q0;
+    const int pix = div_by(active ? rr : r0, ss, g.inv_ss, sub);
     int i, sx;
     const int j = div_by(pix, g.nx, g.inv_nx, i), sy = div_by(sub, g.s, g.inv_s, sx);
     const double ox = g.o0 + sx * g.inv_s, oy = g.o0 + sy * g.inv_s;
@@ -2721,21 +2738,42 @@ __global__ __launch_bounds__(256) void mp_grad_kernel(ModelArgs m, GridArgs g, G
       if (t < n) {
         double v = 0.0;
         for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[buf][wv][t];
-        AT(ga.gpart, slot(k) + t) += v;
+        AT(gpart_mp, slot(k) + t) += v;
       }
     }
   }
 }
 
+// The deflectors' partials of the nb_mp blocks of mp_grad_kernel summed in
+// block order into the first of the nblk slots grad_reduce_kernel reads (the
+// others zero).
+__global__ void mp_partial_reduce(const double *__restrict__ gpart_mp, int nb_mp, int n_img, int n_defl, int n_prof,
+                                  int nblk, double *__restrict__ gpart) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_img * n_defl * LENSIM_NPARAM) return;
+  const int q = i % LENSIM_NPARAM, r = i / LENSIM_NPARAM, k = r % n_defl, img = r / n_defl;
+  double v = 0.0;
+  for (int b = 0; b < nb_mp; ++b) v += AT(gpart_mp, (size_t(r) * nb_mp + b) * LENSIM_NPARAM + q);
+  const size_t o = (size_t(img) * n_prof + k) * nblk;
+  for (int b = 0; b < nblk; ++b) AT(gpart, (o + b) * LENSIM_NPARAM + q) = b == 0 ? v : 0.0;
+}
+
 template <typename T>
 int launch_mp_grad(const ModelArgs &m, const GridArgs &g, const GradArgs &ga, const double *recd, const float *recf,
-                   int nblk, int n_img, hipStream_t st) {
-  const dim3 grid(nblk, n_img);
+                   double *gpart_mp, int nblk, int n_img, hipStream_t st) {
+  const size_t rays = size_t(g.nx) * g.ny * g.s * g.s;
+  const int nb_mp = mp_grad_blocks(rays, n_img);
+  const int spb = int(((rays + 255) / 256 + nb_mp - 1) / nb_mp);
+  const dim3 grid(nb_mp, n_img);
   if (m.has_halo)
-    mp_grad_kernel<T, true><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
+    mp_grad_kernel<T, true><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff, gpart_mp, spb);
   else
-    mp_grad_kernel<T, false><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff);
-  return check_launch("mp_grad_kernel");
+    mp_grad_kernel<T, false><<<grid, 256, 0, st>>>(m, g, ga, recd, recf, ga.coefd, ga.coeff, gpart_mp, spb);
+  int rc = check_launch("mp_grad_kernel");
+  if (rc || m.n_defl == 0) return rc;
+  const int n = n_img * m.n_defl * LENSIM_NPARAM;
+  mp_partial_reduce<<<(n + 255) / 256, 256, 0, st>>>(gpart_mp, nb_mp, n_img, m.n_defl, m.n_prof, nblk, ga.gpart);
+  return check_launch("mp_partial_reduce");
 }
 
 __global__ void grad_reduce_kernel(const double *__restrict__ gpart, int nblk, int n, double *__restrict__ grad) {
@@ -3399,7 +3437,9 @@ int run_grad(const ModelArgs &m, const GridArgs &g, const lensim_psf *psf, int k
   const float *recf = reinterpret_cast<const float *>(ws + L.recf);
   // multi-plane: every deflector in one launch, which also stores beta at every
   // ray (ga.gbeta) for the source launches
-  if (m.mp_n && (rc = launch_mp_grad<T>(m, gg, ga, recd, recf, nblk, n_img, st))) return rc;
+  if (m.mp_n && (rc = launch_mp_grad<T>(m, gg, ga, recd, recf, reinterpret_cast<double *>(ws + L.gpart_mp), nblk,
+                                        n_img, st)))
+    return rc;
   // the sources first: the first source's launch stores dS/dbeta at every ray
   // (ga.gbeta), which the inline lens launches after it read
   for (int q = 0; q < m.n_prof; ++q) {
@@ -4268,6 +4308,8 @@ int lensim_loglike_grad(const lensim_model *model, const lensim_grid *grid, cons
   if (n_img < 0) return fail(LENSIM_EINVAL, "negative n_img");
   if (n_img == 0) return LENSIM_OK;
   if (n_img > 65535 || m.n_prof > 65535) return fail(LENSIM_EINVAL, "batch too large");
+  if (m.mp_n && int64_t(g.nx) * g.ny * g.s * g.s >= (int64_t(1) << 30))
+    return fail(LENSIM_EINVAL, "multi-plane gradients take fewer than 2^30 rays per image");
   if (!d_logl || !d_grad || !d_params) return fail(LENSIM_EINVAL, "NULL buffer");
   if (precision != LENSIM_FP64 && precision != LENSIM_MIXED) return fail(LENSIM_EINVAL, "bad precision");
   const Layout L = grad_layout(m, g.nx, g.ny, kx, ky, n_img, precision, edge, ss, g.s);
