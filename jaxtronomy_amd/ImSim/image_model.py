@@ -115,6 +115,8 @@ class ImageModel(object):
             list(self.SourceModel.param_name_list) + list(self.LensLightModel.param_name_list)
         self._model = backend.make_model(self._lens_types, self._src_types, self._light_types,
                                          alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
+        if not self._bulk:  # LensModel(multi_plane=True): its planes (kept alive by self.LensModel)
+            self._model.multiplane = self.LensModel.multiplane_address() or None
         self._submodels = {}
         self._ws = backend.Workspace()
 
@@ -253,6 +255,10 @@ class ImageModel(object):
             m = backend.make_model([self._lens_types[i] for i in lens_idx], [self._src_types[i] for i in src_idx],
                                    [self._light_types[i] for i in light_idx],
                                    alpha_scaling=self.LensModel.alpha_scaling, flux_scaling=self._flux_scaling)
+            if self._model.multiplane:  # the planes hold every deflector of the list
+                if list(lens_idx) != list(range(nl)):
+                    raise ValueError("no individual deflector can be evaluated in multi-plane lensing")
+                m.multiplane = self._model.multiplane
             self._submodels[key] = (m, rows)
         return self._submodels[key]
 
