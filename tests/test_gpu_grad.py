@@ -332,6 +332,53 @@ def test_inline_halo_grads_fp64_vs_fd_and_mixed_vs_fp64():
     assert np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))) < 1e-5
 
 
+def test_inline_sie_nie_grads_fp64_vs_fd_and_mixed_vs_fp64():
+    """SIE + NIE + SHEAR inline (SIE / NIE duals from their own axis ratio and
+    orientation, nie.py:86-107, :182-200): every parameter's FP64 gradient
+    against central differences of the FP64 logL, and the mixed gradient within
+    1e-4 of the FP64 one (per parameter, relative to its scale over 16 points)."""
+    cfg = dict(W.EPL180, numPix=72, deltaPix=0.05, supersampling=2)
+    kernel = W.moffat_kernel(11, 2.0, 3.0)
+    _, ks, kll = W.epl180_draw(6)
+    names = ["SIE", "NIE", "SHEAR"]
+    kl = [dict(theta_E=1.05, e1=0.08, e2=-0.05, center_x=0.02, center_y=-0.03),
+          dict(theta_E=0.25, e1=-0.1, e2=0.06, s_scale=0.08, center_x=0.9, center_y=0.5),
+          dict(gamma1=0.03, gamma2=-0.02, ra_0=0.0, dec_0=0.0)]
+
+    def model(precision, data=None):
+        return ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+                          LensModel(names), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                          kwargs_numerics={"supersampling_factor": 2}, precision=precision)
+
+    clean = model("fp64").image(kl, ks, kll).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(19))
+    im64, im32 = model("fp64", data), model("mixed", data)
+    base, _ = im64.pack(kl, ks, kll)
+    rng = np.random.default_rng(20)
+    nprm = [5, 6, 2, 7, 7]
+    p = np.repeat(base, 16, axis=0)
+    for k, n in enumerate(nprm):
+        for i in range(n):
+            p[:, k, i] += rng.normal(0, 0.02, 16) * max(abs(base[0, k, i]), 0.1)
+    pt = torch.as_tensor(p, device="cuda")
+    l64, g64 = im64.likelihood_and_grad_batch(pt)
+    l32, g32 = im32.likelihood_and_grad_batch(pt)
+    g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
+    slots = [(k, i) for k, n in enumerate(nprm) for i in range(n)]
+    for (k, i) in slots:
+        scale = np.max(np.abs(g64[:, k, i]))
+        assert np.max(np.abs(g32[:, k, i] - g64[:, k, i])) / scale < 1e-4, (k, i)
+    reps = np.repeat(p[:1], 2 * len(slots), axis=0)
+    hs = np.array([1e-6 * max(abs(p[0, k, i]), 0.05) for (k, i) in slots])
+    for q, (k, i) in enumerate(slots):
+        reps[2 * q, k, i] += hs[q]
+        reps[2 * q + 1, k, i] -= hs[q]
+    ll = im64.likelihood_batch(torch.as_tensor(reps, device="cuda")).cpu().numpy()
+    fd = (ll[0::2] - ll[1::2]) / (2 * hs)
+    got = np.array([g64[0, k, i] for (k, i) in slots])
+    assert np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))) < 1e-5
+
+
 def test_convergence_sheet_grads():
     """EPL + CONVERGENCE (a mass sheet; its kappa and reference point are free):
     the FP64 gradient against central differences of the FP64 logL, and the
