@@ -169,17 +169,14 @@ __device__ void epl_g(const S *p, const T *__restrict__ c, const T *__restrict__
   epl_g_geo<T>(p, q, cos(phi), sin(phi), c, dc, nser, x, y, ax, ay);
 }
 
-// NIE / SIE, LensModel/Profiles/nie.py:86-107, :182-200, :280-295.
-// p: theta_E, e1, e2, s_scale, center_x, center_y (SIE: s_scale = 1e-10, centre at p[3..4]).
+// NIE / SIE, LensModel/Profiles/nie.py:86-107, :182-200, :280-295, from the
+// axis ratio q and the orientation's cos / sin (cs, sn).
 template <typename T, typename S>
-__device__ void nie_g(const S &theta_E, const S &e1, const S &e2, const S &s_scale, const S &cx, const S &cy,
-                      const S &x, const S &y, S &ax, S &ay) {
-  S phi, q;
-  phi_q<T>(e1, e2, phi, q);
+__device__ __forceinline__ void nie_g_geo(const S &theta_E, const S &q, const S &cs, const S &sn, const S &s_scale,
+                                          const S &cx, const S &cy, const S &x, const S &y, S &ax, S &ay) {
   const S theta_E_conv = theta_E / sqrt((T(1) + q * q) / (T(2) * q));
   const S b = theta_E_conv * sqrt((T(1) + q * q) * T(0.5));
   const S s = s_scale / sqrt(q);
-  const S cs = cos(phi), sn = sin(phi);
   const S dx = x - cx, dy = y - cy;
   const S xr = dx * cs + dy * sn;
   const S yr = dy * cs - dx * sn;
@@ -190,6 +187,15 @@ __device__ void nie_g(const S &theta_E, const S &e1, const S &e2, const S &s_sca
   const S fy = b / sq * atanh(sq * yr / (psi + qc * qc * s));
   ax = ax + (fx * cs - fy * sn);
   ay = ay + (fx * sn + fy * cs);
+}
+
+// p: theta_E, e1, e2, s_scale, center_x, center_y (SIE: s_scale = 1e-10, centre at p[3..4]).
+template <typename T, typename S>
+__device__ void nie_g(const S &theta_E, const S &e1, const S &e2, const S &s_scale, const S &cx, const S &cy,
+                      const S &x, const S &y, S &ax, S &ay) {
+  S phi, q;
+  phi_q<T>(e1, e2, phi, q);
+  nie_g_geo<T>(theta_E, q, cos(phi), sin(phi), s_scale, cx, cy, x, y, ax, ay);
 }
 
 // F(X) of the NFW family: arctanh(sqrt(1 - X^2)) / sqrt(1 - X^2) = arccosh(1/X)

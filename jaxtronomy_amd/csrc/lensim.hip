@@ -2234,9 +2234,59 @@ __device__ __forceinline__ void epl_dalpha(const ModelArgs &m, const GradArgs &g
   day[5] = -fyy;
 }
 
-// Contribution of one ray to d flux / d(kwargs of the EPL deflector k).
-template <typename T, bool HALO>
-__device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
+// The deflection of the SIE / NIE deflector k at (x, y) and its derivatives by
+// its kwargs (theta_E, e1, e2[, s_scale], center_x, center_y).  Dual seeds
+// carry q, s_scale and the centre; the axis ratio and orientation come from
+// the ellipticity (q, by Util/param_util.py:88-100) and the record (cos phi,
+// sin phi), not per-ray dual atan2 / cos / sin; alpha is linear in theta_E
+// (b = theta_E sqrt(q)); d alpha / d phi = J alpha - H J (x - c) as for the
+// EPL, H = -(d alpha / d center).
+template <typename T, int TYPE>
+__device__ __forceinline__ void nie_dalpha(const double *rd, const double *pk, double x, double y, T &a_x, T &a_y,
+                                           T *dax, T *day) {
+  using S = gm::Dual<T, 4>;  // q, s_scale, center_x, center_y
+  constexpr bool kNie = TYPE == LENSIM_NIE;
+  const T e1 = T(pk[1]), e2 = T(pk[2]);
+  const T c2 = e1 * e1 + e2 * e2, ce = gm::sqrt(c2);
+  const T cc = ce < T(0.9999) ? ce : T(0.9999);
+  const T q = (T(1) - cc) / (T(1) + cc);
+  const T s_scale = kNie ? T(pk[3]) : T(0.0000000001);
+  const T cx = T(pk[kNie ? 4 : 3]), cy = T(pk[kNie ? 5 : 4]);
+  S ax = S(T(0)), ay = S(T(0));
+  gm::nie_g_geo<T>(S(T(pk[0])), S::seed(q, 0), S(T(rd[NIE_C])), S(T(rd[NIE_S])), S::seed(s_scale, 1),
+                   S::seed(cx, 2), S::seed(cy, 3), S(T(x)), S(T(y)), ax, ay);
+  a_x = ax.v;
+  a_y = ay.v;
+  const T thetaE = T(pk[0]);
+  const T ith = thetaE != T(0) ? T(1) / thetaE : T(0);
+  dax[0] = a_x * ith;
+  day[0] = a_y * ith;
+  // H = d alpha / d(x, y) = -(d alpha / d center)
+  const T hxx = -ax.d[2], hxy = -ax.d[3], hyx = -ay.d[2], hyy = -ay.d[3];
+  const T dx = T(x) - cx, dy = T(y) - cy;
+  const T dphx = hxx * dy - hxy * dx - a_y, dphy = hyx * dy - hyy * dx + a_x;
+  const T h = T(0.5) / c2;  // d phi / d(e1, e2) = (-e2, e1) / (2 c^2)
+  const T dp1 = -e2 * h, dp2 = e1 * h;
+  const T dqdc = ce < T(0.9999) ? T(-2) / ((T(1) + ce) * (T(1) + ce)) : T(0);
+  const T dq1 = dqdc * e1 / ce, dq2 = dqdc * e2 / ce;
+  dax[1] = ax.d[0] * dq1 + dphx * dp1;
+  day[1] = ay.d[0] * dq1 + dphy * dp1;
+  dax[2] = ax.d[0] * dq2 + dphx * dp2;
+  day[2] = ay.d[0] * dq2 + dphy * dp2;
+  constexpr int kc = kNie ? 4 : 3;  // the centre's slots
+  if constexpr (kNie) {
+    dax[3] = ax.d[1];
+    day[3] = ay.d[1];
+  }
+  dax[kc] = ax.d[2];
+  day[kc] = ay.d[2];
+  dax[kc + 1] = ax.d[3];
+  day[kc + 1] = ay.d[3];
+}
+
+// Contribution of one ray to d flux / d(kwargs of the EPL, SIE or NIE deflector k).
+template <typename T, bool HALO, int TYPE = LENSIM_EPL>
+__device__ __forceinline__ void lens_ray_grad(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp, int img,
                                              int k, double x, double y, double w, double *acc, const T *gb) {
   const double *pk = ga.params + (size_t(img) * m.n_prof + k) * LENSIM_NPARAM;
   CHECK_SPAN(pk, LENSIM_NPARAM * sizeof(double));
@@ -2252,7 +2302,11 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
     deflect_bulk<T, HALO>(m, rp, x, y, axp, ayp);
   }
   T a_x, a_y, dax[6], day[6];
-  epl_dalpha<T>(m, ga, rp, img, k, pk, x, y, a_x, a_y, dax, day);
+  if constexpr (TYPE == LENSIM_EPL)
+    epl_dalpha<T>(m, ga, rp, img, k, pk, x, y, a_x, a_y, dax, day);
+  else
+    nie_dalpha<T, TYPE>(rp.recd + k * kRec, pk, x, y, a_x, a_y, dax, day);
+  constexpr int N = n_params_of(TYPE);
   const T as = T(m.alpha_scaling);
   T gx, gy;
   if (gb) {
@@ -2265,7 +2319,7 @@ __device__ __forceinline__ void epl_ray_grad(const ModelArgs &m, const GradArgs 
     gy = f2.d[1];
   }
 #pragma unroll
-  for (int i = 0; i < 6; ++i) acc[i] += w * double(-as * (gx * dax[i] + gy * day[i]));
+  for (int i = 0; i < N; ++i) acc[i] += w * double(-as * (gx * dax[i] + gy * day[i]));
 }
 
 // One block = kGradPix x 256 pixels x one image of profile k, whose type
@@ -2323,9 +2377,9 @@ grad_kernel(ModelArgs m, GridArgs g, GradArgs ga,
                   ? gbeta + ((size_t(img) * npix + pix) * size_t(g.s * g.s) + size_t(sy * g.s + sx)) * 2
                   : nullptr;
       if (gb) CHECK_SPAN(gb, 2 * sizeof(T));
-      if constexpr (type == LENSIM_EPL) epl_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc, gb);
-      else if constexpr (type == LENSIM_SIE) ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
-      else if constexpr (type == LENSIM_NIE) ray_grad<T, 6, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
+      if constexpr (type == LENSIM_EPL) lens_ray_grad<T, HALO>(m, ga, rp, img, k, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_SIE) lens_ray_grad<T, HALO, LENSIM_SIE>(m, ga, rp, img, k, x, y, w, acc, gb);
+      else if constexpr (type == LENSIM_NIE) lens_ray_grad<T, HALO, LENSIM_NIE>(m, ga, rp, img, k, x, y, w, acc, gb);
       else if constexpr (type == LENSIM_NFW) halo_ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
       else if constexpr (type == LENSIM_TNFW) halo_ray_grad<T, 5, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
       else if constexpr (type == LENSIM_SHEAR) ray_grad<T, 4, HALO>(m, ga, rp, img, k, type, x, y, w, acc, gb);
