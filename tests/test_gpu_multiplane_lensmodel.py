@@ -1,8 +1,9 @@
 """LensModel(multi_plane=True) (MultiPlane, LensModel/MultiPlane/multi_plane.py):
 its distances against LensModelBulk's and the background cosmology's, ray
 shooting and alpha against the oracle's plane scan (also with a source
-redshift other than the convention's), and an ImageModel on it against the
-same scene through LensModelBulk(multi_plane=True): image, logL and gradient."""
+redshift other than the convention's), an ImageModel on it against the same
+scene through LensModelBulk(multi_plane=True) (image, logL and gradient), and
+the request server on it."""
 
 import numpy as np
 import numpy.testing as npt
@@ -125,3 +126,40 @@ def test_image_model_on_multi_plane_lens_model(precision, tol, gtol):
             assert abs(glens[j][pn] - glb[pn][d]) <= gtol * scale, (nm, pn, glens[j][pn], glb[pn][d])
     for pn in gsrc[0]:
         assert abs(gsrc[0][pn] - gsb[0][pn]) <= gtol * max(abs(gsb[0][pn]), 1.0)
+
+
+def test_served_multi_plane_lens_model_equals_alone():
+    """The request server on an ImageModel over LensModel(multi_plane=True):
+    each request's logL and image equal the same request run alone."""
+    from jaxtronomy_amd.serving import LensServer
+
+    names, kw, ks, z, cosmo = _scene(n_sub=6)
+    n = 48
+    cfg = dict(W.SUBSTRUCTURE, numPix=n, deltaPix=0.08, supersampling=2)
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))),
+                    PSF("PIXEL", kernel_point_source=W.moffat_kernel(9, 1.5, 3.0)),
+                    LensModel(names, z_source=1.5, lens_redshift_list=z, cosmo=cosmo, multi_plane=True,
+                              precision="mixed"),
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2}, precision="mixed")
+    srv = LensServer(n_streams=2, use_graphs=True)
+    srv.register("mp", im, max_batch=8, n_slots=2, n_lanes=2)
+    obs = [np.random.default_rng(s).normal(0, 0.05, (n, n)) for s in (1, 2)]
+    slots = [srv.put_data("mp", o) for o in obs]
+    base, _ = im.pack(kw, ks)
+    rng = np.random.default_rng(3)
+    params = np.repeat(base, 11, axis=0)
+    params[:, :, :6] *= 1 + 0.01 * rng.normal(size=params[:, :, :6].shape)
+    futs = []
+    for i in range(11):
+        kind = "image" if i % 3 == 2 else "loglike"
+        futs.append((i, kind, srv.submit("mp", params[i], kind=kind, slot=slots[i % 2])))
+        if i % 4 == 3:
+            srv.step()
+    srv.drain()
+    for i, kind, f in futs:
+        p = torch.as_tensor(params[i][None], device="cuda")
+        if kind == "loglike":
+            data = torch.as_tensor(obs[i % 2][None], device="cuda").contiguous()
+            assert f.result() == im.likelihood_batch(p, data=data).item(), i
+        else:
+            assert torch.equal(f.result().double(), im.image_batch(p).double().cpu()[0]), i
