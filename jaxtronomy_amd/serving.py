@@ -27,6 +27,11 @@ alone, so a request's result does not depend on the requests it was batched
 with; batches are padded to a power of two with copies of their first request
 so that a handful of graphs covers every batch size.
 
+A model with a ``flux_evaluate_indexes`` sub-frame runs on its square: an
+observation is cut to it when it is put, together with the log-likelihood of
+a zero model on the pixels outside (which no parameter changes), added to each
+request's log-likelihood; images are completed to the frame (zero outside).
+
 Admission: every bucket's device memory is charged against an HBM budget at
 registration; ``put_data`` returns None while all of a bucket's slots are in
 use.  All device work is on streams of the default priority.
@@ -140,6 +145,9 @@ class _Lane(object):
         self.logl = torch.zeros(mb, dtype=torch.float64, device=dev)
         self.logl_host = torch.zeros(mb, dtype=torch.float64).pin_memory()
         self.images = torch.zeros((mb, ny, nx), dtype=b.dtype, device=dev)
+        # a sub-framed model's images completed to the frame (zero outside the square, written once)
+        self.full_images = (torch.zeros((mb,) + b.full_shape, dtype=b.dtype, device=dev) if b.frame is not None
+                            else self.images)
         self.ws = torch.empty(b.ws_bytes, dtype=torch.uint8, device=dev)
         self.data = backend.LensimData()
         ctypes.memmove(ctypes.byref(self.data), ctypes.byref(b.data_template), ctypes.sizeof(self.data))
@@ -150,6 +158,8 @@ class _Lane(object):
         self.graphs = {}
         self.nbytes = sum(t.numel() * t.element_size() for t in (self.params, self.slot_idx, self.batch_data,
                                                                   self.logl, self.images, self.ws))
+        if b.frame is not None:
+            self.nbytes += self.full_images.numel() * self.full_images.element_size()
         if self.batch_noise is not None:
             self.nbytes += self.batch_noise.numel() * 8
 
@@ -158,8 +168,6 @@ class _Bucket(object):
     def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs, noise_maps):
         if image_model._bulk:
             raise NotImplementedError("the server takes ImageModels with a plain lens model list")
-        if image_model._frame is not None:
-            raise NotImplementedError("the server takes ImageModels without flux_evaluate_indexes sub-frames")
         self.key = key
         self.im = image_model
         self.max_batch = int(max_batch)
@@ -167,7 +175,11 @@ class _Bucket(object):
             raise ValueError("max_batch must be in [1, 65535]")
         self.use_graphs = use_graphs
         self.noise_maps = bool(noise_maps)
+        # a sub-framed model (flux_evaluate_indexes) runs on its square: observations are cut to it when they
+        # are put, each with the logL of a zero model outside the square, and images are completed to the frame
         self.shape = (image_model._grid.ny, image_model._grid.nx)
+        self.full_shape = tuple(image_model._full_shape)
+        self.frame = image_model._frame
         self.n_prof = len(image_model._names)
         self.dtype = torch.float64 if image_model._prec == backend.FP64 else torch.float32
         dev = backend.device()
@@ -175,8 +187,16 @@ class _Bucket(object):
         if buf["noise"] is not None and not self.noise_maps:
             raise ValueError("the ImageData carries a noise map: register the bucket with noise_maps=True and "
                              "give each observation its noise map")
-        self.mask = backend.to_device_f64(image_model.likelihood_mask.astype(np.float64))
-        self.expo_map = buf["expo_map"]
+        self.outside = None
+        if self.frame is None:
+            self.mask = backend.to_device_f64(image_model.likelihood_mask.astype(np.float64))
+            self.expo_map = buf["expo_map"]
+        else:  # the likelihood mask and exposure map cut to the square
+            sub, _, self._sub_keep = image_model._sub_frame_data()
+            x0, y0, n = self.frame
+            self.mask = self._sub_keep[3]
+            self.expo_map = self._sub_keep[2]
+            self.outside = torch.zeros(int(n_slots), dtype=torch.float64, device=dev)
         d = backend.LensimData()
         d.d_mask = self.mask.data_ptr()
         d.d_exposure_map = None if self.expo_map is None else self.expo_map.data_ptr()
@@ -210,12 +230,17 @@ class _Bucket(object):
                                     ctypes.byref(lane.data), backend.ptr(lane.params), n, im._prec,
                                     backend.ptr(lane.logl), None, backend.ptr(lane.ws), lane.ws.numel(), st)
             backend.check(rc, "lensim_loglike")
+            if self.outside is not None:  # the observation's pixels outside the square
+                lane.logl[:n] += torch.index_select(self.outside, 0, idx)
         else:
             rc = lib.lensim_image(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
                                   backend.ptr(lane.params), n, im._prec,
                                   backend.IMAGE_SOURCE | backend.IMAGE_LENS_LIGHT, backend.ptr(lane.images),
                                   backend.ptr(lane.ws), lane.ws.numel(), st)
             backend.check(rc, "lensim_image")
+            if self.frame is not None:
+                x0, y0, m = self.frame
+                lane.full_images[:n, x0:x0 + m, y0:y0 + m] = lane.images[:n]
 
     def _batch(self, lane, kind, npad):
         """One batch of npad rows: staging in, launches and, for log-likelihoods,
@@ -254,8 +279,9 @@ class _Bucket(object):
         images = None
         if kind == "image":
             # padded size: few distinct sizes keep torch's pinned-memory cache warm
-            images = torch.empty((npad,) + self.shape, dtype=self.dtype, pin_memory=True)[:n]
-            rc = _hip().hipMemcpyAsync(ctypes.c_void_p(images.data_ptr()), ctypes.c_void_p(lane.images.data_ptr()),
+            images = torch.empty((npad,) + self.full_shape, dtype=self.dtype, pin_memory=True)[:n]
+            rc = _hip().hipMemcpyAsync(ctypes.c_void_p(images.data_ptr()),
+                                       ctypes.c_void_p(lane.full_images.data_ptr()),
                                        images.numel() * images.element_size(), _D2H,
                                        ctypes.c_void_p(stream.cuda_stream))
             if rc != 0:
@@ -355,23 +381,33 @@ class LensServer(object):
         bucket's slots are in use."""
         b = self.buckets[key]
         img = np.asarray(image, dtype=np.float64)
-        if img.shape != b.shape:
-            raise ValueError("observation of shape %s for a %s bucket" % (img.shape, b.shape))
+        if img.shape != b.full_shape:
+            raise ValueError("observation of shape %s for a %s bucket" % (img.shape, b.full_shape))
         if b.noise_maps:
             if noise_map is None:
                 raise ValueError("this bucket needs a noise map with each observation")
             noise_map = np.asarray(noise_map, dtype=np.float64)
-            if noise_map.shape != b.shape:
-                raise ValueError("noise map of shape %s for a %s bucket" % (noise_map.shape, b.shape))
+            if noise_map.shape != b.full_shape:
+                raise ValueError("noise map of shape %s for a %s bucket" % (noise_map.shape, b.full_shape))
         elif noise_map is not None:
             raise ValueError("bucket %r was registered without noise maps" % (key,))
         with self.lock:
             if not b.free_slots:
                 return None
             slot = b.free_slots.pop()
-        b.slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(img)))
-        if b.noise_maps:
-            b.noise_slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(noise_map)))
+        if b.frame is None:
+            b.slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(img)))
+            if b.noise_maps:
+                b.noise_slots[slot].copy_(torch.from_numpy(np.ascontiguousarray(noise_map)))
+        else:  # cut to the square, with the zero model's logL outside it
+            dev = backend.device()
+            obs = torch.as_tensor(np.ascontiguousarray(img[None]), device=dev)
+            nm = None if noise_map is None else torch.as_tensor(np.ascontiguousarray(noise_map[None]), device=dev)
+            _, out_logl, keep = b.im._sub_frame_data(obs, nm, 1)
+            b.slots[slot].copy_(keep[0][0])
+            if b.noise_maps:
+                b.noise_slots[slot].copy_(keep[1][0])
+            b.outside[slot:slot + 1].copy_(out_logl)
         torch.cuda.current_stream().synchronize()
         return slot
 

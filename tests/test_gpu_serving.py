@@ -430,3 +430,44 @@ def test_serving_at_stated_scale():
         want = O.log_likelihood(want_img, obs[r % 4], noise_map=noise[r % 4])
         tol = 1e-10 if precision == "fp64" else 1e-6
         assert abs(futs[q].value - want) < tol * abs(want), (s["key"], futs[q].value, want)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sub_frame_bucket(precision):
+    """A model with flux_evaluate_indexes: each observation is cut to the square
+    around the evaluated pixels when it is put (with the zero model's logL on
+    the pixels outside), images come back completed to the frame.  logL equals
+    likelihood_batch on the whole frame to rounding, images are bit-equal."""
+    n = 48
+    cfg = dict(W.EPL180, numPix=n, supersampling=2)
+    kernel = W.moffat_kernel(9, cfg["psf_fwhm"] / cfg["deltaPix"], cfg["psf_beta"])
+    ev = np.zeros((n, n), bool)
+    yy, xx = np.mgrid[0:n, 0:n]
+    ev[(yy - 22) ** 2 + (xx - 25) ** 2 < 15 ** 2] = True
+    lmask = np.ones((n, n))
+    lmask[:, :12] = 0
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel),
+                    LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
+                    kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": ev},
+                    likelihood_mask=lmask, precision=precision)
+    assert im._frame is not None and im._frame[2] < n
+    srv = LensServer(n_streams=2, use_graphs=True)
+    srv.register("sf", im, max_batch=8, n_slots=3, n_lanes=2)
+    obs = _observations(im, cfg, 3, 5)
+    slots = [srv.put_data("sf", o) for o in obs]
+    params = _requests(im, 13, 31)
+    futs = []
+    for i in range(len(params)):
+        kind = "image" if i % 4 == 1 else "loglike"
+        futs.append((i, kind, srv.submit("sf", params[i], kind=kind, slot=slots[i % 3])))
+        if i % 5 == 4:
+            srv.step()
+    srv.drain()
+    for i, kind, f in futs:
+        if kind == "loglike":
+            ref = _alone_logl(im, params[i], obs[i % 3])
+            assert abs(f.result() - ref) <= 1e-12 * abs(ref), (i, f.result(), ref)
+        else:
+            got = f.result().double()
+            assert tuple(got.shape) == (n, n)
+            assert torch.equal(got, _alone_image(im, params[i])), i
