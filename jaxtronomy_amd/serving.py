@@ -27,6 +27,10 @@ alone, so a request's result does not depend on the requests it was batched
 with; batches are padded to a power of two with copies of their first request
 so that a handful of graphs covers every batch size.
 
+A bucket of an ImageModel with a LensModelBulk deflector fixes its deflector
+structure at registration (``register(..., bulk=...)``); its requests are
+parameter arrays in that structure's ImageModel.pack_bulk layout.
+
 A model with a ``flux_evaluate_indexes`` sub-frame runs on its square: an
 observation is cut to it when it is put, together with the log-likelihood of
 a zero model on the pixels outside (which no parameter changes), added to each
@@ -165,11 +169,22 @@ class _Lane(object):
 
 
 class _Bucket(object):
-    def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs, noise_maps):
-        if image_model._bulk:
-            raise NotImplementedError("the server takes ImageModels with a plain lens model list")
+    def __init__(self, key, image_model, max_batch, n_slots, n_lanes, use_graphs, noise_maps, bulk=None):
         self.key = key
         self.im = image_model
+        # the lensim_model of the bucket's requests: the ImageModel's own, or for a LensModelBulk deflector the
+        # one of ImageModel.pack_bulk for the bucket's deflector structure (requests in that packed layout)
+        self.model, self._bulk_keep = image_model._model, None
+        if image_model._bulk:
+            if bulk is None:
+                raise ValueError("a LensModelBulk bucket needs bulk=(ray_shooting_kwargs, kwargs_source, "
+                                 "kwargs_lens_light) fixing its deflector structure")
+            params, _, self.model, self._bulk_keep = image_model.pack_bulk(*bulk)
+            self.n_prof = params.shape[1]
+        elif bulk is not None:
+            raise ValueError("bulk is for ImageModels with a LensModelBulk deflector")
+        else:
+            self.n_prof = len(image_model._names)
         self.max_batch = int(max_batch)
         if not 1 <= self.max_batch <= 65535:
             raise ValueError("max_batch must be in [1, 65535]")
@@ -180,7 +195,6 @@ class _Bucket(object):
         self.shape = (image_model._grid.ny, image_model._grid.nx)
         self.full_shape = tuple(image_model._full_shape)
         self.frame = image_model._frame
-        self.n_prof = len(image_model._names)
         self.dtype = torch.float64 if image_model._prec == backend.FP64 else torch.float32
         dev = backend.device()
         buf = image_model.Data.device_buffers()
@@ -203,7 +217,7 @@ class _Bucket(object):
         d.exposure_time = buf["expo_time"]
         d.background_rms = image_model.Data.background_rms
         self.data_template = d
-        self.ws_bytes = max(image_model.workspace_bytes(self.max_batch), 256)
+        self.ws_bytes = max(image_model.workspace_bytes(self.max_batch, self.model), 256)
         self.slots = torch.zeros((int(n_slots),) + self.shape, dtype=torch.float64, device=dev)
         self.noise_slots = torch.ones_like(self.slots) if self.noise_maps else None
         self.free_slots = list(range(int(n_slots)))[::-1]
@@ -226,14 +240,14 @@ class _Bucket(object):
             torch.index_select(self.slots, 0, idx, out=lane.batch_data[:n])
             if self.noise_maps:
                 torch.index_select(self.noise_slots, 0, idx, out=lane.batch_noise[:n])
-            rc = lib.lensim_loglike(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
+            rc = lib.lensim_loglike(ctypes.byref(self.model), ctypes.byref(im._grid), ctypes.byref(im._psf),
                                     ctypes.byref(lane.data), backend.ptr(lane.params), n, im._prec,
                                     backend.ptr(lane.logl), None, backend.ptr(lane.ws), lane.ws.numel(), st)
             backend.check(rc, "lensim_loglike")
             if self.outside is not None:  # the observation's pixels outside the square
                 lane.logl[:n] += torch.index_select(self.outside, 0, idx)
         else:
-            rc = lib.lensim_image(ctypes.byref(im._model), ctypes.byref(im._grid), ctypes.byref(im._psf),
+            rc = lib.lensim_image(ctypes.byref(self.model), ctypes.byref(im._grid), ctypes.byref(im._psf),
                                   backend.ptr(lane.params), n, im._prec,
                                   backend.IMAGE_SOURCE | backend.IMAGE_LENS_LIGHT, backend.ptr(lane.images),
                                   backend.ptr(lane.ws), lane.ws.numel(), st)
@@ -339,10 +353,15 @@ class LensServer(object):
         self._stop = False
 
     # ---------------------------------------------------------- admission --
-    def register(self, key, image_model, max_batch=64, n_slots=16, n_lanes=2, noise_maps=False):
+    def register(self, key, image_model, max_batch=64, n_slots=16, n_lanes=2, noise_maps=False, bulk=None):
+        """Register a bucket.  For an ImageModel with a LensModelBulk deflector,
+        ``bulk`` = (ray_shooting_kwargs, kwargs_source, kwargs_lens_light) fixes
+        the deflector structure (types, count, planes); its requests are then
+        [n_profiles, 8] parameter arrays in that structure's ImageModel.pack_bulk
+        layout."""
         if key in self.buckets:
             raise ValueError("bucket %r is already registered" % (key,))
-        b = _Bucket(key, image_model, max_batch, n_slots, n_lanes, self.use_graphs, noise_maps)
+        b = _Bucket(key, image_model, max_batch, n_slots, n_lanes, self.use_graphs, noise_maps, bulk)
         if self.hbm_used + b.nbytes > self.hbm_budget:
             raise MemoryError("bucket %r needs %d B; %d of %d B already used" % (key, b.nbytes, self.hbm_used,
                                                                               self.hbm_budget))

@@ -471,3 +471,51 @@ def test_sub_frame_bucket(precision):
             got = f.result().double()
             assert tuple(got.shape) == (n, n)
             assert torch.equal(got, _alone_image(im, params[i])), i
+
+
+@pytest.mark.parametrize("multi_plane", [False, True])
+def test_bulk_deflector_bucket(multi_plane):
+    """An ImageModel with a LensModelBulk deflector (EPL + SHEAR inline, 30
+    TNFW in the bulk loop; on one plane or on their own planes): the bucket's
+    deflector structure is fixed at registration, requests come in its
+    pack_bulk layout, and every result equals the same request alone."""
+    from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+    from jaxtronomy_amd.Util.cosmo import FlatLambdaCDM
+
+    n = 48
+    cfg = dict(W.SUBSTRUCTURE, numPix=n, deltaPix=0.08, supersampling=2)
+    kernel = W.moffat_kernel(9, 1.5, 3.0)
+    kl, subs, ks = W.substructure_draw(2, n_sub=30)
+    names = ["EPL", "SHEAR"] + ["TNFW"] * 30
+    lm = LensModelBulk(["EPL", "SHEAR", "TNFW"], multi_plane=multi_plane, cosmo=FlatLambdaCDM(70.0, 0.3),
+                       precision="mixed")
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, np.zeros((n, n)))), PSF("PIXEL", kernel_point_source=kernel), lm,
+                    LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 2}, precision="mixed")
+    if multi_plane:
+        z = [0.5, 0.5] + list(np.random.default_rng(8).uniform(0.2, 1.3, 30))
+        rs = lm.prepare_ray_shooting_kwargs(names, kl + subs, 1.5, z)
+    else:
+        rs = lm.prepare_ray_shooting_kwargs(names, kl + subs)
+    srv = LensServer(n_streams=2, use_graphs=True)
+    with pytest.raises(ValueError):
+        srv.register("no", im)
+    b = srv.register("bulk", im, max_batch=8, n_slots=2, n_lanes=2, bulk=(rs, ks, None))
+    base, _, _, _ = im.pack_bulk(rs, ks)
+    rng = np.random.default_rng(4)
+    params = np.repeat(base, 9, axis=0)
+    params[:, :, :5] *= 1 + 0.01 * rng.normal(size=params[:, :, :5].shape)
+    obs = [np.random.default_rng(s).normal(0, 0.05, (n, n)) for s in (1, 2)]
+    slots = [srv.put_data("bulk", o) for o in obs]
+    futs = []
+    for i in range(len(params)):
+        kind = "image" if i % 3 == 1 else "loglike"
+        futs.append((i, kind, srv.submit("bulk", params[i], kind=kind, slot=slots[i % 2])))
+    srv.drain()
+    for i, kind, f in futs:
+        p = torch.as_tensor(params[i][None], device="cuda")
+        if kind == "loglike":
+            data = torch.as_tensor(obs[i % 2][None], device="cuda").contiguous()
+            assert f.result() == im.likelihood_batch(p, model=b.model, data=data).item(), i
+        else:
+            ref = im.image_batch(p, model=b.model).double().cpu()[0]
+            assert torch.equal(f.result().double(), ref), i
