@@ -2455,9 +2455,20 @@ int launch_grad_profile(const ModelArgs &m, const GridArgs &g, const GradArgs &g
 // rewrites them; their duals would not follow), which costs a float dual all
 // its precision in the derivative by the centre.
 template <int TYPE, int N>
-__device__ __forceinline__ void bulk_tile_duals(const double *pk, const double *lx, const double *ly,
-                                                const double *lgx, const double *lgy, int r0, int step, int r1,
-                                                double *acc) {
+__device__ __forceinline__ void bulk_tile_duals(const double *pk, const double *rd, const double *lx,
+                                                const double *ly, const double *lgx, const double *lgy, int r0,
+                                                int step, int r1, double *acc) {
+  if constexpr (TYPE == LENSIM_SIE || TYPE == LENSIM_NIE) {  // orientation from the record (nie_dalpha)
+    for (int r = r0; r < r1; r += step) {
+      const double gx = lgx[r], gy = lgy[r];
+      if (gx == 0.0 && gy == 0.0) continue;
+      double a_x, a_y, dax[6], day[6];
+      nie_dalpha<double, TYPE>(rd, pk, lx[r], ly[r], a_x, a_y, dax, day);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] += gx * dax[i] + gy * day[i];
+    }
+    return;
+  }
   using S = gm::Dual<double, N>;
   S ps[LENSIM_NPARAM];
 #pragma unroll
@@ -2493,6 +2504,7 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
     pk[q] = mine ? AT(ga.params, (size_t(img) * m.n_prof + k) * LENSIM_NPARAM + q) : 0.0;
     acc[q] = 0.0;
   }
+  const double *rdk = rp.recd + (mine ? k : 0) * kRec;  // this deflector's record (SIE / NIE orientation)
   __shared__ double lx[256], ly[256], lgx[256], lgy[256];
   __shared__ double red[256][LENSIM_NPARAM];
   const int p0 = blockIdx.x * kGradPix * 256, p1 = min(npix, p0 + kGradPix * 256);
@@ -2530,20 +2542,20 @@ __global__ __launch_bounds__(256) void bulk_grad_kernel(ModelArgs m, GridArgs g,
     __syncthreads();
     const int r1 = min(256, r_end - base);
     if constexpr (TYPE != LENSIM_NULL) {
-      if (mine) bulk_tile_duals<TYPE, n_params_of(TYPE)>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+      if (mine) bulk_tile_duals<TYPE, n_params_of(TYPE)>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
     } else {
       switch (type) {
-        case LENSIM_SIE: bulk_tile_duals<LENSIM_SIE, 5>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
-        case LENSIM_NIE: bulk_tile_duals<LENSIM_NIE, 6>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_SIE: bulk_tile_duals<LENSIM_SIE, 5>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_NIE: bulk_tile_duals<LENSIM_NIE, 6>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
         case LENSIM_NFW:
-          if constexpr (HALO) bulk_tile_duals<LENSIM_NFW, 4>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          if constexpr (HALO) bulk_tile_duals<LENSIM_NFW, 4>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
           break;
         case LENSIM_TNFW:
-          if constexpr (HALO) bulk_tile_duals<LENSIM_TNFW, 5>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          if constexpr (HALO) bulk_tile_duals<LENSIM_TNFW, 5>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
           break;
-        case LENSIM_SHEAR: bulk_tile_duals<LENSIM_SHEAR, 4>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
+        case LENSIM_SHEAR: bulk_tile_duals<LENSIM_SHEAR, 4>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc); break;
         case LENSIM_CONVERGENCE:
-          bulk_tile_duals<LENSIM_CONVERGENCE, 3>(pk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
+          bulk_tile_duals<LENSIM_CONVERGENCE, 3>(pk, rdk, lx, ly, lgx, lgy, grp, ngrp, r1, acc);
           break;
         default: break;  // no kwargs: zero partials
       }
@@ -2617,6 +2629,24 @@ __device__ __forceinline__ void dalpha_duals(const double *pk, double x, double 
   dcen[3] = sy.d[N - 1];
 }
 
+// nie_dalpha in double, laid out as dalpha_duals'.
+template <int TYPE>
+__device__ __forceinline__ void nie_dalpha_mp(const double *rd, const double *pk, double x, double y, double &ax,
+                                              double &ay, double *dax, double *day, double *dcen) {
+  constexpr int N = n_params_of(TYPE);
+  double dx[6], dy[6];
+  nie_dalpha<double, TYPE>(rd, pk, x, y, ax, ay, dx, dy);
+#pragma unroll
+  for (int i = 0; i < LENSIM_NPARAM; ++i) {
+    dax[i] = i < N ? dx[i] : 0.0;
+    day[i] = i < N ? dy[i] : 0.0;
+  }
+  dcen[0] = dx[N - 2];
+  dcen[1] = dy[N - 2];
+  dcen[2] = dx[N - 1];
+  dcen[3] = dy[N - 1];
+}
+
 template <typename T, bool HALO>
 __device__ __forceinline__ void deflector_dalpha(const ModelArgs &m, const GradArgs &ga, const RecPtrs<T> &rp,
                                                  int img, int k, int type, double x, double y, double &ax,
@@ -2640,8 +2670,8 @@ __device__ __forceinline__ void deflector_dalpha(const ModelArgs &m, const GradA
       dcen[3] = day[5];
       break;
     }
-    case LENSIM_SIE: dalpha_duals<LENSIM_SIE, 5>(pk, x, y, ax, ay, dax, day, dcen); break;
-    case LENSIM_NIE: dalpha_duals<LENSIM_NIE, 6>(pk, x, y, ax, ay, dax, day, dcen); break;
+    case LENSIM_SIE: nie_dalpha_mp<LENSIM_SIE>(rp.recd + k * kRec, pk, x, y, ax, ay, dax, day, dcen); break;
+    case LENSIM_NIE: nie_dalpha_mp<LENSIM_NIE>(rp.recd + k * kRec, pk, x, y, ax, ay, dax, day, dcen); break;
     case LENSIM_NFW:
       if constexpr (HALO) dalpha_duals<LENSIM_NFW, 4>(pk, x, y, ax, ay, dax, day, dcen);
       break;
