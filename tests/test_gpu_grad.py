@@ -166,26 +166,34 @@ def test_grad_gaussian_psf_vs_oracle_fd(precision, tol):
                 assert abs(grads[j][name] - fd) <= tol * max(abs(fd), 1.0), (which, j, name, grads[j][name], fd)
 
 
+@pytest.mark.parametrize("psf_type", ["PIXEL", "GAUSSIAN"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-5), ("mixed", 1e-4)])
-def test_sub_frame_grad_vs_oracle_fd(precision, tol):
+def test_sub_frame_grad_vs_oracle_fd(precision, tol, psf_type):
     """flux_evaluate_indexes sub-frame: the gradient runs on the square around
     the evaluated pixels, logL covers the whole frame (the pixels outside the
     square, where the model is zero, add a constant); logL against the oracle's
     completed-frame likelihood and the gradient against its central
-    differences, with a likelihood mask that cuts through the square."""
+    differences, with a likelihood mask that cuts through the square (GAUSSIAN:
+    the convolution's edge padding at the square's border)."""
     n = 24
     cfg = dict(W.EPL180, numPix=n, supersampling=2, deltaPix=0.15)
     g = W.grid_dict(cfg)
-    kernel = W.moffat_kernel(5, 1.5, 3.0)
+    if psf_type == "PIXEL":
+        kernel = W.moffat_kernel(5, 1.5, 3.0)
+        psf = PSF("PIXEL", kernel_point_source=kernel)
+    else:
+        psf = PSF("GAUSSIAN", fwhm=0.3, truncation=3)
+        kernel = O.gaussian_kernel(0.3, cfg["deltaPix"], 3)
     kl, ks, kll = W.epl180_draw(2)
     ev = np.zeros((n, n), bool)
     yy, xx = np.mgrid[0:n, 0:n]
     ev[(yy - 11) ** 2 + (xx - 13) ** 2 < 8 ** 2] = True
     lmask = np.ones((n, n))
     lmask[:, :9] = 0
-    m = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel)
+    m = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, psf_type=psf_type,
+                kernel=kernel)
     data = W.noisy(m, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(7))
-    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel),
+    im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), psf,
                     LensModel(["EPL", "SHEAR"]), LightModel(["SERSIC_ELLIPSE"]), LightModel(["SERSIC_ELLIPSE"]),
                     kwargs_numerics={"supersampling_factor": 2, "flux_evaluate_indexes": ev},
                     likelihood_mask=lmask, precision=precision)
@@ -194,8 +202,8 @@ def test_sub_frame_grad_vs_oracle_fd(precision, tol):
     logl, glens, gsrc, glight = im.likelihood_gradient(kl, ks, kll)
 
     def ll(klx, ksx, kllx):
-        mm = O.image(g, ["EPL", "SHEAR"], klx, ["SERSIC_ELLIPSE"], ksx, ["SERSIC_ELLIPSE"], kllx, kernel=kernel,
-                     flux_evaluate_indexes=ev)
+        mm = O.image(g, ["EPL", "SHEAR"], klx, ["SERSIC_ELLIPSE"], ksx, ["SERSIC_ELLIPSE"], kllx,
+                     psf_type=psf_type, kernel=kernel, flux_evaluate_indexes=ev)
         return O.log_likelihood(mm, data, mask=lmask, background_rms=cfg["background_rms"],
                                 exposure_map=cfg["exposure_time"])
 
@@ -219,8 +227,8 @@ def test_sub_frame_grad_vs_oracle_fd(precision, tol):
     assert float(logl2) == float(logl)
     im.Data.update_data(data * 1.01)
     logl3, _, _, _ = im.likelihood_gradient(kl, ks, kll)
-    mm = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, kernel=kernel,
-                 flux_evaluate_indexes=ev)
+    mm = O.image(g, ["EPL", "SHEAR"], kl, ["SERSIC_ELLIPSE"], ks, ["SERSIC_ELLIPSE"], kll, psf_type=psf_type,
+                 kernel=kernel, flux_evaluate_indexes=ev)
     ll3 = O.log_likelihood(mm, data * 1.01, mask=lmask, background_rms=cfg["background_rms"],
                            exposure_map=cfg["exposure_time"])
     assert abs(float(logl3) - ll3) < ltol * abs(ll3)
