@@ -1193,6 +1193,20 @@ __device__ __forceinline__ double quad_sum(double v) {
   return v;
 }
 
+// The sum over a wave (every lane active), uniform: the quads' sums, then
+// the row's (row_ror 4 and 8 within each row of 16 lanes), then the four rows'
+// sums read from their first lanes and added in order.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = quad_sum(v);
+  v += dpp_quad_f64<0x124>(v);  // row_ror:4
+  v += dpp_quad_f64<0x128>(v);  // row_ror:8
+  auto row = [&](int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+  };
+  return (row(0) + row(16)) + (row(32) + row(48));
+}
+
 // Multi-plane ray tracing (MultiPlaneBulk.ray_shooting,
 // LensModel/MultiPlane/multi_plane_bulk.py:152-212) with kMpG lanes per ray.
 // Deflectors on one plane follow each other in the plane order with T_ij = 0
@@ -2814,15 +2828,18 @@ __global__ __launch_bounds__(256) void mp_grad_kernel(ModelArgs m, GridArgs g, G
       const int buf = step++ & 1;
 #pragma unroll
       for (int q = 0; q < LENSIM_NPARAM; ++q) {
-        double v = c[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        const double v = wave_sum(c[q]);
         if ((t & 63) == 0) red[buf][t >> 6][q] = v;
       }
       __syncthreads();
       if (t < n) {
         double v = 0.0;
         for (int wv = 0; wv < int(blockDim.x >> 6); ++wv) v += red[buf][wv][t];
-        AT(gpart_mp, slot(k) + t) += v;
+        // the block's first sweep stores (no wait for a load), later ones add
+        if (base == r0)
+          AT(gpart_mp, slot(k) + t) = v;
+        else
+          AT(gpart_mp, slot(k) + t) += v;
       }
     }
   }
