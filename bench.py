@@ -849,6 +849,93 @@ def main_los(args):
     print(json.dumps(line), flush=True)
 
 
+def main_los_grad(args):
+    """logL and its gradient with respect to every deflector and source
+    parameter of the line-of-sight scene (workloads.los_halo_draw: 5130 TNFW +
+    126 CONVERGENCE sheets on 126 planes, z_source 2.5) for a SERSIC_ELLIPSE
+    source on a 100 x 100 grid (0.2"/pixel, no supersampling, 5x5 Moffat PSF):
+    jax.grad of likelihood_data_given_model through
+    LensModelBulk(multi_plane=True) (MultiPlaneBulk.ray_shooting,
+    LensModel/MultiPlane/multi_plane_bulk.py:152-212, differentiated in reverse).
+    value: milliseconds per call of one image, FP64; the mixed form and a batch
+    of 8 images beside it."""
+    import torch
+
+    from jaxtronomy_amd import backend
+    from jaxtronomy_amd import workloads as W
+    from jaxtronomy_amd.Data.imaging_data import ImageData
+    from jaxtronomy_amd.Data.psf import PSF
+    from jaxtronomy_amd.ImSim.image_model import ImageModel
+    from jaxtronomy_amd.LensModel.lens_model_bulk import LensModelBulk
+    from jaxtronomy_amd.LightModel.light_model import LightModel
+
+    names, kw, z = W.los_halo_draw(1)
+    ks = [dict(amp=40.0, R_sersic=0.8, n_sersic=1.5, e1=0.1, e2=-0.05, center_x=0.3, center_y=-0.2)]
+    npix = 100
+    cfg = dict(W.EPL180, numPix=npix, deltaPix=20.0 / npix, supersampling=1)
+    kernel = W.moffat_kernel(5, 1.2, 3.0)
+
+    def model(prec, data):
+        lm = LensModelBulk(["TNFW", "CONVERGENCE"], multi_plane=True, precision=prec)
+        im = ImageModel(ImageData(**W.data_kwargs(cfg, data)), PSF("PIXEL", kernel_point_source=kernel), lm,
+                        LightModel(["SERSIC_ELLIPSE"]), kwargs_numerics={"supersampling_factor": 1}, precision=prec)
+        rs = lm.prepare_ray_shooting_kwargs(names, kw, z_source=W.LOS["z_source"], lens_redshift_list=z)
+        return im, rs
+
+    im0, rs0 = model("fp64", np.zeros((npix, npix)))
+    clean = im0.image(rs0, ks).cpu().numpy()
+    data = W.noisy(clean, cfg["exposure_time"], cfg["background_rms"], np.random.default_rng(3))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        im, rs = model(prec, data)
+        params, _, m, keep = im.pack_bulk(rs, ks)
+        for B in (1, 8):
+            pt = backend.to_device_f64(np.repeat(params, B, axis=0))
+            log("bench: los-grad %s, batch %d (%d steps)" % (prec, B, args.steps))
+            for _ in range(max(1, args.warmup)):
+                im.likelihood_and_grad_batch(pt, B, model=m)
+            torch.cuda.synchronize()
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+            evs[0].record()
+            for i in range(args.steps):
+                ll, g = im.likelihood_and_grad_batch(pt, B, model=m)
+                evs[i + 1].record()
+            torch.cuda.synchronize()
+            per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)])
+            res[(prec, B)] = dict(ms=float(np.mean(per)), p99=float(np.percentile(per, 99)),
+                                  ll=ll.cpu().numpy(), g=g.cpu().numpy())
+        del keep
+    g64, g32 = res[("fp64", 1)]["g"], res[("mixed", 1)]["g"]
+    scale = np.maximum(np.max(np.abs(g64), axis=(0, 1)), 1e-300)  # per parameter slot, over all profiles
+    err = float(np.max(np.max(np.abs(g32 - g64), axis=(0, 1)) / scale))
+    ll64, ll32 = res[("fp64", 1)]["ll"], res[("mixed", 1)]["ll"]
+    n_par = int(np.count_nonzero(g64[0]))
+    line = {
+        "metric": "ms per logL + gradient call, line-of-sight multi-plane scene (5130 TNFW + 126 CONVERGENCE "
+                  "planes), 100^2, one image, FP64",
+        "value": round(res[("fp64", 1)]["ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res[("fp64", 1)]["ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: a seeded line-of-sight realisation and mock data from it",
+        "config": {"workload": "LensModelBulk(['TNFW', 'CONVERGENCE'], multi_plane=True) + SERSIC_ELLIPSE source, "
+                               "100x100 @0.2\", supersampling 1, 5x5 Moffat PSF, logL and its gradient in every "
+                               "deflector and source parameter", "batch": 1, "nonzero_gradient_entries": n_par},
+        "p99_ms": round(res[("fp64", 1)]["p99"], 4),
+        "batch8_ms": round(res[("fp64", 8)]["ms"], 4),
+        "mixed": {"ms_per_step": round(res[("mixed", 1)]["ms"], 4), "batch8_ms": round(res[("mixed", 8)]["ms"], 4)},
+        "accuracy": {"mixed_grad_max_err_rel_to_slot_scale_vs_fp64": err,
+                     "mixed_logl_max_rel_vs_fp64": float(np.max(np.abs(ll32 - ll64) / np.abs(ll64)))},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -861,7 +948,7 @@ def main():
     ap.add_argument("--settle", type=float, default=0.5, help="seconds of untimed sustained load before the timed "
                     "phases (epl180)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60", "los"),
+    ap.add_argument("--workload", choices=("epl180", "substructure", "grad", "serve-mix", "sie60", "los", "los-grad"),
                     default="epl180")
     ap.add_argument("--in-flight", type=int, default=3, help="batches in flight for the headline value (epl180)")
     ap.add_argument("--mix-slots", type=int, default=256, help="resident observations per bucket (serve-mix)")
@@ -902,6 +989,8 @@ def main():
         return main_sie60(args)
     if args.workload == "los":
         return main_los(args)
+    if args.workload == "los-grad":
+        return main_los_grad(args)
     log("bench: building models")
     cfg, kernel, draws, data, params, params_np, im64, im32 = build_models(args.batch, args.in_flight)
     n_ray = (cfg["numPix"] * cfg["supersampling"]) ** 2

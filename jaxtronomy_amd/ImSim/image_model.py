@@ -417,7 +417,7 @@ class ImageModel(object):
         """(logL [B], dlogL/dparams [B, n_profiles, 8]) of a packed device parameter
         block; the gradient slot layout is the parameter-block layout.  A
         LensModelBulk deflector passes the lensim_model of pack_bulk as model
-        (single-plane lists: multi-plane gradients are not available)."""
+        (single-plane and multi-plane lists alike)."""
         lib = backend.lib()
         if self._bulk and model is None:
             raise ValueError("a LensModelBulk deflector needs the model of pack_bulk")

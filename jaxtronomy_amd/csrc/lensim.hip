@@ -1098,6 +1098,28 @@ __global__ __launch_bounds__(256) void render_fast_kernel(ModelArgs m, GridArgs 
 
 int check_launch(const char *where);
 
+// The per-256-pixel block maxima of |image| that render_kernel stores for the
+// MFMA convolution, from an image rendered by a kernel that keeps none.
+template <typename T>
+__global__ __launch_bounds__(256) void block_max_kernel(const T *__restrict__ v, int npix,
+                                                        float *__restrict__ blkmax) {
+  const int img = blockIdx.y;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const float a = pix < npix ? fabsf(float(AT(v, size_t(img) * npix + pix))) : 0.f;
+  block_max_store(a, blkmax + size_t(img) * gridDim.x + blockIdx.x);
+}
+
+// Lanes per ray of the multi-plane kernels that share a plane's deflectors
+// among a quad (raytrace_mp_kernel, render_mp_kernel).
+constexpr int kMpG = 4;
+
+template <typename T, bool HALO, bool DBL>
+__global__ __launch_bounds__(256) void render_mp_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
+                                                        const float *__restrict__ recf,
+                                                        const double *__restrict__ coefd,
+                                                        const float *__restrict__ coeff, unsigned flags,
+                                                        T *__restrict__ out);
+
 // Launches render_fast_kernel when the model is one of its compile-time lists
 // (EPL, SIE / NIE or NFW, each with or without SHEAR after it, NFW + SIE / NIE
 // in either order; one Sersic source, none or one Sersic lens light), else
@@ -1118,6 +1140,32 @@ int launch_render(const ModelArgs &m, const GridArgs &g, const double *recd, con
                       (d0 == LENSIM_NFW && d1 == LENSIM_NIE) || (d0 == LENSIM_NIE && d1 == LENSIM_NFW);
     fast = pair && (d0 != LENSIM_EPL || m.epl_index[0] == 0);
     for (int k = m.n_lens; fast && k < m.n_lens + m.n_source + m.n_light; ++k) fast = sersic(m.types[k]);
+  }
+  // a multi-plane launch of fewer than 2 waves per SIMD at one lane per pixel
+  // takes a quad per pixel (render_mp_kernel; the block maxima for the MFMA
+  // convolution follow from the image in a second launch)
+  const bool quad = m.mp_n && size_t(g.nx) * g.ny * size_t(n_img) < 131072;
+  if (!fast && quad) {
+    const dim3 qgrid((unsigned)((size_t(g.nx) * g.ny * kMpG + 255) / 256), n_img);
+#define LENSIM_RENDER(H, D) \
+  render_mp_kernel<T, H, D><<<qgrid, 256, 0, st>>>(m, g, recd, recf, coefd, coeff, flags, out)
+    bool done = false;
+    if constexpr (sizeof(T) == sizeof(float)) {
+      if (m.has_nie_nfw) {
+        if (m.has_halo) LENSIM_RENDER(true, true);
+        else LENSIM_RENDER(false, true);
+        done = true;
+      }
+    }
+    if (!done) {
+      if (m.has_halo) LENSIM_RENDER(true, false);
+      else LENSIM_RENDER(false, false);
+    }
+#undef LENSIM_RENDER
+    int rc = check_launch("render_mp_kernel");
+    if (rc || !blkmax) return rc;
+    block_max_kernel<T><<<rgrid, 256, 0, st>>>(out, g.nx * g.ny, blkmax);
+    return check_launch("block_max_kernel");
   }
   if (!fast) {
     // SIE / NIE and NFW deflect in double in the float form too; the kernel
@@ -1222,7 +1270,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 // FP64 form's software transcendentals keep the vector ALU busy at 2.5 waves
 // per SIMD already, and the gathers and quad sums made it slower (10.8 ->
 // 11.9 ms), so it keeps one lane per ray.
-constexpr int kMpG = 4;
 
 template <typename T, bool HALO, bool DBL>
 __global__ __launch_bounds__(256) void raytrace_mp_kernel(ModelArgs m, const double *__restrict__ recd,
@@ -1275,6 +1322,83 @@ __global__ __launch_bounds__(256) void raytrace_mp_kernel(ModelArgs m, const dou
     AT(bx, size_t(img) * n_ray + ray) = px / tz;
     AT(by, size_t(img) * n_ray + ray) = py / tz;
   }
+}
+
+// render_kernel for multi-plane lists with a quad per pixel: the quad walks
+// the pixel's sub-rays together and shares each plane's deflectors among its
+// lanes as raytrace_mp_kernel does (with render_kernel's deflection
+// arithmetic, trace_ray's), so every lane ends with the sub-ray's beta; every
+// lane adds the light, lane 0 stores.  For launches too small to fill the
+// SIMDs at one lane per pixel (a gradient's image of one line-of-sight scene).
+template <typename T, bool HALO, bool DBL>
+__global__ __launch_bounds__(256) void render_mp_kernel(ModelArgs m, GridArgs g, const double *__restrict__ recd,
+                                                        const float *__restrict__ recf,
+                                                        const double *__restrict__ coefd,
+                                                        const float *__restrict__ coeff, unsigned flags,
+                                                        T *__restrict__ out) {
+  const int img = blockIdx.y;
+  const int npix = g.nx * g.ny;
+  const int slot = threadIdx.x & (kMpG - 1);
+  const int pix0 = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) / kMpG);
+  const bool active = pix0 < npix;
+  const int pix = active ? pix0 : npix - 1;
+  int i;
+  const int j = div_by(pix, g.nx, g.inv_nx, i);
+  const RecPtrs<T> rp = rec_ptrs<T>(m, recd, recf, coefd, coeff, img);
+  const double ra_c = i * g.m00 + j * g.m01 + g.ra0;
+  const double dec_c = i * g.m10 + j * g.m11 + g.dec0;
+  const double inv_s = g.inv_s;
+  const double o0 = g.o0;
+  const bool do_src = (flags & LENSIM_IMAGE_SOURCE) && m.n_source > 0;
+  const bool do_light = (flags & LENSIM_IMAGE_LENS_LIGHT) && m.n_light > 0;
+  const int k_src0 = m.n_defl, k_src1 = m.n_defl + m.n_source, k_l1 = m.n_prof;
+  const size_t o = size_t(img) * size_t(m.mp_stride);
+  const double *tij = m.mp_tij + o, *tzs = m.mp_tz + o, *r2p = m.mp_r2p + o;
+  const int nd = m.mp_n - 1;  // planes with a deflector; the last one is the source's
+  double acc = 0.0;
+  const bool eval = active && evaluated(g, i, j);  // one value per quad (one pixel)
+  for (int sy = 0; eval && sy < g.s; ++sy) {
+    const double oy = o0 + sy * inv_s;
+    for (int sx = 0; sx < g.s; ++sx) {
+      const double ox = o0 + sx * inv_s;
+      const double x = ra_c + g.m00 * ox + g.m01 * oy;
+      const double y = dec_c + g.m10 * ox + g.m11 * oy;
+      if (do_src) {
+        double angx = x, angy = y, px = 0.0, py = 0.0;
+        int p = 0;
+        while (p < nd) {
+          const double dt = AT(tij, p);
+          px += angx * dt;
+          py += angy * dt;
+          int mc = 1;  // deflectors p .. p + mc - 1 see the same position
+          while (mc < kMpG && p + mc < nd && AT(tij, p + mc) == 0.0) ++mc;
+          double ax = 0.0, ay = 0.0;
+          if (slot < mc) {
+            const int q = p + slot;
+            int k = AT(m.mp_order, q);
+            k = k < 0 ? 0 : (k >= m.n_defl ? m.n_defl - 1 : k);  // a bad index cannot read outside the records
+            const double tz = AT(tzs, q);
+            double dx = 0.0, dy = 0.0;
+            deflect_one<T, HALO, DBL, DBL>(m, rp, img, k, px / tz, py / tz, dx, dy);
+            const double f = AT(r2p, q);
+            ax = dx * f;
+            ay = dy * f;
+          }
+          angx -= quad_sum(ax);
+          angy -= quad_sum(ay);
+          p += mc;
+        }
+        const double dt = AT(tij, nd);
+        px += angx * dt;
+        py += angy * dt;
+        const double tz = AT(tzs, nd);
+        acc += light_sum<T>(m, rp, k_src0, k_src1, px / tz, py / tz);
+      }
+      if (do_light) acc += light_sum<T>(m, rp, k_src1, k_l1, x, y);
+    }
+  }
+  const T v = T(acc * (g.scale * m.flux_scaling));
+  if (active && slot == 0) AT(out, size_t(img) * npix + pix) = v;
 }
 
 template <typename T>
